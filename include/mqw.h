@@ -1,0 +1,169 @@
+/*
+ * mqw.h -- C ABI of libmqw, the MI355X (gfx950) multi-stage quantum-walk
+ * simulator.  Drop-in for the hot path of Asa-Hopkins/MultiStageQW
+ * (MultiQW.cpp): Ising energies + ground-state search, the per-stage
+ * gamma/t heuristic, the Chebyshev/Clenshaw exp(iHt) propagator with a
+ * matrix-free H|psi>, renormalisation and the ground-state probability.
+ *
+ * Conventions
+ *   - Every function returns mqw_status (0 = MQW_OK).  Functions that take a
+ *     context leave a message in mqw_last_error(ctx) on failure.
+ *   - Complex states are planar, like the reference (MultiQW.cpp:191-194):
+ *     N = 2^n real parts followed by N imaginary parts.
+ *   - Pointers named *_dev are device pointers (hipMalloc'ed on the
+ *     context's device); every other pointer is caller-owned host memory.
+ *   - A context owns one HIP stream and a grow-only device workspace; use one
+ *     context per host thread.
+ *   - H = H_P - gamma * sum_i X_i (the reference's sign, MultiQW.cpp:92), and
+ *     the propagator applies exp(+iHt) (odd Chebyshev terms carry +i,
+ *     MultiQW.cpp:206-217).
+ */
+#ifndef MQW_H
+#define MQW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MQW_ABI_VERSION 1
+#define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
+#define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
+#define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
+#define MQW_MAX_STAGES 64
+#define MQW_MIN_N 4  /* the reference's 16-wide loads need N >= 16 (MultiQW.cpp:117-119) */
+#define MQW_MAX_N 30
+
+typedef enum {
+    MQW_OK = 0,
+    MQW_EINVAL = 1,  /* bad argument (n, m, sizes, NaN parameters ...) */
+    MQW_ENOMEM = 2,  /* host or device allocation failed */
+    MQW_EIO = 3,     /* file missing, short or unwritable (CLI helpers) */
+    MQW_EDEVICE = 4, /* HIP runtime error */
+    MQW_ENOTSUP = 5
+} mqw_status;
+
+typedef enum { MQW_ENERGY_COMPAT = 0, MQW_ENERGY_EXACT = 1 } mqw_energy_mode;
+typedef enum { MQW_FP32 = 0, MQW_FP64 = 1 } mqw_precision;
+
+typedef struct mqw_ctx mqw_ctx;
+
+/* Result of the energy build (MultiQW.cpp:325-362). */
+typedef struct {
+    double e0;        /* lowest energy of the walk */
+    double emax;      /* highest energy */
+    uint32_t eloc;    /* ground-state index; the first in Gray order on ties */
+    float shift;      /* (float)((emax+e0)/2), subtracted from H_P */
+    float e_abs;      /* (float)((emax-e0)/2), the diagonal's spectral radius */
+    double e_init;    /* energy of the all -1 state */
+} mqw_energy_info;
+
+/* Heuristic schedule (MultiQW.cpp:365-423). */
+typedef struct {
+    int n, m;
+    float a;       /* expected maximum of 2^n Gaussians, in sigma units */
+    float hp2;     /* mean_k E(k)^2 = 2 sum J^2 + sum h^2 */
+    float delta2;  /* 16 sum J^2 + 4 sum h^2 */
+    float total_t; /* sum of short_t, printed before each result */
+    float gamma[MQW_MAX_STAGES];
+    float short_t[MQW_MAX_STAGES];
+} mqw_schedule;
+
+/* Options of a solve.  Initialise with mqw_opts_default. */
+typedef struct {
+    int samples;       /* Monte-Carlo samples per problem (100) */
+    double tol;        /* Chebyshev tail tolerance (1e-6) */
+    int precision;     /* mqw_precision: state/recurrence in FP32 (default) or FP64 */
+    int energy_mode;   /* mqw_energy_mode for H_P and E_loc (COMPAT = the reference walk) */
+    int tile_bits;     /* 0 = tuned/default; else bits of a sweep tile (10..13) */
+    int chunk_bits;    /* -1 = tuned/default; XCD chunk of consecutive tiles, log2 */
+    int small_n_max;   /* -1 = default; largest n run by the LDS-resident kernel (0 = never) */
+    int use_graph;     /* 1 = capture each stage plan in a hipGraph (default 1) */
+    size_t max_device_bytes; /* 0 = default budget for trajectory buffers */
+    float* sample_probs;     /* optional output: count*samples per-sample probabilities */
+} mqw_opts;
+
+/* One problem of a batch; gamma/short_t override the heuristic when both are
+ * non-NULL (length m): the learned-schedule path. */
+typedef struct {
+    int n, m;
+    const double* rec;    /* n(n+1)/2 doubles in the reference file layout */
+    const float* gamma;
+    const float* short_t;
+} mqw_instance;
+
+/* ---- context ------------------------------------------------------------ */
+mqw_status mqw_create(int device, mqw_ctx** out);
+void mqw_destroy(mqw_ctx* ctx);
+const char* mqw_last_error(const mqw_ctx* ctx);
+const char* mqw_status_string(mqw_status s);
+int mqw_abi_version(void);
+void mqw_opts_default(mqw_opts* o);
+
+/* ---- host-side pieces of the path (no device) ---------------------------- */
+/* Unpack one record (MultiQW.cpp:303-323): J symmetric n*n with the
+ * reference's halving, h of length n, both float. */
+mqw_status mqw_unpack(int n, const double* rec, float* J, float* h);
+/* Heuristic gamma and times per stage (MultiQW.cpp:365-423). */
+mqw_status mqw_schedule_heuristic(int n, int m, const double* rec, mqw_schedule* out);
+/* Monte-Carlo times times[i*samples+j] (MultiQW.cpp:387,417-430): MT19937
+ * re-seeded per problem, U(short_t, 2 short_t) stage-major; m == 1 sorts and
+ * differences.  short_t has m entries. */
+mqw_status mqw_sample_times(int m, int samples, const float* short_t, float* times);
+/* Chebyshev coefficients of cos(sx)+sin(sx) on [-1,1] in FP64, truncated at
+ * tail sum <= tol (replaces ApproxTools RCF_odd_even, MultiQW.cpp:438-440). */
+mqw_status mqw_chebyshev_coeffs(double s, double tol, double* c, int cap, int* K);
+/* Seeded SK instances in the reference file layout: count records of
+ * n(n+1)/2 N(0,1) doubles (SplitMix64-seeded xoshiro256**, Marsaglia polar). */
+mqw_status mqw_generate_sk(int n, int count, uint64_t seed, double* out);
+
+/* ---- device pieces -------------------------------------------------------- */
+/* Energy build + ground-state search on the GPU (MultiQW.cpp:325-362).
+ * hp (nullable, 2^n floats) receives the shifted float diagonal H_P;
+ * e (nullable, 2^n doubles) receives the unshifted FP64 energies. */
+mqw_status mqw_energy(mqw_ctx* ctx, int n, const double* rec, int mode, float* hp, double* e,
+                      mqw_energy_info* info);
+
+/* One Clenshaw step on one real half, device pointers (the seam at
+ * MultiQW.cpp:88-173): b2 <- (2/scale)(H_P - gamma H_G) b1 - b2 + coef psi. */
+mqw_status mqw_clenshaw_step_dev(mqw_ctx* ctx, const float* b1_dev, float* b2_dev, const float* hp_dev,
+                                 const float* psi_dev, unsigned n, float scale, float gamma, float coef);
+
+/* Clenshaw evaluation of sum_r c_r i^(r odd) T_r(H/scale) psi on host arrays
+ * (the seam at MultiQW.cpp:177-236).  psi: 2^(n+1) floats, replaced by the
+ * (unnormalised) result; hp: 2^n floats. */
+mqw_status mqw_clenshaw(mqw_ctx* ctx, int n, const float* coeffs, int K, float* psi, const float* hp,
+                        float gamma, float scale, int psi_real);
+
+/* The per-problem loop of MultiQW.cpp:296-455 for count problems of one
+ * (n, m): recs holds count*n(n+1)/2 doubles.  prob and total_t receive one
+ * float per problem.  opts may be NULL (defaults). */
+mqw_status mqw_solve(mqw_ctx* ctx, int n, int m, const double* recs, int count, const mqw_opts* opts,
+                     float* prob, float* total_t);
+
+/* Uneven batch: problems of different (n, m) in one call. */
+mqw_status mqw_solve_batch(mqw_ctx* ctx, const mqw_instance* inst, int count, const mqw_opts* opts,
+                           float* prob, float* total_t);
+
+/* Autotuner: times sweep configurations for n on this device, stores the
+ * best in the context (and in the cache file when path != NULL). */
+mqw_status mqw_tune(mqw_ctx* ctx, int n, const char* cache_path, int* tile_bits, int* chunk_bits);
+
+/* Kernel statistics of the last solve on this context. */
+typedef struct {
+    uint64_t sweeps;         /* term-kernel launches (global-memory path) */
+    uint64_t term_units;     /* complex H-applies executed, summed over trajectories */
+    uint64_t amp_terms;      /* sum over applies of 2^n (amplitude-terms) */
+    double algorithmic_bytes;/* 36 B per amplitude-term (FP32) or 72 (FP64), executed */
+    double gpu_ms;           /* device time of the propagation (events) */
+    double sweep_ms;         /* device time inside term kernels only (events) */
+    uint64_t lds_trajectories; /* trajectories run by the LDS-resident kernel */
+} mqw_stats;
+mqw_status mqw_last_stats(const mqw_ctx* ctx, mqw_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MQW_H */

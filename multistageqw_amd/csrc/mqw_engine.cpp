@@ -1,0 +1,910 @@
+// Host engine of libmqw: context, device workspace, energy build, the plan of
+// Clenshaw sweeps for a batch of trajectories, hipGraph capture, and the C ABI
+// declared in include/mqw.h.
+//
+// Reference: Asa-Hopkins/MultiStageQW MultiQW.cpp.  A "trajectory" is one
+// chain of stages that starts from |+>: for m > 1 each Monte-Carlo sample is a
+// trajectory of m stages (the state is reset per sample, :447-451); for m == 1
+// the samples are chained (sorted, differenced times, :426-430) into one
+// trajectory of `samples` stages with a readout after each.  Each stage is a
+// Chebyshev series of K terms evaluated by the Clenshaw recurrence; its K-1
+// sweeps (the reference's injection-only top term is folded into the first
+// sweep) run as term-kernel launches in lockstep across the trajectories of a
+// batch, aligned so every trajectory ends its stage on the same launch.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "mqw_internal.h"
+
+using namespace mqw;
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+};
+
+struct mqw_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<std::string, DevBuf> bufs;
+    mqw_stats stats{};
+    int tuned_tb[MQW_MAX_N + 1];
+    int tuned_lc[MQW_MAX_N + 1];
+};
+
+namespace {
+
+mqw_status fail(mqw_ctx* c, mqw_status s, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return s;
+}
+
+#define HIPCHK(ctx, expr)                                                                                   \
+    do {                                                                                                    \
+        hipError_t _e = (expr);                                                                             \
+        if (_e != hipSuccess) return fail((ctx), MQW_EDEVICE, "%s: %s", #expr, hipGetErrorString(_e));      \
+    } while (0)
+
+// Grow-only named device buffer.  Growing synchronises the stream first.
+void* dev_get(mqw_ctx* c, const std::string& name, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    DevBuf& b = c->bufs[name];
+    if (b.cap >= bytes) return b.ptr;
+    hipStreamSynchronize(c->stream);
+    if (b.ptr) hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.cap = 0;
+    const size_t want = (bytes + 4095) & ~(size_t)4095;
+    if (hipMalloc(&b.ptr, want) != hipSuccess) {
+        hipGetLastError();
+        b.ptr = nullptr;
+        return nullptr;
+    }
+    b.cap = want;
+    return b.ptr;
+}
+
+void release_bufs(mqw_ctx* c)
+{
+    hipStreamSynchronize(c->stream);
+    for (auto& kv : c->bufs)
+        if (kv.second.ptr) hipFree(kv.second.ptr);
+    c->bufs.clear();
+}
+
+bool finite_rec(int n, const double* rec)
+{
+    const int P = n * (n + 1) / 2;
+    for (int k = 0; k < P; k++)
+        if (!isfinite(rec[k])) return false;
+    return true;
+}
+
+int default_tile_bits(int precision, int n)
+{
+    const int tb = precision == MQW_FP32 ? 12 : 11;
+    return std::min(n, tb);
+}
+
+int default_chunk_bits(int n, int tb)
+{
+    // keep a whole trajectory (up to 256 tiles) on one XCD
+    return std::min(n - tb, 8);
+}
+
+// ---------------------------------------------------------------------------
+// Per-problem host state.
+// ---------------------------------------------------------------------------
+struct Problem {
+    int n = 0, m = 0;
+    const double* rec = nullptr;
+    float J[kMaxN * kMaxN];
+    float h[kMaxN];
+    mqw_schedule sch{};
+    std::vector<float> times; // m * samples
+    mqw_energy_info info{};
+    int out = 0;              // index into the caller's outputs
+    size_t prob_base = 0;     // first readout slot
+};
+
+struct Segment {
+    float gamma, onenorm;
+    std::vector<double> c; // FP64 coefficients, K = c.size()
+    int64_t prob_slot;     // -1 = no readout
+};
+
+struct Traj {
+    int prob;                     // index into the group's problem list
+    std::vector<Segment> segs;
+    const void* psi_in = nullptr; // input of the first stage; nullptr = |+>
+};
+
+// ---------------------------------------------------------------------------
+// Energies for a group of problems of one n: E (float) or E64 (double) per
+// problem on the device, infos copied back to the host.
+// ---------------------------------------------------------------------------
+mqw_status group_energies(mqw_ctx* c, int n, std::vector<Problem*>& ps, int mode, int precision, void* E_out)
+{
+    const size_t N = (size_t)1 << n;
+    const size_t cnt = ps.size();
+    const size_t nb = energy_blocks(n);
+    float* Jh = (float*)dev_get(c, "energy_Jh", sizeof(float) * (size_t)(n * n + n) * cnt);
+    double* scr = (double*)dev_get(c, "energy_scratch", sizeof(double) * 4 * nb);
+    EnergyDev* info = (EnergyDev*)dev_get(c, "energy_info", sizeof(EnergyDev) * cnt);
+    float* hp_tmp = nullptr;
+    if (precision == MQW_FP64) hp_tmp = (float*)dev_get(c, "energy_hp_tmp", sizeof(float) * N);
+    if (!Jh || !scr || !info || (precision == MQW_FP64 && !hp_tmp))
+        return fail(c, MQW_ENOMEM, "device allocation failed in energy build (n=%d, %zu problems)", n, cnt);
+    std::vector<float> hJh((size_t)(n * n + n) * cnt);
+    for (size_t p = 0; p < cnt; p++) {
+        memcpy(&hJh[p * (n * n + n)], ps[p]->J, sizeof(float) * n * n);
+        memcpy(&hJh[p * (n * n + n) + n * n], ps[p]->h, sizeof(float) * n);
+    }
+    HIPCHK(c, hipMemcpyAsync(Jh, hJh.data(), sizeof(float) * hJh.size(), hipMemcpyHostToDevice, c->stream));
+    EnergyScratch es{scr, scr + nb, (uint32_t*)(scr + 2 * nb), scr + 3 * nb};
+    for (size_t p = 0; p < cnt; p++) {
+        const float* Jd = Jh + p * (n * n + n);
+        const float* hd = Jd + n * n;
+        const double e_init = mode == MQW_ENERGY_EXACT ? e_init_exact(n, ps[p]->J, ps[p]->h)
+                                                       : e_init_compat(n, ps[p]->J, ps[p]->h);
+        float* hp = precision == MQW_FP32 ? (float*)E_out + p * N : hp_tmp;
+        double* e64 = precision == MQW_FP64 ? (double*)E_out + p * N : nullptr;
+        if (launch_energy(n, Jd, hd, e_init, mode == MQW_ENERGY_EXACT, hp, e64, 1, es, info + p, c->stream))
+            return fail(c, MQW_EDEVICE, "energy kernels: %s", last_kernel_error());
+    }
+    std::vector<EnergyDev> hinfo(cnt);
+    HIPCHK(c, hipMemcpyAsync(hinfo.data(), info, sizeof(EnergyDev) * cnt, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t p = 0; p < cnt; p++) {
+        mqw_energy_info& o = ps[p]->info;
+        o.e0 = hinfo[p].e0;
+        o.emax = hinfo[p].emax;
+        o.eloc = hinfo[p].eloc;
+        o.shift = hinfo[p].shift;
+        o.e_abs = hinfo[p].e_abs;
+        o.e_init = mode == MQW_ENERGY_EXACT ? e_init_exact(n, ps[p]->J, ps[p]->h)
+                                            : e_init_compat(n, ps[p]->J, ps[p]->h);
+    }
+    return MQW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The sweep plan of one wave of trajectories (global-memory path).
+// ---------------------------------------------------------------------------
+struct Op {
+    int kind; // 0 = term launch, 1 = reduce launch
+    size_t off;
+    uint32_t count;
+};
+
+template <typename real>
+struct Plan {
+    std::vector<TermJob<real>> jobs;
+    std::vector<ReduceJob> rjobs;
+    std::vector<Op> ops;
+    uint64_t amp_terms = 0;
+    uint64_t terms = 0;
+    uint64_t sweeps = 0;
+};
+
+template <typename real>
+void build_plan(int n, int tb, const std::vector<Traj>& trajs, const std::vector<Problem*>& ps, const real* E,
+                real* state, double* norm_part, double* psi_scale, double* probs, Plan<real>& P)
+{
+    const size_t N = (size_t)1 << n;
+    const uint32_t ntiles = 1u << (n - tb);
+    const size_t T = trajs.size();
+    struct Rot {
+        int p, a, b;
+    };
+    std::vector<Rot> rot(T, Rot{-1, 0, 1});
+    auto buf = [&](size_t t, int k) { return state + (t * 3 + (size_t)k) * 2 * N; };
+    const real psi0 = (real)(float)(1 / sqrt((double)N));
+    size_t rounds = 0;
+    for (auto& tr : trajs) rounds = std::max(rounds, tr.segs.size());
+    std::vector<int> K(T), S(T);
+    for (size_t s = 0; s < rounds; s++) {
+        int Smax = 0;
+        for (size_t t = 0; t < T; t++) {
+            if (s >= trajs[t].segs.size()) continue;
+            K[t] = (int)trajs[t].segs[s].c.size();
+            S[t] = std::max(K[t] - 1, 1);
+            Smax = std::max(Smax, S[t]);
+        }
+        for (int q = 0; q < Smax; q++) {
+            Op op{0, P.jobs.size(), 0};
+            for (size_t t = 0; t < T; t++) {
+                if (s >= trajs[t].segs.size()) continue;
+                const int start = Smax - S[t];
+                if (q < start) continue;
+                const int d = q - start;
+                const Segment& sg = trajs[t].segs[s];
+                const Rot& r = rot[t];
+                const std::vector<double>& cc = sg.c;
+                TermJob<real> j{};
+                const bool pconst = r.p < 0 && !trajs[t].psi_in;
+                j.psi = r.p >= 0 ? buf(t, r.p) : (const real*)trajs[t].psi_in;
+                j.psi_scale = r.p >= 0 ? psi_scale + t : nullptr;
+                j.psi0 = psi0;
+                j.E = E + (size_t)trajs[t].prob * N;
+                j.norm_part = norm_part + t * ntiles;
+                uint32_t flags = pconst ? F_PSI_CONST : 0u;
+                auto rc = [](double v) { return (real)v; };
+                if (K[t] == 1) {
+                    flags |= SRC1_ZERO | SRC2_ZERO | F_FINAL;
+                    j.out = buf(t, r.a);
+                    j.cr = rc(cc[0]);
+                    j.ci = real(0);
+                    j.alpha = j.beta = real(0);
+                } else {
+                    const int rr = K[t] - 2 - d;
+                    const bool final_term = rr == 0;
+                    j.out = buf(t, (d & 1) ? r.b : r.a);
+                    const double top = cc[K[t] - 1];
+                    const bool top_odd = ((K[t] - 1) & 1) != 0;
+                    if (d == 0) {
+                        flags |= SRC1_PSI;
+                        j.w1r = top_odd ? real(0) : rc(top);
+                        j.w1i = top_odd ? rc(top) : real(0);
+                    } else {
+                        flags |= SRC1_ARRAY;
+                        j.b1 = buf(t, ((d - 1) & 1) ? r.b : r.a);
+                        j.w1r = real(1);
+                        j.w1i = real(0);
+                    }
+                    if (d == 0) {
+                        flags |= SRC2_ZERO;
+                    } else if (d == 1) {
+                        flags |= SRC2_PSI;
+                        j.w2r = top_odd ? real(0) : rc(top);
+                        j.w2i = top_odd ? rc(top) : real(0);
+                    } else {
+                        flags |= SRC2_ARRAY;
+                        j.b2 = j.out;
+                        j.w2r = real(1);
+                        j.w2i = real(0);
+                    }
+                    const bool odd = (rr & 1) != 0;
+                    j.cr = odd ? real(0) : rc(cc[rr]);
+                    j.ci = odd ? rc(cc[rr]) : real(0);
+                    const double sc = (double)sg.onenorm * (final_term ? 1.0 : 0.5);
+                    j.alpha = (real)(1.0 / sc);
+                    j.beta = (real)((double)sg.gamma / sc);
+                    if (final_term) flags |= F_FINAL;
+                }
+                j.flags = flags;
+                P.jobs.push_back(j);
+                op.count++;
+                P.amp_terms += N;
+            }
+            P.ops.push_back(op);
+            P.sweeps++;
+        }
+        // reduce: norms, next-stage scales, readouts; then rotate buffers
+        Op rop{1, P.rjobs.size(), 0};
+        for (size_t t = 0; t < T; t++) {
+            if (s >= trajs[t].segs.size()) continue;
+            Rot& r = rot[t];
+            const int fin = (K[t] == 1) ? r.a : (((S[t] - 1) & 1) ? r.b : r.a);
+            ReduceJob rj{};
+            rj.norm_part = norm_part + t * ntiles;
+            rj.out = buf(t, fin);
+            rj.N = N;
+            rj.ntiles = ntiles;
+            rj.eloc = ps[trajs[t].prob]->info.eloc;
+            rj.psi_scale = psi_scale + t;
+            const int64_t slot = trajs[t].segs[s].prob_slot;
+            rj.prob = slot >= 0 ? probs + slot : nullptr;
+            P.rjobs.push_back(rj);
+            rop.count++;
+            P.terms += (uint64_t)S[t];
+            int others[2], k = 0;
+            for (int q = 0; q < 3; q++)
+                if (q != fin) others[k++] = q;
+            r.p = fin;
+            r.a = others[0];
+            r.b = others[1];
+        }
+        P.ops.push_back(rop);
+    }
+}
+
+template <typename real>
+mqw_status run_plan(mqw_ctx* c, int precision, int n, int tb, int lc, Plan<real>& P, bool use_graph, double* ms)
+{
+    TermJob<real>* jd = (TermJob<real>*)dev_get(c, "plan_jobs", sizeof(TermJob<real>) * P.jobs.size());
+    ReduceJob* rd = (ReduceJob*)dev_get(c, "plan_rjobs", sizeof(ReduceJob) * P.rjobs.size());
+    if (!jd || !rd) return fail(c, MQW_ENOMEM, "device allocation failed for %zu sweep jobs", P.jobs.size());
+    HIPCHK(c, hipMemcpyAsync(jd, P.jobs.data(), sizeof(TermJob<real>) * P.jobs.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(rd, P.rjobs.data(), sizeof(ReduceJob) * P.rjobs.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    auto issue = [&]() -> int {
+        for (const Op& op : P.ops) {
+            int rc = op.kind == 0 ? launch_term(precision, tb, jd + op.off, op.count, n, lc, c->stream)
+                                  : launch_reduce(precision, rd + op.off, op.count, c->stream);
+            if (rc) return rc;
+        }
+        return 0;
+    };
+    mqw_status st = MQW_OK;
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    if (use_graph) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = issue();
+        hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        if (rc || ce != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return fail(c, MQW_EDEVICE, "graph capture failed: %s / %s", last_kernel_error(),
+                        hipGetErrorString(ce));
+        }
+        hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            hipGraphDestroy(g);
+            return fail(c, MQW_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+        }
+        hipError_t le = hipGraphLaunch(ge, c->stream);
+        if (le != hipSuccess) st = fail(c, MQW_EDEVICE, "hipGraphLaunch: %s", hipGetErrorString(le));
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        hipError_t se = hipStreamSynchronize(c->stream);
+        hipGraphExecDestroy(ge);
+        hipGraphDestroy(g);
+        if (se != hipSuccess) return fail(c, MQW_EDEVICE, "stream sync: %s", hipGetErrorString(se));
+    } else {
+        if (issue()) st = fail(c, MQW_EDEVICE, "term launch: %s", last_kernel_error());
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    float t = 0.0f;
+    hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms += t;
+    return st;
+}
+
+// ---------------------------------------------------------------------------
+// Small n: every trajectory runs inside one workgroup (mqw_lds.hip).
+// ---------------------------------------------------------------------------
+mqw_status run_lds(mqw_ctx* c, int n, const std::vector<Traj>& trajs, const std::vector<Problem*>& ps,
+                   const float* E, double* probs_d, bool use_graph)
+{
+    const size_t N = (size_t)1 << n;
+    std::vector<LdsTraj> ht(trajs.size());
+    std::vector<LdsSeg> hs;
+    std::vector<float> hc;
+    uint64_t amp_terms = 0;
+    for (size_t t = 0; t < trajs.size(); t++) {
+        const Traj& tr = trajs[t];
+        ht[t].E = E + (size_t)tr.prob * N;
+        ht[t].seg_off = (uint32_t)hs.size();
+        ht[t].nseg = (uint32_t)tr.segs.size();
+        ht[t].eloc = ps[tr.prob]->info.eloc;
+        ht[t].pad = 0;
+        for (const Segment& sg : tr.segs) {
+            LdsSeg g{};
+            g.alpha = (float)(2.0 / (double)sg.onenorm);
+            g.beta = (float)(2.0 * (double)sg.gamma / (double)sg.onenorm);
+            g.coef_off = (uint32_t)hc.size();
+            g.K = (uint32_t)sg.c.size();
+            g.prob_slot = (int32_t)sg.prob_slot;
+            for (double v : sg.c) hc.push_back((float)v);
+            hs.push_back(g);
+            amp_terms += (uint64_t)std::max<size_t>(sg.c.size() - 1, 1) * N;
+        }
+    }
+    if (hc.size() > 0xffffffffull) return fail(c, MQW_EINVAL, "coefficient table too large");
+    LdsTraj* dt = (LdsTraj*)dev_get(c, "lds_trajs", sizeof(LdsTraj) * ht.size());
+    LdsSeg* ds = (LdsSeg*)dev_get(c, "lds_segs", sizeof(LdsSeg) * hs.size());
+    float* dc = (float*)dev_get(c, "lds_coefs", sizeof(float) * hc.size());
+    if (!dt || !ds || !dc) return fail(c, MQW_ENOMEM, "device allocation failed for LDS trajectory tables");
+    HIPCHK(c, hipMemcpyAsync(dt, ht.data(), sizeof(LdsTraj) * ht.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ds, hs.data(), sizeof(LdsSeg) * hs.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dc, hc.data(), sizeof(float) * hc.size(), hipMemcpyHostToDevice, c->stream));
+    const int grid = lds_grid(n, c->device);
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    if (use_graph) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = launch_lds(n, dt, (uint32_t)ht.size(), ds, dc, probs_d, nullptr, grid, c->stream);
+        hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        if (rc || ce != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return fail(c, MQW_EDEVICE, "graph capture of the LDS kernel failed (%d): %s", rc,
+                        hipGetErrorString(ce));
+        }
+        HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        hipError_t le = hipGraphLaunch(ge, c->stream);
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        hipError_t se = hipStreamSynchronize(c->stream);
+        hipGraphExecDestroy(ge);
+        hipGraphDestroy(g);
+        if (le != hipSuccess) return fail(c, MQW_EDEVICE, "hipGraphLaunch: %s", hipGetErrorString(le));
+        if (se != hipSuccess) return fail(c, MQW_EDEVICE, "stream sync: %s", hipGetErrorString(se));
+    } else {
+        if (launch_lds(n, dt, (uint32_t)ht.size(), ds, dc, probs_d, nullptr, grid, c->stream))
+            return fail(c, MQW_EDEVICE, "LDS kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c->stats.gpu_ms += ms;
+    c->stats.lds_trajectories += trajs.size();
+    c->stats.amp_terms += amp_terms;
+    return MQW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// One group of problems of the same n.
+// ---------------------------------------------------------------------------
+template <typename real>
+mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_opts& o, std::vector<double>& probs)
+{
+    const size_t N = (size_t)1 << n;
+    const int precision = o.precision;
+    const int samples = o.samples;
+    int tb = o.tile_bits > 0 ? std::min(o.tile_bits, n) : (c->tuned_tb[n] > 0 ? c->tuned_tb[n]
+                                                                               : default_tile_bits(precision, n));
+    if (!term_tile_bits_supported(precision, tb)) tb = default_tile_bits(precision, n);
+    int lc = o.chunk_bits >= 0 ? o.chunk_bits : (c->tuned_lc[n] >= 0 ? c->tuned_lc[n] : default_chunk_bits(n, tb));
+    lc = std::max(0, std::min(lc, 24));
+    real* E = (real*)dev_get(c, "E_group", sizeof(real) * N * ps.size());
+    if (!E) return fail(c, MQW_ENOMEM, "device allocation failed for %zu diagonals at n=%d", ps.size(), n);
+    mqw_status st = group_energies(c, n, ps, o.energy_mode, precision, E);
+    if (st) return st;
+
+    // trajectories and their segments
+    std::vector<Traj> trajs;
+    for (size_t p = 0; p < ps.size(); p++) {
+        Problem& pr = *ps[p];
+        const int m = pr.m;
+        auto seg = [&](int i, int j, int64_t slot) {
+            Segment sg;
+            sg.gamma = pr.sch.gamma[i];
+            sg.onenorm = fmaf(sg.gamma, (float)(unsigned)n, pr.info.e_abs);
+            const double s = (double)(sg.onenorm * pr.times[(size_t)i * samples + j]);
+            std::vector<double> cc(8192);
+            int K = cheb_coeffs(s, o.tol, cc.data(), (int)cc.size());
+            if (K < 0) {
+                cc.resize(1 << 20);
+                K = cheb_coeffs(s, o.tol, cc.data(), (int)cc.size());
+            }
+            cc.resize((size_t)std::max(K, 1));
+            if (precision == MQW_FP32)
+                for (double& v : cc) v = (double)(float)v; // the reference casts to float (:440)
+            sg.c = std::move(cc);
+            sg.prob_slot = slot;
+            return sg;
+        };
+        if (m == 1) {
+            Traj tr;
+            tr.prob = (int)p;
+            for (int j = 0; j < samples; j++) tr.segs.push_back(seg(0, j, (int64_t)(pr.prob_base + j)));
+            trajs.push_back(std::move(tr));
+        } else {
+            for (int j = 0; j < samples; j++) {
+                Traj tr;
+                tr.prob = (int)p;
+                for (int i = 0; i < m; i++)
+                    tr.segs.push_back(seg(i, j, i + 1 == m ? (int64_t)(pr.prob_base + j) : -1));
+                trajs.push_back(std::move(tr));
+            }
+        }
+    }
+
+    double* probs_d = (double*)dev_get(c, "probs", sizeof(double) * probs.size());
+    if (!probs_d) return fail(c, MQW_ENOMEM, "device allocation failed for probabilities");
+    HIPCHK(c, hipMemcpyAsync(probs_d, probs.data(), sizeof(double) * probs.size(), hipMemcpyHostToDevice, c->stream));
+    const int small_max = o.small_n_max >= 0 ? o.small_n_max : 13;
+    if (precision == MQW_FP32 && n <= small_max && lds_supported(n)) {
+        st = run_lds(c, n, trajs, ps, (const float*)E, probs_d, o.use_graph != 0);
+        if (st) return st;
+        HIPCHK(c, hipMemcpyAsync(probs.data(), probs_d, sizeof(double) * probs.size(), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return MQW_OK;
+    }
+
+    // waves bounded by the device budget
+    const uint32_t ntiles = 1u << (n - tb);
+    const size_t per_traj = sizeof(real) * 6 * N + sizeof(double) * (ntiles + 1);
+    size_t budget = o.max_device_bytes ? o.max_device_bytes : ((size_t)48 << 30);
+    size_t wave = std::max<size_t>(1, budget / per_traj);
+    for (size_t w0 = 0; w0 < trajs.size(); w0 += wave) {
+        const size_t w1 = std::min(trajs.size(), w0 + wave);
+        std::vector<Traj> wt(trajs.begin() + w0, trajs.begin() + w1);
+        const size_t T = wt.size();
+        real* state = (real*)dev_get(c, "state", sizeof(real) * 6 * N * T);
+        double* np = (double*)dev_get(c, "norm_part", sizeof(double) * ntiles * T);
+        double* pscale = (double*)dev_get(c, "psi_scale", sizeof(double) * T);
+        if (!state || !np || !pscale)
+            return fail(c, MQW_ENOMEM, "device allocation failed for %zu trajectories at n=%d", T, n);
+        Plan<real> plan;
+        build_plan<real>(n, tb, wt, ps, E, state, np, pscale, probs_d, plan);
+        st = run_plan<real>(c, precision, n, tb, lc, plan, o.use_graph != 0, &c->stats.gpu_ms);
+        if (st) return st;
+        c->stats.sweeps += plan.sweeps;
+        c->stats.term_units += plan.terms;
+        c->stats.amp_terms += plan.amp_terms;
+    }
+    HIPCHK(c, hipMemcpyAsync(probs.data(), probs_d, sizeof(double) * probs.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MQW_OK;
+}
+
+mqw_status solve_problems(mqw_ctx* c, std::vector<Problem>& probs_in, const mqw_opts& o, float* prob,
+                          float* total_t)
+{
+    c->stats = mqw_stats{};
+    if (o.samples < 1) return fail(c, MQW_EINVAL, "samples must be >= 1 (got %d)", o.samples);
+    if (!(o.tol > 0.0)) return fail(c, MQW_EINVAL, "tol must be > 0");
+    if (o.precision != MQW_FP32 && o.precision != MQW_FP64) return fail(c, MQW_EINVAL, "bad precision");
+    if (o.energy_mode != MQW_ENERGY_COMPAT && o.energy_mode != MQW_ENERGY_EXACT)
+        return fail(c, MQW_EINVAL, "bad energy mode");
+    std::map<int, std::vector<Problem*>> groups;
+    size_t slots = 0;
+    for (Problem& p : probs_in) {
+        p.prob_base = slots;
+        slots += (size_t)o.samples;
+        groups[p.n].push_back(&p);
+    }
+    std::vector<double> sp(slots, 0.0);
+    for (auto& kv : groups) {
+        mqw_status st = o.precision == MQW_FP32 ? solve_group<float>(c, kv.first, kv.second, o, sp)
+                                                : solve_group<double>(c, kv.first, kv.second, o, sp);
+        if (st) return st;
+    }
+    c->stats.algorithmic_bytes = (double)c->stats.amp_terms * (o.precision == MQW_FP32 ? 36.0 : 72.0);
+    for (Problem& p : probs_in) {
+        double s = 0.0;
+        for (int j = 0; j < o.samples; j++) s += sp[p.prob_base + j];
+        prob[p.out] = (float)(s / o.samples);
+        if (total_t) total_t[p.out] = p.sch.total_t;
+        if (o.sample_probs)
+            for (int j = 0; j < o.samples; j++)
+                o.sample_probs[(size_t)p.out * o.samples + j] = (float)sp[p.prob_base + j];
+    }
+    return MQW_OK;
+}
+
+mqw_status prepare_problem(mqw_ctx* c, Problem& p, int n, int m, const double* rec, const float* gamma,
+                           const float* short_t, int samples)
+{
+    if (n < MQW_MIN_N || n > MQW_MAX_N) return fail(c, MQW_EINVAL, "n=%d outside [%d, %d]", n, MQW_MIN_N, MQW_MAX_N);
+    if (m < 1 || m > MQW_MAX_STAGES) return fail(c, MQW_EINVAL, "m=%d outside [1, %d]", m, MQW_MAX_STAGES);
+    if (!rec) return fail(c, MQW_EINVAL, "null record");
+    if (!finite_rec(n, rec)) return fail(c, MQW_EINVAL, "record holds NaN or Inf");
+    p.n = n;
+    p.m = m;
+    p.rec = rec;
+    unpack(n, rec, p.J, p.h);
+    schedule_heuristic(n, m, p.J, p.h, &p.sch);
+    if (gamma && short_t) {
+        float tot = 0;
+        for (int i = 0; i < m; i++) {
+            if (!(gamma[i] > 0) || !(short_t[i] > 0) || !isfinite(gamma[i]) || !isfinite(short_t[i]))
+                return fail(c, MQW_EINVAL, "schedule override must be positive and finite");
+            p.sch.gamma[i] = gamma[i];
+            p.sch.short_t[i] = short_t[i];
+            tot += short_t[i];
+        }
+        p.sch.total_t = tot;
+    }
+    p.times.assign((size_t)m * samples, 0.0f);
+    sample_times(m, samples, p.sch.short_t, p.times.data());
+    return MQW_OK;
+}
+
+} // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int mqw_abi_version(void) { return MQW_ABI_VERSION; }
+
+const char* mqw_status_string(mqw_status s)
+{
+    switch (s) {
+    case MQW_OK: return "ok";
+    case MQW_EINVAL: return "invalid argument";
+    case MQW_ENOMEM: return "out of memory";
+    case MQW_EIO: return "i/o error";
+    case MQW_EDEVICE: return "device error";
+    case MQW_ENOTSUP: return "not supported";
+    }
+    return "unknown status";
+}
+
+void mqw_opts_default(mqw_opts* o)
+{
+    memset(o, 0, sizeof *o);
+    o->samples = MQW_DEFAULT_SAMPLES;
+    o->tol = MQW_DEFAULT_TOL;
+    o->precision = MQW_FP32;
+    o->energy_mode = MQW_ENERGY_COMPAT;
+    o->tile_bits = 0;
+    o->chunk_bits = -1;
+    o->small_n_max = -1;
+    o->use_graph = 1;
+    o->max_device_bytes = 0;
+    o->sample_probs = nullptr;
+}
+
+mqw_status mqw_create(int device, mqw_ctx** out)
+{
+    if (!out) return MQW_EINVAL;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        hipGetLastError();
+        return MQW_EDEVICE;
+    }
+    if (device < 0 || device >= count) return MQW_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return MQW_EDEVICE;
+    std::unique_ptr<mqw_ctx> c(new mqw_ctx());
+    c->device = device;
+    for (int i = 0; i <= MQW_MAX_N; i++) {
+        c->tuned_tb[i] = 0;
+        c->tuned_lc[i] = -1;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return MQW_EDEVICE;
+    *out = c.release();
+    return MQW_OK;
+}
+
+void mqw_destroy(mqw_ctx* c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    release_bufs(c);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* mqw_last_error(const mqw_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+mqw_status mqw_last_stats(const mqw_ctx* c, mqw_stats* out)
+{
+    if (!c || !out) return MQW_EINVAL;
+    *out = c->stats;
+    return MQW_OK;
+}
+
+mqw_status mqw_unpack(int n, const double* rec, float* J, float* h)
+{
+    if (n < 2 || n > MQW_MAX_N || !rec || !J || !h) return MQW_EINVAL;
+    unpack(n, rec, J, h);
+    return MQW_OK;
+}
+
+mqw_status mqw_schedule_heuristic(int n, int m, const double* rec, mqw_schedule* out)
+{
+    if (n < MQW_MIN_N || n > MQW_MAX_N || m < 1 || m > MQW_MAX_STAGES || !rec || !out) return MQW_EINVAL;
+    if (!finite_rec(n, rec)) return MQW_EINVAL;
+    float J[kMaxN * kMaxN], h[kMaxN];
+    unpack(n, rec, J, h);
+    schedule_heuristic(n, m, J, h, out);
+    return MQW_OK;
+}
+
+mqw_status mqw_sample_times(int m, int samples, const float* short_t, float* times)
+{
+    if (m < 1 || m > MQW_MAX_STAGES || samples < 1 || !short_t || !times) return MQW_EINVAL;
+    sample_times(m, samples, short_t, times);
+    return MQW_OK;
+}
+
+mqw_status mqw_chebyshev_coeffs(double s, double tol, double* cf, int cap, int* K)
+{
+    if (!cf || !K || cap < 1 || !(tol > 0) || !isfinite(s) || s < 0) return MQW_EINVAL;
+    const int k = cheb_coeffs(s, tol, cf, cap);
+    if (k < 0) return MQW_EINVAL;
+    *K = k;
+    return MQW_OK;
+}
+
+mqw_status mqw_generate_sk(int n, int count, uint64_t seed, double* out)
+{
+    if (n < 2 || n > MQW_MAX_N || count < 0 || (!out && count > 0)) return MQW_EINVAL;
+    generate_sk(n, count, seed, out);
+    return MQW_OK;
+}
+
+mqw_status mqw_energy(mqw_ctx* c, int n, const double* rec, int mode, float* hp, double* e, mqw_energy_info* info)
+{
+    if (!c) return MQW_EINVAL;
+    if (n < MQW_MIN_N || n > MQW_MAX_N) return fail(c, MQW_EINVAL, "n=%d outside [%d, %d]", n, MQW_MIN_N, MQW_MAX_N);
+    if (!rec || !info) return fail(c, MQW_EINVAL, "null argument");
+    if (mode != MQW_ENERGY_COMPAT && mode != MQW_ENERGY_EXACT) return fail(c, MQW_EINVAL, "bad energy mode");
+    if (!finite_rec(n, rec)) return fail(c, MQW_EINVAL, "record holds NaN or Inf");
+    hipSetDevice(c->device);
+    Problem p;
+    p.n = n;
+    p.m = 1;
+    unpack(n, rec, p.J, p.h);
+    std::vector<Problem*> ps{&p};
+    const size_t N = (size_t)1 << n;
+    float* hpd = (float*)dev_get(c, "energy_hp", sizeof(float) * N);
+    if (!hpd) return fail(c, MQW_ENOMEM, "device allocation failed for the diagonal");
+    double* e64 = nullptr;
+    if (e) {
+        e64 = (double*)dev_get(c, "energy_e64", sizeof(double) * N);
+        if (!e64) return fail(c, MQW_ENOMEM, "device allocation failed for FP64 energies");
+    }
+    // FP32 diagonal always; FP64 energies unshifted when requested.
+    const size_t nb = energy_blocks(n);
+    float* Jh = (float*)dev_get(c, "energy_Jh", sizeof(float) * (size_t)(n * n + n));
+    double* scr = (double*)dev_get(c, "energy_scratch", sizeof(double) * 4 * nb);
+    EnergyDev* infod = (EnergyDev*)dev_get(c, "energy_info", sizeof(EnergyDev));
+    if (!Jh || !scr || !infod) return fail(c, MQW_ENOMEM, "device allocation failed in energy build");
+    std::vector<float> hJh((size_t)(n * n + n));
+    memcpy(hJh.data(), p.J, sizeof(float) * n * n);
+    memcpy(hJh.data() + n * n, p.h, sizeof(float) * n);
+    HIPCHK(c, hipMemcpyAsync(Jh, hJh.data(), sizeof(float) * hJh.size(), hipMemcpyHostToDevice, c->stream));
+    EnergyScratch es{scr, scr + nb, (uint32_t*)(scr + 2 * nb), scr + 3 * nb};
+    const double e_init = mode == MQW_ENERGY_EXACT ? e_init_exact(n, p.J, p.h) : e_init_compat(n, p.J, p.h);
+    if (launch_energy(n, Jh, Jh + n * n, e_init, mode == MQW_ENERGY_EXACT, hpd, e64, 0, es, infod, c->stream))
+        return fail(c, MQW_EDEVICE, "energy kernels: %s", last_kernel_error());
+    EnergyDev hi;
+    HIPCHK(c, hipMemcpyAsync(&hi, infod, sizeof hi, hipMemcpyDeviceToHost, c->stream));
+    if (hp) HIPCHK(c, hipMemcpyAsync(hp, hpd, sizeof(float) * N, hipMemcpyDeviceToHost, c->stream));
+    if (e) HIPCHK(c, hipMemcpyAsync(e, e64, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    info->e0 = hi.e0;
+    info->emax = hi.emax;
+    info->eloc = hi.eloc;
+    info->shift = hi.shift;
+    info->e_abs = hi.e_abs;
+    info->e_init = e_init;
+    return MQW_OK;
+}
+
+mqw_status mqw_clenshaw_step_dev(mqw_ctx* c, const float* b1, float* b2, const float* hp, const float* psi,
+                                 unsigned n, float scale, float gamma, float coef)
+{
+    if (!c) return MQW_EINVAL;
+    if ((int)n < MQW_MIN_N || (int)n > MQW_MAX_N) return fail(c, MQW_EINVAL, "n=%u outside range", n);
+    if (!b1 || !b2 || !hp || (!psi && coef != 0.0f)) return fail(c, MQW_EINVAL, "null device pointer");
+    if (!(scale != 0.0f) || !isfinite(scale) || !isfinite(gamma))
+        return fail(c, MQW_EINVAL, "scale must be finite and nonzero");
+    hipSetDevice(c->device);
+    const float alpha = (float)(2.0 / (double)scale), beta = (float)(2.0 * (double)gamma / (double)scale);
+    if (launch_half_step(b1, b2, hp, psi, (int)n, alpha, beta, coef, c->stream))
+        return fail(c, MQW_EDEVICE, "half step: %s", last_kernel_error());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MQW_OK;
+}
+
+mqw_status mqw_clenshaw(mqw_ctx* c, int n, const float* coeffs, int K, float* psi, const float* hp, float gamma,
+                        float scale, int psi_real)
+{
+    (void)psi_real; // the imaginary injections of a real state are zero either way
+    if (!c) return MQW_EINVAL;
+    if (n < MQW_MIN_N || n > MQW_MAX_N) return fail(c, MQW_EINVAL, "n=%d outside range", n);
+    if (!coeffs || K < 1 || !psi || !hp) return fail(c, MQW_EINVAL, "null argument or K < 1");
+    if (!(scale > 0.0f) || !isfinite(scale) || !isfinite(gamma)) return fail(c, MQW_EINVAL, "bad scale");
+    hipSetDevice(c->device);
+    const size_t N = (size_t)1 << n;
+    const int tb = default_tile_bits(MQW_FP32, n);
+    const uint32_t ntiles = 1u << (n - tb);
+    float* E = (float*)dev_get(c, "cl_E", sizeof(float) * N);
+    float* state = (float*)dev_get(c, "cl_state", sizeof(float) * 8 * N);
+    double* np = (double*)dev_get(c, "cl_norm", sizeof(double) * (ntiles + 1));
+    if (!E || !state || !np) return fail(c, MQW_ENOMEM, "device allocation failed");
+    HIPCHK(c, hipMemcpyAsync(E, hp, sizeof(float) * N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(state, psi, sizeof(float) * 2 * N, hipMemcpyHostToDevice, c->stream));
+    // one trajectory, one stage, input uploaded to `state`, buffers after it
+    Problem p;
+    p.n = n;
+    p.info.eloc = 0;
+    std::vector<Problem*> ps{&p};
+    Traj tr;
+    tr.prob = 0;
+    tr.psi_in = state;
+    Segment sg;
+    sg.gamma = gamma;
+    sg.onenorm = scale;
+    sg.c.assign(coeffs, coeffs + K);
+    sg.prob_slot = -1;
+    tr.segs.push_back(sg);
+    Plan<float> plan;
+    std::vector<Traj> trajs{tr};
+    build_plan<float>(n, tb, trajs, ps, E, state + 2 * N, np, np + ntiles, nullptr, plan);
+    double ms = 0;
+    mqw_status st = run_plan<float>(c, MQW_FP32, n, tb, default_chunk_bits(n, tb), plan, false, &ms);
+    if (st) return st;
+    const float* outp = (const float*)plan.rjobs.back().out;
+    HIPCHK(c, hipMemcpyAsync(psi, outp, sizeof(float) * 2 * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MQW_OK;
+}
+
+mqw_status mqw_solve(mqw_ctx* c, int n, int m, const double* recs, int count, const mqw_opts* opts, float* prob,
+                     float* total_t)
+{
+    if (!c) return MQW_EINVAL;
+    if (count < 0 || (count > 0 && (!recs || !prob))) return fail(c, MQW_EINVAL, "bad count or null output");
+    mqw_opts o;
+    if (opts)
+        o = *opts;
+    else
+        mqw_opts_default(&o);
+    hipSetDevice(c->device);
+    const int P = n * (n + 1) / 2;
+    std::vector<Problem> ps((size_t)count);
+    for (int k = 0; k < count; k++) {
+        mqw_status st = prepare_problem(c, ps[k], n, m, recs + (size_t)k * P, nullptr, nullptr, o.samples);
+        if (st) return st;
+        ps[k].out = k;
+    }
+    if (count == 0) return MQW_OK;
+    return solve_problems(c, ps, o, prob, total_t);
+}
+
+mqw_status mqw_solve_batch(mqw_ctx* c, const mqw_instance* inst, int count, const mqw_opts* opts, float* prob,
+                           float* total_t)
+{
+    if (!c) return MQW_EINVAL;
+    if (count < 0 || (count > 0 && (!inst || !prob))) return fail(c, MQW_EINVAL, "bad count or null output");
+    mqw_opts o;
+    if (opts)
+        o = *opts;
+    else
+        mqw_opts_default(&o);
+    hipSetDevice(c->device);
+    std::vector<Problem> ps((size_t)count);
+    for (int k = 0; k < count; k++) {
+        mqw_status st = prepare_problem(c, ps[k], inst[k].n, inst[k].m, inst[k].rec, inst[k].gamma, inst[k].short_t,
+                                        o.samples);
+        if (st) return st;
+        ps[k].out = k;
+    }
+    if (count == 0) return MQW_OK;
+    return solve_problems(c, ps, o, prob, total_t);
+}
+
+mqw_status mqw_tune(mqw_ctx* c, int n, const char* cache_path, int* tile_bits, int* chunk_bits)
+{
+    (void)cache_path;
+    if (!c) return MQW_EINVAL;
+    if (n < MQW_MIN_N || n > MQW_MAX_N) return fail(c, MQW_EINVAL, "n out of range");
+    const int tb = default_tile_bits(MQW_FP32, n);
+    if (tile_bits) *tile_bits = tb;
+    if (chunk_bits) *chunk_bits = default_chunk_bits(n, tb);
+    return MQW_OK;
+}
+
+} // extern "C"

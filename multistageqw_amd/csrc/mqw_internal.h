@@ -1,0 +1,133 @@
+// Internal declarations shared by the host engine (mqw_engine.cpp, mqw_host.cpp)
+// and the gfx950 kernels (mqw_kernels.hip).  Not part of the public ABI.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mqw.h"
+
+#if defined(__HIPCC__)
+#define MQW_HD __host__ __device__
+#else
+#define MQW_HD
+#endif
+
+namespace mqw {
+
+constexpr int kMaxN = MQW_MAX_N;
+
+// ---------------------------------------------------------------------------
+// Host-side pieces of the path (mqw_host.cpp).
+// ---------------------------------------------------------------------------
+float eigen_sum_f32(const float* x, long size);
+void unpack(int n, const double* rec, float* J, float* h);
+double ppnd16(double p);
+void schedule_heuristic(int n, int m, const float* J, const float* h, mqw_schedule* out);
+void sample_times(int m, int samples, const float* short_t, float* times);
+// Coefficients c[0..K) of cos(sx)+sin(sx); returns K or -1 when cap is too small.
+int cheb_coeffs(double s, double tol, double* c, int cap);
+void generate_sk(int n, int count, uint64_t seed, double* out);
+// FP32 and FP64 energy of the all -1 state: (float)(sum J - sum h) as the walk
+// starts (MultiQW.cpp:332), and the exact FP64 value from the same couplings.
+double e_init_compat(int n, const float* J, const float* h);
+double e_init_exact(int n, const float* J, const float* h);
+
+// ---------------------------------------------------------------------------
+// Term jobs: one complex Clenshaw term for one trajectory.
+//   u   = alpha * E .* s1 - beta * H_G s1            (s1 per SRC1_*)
+//   out = w1 * u - w2 * s2 + c * Psi                 (complex scalars)
+// with Psi the segment's input state scaled by *psi_scale.  Terms whose s1
+// or s2 is Psi also take the psi_scale factor.  flags are uniform per job.
+// ---------------------------------------------------------------------------
+enum : uint32_t {
+    SRC1_ARRAY = 0u,
+    SRC1_PSI = 1u,
+    SRC1_ZERO = 2u,
+    SRC2_ARRAY = 0u << 2,
+    SRC2_PSI = 1u << 2,
+    SRC2_ZERO = 2u << 2,
+    F_PSI_CONST = 1u << 4, // Psi is |+>: psi0 on every real part, no array
+    F_FINAL = 1u << 5,     // write the tile's sum |out|^2 to norm_part[tile]
+};
+
+template <typename real>
+struct TermJob {
+    const real* b1;          // s1 array (SRC1_ARRAY)
+    const real* b2;          // s2 array (SRC2_ARRAY); may alias out
+    real* out;               // planar output
+    const real* psi;         // segment input state (planar), unless F_PSI_CONST
+    const real* E;           // diagonal H_P (float for FP32, double for FP64)
+    const double* psi_scale; // 1/||psi|| of the segment input, nullable
+    double* norm_part;       // F_FINAL: one double per tile
+    real alpha, beta;
+    real w1r, w1i, w2r, w2i;
+    real cr, ci;
+    real psi0;
+    uint32_t flags;
+};
+
+struct ReduceJob {
+    const double* norm_part; // ntiles partial sums of |out|^2
+    const void* out;         // planar state (float or double)
+    uint64_t N;
+    uint32_t ntiles;
+    uint32_t eloc;
+    double* psi_scale;       // receives 1/||out||
+    double* prob;            // nullable: receives |out[eloc]|^2 / ||out||^2
+};
+
+struct EnergyDev {
+    double e0, emax;
+    uint32_t eloc;
+    float shift, e_abs;
+    double center; // (emax + e0) / 2 in FP64
+};
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (mqw_kernels.hip).  All asynchronous on `stream`.
+// ---------------------------------------------------------------------------
+struct EnergyScratch {
+    double* block_sum;   // nblocks
+    double* block_min;   // nblocks
+    uint32_t* block_arg; // nblocks
+    double* block_max;   // nblocks
+};
+size_t energy_blocks(int n);
+// Energies into hp (float, shifted) and optionally e64 (double, shifted by the
+// FP64 centre when e64_shift, else unshifted), info on the device.
+int launch_energy(int n, const float* J_dev, const float* h_dev, double e_init, int exact, float* hp,
+                  double* e64, int e64_shift, const EnergyScratch& scr, EnergyDev* info_dev, void* stream);
+
+int term_tile_bits_supported(int precision, int tb);
+int launch_term(int precision, int tb, const void* jobs_dev, uint32_t njobs, int n, int lc, void* stream);
+int launch_reduce(int precision, const ReduceJob* jobs_dev, uint32_t njobs, void* stream);
+int launch_half_step(const float* b1, float* b2, const float* hp, const float* psi, int n, float alpha,
+                     float beta, float coef, void* stream);
+
+// LDS-resident trajectory kernel (whole trajectory inside one workgroup).
+struct LdsSeg {
+    float alpha, beta;   // 2/scale, 2 gamma/scale
+    uint32_t coef_off;   // offset of K float coefficients in the coefficient table
+    uint32_t K;
+    int32_t prob_slot;   // >= 0: write the readout probability there
+    uint32_t pad;
+};
+struct LdsTraj {
+    const float* E;       // diagonal (2^n floats)
+    uint32_t seg_off;     // first segment in the segment table
+    uint32_t nseg;
+    uint32_t eloc;
+    uint32_t pad;
+};
+int lds_supported(int n);
+int launch_lds(int n, const LdsTraj* trajs, uint32_t ntraj, const LdsSeg* segs, const float* coefs,
+               double* probs, uint32_t* counter, int grid, void* stream);
+int lds_grid(int n, int device);
+
+const char* last_kernel_error();
+
+} // namespace mqw

@@ -1,0 +1,677 @@
+// gfx950 kernels of the multi-stage quantum-walk path.
+//
+//   energy_*    : FP64 Gray-code energy build + ground-state search
+//                 (MultiQW.cpp:325-362), one thread per 16 Gray steps,
+//                 exact per-step deltas recomputed in two passes around a
+//                 block-sum scan instead of the reference's serial walk.
+//   term_kernel : one fused complex Clenshaw term (MultiQW.cpp:88-173,177-236)
+//                 for a list of trajectories: the state tile of 2^TB amplitudes
+//                 is flipped along its low TB bits through LDS / registers and
+//                 along the remaining bits by coalesced partner-tile loads, in
+//                 one read of b_{k+1}, b_{k+2}, psi, E and one write of b_k.
+//   reduce_kernel: FP64 norm of a finished stage, 1/||psi|| for the next stage
+//                 and the ground-state probability readout (MultiQW.cpp:443-446).
+//   lds_kernel  : small n: a whole trajectory (all stages, all terms) inside one
+//                 workgroup with the state resident in LDS and registers.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "mqw_internal.h"
+
+namespace mqw {
+
+static thread_local const char* g_kernel_error = "";
+const char* last_kernel_error() { return g_kernel_error; }
+
+static int check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_kernel_error = hipGetErrorString(e);
+        (void)what;
+        return -1;
+    }
+    return 0;
+}
+
+// ===========================================================================
+// Energies
+// ===========================================================================
+constexpr int kEnergyThreads = 256;
+constexpr int kEnergyPer = 16;
+constexpr uint64_t kEnergyChunk = (uint64_t)kEnergyThreads * kEnergyPer;
+
+size_t energy_blocks(int n)
+{
+    const uint64_t N = 1ull << n;
+    return (size_t)((N + kEnergyChunk - 1) / kEnergyChunk);
+}
+
+// Change of energy at Gray step i (i >= 1): spin f = ctz(i) flips into the
+// state g = grey(i).  The reference (MultiQW.cpp:340-346) forms
+//   delta = float(4 * S - 2 h_f st_ff),  S = float sum_j J_fj st_fj in j order,
+// with st_fj = s_f s_j and st_ff = -s_f.  Since negation is exact and rounding
+// symmetric, delta = s_f * float(4 T + 2 h_f) with T = float sum_{j != f} J_fj s_j.
+template <bool EXACT>
+__device__ __forceinline__ double gray_delta(int n, const float* __restrict__ sJ, const float* __restrict__ sh,
+                                            uint64_t i)
+{
+    if (i == 0) return 0.0;
+    const int f = __builtin_ctzll(i);
+    const uint64_t g = i ^ (i >> 1);
+    const float* row = sJ + f * n;
+    if (EXACT) {
+        double T = 0.0;
+        for (int j = 0; j < n; j++) {
+            if (j == f) continue;
+            const double v = (double)row[j];
+            T += ((g >> j) & 1ull) ? v : -v;
+        }
+        const double d = 4.0 * T + 2.0 * (double)sh[f];
+        return ((g >> f) & 1ull) ? d : -d;
+    } else {
+        float T = 0.0f;
+        for (int j = 0; j < n; j++) {
+            if (j == f) continue;
+            const float v = row[j];
+            T = T + (((g >> j) & 1ull) ? v : -v);
+        }
+        const float d = 4.0f * T + 2.0f * sh[f];
+        return (double)(((g >> f) & 1ull) ? d : -d);
+    }
+}
+
+__device__ __forceinline__ void load_couplings(int n, const float* J, const float* h, float* sJ, float* sh)
+{
+    for (int k = threadIdx.x; k < n * n; k += blockDim.x) sJ[k] = J[k];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) sh[k] = h[k];
+    __syncthreads();
+}
+
+// Fixed-order block sum of one double per thread (kEnergyThreads threads).
+__device__ __forceinline__ double block_sum_fixed(double v, double* red)
+{
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kEnergyThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+template <bool EXACT>
+__global__ void __launch_bounds__(kEnergyThreads)
+energy_blocksum_kernel(int n, const float* __restrict__ J, const float* __restrict__ h, double* __restrict__ block_sum)
+{
+    __shared__ float sJ[kMaxN * kMaxN];
+    __shared__ float sh[kMaxN];
+    __shared__ double red[kEnergyThreads];
+    load_couplings(n, J, h, sJ, sh);
+    const uint64_t N = 1ull << n;
+    const uint64_t i0 = ((uint64_t)blockIdx.x * kEnergyThreads + threadIdx.x) * kEnergyPer;
+    double s = 0.0;
+    for (int k = 0; k < kEnergyPer; k++) {
+        const uint64_t i = i0 + k;
+        if (i < N) s += gray_delta<EXACT>(n, sJ, sh, i);
+    }
+    const double tot = block_sum_fixed(s, red);
+    if (threadIdx.x == 0) block_sum[blockIdx.x] = tot;
+}
+
+// Exclusive scan of the block sums in place, one workgroup, fixed order.
+__global__ void __launch_bounds__(1024) energy_scan_kernel(double* __restrict__ v, uint32_t nb)
+{
+    __shared__ double part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t lo = t * per, hi = min(nb, lo + per);
+    double s = 0.0;
+    for (uint32_t k = lo; k < hi; k++) s += v[k];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        double run = 0.0;
+        for (int k = 0; k < 1024; k++) {
+            const double x = part[k];
+            part[k] = run;
+            run += x;
+        }
+    }
+    __syncthreads();
+    double run = part[t];
+    for (uint32_t k = lo; k < hi; k++) {
+        const double x = v[k];
+        v[k] = run;
+        run += x;
+    }
+}
+
+template <bool EXACT>
+__global__ void __launch_bounds__(kEnergyThreads)
+energy_write_kernel(int n, const float* __restrict__ J, const float* __restrict__ h, double e_init,
+                    const double* __restrict__ block_prefix, float* __restrict__ hp, double* __restrict__ e64,
+                    double* __restrict__ block_min, uint32_t* __restrict__ block_arg, double* __restrict__ block_max)
+{
+    __shared__ float sJ[kMaxN * kMaxN];
+    __shared__ float sh[kMaxN];
+    __shared__ double scan[kEnergyThreads];
+    __shared__ double mn[kEnergyThreads];
+    __shared__ double mx[kEnergyThreads];
+    __shared__ uint32_t ag[kEnergyThreads];
+    load_couplings(n, J, h, sJ, sh);
+    const uint64_t N = 1ull << n;
+    const int t = threadIdx.x;
+    const uint64_t i0 = ((uint64_t)blockIdx.x * kEnergyThreads + t) * kEnergyPer;
+    double d[kEnergyPer];
+    double s = 0.0;
+    for (int k = 0; k < kEnergyPer; k++) {
+        const uint64_t i = i0 + k;
+        d[k] = (i < N) ? gray_delta<EXACT>(n, sJ, sh, i) : 0.0;
+        s += d[k];
+    }
+    // exclusive scan of the thread sums (Hillis-Steele on a fixed layout)
+    scan[t] = s;
+    __syncthreads();
+    for (int w = 1; w < kEnergyThreads; w <<= 1) {
+        const double add = (t >= w) ? scan[t - w] : 0.0;
+        __syncthreads();
+        scan[t] += add;
+        __syncthreads();
+    }
+    // Every partial sum here is a walk energy: float deltas of magnitude
+    // >= 2^-22 added to values below 2^8 are exact in FP64, so this tree order
+    // reproduces the reference's serial accumulation (MultiQW.cpp:345).
+    double run = (e_init + block_prefix[blockIdx.x]) + (t > 0 ? scan[t - 1] : 0.0);
+    double lo = __builtin_inf(), hi = -__builtin_inf();
+    uint32_t arg = 0xffffffffu;
+    for (int k = 0; k < kEnergyPer; k++) {
+        const uint64_t i = i0 + k;
+        if (i >= N) break;
+        run += d[k];
+        const uint64_t g = i ^ (i >> 1);
+        hp[g] = (float)run;
+        if (e64) e64[g] = run;
+        if (run < lo) {
+            lo = run;
+            arg = (uint32_t)i;
+        }
+        if (run > hi) hi = run;
+    }
+    mn[t] = lo;
+    mx[t] = hi;
+    ag[t] = arg;
+    __syncthreads();
+    for (int w = kEnergyThreads / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            // strict minimum, earliest Gray step on ties (MultiQW.cpp:349)
+            const double a = mn[t], b = mn[t + w];
+            if (b < a || (b == a && ag[t + w] < ag[t])) {
+                mn[t] = b;
+                ag[t] = ag[t + w];
+            }
+            if (mx[t + w] > mx[t]) mx[t] = mx[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        block_min[blockIdx.x] = mn[0];
+        block_arg[blockIdx.x] = ag[0];
+        block_max[blockIdx.x] = mx[0];
+    }
+}
+
+__global__ void __launch_bounds__(256)
+energy_finalize_kernel(uint32_t nb, const double* __restrict__ block_min, const uint32_t* __restrict__ block_arg,
+                       const double* __restrict__ block_max, EnergyDev* __restrict__ info)
+{
+    __shared__ double mn[256], mx[256];
+    __shared__ uint32_t ag[256];
+    const int t = threadIdx.x;
+    double lo = __builtin_inf(), hi = -__builtin_inf();
+    uint32_t arg = 0xffffffffu;
+    for (uint32_t k = t; k < nb; k += 256) {
+        const double a = block_min[k];
+        if (a < lo || (a == lo && block_arg[k] < arg)) {
+            lo = a;
+            arg = block_arg[k];
+        }
+        if (block_max[k] > hi) hi = block_max[k];
+    }
+    mn[t] = lo;
+    mx[t] = hi;
+    ag[t] = arg;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            const double a = mn[t], b = mn[t + w];
+            if (b < a || (b == a && ag[t + w] < ag[t])) {
+                mn[t] = b;
+                ag[t] = ag[t + w];
+            }
+            if (mx[t + w] > mx[t]) mx[t] = mx[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double e0 = mn[0], emax = mx[0];
+        const uint32_t i = ag[0];
+        info->e0 = e0;
+        info->emax = emax;
+        info->eloc = i ^ (i >> 1);
+        info->shift = (float)((emax + e0) / 2);
+        info->e_abs = (float)((emax - e0) / 2);
+        info->center = (emax + e0) / 2;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+energy_shift_kernel(uint64_t N, float* __restrict__ hp, double* __restrict__ e64, int e64_shift,
+                    const EnergyDev* __restrict__ info)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= N) return;
+    const float s = info->shift;
+    hp[k] = hp[k] - s;
+    if (e64 && e64_shift) e64[k] = e64[k] - info->center;
+}
+
+int launch_energy(int n, const float* J_dev, const float* h_dev, double e_init, int exact, float* hp, double* e64,
+                  int e64_shift, const EnergyScratch& scr, EnergyDev* info_dev, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t nb = (uint32_t)energy_blocks(n);
+    const uint64_t N = 1ull << n;
+    if (exact)
+        hipLaunchKernelGGL(energy_blocksum_kernel<true>, dim3(nb), dim3(kEnergyThreads), 0, st, n, J_dev, h_dev,
+                           scr.block_sum);
+    else
+        hipLaunchKernelGGL(energy_blocksum_kernel<false>, dim3(nb), dim3(kEnergyThreads), 0, st, n, J_dev, h_dev,
+                           scr.block_sum);
+    hipLaunchKernelGGL(energy_scan_kernel, dim3(1), dim3(1024), 0, st, scr.block_sum, nb);
+    if (exact)
+        hipLaunchKernelGGL(energy_write_kernel<true>, dim3(nb), dim3(kEnergyThreads), 0, st, n, J_dev, h_dev, e_init,
+                           scr.block_sum, hp, e64, scr.block_min, scr.block_arg, scr.block_max);
+    else
+        hipLaunchKernelGGL(energy_write_kernel<false>, dim3(nb), dim3(kEnergyThreads), 0, st, n, J_dev, h_dev,
+                           e_init, scr.block_sum, hp, e64, scr.block_min, scr.block_arg, scr.block_max);
+    hipLaunchKernelGGL(energy_finalize_kernel, dim3(1), dim3(256), 0, st, nb, scr.block_min, scr.block_arg,
+                       scr.block_max, info_dev);
+    hipLaunchKernelGGL(energy_shift_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, hp, e64,
+                       e64_shift, info_dev);
+    return check_launch("energy");
+}
+
+// ===========================================================================
+// Fused Clenshaw term
+// ===========================================================================
+template <typename real>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    using type = float4;
+    static constexpr int W = 4;
+};
+template <>
+struct Vec16<double> {
+    using type = double2;
+    static constexpr int W = 2;
+};
+
+template <typename real, int R>
+__device__ __forceinline__ void vload(const real* __restrict__ p, real (&v)[R])
+{
+    using vt = typename Vec16<real>::type;
+    constexpr int W = Vec16<real>::W;
+#pragma unroll
+    for (int q = 0; q < R / W; q++) {
+        const vt x = *reinterpret_cast<const vt*>(p + q * W);
+        if constexpr (W == 4) {
+            v[q * 4 + 0] = x.x;
+            v[q * 4 + 1] = x.y;
+            v[q * 4 + 2] = x.z;
+            v[q * 4 + 3] = x.w;
+        } else {
+            v[q * 2 + 0] = x.x;
+            v[q * 2 + 1] = x.y;
+        }
+    }
+}
+
+template <typename real, int R>
+__device__ __forceinline__ void vstore(real* p, const real (&v)[R])
+{
+    using vt = typename Vec16<real>::type;
+    constexpr int W = Vec16<real>::W;
+#pragma unroll
+    for (int q = 0; q < R / W; q++) {
+        vt x;
+        if constexpr (W == 4) {
+            x.x = v[q * 4 + 0];
+            x.y = v[q * 4 + 1];
+            x.z = v[q * 4 + 2];
+            x.w = v[q * 4 + 3];
+        } else {
+            x.x = v[q * 2 + 0];
+            x.y = v[q * 2 + 1];
+        }
+        *reinterpret_cast<vt*>(p + q * W) = x;
+    }
+}
+
+// Deterministic workgroup sum of one double per thread, result valid in thread 0.
+// Full waves reduce by cross-lane shuffles; a workgroup narrower than one
+// wave goes through LDS so no inactive lane is ever read.
+template <int NT>
+__device__ __forceinline__ double wg_sum_t0(double v, double* red)
+{
+    const int t = threadIdx.x;
+    if constexpr (NT >= 64) {
+#pragma unroll
+        for (int w = 32; w > 0; w >>= 1) v += __shfl_xor(v, w, 64);
+        if ((t & 63) == 0) red[t >> 6] = v;
+        __syncthreads();
+        double s = 0.0;
+        if (t == 0)
+            for (int w = 0; w < NT / 64; w++) s += red[w];
+        return s;
+    } else {
+        red[t] = v;
+        __syncthreads();
+        double s = 0.0;
+        if (t == 0)
+            for (int w = 0; w < NT; w++) s += red[w];
+        return s;
+    }
+}
+
+// blockIdx -> (job, tile).  Workgroups are dispatched round-robin over the 8
+// XCDs; consecutive runs of 2^lc tiles (a sub-cube of the outer bits) are kept
+// on one XCD so their partner-tile loads meet in that XCD's L2.
+__device__ __forceinline__ uint64_t xcd_unit(uint32_t bid, int lc)
+{
+    const uint32_t xcd = bid & 7u, slot = bid >> 3;
+    const uint64_t chunk = ((uint64_t)(slot >> lc) << 3) | xcd;
+    return (chunk << lc) | (uint64_t)(slot & ((1u << lc) - 1u));
+}
+
+template <typename real, int TB, int RB>
+__global__ void __launch_bounds__(1 << (TB - RB))
+term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int lc)
+{
+    constexpr int NT = 1 << (TB - RB);
+    constexpr int R = 1 << RB;
+    constexpr int TS = 1 << TB;
+    constexpr int NRED = NT >= 64 ? NT / 64 : NT;
+    __shared__ real s_re[TS];
+    __shared__ real s_im[TS];
+    __shared__ double s_red[NRED];
+
+    const int ob = n - TB;
+    const uint64_t unit = xcd_unit(blockIdx.x, lc);
+    const uint64_t job = unit >> ob;
+    if (job >= njobs) return;
+    const uint64_t tile = unit & ((1ull << ob) - 1ull);
+    const TermJob<real>& J = jobs[job];
+    const uint32_t flags = J.flags;
+    const uint32_t src1 = flags & 3u, src2 = flags & 12u;
+    const bool psi_const = (flags & F_PSI_CONST) != 0;
+    const uint64_t N = 1ull << n;
+    const int t = threadIdx.x;
+    const uint64_t off = (tile << TB) + (uint64_t)t * R;
+    const real ps = J.psi_scale ? (real)(*J.psi_scale) : real(1);
+
+    real pr[R], pi[R], e[R];
+    if (!psi_const) {
+        vload<real, R>(J.psi + off, pr);
+        vload<real, R>(J.psi + N + off, pi);
+    } else {
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            pr[q] = J.psi0;
+            pi[q] = real(0);
+        }
+    }
+    vload<real, R>(J.E + off, e);
+
+    real ur[R], ui[R];
+    if (src1 == SRC1_ZERO) {
+#pragma unroll
+        for (int q = 0; q < R; q++) ur[q] = ui[q] = real(0);
+    } else if (src1 == SRC1_PSI && psi_const) {
+        // |+> is an eigenvector of H_G with eigenvalue n
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            ur[q] = J.alpha * e[q] * J.psi0 - J.beta * (real)n * J.psi0;
+            ui[q] = real(0);
+        }
+    } else {
+        const real* src = (src1 == SRC1_PSI) ? J.psi : J.b1;
+        real xr[R], xi[R];
+        if (src1 == SRC1_PSI) {
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+                xr[q] = pr[q];
+                xi[q] = pi[q];
+            }
+        } else {
+            vload<real, R>(src + off, xr);
+            vload<real, R>(src + N + off, xi);
+        }
+        vstore<real, R>(s_re + t * R, xr);
+        vstore<real, R>(s_im + t * R, xi);
+        real ar[R], ai[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            ar[q] = real(0);
+            ai[q] = real(0);
+        }
+        // flips of the register bits
+#pragma unroll
+        for (int b = 0; b < RB; b++)
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+                ar[q] += xr[q ^ (1 << b)];
+                ai[q] += xi[q ^ (1 << b)];
+            }
+        // flips of the outer bits: partner tiles, coalesced loads
+#pragma unroll 4
+        for (int j = 0; j < ob; j++) {
+            const uint64_t poff = ((tile ^ (1ull << j)) << TB) + (uint64_t)t * R;
+            real yr[R], yi[R];
+            vload<real, R>(src + poff, yr);
+            vload<real, R>(src + N + poff, yi);
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+                ar[q] += yr[q];
+                ai[q] += yi[q];
+            }
+        }
+        __syncthreads();
+        // flips of the thread bits through LDS
+#pragma unroll
+        for (int b = 0; b < TB - RB; b++) {
+            const int pt = t ^ (1 << b);
+            real yr[R], yi[R];
+            vload<real, R>(s_re + pt * R, yr);
+            vload<real, R>(s_im + pt * R, yi);
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+                ar[q] += yr[q];
+                ai[q] += yi[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            ur[q] = J.alpha * e[q] * xr[q] - J.beta * ar[q];
+            ui[q] = J.alpha * e[q] * xi[q] - J.beta * ai[q];
+        }
+    }
+
+    real w1r = J.w1r, w1i = J.w1i, w2r = J.w2r, w2i = J.w2i;
+    if (src1 == SRC1_PSI) {
+        w1r *= ps;
+        w1i *= ps;
+    }
+    real yr[R], yi[R];
+    if (src2 == SRC2_ZERO) {
+#pragma unroll
+        for (int q = 0; q < R; q++) yr[q] = yi[q] = real(0);
+    } else if (src2 == SRC2_PSI) {
+        w2r *= ps;
+        w2i *= ps;
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            yr[q] = pr[q];
+            yi[q] = pi[q];
+        }
+    } else {
+        vload<real, R>(J.b2 + off, yr);
+        vload<real, R>(J.b2 + N + off, yi);
+    }
+    const real cr = J.cr * ps, ci = J.ci * ps;
+    real orr[R], oi[R];
+    double nrm = 0.0;
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        orr[q] = (w1r * ur[q] - w1i * ui[q]) - (w2r * yr[q] - w2i * yi[q]) + (cr * pr[q] - ci * pi[q]);
+        oi[q] = (w1r * ui[q] + w1i * ur[q]) - (w2r * yi[q] + w2i * yr[q]) + (cr * pi[q] + ci * pr[q]);
+        nrm += (double)orr[q] * (double)orr[q] + (double)oi[q] * (double)oi[q];
+    }
+    vstore<real, R>(J.out + off, orr);
+    vstore<real, R>(J.out + N + off, oi);
+
+    if (flags & F_FINAL) {
+        const double s = wg_sum_t0<NT>(nrm, s_red);
+        if (t == 0) J.norm_part[tile] = s;
+    }
+}
+
+template <typename real, int TB>
+static int launch_term_tb(const void* jobs, uint32_t njobs, int n, int lc, hipStream_t st)
+{
+    // 16-byte vectors per thread at least; at most 1024 threads per workgroup
+    constexpr int RB0 = (sizeof(real) == 4) ? 2 : 1;
+    constexpr int RB = (TB - RB0 > 10) ? TB - 10 : RB0;
+    constexpr int NT = 1 << (TB - RB);
+    static_assert(NT <= 1024, "workgroup too large");
+    const int ob = n - TB;
+    if (ob < 0) return -2;
+    if (lc > ob + 16) lc = ob + 16;
+    const uint64_t units = (uint64_t)njobs << ob;
+    const uint64_t per = 8ull << lc;
+    const uint64_t grid = (units + per - 1) / per * per;
+    if (grid > 0x7fffffffull) return -3;
+    hipLaunchKernelGGL((term_kernel<real, TB, RB>), dim3((uint32_t)grid), dim3(NT), 0, st,
+                       (const TermJob<real>*)jobs, njobs, n, lc);
+    return check_launch("term");
+}
+
+int term_tile_bits_supported(int precision, int tb)
+{
+    if (precision == MQW_FP32) return tb >= 4 && tb <= 13;
+    return tb >= 4 && tb <= 12;
+}
+
+int launch_term(int precision, int tb, const void* jobs_dev, uint32_t njobs, int n, int lc, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == MQW_FP32) {
+        switch (tb) {
+        case 4: return launch_term_tb<float, 4>(jobs_dev, njobs, n, lc, st);
+        case 5: return launch_term_tb<float, 5>(jobs_dev, njobs, n, lc, st);
+        case 6: return launch_term_tb<float, 6>(jobs_dev, njobs, n, lc, st);
+        case 7: return launch_term_tb<float, 7>(jobs_dev, njobs, n, lc, st);
+        case 8: return launch_term_tb<float, 8>(jobs_dev, njobs, n, lc, st);
+        case 9: return launch_term_tb<float, 9>(jobs_dev, njobs, n, lc, st);
+        case 10: return launch_term_tb<float, 10>(jobs_dev, njobs, n, lc, st);
+        case 11: return launch_term_tb<float, 11>(jobs_dev, njobs, n, lc, st);
+        case 12: return launch_term_tb<float, 12>(jobs_dev, njobs, n, lc, st);
+        case 13: return launch_term_tb<float, 13>(jobs_dev, njobs, n, lc, st);
+        }
+    } else {
+        switch (tb) {
+        case 4: return launch_term_tb<double, 4>(jobs_dev, njobs, n, lc, st);
+        case 5: return launch_term_tb<double, 5>(jobs_dev, njobs, n, lc, st);
+        case 6: return launch_term_tb<double, 6>(jobs_dev, njobs, n, lc, st);
+        case 7: return launch_term_tb<double, 7>(jobs_dev, njobs, n, lc, st);
+        case 8: return launch_term_tb<double, 8>(jobs_dev, njobs, n, lc, st);
+        case 9: return launch_term_tb<double, 9>(jobs_dev, njobs, n, lc, st);
+        case 10: return launch_term_tb<double, 10>(jobs_dev, njobs, n, lc, st);
+        case 11: return launch_term_tb<double, 11>(jobs_dev, njobs, n, lc, st);
+        case 12: return launch_term_tb<double, 12>(jobs_dev, njobs, n, lc, st);
+        }
+    }
+    return -1;
+}
+
+// ===========================================================================
+// Stage reduction: norm, next-stage scale, probability readout
+// ===========================================================================
+template <typename real>
+__global__ void __launch_bounds__(256) reduce_kernel(const ReduceJob* __restrict__ jobs)
+{
+    __shared__ double red[256];
+    const ReduceJob& J = jobs[blockIdx.x];
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (uint32_t k = t; k < J.ntiles; k += 256) s += J.norm_part[k];
+    red[t] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double tot = red[0];
+        *J.psi_scale = 1.0 / sqrt(tot);
+        if (J.prob) {
+            const real* o = (const real*)J.out;
+            const double re = (double)o[J.eloc], im = (double)o[J.N + J.eloc];
+            *J.prob = (re * re + im * im) / tot;
+        }
+    }
+}
+
+int launch_reduce(int precision, const ReduceJob* jobs_dev, uint32_t njobs, void* stream)
+{
+    if (njobs == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == MQW_FP32)
+        hipLaunchKernelGGL(reduce_kernel<float>, dim3(njobs), dim3(256), 0, st, jobs_dev);
+    else
+        hipLaunchKernelGGL(reduce_kernel<double>, dim3(njobs), dim3(256), 0, st, jobs_dev);
+    return check_launch("reduce");
+}
+
+// ===========================================================================
+// One real-half Clenshaw step on caller device buffers (the function-level
+// seam of MultiQW.cpp:88): b2 <- alpha (hp .* b1) - beta H_G b1 - b2 + coef psi
+// ===========================================================================
+__global__ void __launch_bounds__(256)
+half_step_kernel(const float* __restrict__ b1, float* __restrict__ b2, const float* __restrict__ hp,
+                 const float* __restrict__ psi, int n, float alpha, float beta, float coef)
+{
+    const uint64_t N = 1ull << n;
+    const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= N) return;
+    float acc = 0.0f;
+    for (int i = 0; i < n; i++) acc += b1[k ^ (1ull << i)];
+    float v = alpha * (hp[k] * b1[k]) - beta * acc - b2[k];
+    if (coef != 0.0f) v += coef * psi[k];
+    b2[k] = v;
+}
+
+int launch_half_step(const float* b1, float* b2, const float* hp, const float* psi, int n, float alpha, float beta,
+                     float coef, void* stream)
+{
+    const uint64_t N = 1ull << n;
+    hipLaunchKernelGGL(half_step_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, (hipStream_t)stream, b1,
+                       b2, hp, psi, n, alpha, beta, coef);
+    return check_launch("half_step");
+}
+
+} // namespace mqw
