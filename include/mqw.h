@@ -81,7 +81,10 @@ typedef struct {
     int chunk_bits;    /* -1 = tuned/default; XCD chunk of consecutive tiles, log2 */
     int small_n_max;   /* -1 = default; largest n run by the LDS-resident kernel (0 = never) */
     int use_graph;     /* 1 = capture each stage plan in a hipGraph (default 1) */
-    size_t max_device_bytes; /* 0 = default budget for trajectory buffers */
+    size_t max_device_bytes; /* 0 = no cap; else a cap on the trajectory buffers of a group */
+    size_t group_bytes;      /* 0 = default (32 GiB): trajectory buffers of one group; a group's
+                                trajectories share every sweep launch */
+    int variant;             /* -1 = tuned/default; else a sweep-kernel variant (loads in flight) */
     float* sample_probs;     /* optional output: count*samples per-sample probabilities */
 } mqw_opts;
 

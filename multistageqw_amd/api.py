@@ -108,7 +108,8 @@ def write_output(n: int, m: int, probs, directory: str = ".") -> str:
 
 
 def make_opts(samples=L.DEFAULT_SAMPLES, tol=L.DEFAULT_TOL, precision=L.FP32, energy_mode=L.ENERGY_COMPAT,
-              tile_bits=0, chunk_bits=-1, small_n_max=-1, use_graph=True, max_device_bytes=0) -> L.Opts:
+              tile_bits=0, chunk_bits=-1, small_n_max=-1, use_graph=True, max_device_bytes=0,
+              group_bytes=0, variant=-1) -> L.Opts:
     o = L.Opts()
     L.load().mqw_opts_default(C.byref(o))
     o.samples = samples
@@ -120,6 +121,8 @@ def make_opts(samples=L.DEFAULT_SAMPLES, tol=L.DEFAULT_TOL, precision=L.FP32, en
     o.small_n_max = small_n_max
     o.use_graph = int(bool(use_graph))
     o.max_device_bytes = max_device_bytes
+    o.group_bytes = group_bytes
+    o.variant = variant
     return o
 
 

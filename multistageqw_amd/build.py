@@ -65,6 +65,9 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    copy_src = os.path.join(CSRC, "hbm_copy.hip")
+    _run([HIPCC, "-x", "hip", f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, "-o", os.path.join(HERE, "hbm_copy"),
+          copy_src])
     cli_src = os.path.join(CSRC, "mqw_cli.cpp")
     if os.path.exists(cli_src):
         _run([HIPCC, *COMMON, "-D__HIP_PLATFORM_AMD__", "-o", CLI, cli_src, f"-L{HERE}", "-lmqw",

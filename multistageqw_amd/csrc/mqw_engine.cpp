@@ -42,6 +42,7 @@ struct mqw_ctx {
     mqw_stats stats{};
     int tuned_tb[MQW_MAX_N + 1];
     int tuned_lc[MQW_MAX_N + 1];
+    int tuned_var[MQW_MAX_N + 1];
 };
 
 namespace {
@@ -107,8 +108,46 @@ int default_tile_bits(int precision, int n)
 
 int default_chunk_bits(int n, int tb)
 {
-    // keep a whole trajectory (up to 256 tiles) on one XCD
-    return std::min(n - tb, 8);
+    // runs of 64 tiles (a sub-cube of the outer bits) per XCD measured best
+    return std::min(n - tb, 6);
+}
+
+// Which sweep kernel runs a group: the real-half kernel (FP32, n >= its tile
+// bits) or the complex-tile kernel.  `tb` is the tile bits of that kernel and
+// `parts` the number of partial norm sums one output leaves.
+struct SweepCfg {
+    bool half = false;
+    int tb = 12;
+    int lc = 0;
+    int variant = 0;
+    uint32_t parts = 1;
+};
+
+SweepCfg choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits, int chunk_bits, int variant = -1)
+{
+    SweepCfg k;
+    k.variant = variant >= 0 ? variant : c->tuned_var[n];
+    int tb = tile_bits > 0 ? tile_bits : c->tuned_tb[n];
+    // Untuned default, from measurements on MI355X: complex 2^12 tiles up to
+    // n = 21, real-half 2^13 tiles (fewer partner-tile loads) above.
+    if (tb <= 0 && precision == MQW_FP32 && n >= 22) {
+        tb = 13;
+        if (variant < 0 && c->tuned_var[n] == 0) k.variant = 1;
+    }
+    if (precision == MQW_FP32 && n >= 13 && tb > 0 && half_tile_bits_supported(tb)) {
+        if (tb > n) tb = 13;
+        k.half = true;
+        k.tb = tb;
+        k.parts = 2u << (n - tb);
+        k.lc = chunk_bits >= 0 ? chunk_bits : (c->tuned_lc[n] >= 0 ? c->tuned_lc[n] : std::min(n - tb, 6));
+    } else {
+        if (tb <= 0 || tb > n || !term_tile_bits_supported(precision, tb)) tb = default_tile_bits(precision, n);
+        k.tb = tb;
+        k.parts = 1u << (n - tb);
+        k.lc = chunk_bits >= 0 ? chunk_bits : (c->tuned_lc[n] >= 0 ? c->tuned_lc[n] : default_chunk_bits(n, tb));
+    }
+    k.lc = std::max(0, std::min(k.lc, 24));
+    return k;
 }
 
 // ---------------------------------------------------------------------------
@@ -188,7 +227,16 @@ mqw_status group_energies(mqw_ctx* c, int n, std::vector<Problem*>& ps, int mode
 }
 
 // ---------------------------------------------------------------------------
-// The sweep plan of one wave of trajectories (global-memory path).
+// Sweep plans (global-memory path).
+//
+// Each trajectory becomes a linear sequence of term jobs across all of its
+// stages.  Trajectories run in groups whose buffers fit the device's memory-
+// side cache (Infinity Cache): a group's launch q holds job q of every member
+// still running, so members progress independently through their stages and
+// nobody idles at stage boundaries.  Groups run one after another on the
+// stream and reuse the same buffer slots.  The stage norm is reduced inside
+// the next stage's first term (F_NORM_IN) and a readout folds into it
+// (F_READOUT); a reduce launch per group reads out the last stage.
 // ---------------------------------------------------------------------------
 struct Op {
     int kind; // 0 = term launch, 1 = reduce launch
@@ -206,145 +254,169 @@ struct Plan {
     uint64_t sweeps = 0;
 };
 
+// Buffers of one trajectory slot: three planar states and two partial-sum arrays.
 template <typename real>
-void build_plan(int n, int tb, const std::vector<Traj>& trajs, const std::vector<Problem*>& ps, const real* E,
-                real* state, double* norm_part, double* psi_scale, double* probs, Plan<real>& P)
+struct Slot {
+    real* st[3];
+    double* part[2];
+};
+
+template <typename real>
+void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, const real* E, uint32_t eloc,
+               double* probs, std::vector<TermJob<real>>& out, ReduceJob& fin, bool& has_fin)
 {
     const size_t N = (size_t)1 << n;
-    const uint32_t ntiles = 1u << (n - tb);
-    const size_t T = trajs.size();
-    struct Rot {
-        int p, a, b;
-    };
-    std::vector<Rot> rot(T, Rot{-1, 0, 1});
-    auto buf = [&](size_t t, int k) { return state + (t * 3 + (size_t)k) * 2 * N; };
     const real psi0 = (real)(float)(1 / sqrt((double)N));
-    size_t rounds = 0;
-    for (auto& tr : trajs) rounds = std::max(rounds, tr.segs.size());
-    std::vector<int> K(T), S(T);
-    for (size_t s = 0; s < rounds; s++) {
-        int Smax = 0;
-        for (size_t t = 0; t < T; t++) {
-            if (s >= trajs[t].segs.size()) continue;
-            K[t] = (int)trajs[t].segs[s].c.size();
-            S[t] = std::max(K[t] - 1, 1);
-            Smax = std::max(Smax, S[t]);
-        }
-        for (int q = 0; q < Smax; q++) {
-            Op op{0, P.jobs.size(), 0};
-            for (size_t t = 0; t < T; t++) {
-                if (s >= trajs[t].segs.size()) continue;
-                const int start = Smax - S[t];
-                if (q < start) continue;
-                const int d = q - start;
-                const Segment& sg = trajs[t].segs[s];
-                const Rot& r = rot[t];
-                const std::vector<double>& cc = sg.c;
-                TermJob<real> j{};
-                const bool pconst = r.p < 0 && !trajs[t].psi_in;
-                j.psi = r.p >= 0 ? buf(t, r.p) : (const real*)trajs[t].psi_in;
-                j.psi_scale = r.p >= 0 ? psi_scale + t : nullptr;
-                j.psi0 = psi0;
-                j.E = E + (size_t)trajs[t].prob * N;
-                j.norm_part = norm_part + t * ntiles;
-                uint32_t flags = pconst ? F_PSI_CONST : 0u;
-                auto rc = [](double v) { return (real)v; };
-                if (K[t] == 1) {
-                    flags |= SRC1_ZERO | SRC2_ZERO | F_FINAL;
-                    j.out = buf(t, r.a);
-                    j.cr = rc(cc[0]);
-                    j.ci = real(0);
-                    j.alpha = j.beta = real(0);
-                } else {
-                    const int rr = K[t] - 2 - d;
-                    const bool final_term = rr == 0;
-                    j.out = buf(t, (d & 1) ? r.b : r.a);
-                    const double top = cc[K[t] - 1];
-                    const bool top_odd = ((K[t] - 1) & 1) != 0;
-                    if (d == 0) {
-                        flags |= SRC1_PSI;
-                        j.w1r = top_odd ? real(0) : rc(top);
-                        j.w1i = top_odd ? rc(top) : real(0);
-                    } else {
-                        flags |= SRC1_ARRAY;
-                        j.b1 = buf(t, ((d - 1) & 1) ? r.b : r.a);
-                        j.w1r = real(1);
-                        j.w1i = real(0);
-                    }
-                    if (d == 0) {
-                        flags |= SRC2_ZERO;
-                    } else if (d == 1) {
-                        flags |= SRC2_PSI;
-                        j.w2r = top_odd ? real(0) : rc(top);
-                        j.w2i = top_odd ? rc(top) : real(0);
-                    } else {
-                        flags |= SRC2_ARRAY;
-                        j.b2 = j.out;
-                        j.w2r = real(1);
-                        j.w2i = real(0);
-                    }
-                    const bool odd = (rr & 1) != 0;
-                    j.cr = odd ? real(0) : rc(cc[rr]);
-                    j.ci = odd ? rc(cc[rr]) : real(0);
-                    const double sc = (double)sg.onenorm * (final_term ? 1.0 : 0.5);
-                    j.alpha = (real)(1.0 / sc);
-                    j.beta = (real)((double)sg.gamma / sc);
-                    if (final_term) flags |= F_FINAL;
+    int p = -1, a = 0, b = 1, par = 0;
+    int64_t prev_slot = -1;
+    auto rc = [](double v) { return (real)v; };
+    for (size_t s = 0; s < tr.segs.size(); s++) {
+        const Segment& sg = tr.segs[s];
+        const std::vector<double>& cc = sg.c;
+        const int K = (int)cc.size();
+        const int S = std::max(K - 1, 1);
+        const bool pconst = p < 0 && !tr.psi_in;
+        const real* psi = p >= 0 ? sl.st[p] : (const real*)tr.psi_in;
+        for (int d = 0; d < S; d++) {
+            TermJob<real> j{};
+            j.psi = psi;
+            j.psi0 = psi0;
+            j.E = E;
+            j.eloc = eloc;
+            uint32_t flags = pconst ? F_PSI_CONST : 0u;
+            if (d == 0 && p >= 0) {
+                flags |= F_NORM_IN;
+                j.norm_in = sl.part[par ^ 1];
+                if (prev_slot >= 0) {
+                    flags |= F_READOUT;
+                    j.prob = probs + prev_slot;
                 }
-                j.flags = flags;
-                P.jobs.push_back(j);
-                op.count++;
-                P.amp_terms += N;
             }
+            if (K == 1) {
+                flags |= SRC1_ZERO | SRC2_ZERO | F_FINAL;
+                j.out = sl.st[a];
+                j.cr = rc(cc[0]);
+            } else {
+                const int r = K - 2 - d;
+                j.out = sl.st[(d & 1) ? b : a];
+                const double top = cc[K - 1];
+                const bool top_odd = ((K - 1) & 1) != 0;
+                if (d == 0) {
+                    flags |= SRC1_PSI;
+                    j.w1r = top_odd ? real(0) : rc(top);
+                    j.w1i = top_odd ? rc(top) : real(0);
+                } else {
+                    flags |= SRC1_ARRAY;
+                    j.b1 = sl.st[((d - 1) & 1) ? b : a];
+                    j.w1r = real(1);
+                }
+                if (d == 0) {
+                    flags |= SRC2_ZERO;
+                } else if (d == 1) {
+                    flags |= SRC2_PSI;
+                    j.w2r = top_odd ? real(0) : rc(top);
+                    j.w2i = top_odd ? rc(top) : real(0);
+                } else {
+                    flags |= SRC2_ARRAY;
+                    j.b2 = j.out;
+                    j.w2r = real(1);
+                }
+                const bool odd = (r & 1) != 0;
+                j.cr = odd ? real(0) : rc(cc[r]);
+                j.ci = odd ? rc(cc[r]) : real(0);
+                const double sc = (double)sg.onenorm * (r == 0 ? 1.0 : 0.5);
+                j.alpha = (real)(1.0 / sc);
+                j.beta = (real)((double)sg.gamma / sc);
+                if (r == 0) flags |= F_FINAL;
+            }
+            if (flags & F_FINAL) j.norm_part = sl.part[par];
+            j.flags = flags;
+            out.push_back(j);
+        }
+        const int fout = (K == 1) ? a : (((S - 1) & 1) ? b : a);
+        int others[2], k = 0;
+        for (int q = 0; q < 3; q++)
+            if (q != fout) others[k++] = q;
+        p = fout;
+        a = others[0];
+        b = others[1];
+        par ^= 1;
+        prev_slot = sg.prob_slot;
+    }
+    has_fin = prev_slot >= 0;
+    if (has_fin) {
+        fin = ReduceJob{};
+        fin.norm_part = sl.part[par ^ 1];
+        fin.out = sl.st[p];
+        fin.N = N;
+        fin.ntiles = parts;
+        fin.eloc = eloc;
+        fin.prob = probs + prev_slot;
+    }
+}
+
+// Groups of trajectories (indices into trajs), each run to completion before the next.
+template <typename real>
+void build_plan(int n, uint32_t parts, const std::vector<Traj>& trajs, const std::vector<std::vector<size_t>>& groups,
+                const std::vector<Problem*>& ps, const real* E, const std::vector<Slot<real>>& slots, double* probs,
+                Plan<real>& P)
+{
+    const size_t N = (size_t)1 << n;
+    for (const auto& grp : groups) {
+        std::vector<std::vector<TermJob<real>>> seq(grp.size());
+        std::vector<ReduceJob> fins;
+        size_t L = 0;
+        for (size_t i = 0; i < grp.size(); i++) {
+            const Traj& tr = trajs[grp[i]];
+            ReduceJob fin;
+            bool has = false;
+            traj_jobs<real>(n, parts, tr, slots[i], E + (size_t)tr.prob * N, ps[tr.prob]->info.eloc, probs, seq[i],
+                            fin, has);
+            if (has) fins.push_back(fin);
+            L = std::max(L, seq[i].size());
+            P.terms += seq[i].size();
+        }
+        for (size_t q = 0; q < L; q++) {
+            Op op{0, P.jobs.size(), 0};
+            for (size_t i = 0; i < grp.size(); i++)
+                if (q < seq[i].size()) {
+                    P.jobs.push_back(seq[i][q]);
+                    op.count++;
+                    P.amp_terms += N;
+                }
             P.ops.push_back(op);
             P.sweeps++;
         }
-        // reduce: norms, next-stage scales, readouts; then rotate buffers
-        Op rop{1, P.rjobs.size(), 0};
-        for (size_t t = 0; t < T; t++) {
-            if (s >= trajs[t].segs.size()) continue;
-            Rot& r = rot[t];
-            const int fin = (K[t] == 1) ? r.a : (((S[t] - 1) & 1) ? r.b : r.a);
-            ReduceJob rj{};
-            rj.norm_part = norm_part + t * ntiles;
-            rj.out = buf(t, fin);
-            rj.N = N;
-            rj.ntiles = ntiles;
-            rj.eloc = ps[trajs[t].prob]->info.eloc;
-            rj.psi_scale = psi_scale + t;
-            const int64_t slot = trajs[t].segs[s].prob_slot;
-            rj.prob = slot >= 0 ? probs + slot : nullptr;
-            P.rjobs.push_back(rj);
-            rop.count++;
-            P.terms += (uint64_t)S[t];
-            int others[2], k = 0;
-            for (int q = 0; q < 3; q++)
-                if (q != fin) others[k++] = q;
-            r.p = fin;
-            r.a = others[0];
-            r.b = others[1];
+        if (!fins.empty()) {
+            P.ops.push_back(Op{1, P.rjobs.size(), (uint32_t)fins.size()});
+            P.rjobs.insert(P.rjobs.end(), fins.begin(), fins.end());
         }
-        P.ops.push_back(rop);
     }
 }
 
 template <typename real>
-mqw_status run_plan(mqw_ctx* c, int precision, int n, int tb, int lc, Plan<real>& P, bool use_graph, double* ms)
+mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<real>& P, bool use_graph, double* ms)
 {
     TermJob<real>* jd = (TermJob<real>*)dev_get(c, "plan_jobs", sizeof(TermJob<real>) * P.jobs.size());
-    ReduceJob* rd = (ReduceJob*)dev_get(c, "plan_rjobs", sizeof(ReduceJob) * P.rjobs.size());
+    ReduceJob* rd = (ReduceJob*)dev_get(c, "plan_rjobs", sizeof(ReduceJob) * std::max<size_t>(1, P.rjobs.size()));
     if (!jd || !rd) return fail(c, MQW_ENOMEM, "device allocation failed for %zu sweep jobs", P.jobs.size());
     HIPCHK(c, hipMemcpyAsync(jd, P.jobs.data(), sizeof(TermJob<real>) * P.jobs.size(), hipMemcpyHostToDevice,
                              c->stream));
-    HIPCHK(c, hipMemcpyAsync(rd, P.rjobs.data(), sizeof(ReduceJob) * P.rjobs.size(), hipMemcpyHostToDevice,
-                             c->stream));
+    if (!P.rjobs.empty())
+        HIPCHK(c, hipMemcpyAsync(rd, P.rjobs.data(), sizeof(ReduceJob) * P.rjobs.size(), hipMemcpyHostToDevice,
+                                 c->stream));
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
     auto issue = [&]() -> int {
         for (const Op& op : P.ops) {
-            int rc = op.kind == 0 ? launch_term(precision, tb, jd + op.off, op.count, n, lc, c->stream)
-                                  : launch_reduce(precision, rd + op.off, op.count, c->stream);
+            int rc;
+            if (op.kind == 1)
+                rc = launch_reduce(precision, rd + op.off, op.count, c->stream);
+            else if (k.half)
+                rc = launch_half_term(k.tb, k.variant, jd + op.off, op.count, n, k.lc, c->stream);
+            else
+                rc = launch_term(precision, k.tb, jd + op.off, op.count, n, k.lc, c->stream);
             if (rc) return rc;
         }
         return 0;
@@ -474,11 +546,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     const size_t N = (size_t)1 << n;
     const int precision = o.precision;
     const int samples = o.samples;
-    int tb = o.tile_bits > 0 ? std::min(o.tile_bits, n) : (c->tuned_tb[n] > 0 ? c->tuned_tb[n]
-                                                                               : default_tile_bits(precision, n));
-    if (!term_tile_bits_supported(precision, tb)) tb = default_tile_bits(precision, n);
-    int lc = o.chunk_bits >= 0 ? o.chunk_bits : (c->tuned_lc[n] >= 0 ? c->tuned_lc[n] : default_chunk_bits(n, tb));
-    lc = std::max(0, std::min(lc, 24));
+    const SweepCfg k = choose_sweep(c, precision, n, o.tile_bits, o.chunk_bits, o.variant);
     real* E = (real*)dev_get(c, "E_group", sizeof(real) * N * ps.size());
     if (!E) return fail(c, MQW_ENOMEM, "device allocation failed for %zu diagonals at n=%d", ps.size(), n);
     mqw_status st = group_energies(c, n, ps, o.energy_mode, precision, E);
@@ -494,13 +562,8 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             sg.gamma = pr.sch.gamma[i];
             sg.onenorm = fmaf(sg.gamma, (float)(unsigned)n, pr.info.e_abs);
             const double s = (double)(sg.onenorm * pr.times[(size_t)i * samples + j]);
-            std::vector<double> cc(8192);
-            int K = cheb_coeffs(s, o.tol, cc.data(), (int)cc.size());
-            if (K < 0) {
-                cc.resize(1 << 20);
-                K = cheb_coeffs(s, o.tol, cc.data(), (int)cc.size());
-            }
-            cc.resize((size_t)std::max(K, 1));
+            std::vector<double> cc;
+            cheb_coeffs_vec(s, o.tol, cc);
             if (precision == MQW_FP32)
                 for (double& v : cc) v = (double)(float)v; // the reference casts to float (:440)
             sg.c = std::move(cc);
@@ -536,27 +599,52 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         return MQW_OK;
     }
 
-    // waves bounded by the device budget
-    const uint32_t ntiles = 1u << (n - tb);
-    const size_t per_traj = sizeof(real) * 6 * N + sizeof(double) * (ntiles + 1);
-    size_t budget = o.max_device_bytes ? o.max_device_bytes : ((size_t)48 << 30);
-    size_t wave = std::max<size_t>(1, budget / per_traj);
-    for (size_t w0 = 0; w0 < trajs.size(); w0 += wave) {
-        const size_t w1 = std::min(trajs.size(), w0 + wave);
-        std::vector<Traj> wt(trajs.begin() + w0, trajs.begin() + w1);
-        const size_t T = wt.size();
-        real* state = (real*)dev_get(c, "state", sizeof(real) * 6 * N * T);
-        double* np = (double*)dev_get(c, "norm_part", sizeof(double) * ntiles * T);
-        double* pscale = (double*)dev_get(c, "psi_scale", sizeof(double) * T);
-        if (!state || !np || !pscale)
-            return fail(c, MQW_ENOMEM, "device allocation failed for %zu trajectories at n=%d", T, n);
+    // groups sized to the memory-side cache, longest trajectories first
+    const uint32_t nparts = k.parts;
+    const size_t per_traj = sizeof(real) * 6 * N + sizeof(double) * 2 * nparts;
+    const size_t budget = o.group_bytes ? o.group_bytes : ((size_t)32 << 30);
+    size_t G = std::max<size_t>(1, budget / per_traj);
+    if (o.max_device_bytes) G = std::min(G, std::max<size_t>(1, o.max_device_bytes / per_traj));
+    G = std::min(G, trajs.size());
+    std::vector<size_t> order(trajs.size());
+    std::vector<size_t> cost(trajs.size(), 0);
+    for (size_t t = 0; t < trajs.size(); t++) {
+        order[t] = t;
+        for (const Segment& sg : trajs[t].segs) cost[t] += std::max<size_t>(sg.c.size() - 1, 1);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+    std::vector<std::vector<size_t>> groups;
+    for (size_t i = 0; i < order.size(); i += G)
+        groups.emplace_back(order.begin() + i, order.begin() + std::min(order.size(), i + G));
+    real* state = (real*)dev_get(c, "state", sizeof(real) * 6 * N * G);
+    double* parts = (double*)dev_get(c, "norm_part", sizeof(double) * 2 * nparts * G);
+    if (!state || !parts) return fail(c, MQW_ENOMEM, "device allocation failed for %zu trajectory slots at n=%d", G, n);
+    std::vector<Slot<real>> slots(G);
+    for (size_t i = 0; i < G; i++) {
+        for (int q = 0; q < 3; q++) slots[i].st[q] = state + (i * 3 + (size_t)q) * 2 * N;
+        for (int q = 0; q < 2; q++) slots[i].part[q] = parts + (i * 2 + (size_t)q) * nparts;
+    }
+    // bound the host job table: run the groups in batches
+    const size_t jobs_cap = (size_t)1 << 22;
+    size_t g0 = 0;
+    while (g0 < groups.size()) {
+        size_t g1 = g0, njobs = 0;
+        while (g1 < groups.size()) {
+            size_t gj = 0;
+            for (size_t t : groups[g1]) gj += cost[t];
+            if (g1 > g0 && njobs + gj > jobs_cap) break;
+            njobs += gj;
+            g1++;
+        }
+        std::vector<std::vector<size_t>> batch(groups.begin() + g0, groups.begin() + g1);
         Plan<real> plan;
-        build_plan<real>(n, tb, wt, ps, E, state, np, pscale, probs_d, plan);
-        st = run_plan<real>(c, precision, n, tb, lc, plan, o.use_graph != 0, &c->stats.gpu_ms);
+        build_plan<real>(n, nparts, trajs, batch, ps, E, slots, probs_d, plan);
+        st = run_plan<real>(c, precision, n, k, plan, o.use_graph != 0, &c->stats.gpu_ms);
         if (st) return st;
         c->stats.sweeps += plan.sweeps;
         c->stats.term_units += plan.terms;
         c->stats.amp_terms += plan.amp_terms;
+        g0 = g1;
     }
     HIPCHK(c, hipMemcpyAsync(probs.data(), probs_d, sizeof(double) * probs.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -660,6 +748,8 @@ void mqw_opts_default(mqw_opts* o)
     o->small_n_max = -1;
     o->use_graph = 1;
     o->max_device_bytes = 0;
+    o->group_bytes = 0;
+    o->variant = -1;
     o->sample_probs = nullptr;
 }
 
@@ -679,6 +769,7 @@ mqw_status mqw_create(int device, mqw_ctx** out)
     for (int i = 0; i <= MQW_MAX_N; i++) {
         c->tuned_tb[i] = 0;
         c->tuned_lc[i] = -1;
+        c->tuned_var[i] = 0;
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return MQW_EDEVICE;
     *out = c.release();
@@ -818,19 +909,15 @@ mqw_status mqw_clenshaw(mqw_ctx* c, int n, const float* coeffs, int K, float* ps
     if (!(scale > 0.0f) || !isfinite(scale) || !isfinite(gamma)) return fail(c, MQW_EINVAL, "bad scale");
     hipSetDevice(c->device);
     const size_t N = (size_t)1 << n;
-    const int tb = default_tile_bits(MQW_FP32, n);
-    const uint32_t ntiles = 1u << (n - tb);
+    const SweepCfg k = choose_sweep(c, MQW_FP32, n, 0, -1);
+    const uint32_t ntiles = k.parts;
     float* E = (float*)dev_get(c, "cl_E", sizeof(float) * N);
     float* state = (float*)dev_get(c, "cl_state", sizeof(float) * 8 * N);
-    double* np = (double*)dev_get(c, "cl_norm", sizeof(double) * (ntiles + 1));
+    double* np = (double*)dev_get(c, "cl_norm", sizeof(double) * 2 * ntiles);
     if (!E || !state || !np) return fail(c, MQW_ENOMEM, "device allocation failed");
     HIPCHK(c, hipMemcpyAsync(E, hp, sizeof(float) * N, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(state, psi, sizeof(float) * 2 * N, hipMemcpyHostToDevice, c->stream));
     // one trajectory, one stage, input uploaded to `state`, buffers after it
-    Problem p;
-    p.n = n;
-    p.info.eloc = 0;
-    std::vector<Problem*> ps{&p};
     Traj tr;
     tr.prob = 0;
     tr.psi_in = state;
@@ -838,13 +925,23 @@ mqw_status mqw_clenshaw(mqw_ctx* c, int n, const float* coeffs, int K, float* ps
     sg.gamma = gamma;
     sg.onenorm = scale;
     sg.c.assign(coeffs, coeffs + K);
-    sg.prob_slot = -1;
+    sg.prob_slot = 0; // a readout gives the reduce job that names the output buffer
     tr.segs.push_back(sg);
+    Slot<float> sl;
+    for (int k = 0; k < 3; k++) sl.st[k] = state + (size_t)(2 + 2 * k) * N;
+    sl.part[0] = np;
+    sl.part[1] = np + ntiles;
+    double* dummy = (double*)dev_get(c, "cl_prob", sizeof(double));
+    if (!dummy) return fail(c, MQW_ENOMEM, "device allocation failed");
     Plan<float> plan;
-    std::vector<Traj> trajs{tr};
-    build_plan<float>(n, tb, trajs, ps, E, state + 2 * N, np, np + ntiles, nullptr, plan);
+    ReduceJob fin;
+    bool has = false;
+    traj_jobs<float>(n, ntiles, tr, sl, E, 0, dummy, plan.jobs, fin, has);
+    for (size_t q = 0; q < plan.jobs.size(); q++) plan.ops.push_back(Op{0, q, 1});
+    plan.rjobs.push_back(fin);
+    plan.ops.push_back(Op{1, 0, 1});
     double ms = 0;
-    mqw_status st = run_plan<float>(c, MQW_FP32, n, tb, default_chunk_bits(n, tb), plan, false, &ms);
+    mqw_status st = run_plan<float>(c, MQW_FP32, n, k, plan, false, &ms);
     if (st) return st;
     const float* outp = (const float*)plan.rjobs.back().out;
     HIPCHK(c, hipMemcpyAsync(psi, outp, sizeof(float) * 2 * N, hipMemcpyDeviceToHost, c->stream));
@@ -901,9 +998,9 @@ mqw_status mqw_tune(mqw_ctx* c, int n, const char* cache_path, int* tile_bits, i
     (void)cache_path;
     if (!c) return MQW_EINVAL;
     if (n < MQW_MIN_N || n > MQW_MAX_N) return fail(c, MQW_EINVAL, "n out of range");
-    const int tb = default_tile_bits(MQW_FP32, n);
-    if (tile_bits) *tile_bits = tb;
-    if (chunk_bits) *chunk_bits = default_chunk_bits(n, tb);
+    const SweepCfg k = choose_sweep(c, MQW_FP32, n, 0, -1);
+    if (tile_bits) *tile_bits = k.tb;
+    if (chunk_bits) *chunk_bits = k.lc;
     return MQW_OK;
 }
 

@@ -186,16 +186,17 @@ void sample_times(int m, int samples, const float* short_t, float* times)
 // J_k(s) by Miller's backward recurrence J_{k-1} = (2k/s) J_k - J_{k+1} from an
 // order well past the turning point k = s, normalised by J_0 + 2 sum J_{2k} = 1.
 // K is the first order whose tail sum of |c_k| is <= tol.
-int cheb_coeffs(double s, double tol, double* c, int cap)
+int cheb_coeffs_vec(double s, double tol, std::vector<double>& c)
 {
     if (!(s > 0.0)) {
-        if (cap < 1) return -1;
-        c[0] = 1.0;
+        c.assign(1, 1.0);
         return 1;
     }
     int top = (int)(s + 12.0 * cbrt(s) + 40.0);
     top += top & 1;
-    std::vector<double> j((size_t)top + 2, 0.0);
+    thread_local std::vector<double> j;
+    j.resize((size_t)top + 2);
+    j[top + 1] = 0.0;
     j[top] = 1e-300;
     for (int k = top; k > 0; k--) {
         j[k - 1] = (2.0 * k / s) * j[k] - j[k + 1];
@@ -213,11 +214,20 @@ int cheb_coeffs(double s, double tol, double* c, int cap)
         tail += ck;
         K = k;
     }
-    if (K > cap) return -1;
+    c.resize((size_t)K);
     for (int k = 0; k < K; k++) {
         const double sgn = ((k >> 1) & 1) ? -1.0 : 1.0;
         c[k] = (k == 0 ? 1.0 : 2.0) * sgn * (j[k] * inv);
     }
+    return K;
+}
+
+int cheb_coeffs(double s, double tol, double* c, int cap)
+{
+    std::vector<double> v;
+    const int K = cheb_coeffs_vec(s, tol, v);
+    if (K > cap) return -1;
+    std::copy(v.begin(), v.end(), c);
     return K;
 }
 

@@ -30,6 +30,7 @@ void schedule_heuristic(int n, int m, const float* J, const float* h, mqw_schedu
 void sample_times(int m, int samples, const float* short_t, float* times);
 // Coefficients c[0..K) of cos(sx)+sin(sx); returns K or -1 when cap is too small.
 int cheb_coeffs(double s, double tol, double* c, int cap);
+int cheb_coeffs_vec(double s, double tol, std::vector<double>& c);
 void generate_sk(int n, int count, uint64_t seed, double* out);
 // FP32 and FP64 energy of the all -1 state: (float)(sum J - sum h) as the walk
 // starts (MultiQW.cpp:332), and the exact FP64 value from the same couplings.
@@ -40,8 +41,10 @@ double e_init_exact(int n, const float* J, const float* h);
 // Term jobs: one complex Clenshaw term for one trajectory.
 //   u   = alpha * E .* s1 - beta * H_G s1            (s1 per SRC1_*)
 //   out = w1 * u - w2 * s2 + c * Psi                 (complex scalars)
-// with Psi the segment's input state scaled by *psi_scale.  Terms whose s1
-// or s2 is Psi also take the psi_scale factor.  flags are uniform per job.
+// with Psi the stage's input state scaled by 1/||Psi||.  That norm comes from
+// the per-tile partial sums the previous stage's final term left in norm_in
+// (F_NORM_IN; every workgroup reduces them in the same fixed order); terms
+// whose s1 or s2 is Psi take the same factor.  flags are uniform per job.
 // ---------------------------------------------------------------------------
 enum : uint32_t {
     SRC1_ARRAY = 0u,
@@ -52,6 +55,8 @@ enum : uint32_t {
     SRC2_ZERO = 2u << 2,
     F_PSI_CONST = 1u << 4, // Psi is |+>: psi0 on every real part, no array
     F_FINAL = 1u << 5,     // write the tile's sum |out|^2 to norm_part[tile]
+    F_NORM_IN = 1u << 6,   // Psi's norm from norm_in (ntiles partial sums)
+    F_READOUT = 1u << 7,   // tile 0 writes |Psi[eloc]|^2 / ||Psi||^2 to *prob
 };
 
 template <typename real>
@@ -59,15 +64,17 @@ struct TermJob {
     const real* b1;          // s1 array (SRC1_ARRAY)
     const real* b2;          // s2 array (SRC2_ARRAY); may alias out
     real* out;               // planar output
-    const real* psi;         // segment input state (planar), unless F_PSI_CONST
+    const real* psi;         // stage input state (planar), unless F_PSI_CONST
     const real* E;           // diagonal H_P (float for FP32, double for FP64)
-    const double* psi_scale; // 1/||psi|| of the segment input, nullable
-    double* norm_part;       // F_FINAL: one double per tile
+    const double* norm_in;   // F_NORM_IN: partial sums of |psi|^2, one per tile
+    double* norm_part;       // F_FINAL: one double per tile of out
+    double* prob;            // F_READOUT
     real alpha, beta;
     real w1r, w1i, w2r, w2i;
     real cr, ci;
     real psi0;
     uint32_t flags;
+    uint32_t eloc;
 };
 
 struct ReduceJob {
@@ -76,7 +83,7 @@ struct ReduceJob {
     uint64_t N;
     uint32_t ntiles;
     uint32_t eloc;
-    double* psi_scale;       // receives 1/||out||
+    double* psi_scale;       // nullable: receives 1/||out||
     double* prob;            // nullable: receives |out[eloc]|^2 / ||out||^2
 };
 
@@ -104,6 +111,11 @@ int launch_energy(int n, const float* J_dev, const float* h_dev, double e_init, 
 
 int term_tile_bits_supported(int precision, int tb);
 int launch_term(int precision, int tb, const void* jobs_dev, uint32_t njobs, int n, int lc, void* stream);
+// Real-half sweep (FP32): hb bits of one real half per workgroup; partial
+// sums are laid out [half][tile], 2 * 2^(n-hb) per output.
+int half_tile_bits_supported(int hb);
+int half_variants(int hb);
+int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc, void* stream);
 int launch_reduce(int precision, const ReduceJob* jobs_dev, uint32_t njobs, void* stream);
 int launch_half_step(const float* b1, float* b2, const float* hp, const float* psi, int n, float alpha,
                      float beta, float coef, void* stream);

@@ -308,56 +308,115 @@ int launch_energy(int n, const float* J_dev, const float* h_dev, double e_init, 
 // ===========================================================================
 // Fused Clenshaw term
 // ===========================================================================
+// Pointers fetched from a job descriptor are generic; casting them to the
+// global address space makes the compiler emit global_load/store (counted by
+// vmcnt only) instead of flat accesses that would also stall LDS waits.
+#define MQW_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const MQW_GLOBAL T* gview(const T* p)
+{
+    return (const MQW_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ MQW_GLOBAL T* gview(T* p)
+{
+    return (MQW_GLOBAL T*)p;
+}
+
 template <typename real>
-struct Vec16;
+struct NVec16;
 template <>
-struct Vec16<float> {
-    using type = float4;
+struct NVec16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
     static constexpr int W = 4;
 };
 template <>
-struct Vec16<double> {
-    using type = double2;
+struct NVec16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
     static constexpr int W = 2;
 };
 
 template <typename real, int R>
 __device__ __forceinline__ void vload(const real* __restrict__ p, real (&v)[R])
 {
-    using vt = typename Vec16<real>::type;
-    constexpr int W = Vec16<real>::W;
+    using vt = typename NVec16<real>::type;
+    constexpr int W = NVec16<real>::W;
 #pragma unroll
     for (int q = 0; q < R / W; q++) {
-        const vt x = *reinterpret_cast<const vt*>(p + q * W);
-        if constexpr (W == 4) {
-            v[q * 4 + 0] = x.x;
-            v[q * 4 + 1] = x.y;
-            v[q * 4 + 2] = x.z;
-            v[q * 4 + 3] = x.w;
-        } else {
-            v[q * 2 + 0] = x.x;
-            v[q * 2 + 1] = x.y;
-        }
+        const vt x = *gview(reinterpret_cast<const vt*>(p + q * W));
+#pragma unroll
+        for (int w = 0; w < W; w++) v[q * W + w] = x[w];
     }
 }
 
 template <typename real, int R>
 __device__ __forceinline__ void vstore(real* p, const real (&v)[R])
 {
-    using vt = typename Vec16<real>::type;
-    constexpr int W = Vec16<real>::W;
+    using vt = typename NVec16<real>::type;
+    constexpr int W = NVec16<real>::W;
 #pragma unroll
     for (int q = 0; q < R / W; q++) {
         vt x;
-        if constexpr (W == 4) {
-            x.x = v[q * 4 + 0];
-            x.y = v[q * 4 + 1];
-            x.z = v[q * 4 + 2];
-            x.w = v[q * 4 + 3];
-        } else {
-            x.x = v[q * 2 + 0];
-            x.y = v[q * 2 + 1];
-        }
+#pragma unroll
+        for (int w = 0; w < W; w++) x[w] = v[q * W + w];
+        *gview(reinterpret_cast<vt*>(p + q * W)) = x;
+    }
+}
+
+// Streaming accesses: data read once per sweep (b_{k+2}, psi, E) carry the
+// non-temporal hint so L2 keeps the b_{k+1} tiles that neighbouring
+// workgroups re-read as partner tiles.
+template <typename real, int R>
+__device__ __forceinline__ void vload_nt(const real* __restrict__ p, real (&v)[R])
+{
+    using vt = typename NVec16<real>::type;
+    constexpr int W = NVec16<real>::W;
+#pragma unroll
+    for (int q = 0; q < R / W; q++) {
+        const vt x = __builtin_nontemporal_load(gview(reinterpret_cast<const vt*>(p + q * W)));
+#pragma unroll
+        for (int w = 0; w < W; w++) v[q * W + w] = x[w];
+    }
+}
+
+template <typename real, int R>
+__device__ __forceinline__ void vstore_nt(real* p, const real (&v)[R])
+{
+    using vt = typename NVec16<real>::type;
+    constexpr int W = NVec16<real>::W;
+#pragma unroll
+    for (int q = 0; q < R / W; q++) {
+        vt x;
+#pragma unroll
+        for (int w = 0; w < W; w++) x[w] = v[q * W + w];
+        __builtin_nontemporal_store(x, gview(reinterpret_cast<vt*>(p + q * W)));
+    }
+}
+
+// LDS accesses of the tile exchange buffer (generic pointers into __shared__).
+template <typename real, int R>
+__device__ __forceinline__ void lds_load(const real* p, real (&v)[R])
+{
+    using vt = typename NVec16<real>::type;
+    constexpr int W = NVec16<real>::W;
+#pragma unroll
+    for (int q = 0; q < R / W; q++) {
+        const vt x = *reinterpret_cast<const vt*>(p + q * W);
+#pragma unroll
+        for (int w = 0; w < W; w++) v[q * W + w] = x[w];
+    }
+}
+
+template <typename real, int R>
+__device__ __forceinline__ void lds_store(real* p, const real (&v)[R])
+{
+    using vt = typename NVec16<real>::type;
+    constexpr int W = NVec16<real>::W;
+#pragma unroll
+    for (int q = 0; q < R / W; q++) {
+        vt x;
+#pragma unroll
+        for (int w = 0; w < W; w++) x[w] = v[q * W + w];
         *reinterpret_cast<vt*>(p + q * W) = x;
     }
 }
@@ -409,6 +468,7 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
     __shared__ real s_re[TS];
     __shared__ real s_im[TS];
     __shared__ double s_red[NRED];
+    __shared__ double s_bcast;
 
     const int ob = n - TB;
     const uint64_t unit = xcd_unit(blockIdx.x, lc);
@@ -422,12 +482,27 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
     const uint64_t N = 1ull << n;
     const int t = threadIdx.x;
     const uint64_t off = (tile << TB) + (uint64_t)t * R;
-    const real ps = J.psi_scale ? (real)(*J.psi_scale) : real(1);
+    real ps = real(1);
+    if (flags & F_NORM_IN) {
+        // 1/||Psi|| from the previous stage's partial sums, same order in every workgroup
+        const uint32_t nt = 1u << ob;
+        double s = 0.0;
+        for (uint32_t k = t; k < nt; k += NT) s += J.norm_in[k];
+        const double tot0 = wg_sum_t0<NT>(s, s_red);
+        if (t == 0) s_bcast = tot0;
+        __syncthreads();
+        const double tot = s_bcast;
+        ps = (real)(1.0 / sqrt(tot));
+        if ((flags & F_READOUT) && tile == 0 && t == 0) {
+            const double re = (double)J.psi[J.eloc], im = (double)J.psi[N + J.eloc];
+            *J.prob = (re * re + im * im) / tot;
+        }
+    }
 
     real pr[R], pi[R], e[R];
     if (!psi_const) {
-        vload<real, R>(J.psi + off, pr);
-        vload<real, R>(J.psi + N + off, pi);
+        vload_nt<real, R>(J.psi + off, pr);
+        vload_nt<real, R>(J.psi + N + off, pi);
     } else {
 #pragma unroll
         for (int q = 0; q < R; q++) {
@@ -435,7 +510,12 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
             pi[q] = real(0);
         }
     }
-    vload<real, R>(J.E + off, e);
+    vload_nt<real, R>(J.E + off, e);
+    real yr[R], yi[R];
+    if (src2 == SRC2_ARRAY) {
+        vload_nt<real, R>(J.b2 + off, yr);
+        vload_nt<real, R>(J.b2 + N + off, yi);
+    }
 
     real ur[R], ui[R];
     if (src1 == SRC1_ZERO) {
@@ -461,8 +541,8 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
             vload<real, R>(src + off, xr);
             vload<real, R>(src + N + off, xi);
         }
-        vstore<real, R>(s_re + t * R, xr);
-        vstore<real, R>(s_im + t * R, xi);
+        lds_store<real, R>(s_re + t * R, xr);
+        lds_store<real, R>(s_im + t * R, xi);
         real ar[R], ai[R];
 #pragma unroll
         for (int q = 0; q < R; q++) {
@@ -496,8 +576,8 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
         for (int b = 0; b < TB - RB; b++) {
             const int pt = t ^ (1 << b);
             real yr[R], yi[R];
-            vload<real, R>(s_re + pt * R, yr);
-            vload<real, R>(s_im + pt * R, yi);
+            lds_load<real, R>(s_re + pt * R, yr);
+            lds_load<real, R>(s_im + pt * R, yi);
 #pragma unroll
             for (int q = 0; q < R; q++) {
                 ar[q] += yr[q];
@@ -516,7 +596,6 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
         w1r *= ps;
         w1i *= ps;
     }
-    real yr[R], yi[R];
     if (src2 == SRC2_ZERO) {
 #pragma unroll
         for (int q = 0; q < R; q++) yr[q] = yi[q] = real(0);
@@ -528,9 +607,6 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
             yr[q] = pr[q];
             yi[q] = pi[q];
         }
-    } else {
-        vload<real, R>(J.b2 + off, yr);
-        vload<real, R>(J.b2 + N + off, yi);
     }
     const real cr = J.cr * ps, ci = J.ci * ps;
     real orr[R], oi[R];
@@ -609,6 +685,202 @@ int launch_term(int precision, int tb, const void* jobs_dev, uint32_t njobs, int
 }
 
 // ===========================================================================
+// Real-half sweep (FP32).  The real and imaginary parts of b obey independent
+// real recurrences, coupled only through the injected c_r i^(r&1) psi and the
+// complex scalars of a stage's first two terms, each of which is purely real
+// or purely imaginary.  So one workgroup produces one real half of a tile of
+// 2^HB amplitudes from one real half of each source: with the same LDS as a
+// complex tile of 2^(HB-1), two more tile bits are exchanged on chip and
+// fewer flips need partner-tile loads.  The diagonal is read by both halves.
+// ===========================================================================
+template <int HB, int UNR, int OCC>
+__global__ void __launch_bounds__(1024, OCC)
+half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n, int lc)
+{
+    constexpr int NT = 1024;
+    constexpr int RB = HB - 10;
+    constexpr int R = 1 << RB;
+    constexpr int TS = 1 << HB;
+    __shared__ float s_x[TS];
+    __shared__ double s_red[NT / 64];
+    __shared__ double s_bcast;
+
+    const int ob = n - HB;
+    const uint64_t unit = xcd_unit(blockIdx.x, lc);
+    const uint64_t jh = unit >> ob;
+    const uint64_t job = jh >> 1;
+    if (job >= njobs) return;
+    const int h = (int)(jh & 1);
+    const uint64_t tile = unit & ((1ull << ob) - 1ull);
+    const TermJob<float>& J = jobs[job];
+    const uint32_t flags = J.flags;
+    const uint32_t src1 = flags & 3u, src2 = flags & 12u;
+    const bool psi_const = (flags & F_PSI_CONST) != 0;
+    const uint64_t N = 1ull << n;
+    const int t = threadIdx.x;
+    const uint64_t off = (tile << HB) + (uint64_t)t * R;
+
+    float ps = 1.0f;
+    if (flags & F_NORM_IN) {
+        const uint32_t nt = 2u << ob;
+        double sacc = 0.0;
+        for (uint32_t k = t; k < nt; k += NT) sacc += J.norm_in[k];
+        const double tot0 = wg_sum_t0<NT>(sacc, s_red);
+        if (t == 0) s_bcast = tot0;
+        __syncthreads();
+        const double tot = s_bcast;
+        ps = (float)(1.0 / sqrt(tot));
+        if ((flags & F_READOUT) && tile == 0 && h == 0 && t == 0) {
+            const double re = (double)J.psi[J.eloc], im = (double)J.psi[N + J.eloc];
+            *J.prob = (re * re + im * im) / tot;
+        }
+    }
+    // out_h = a1 u_{h1} - a2 s2_{h2} + ac Psi_{hc}: each complex scalar is real
+    // or imaginary, so it maps one source half onto this output half.
+    float a1 = J.w1r, a2 = J.w2r, ac = J.cr;
+    int h1 = h, h2 = h, hc = h;
+    if (J.w1i != 0.0f) {
+        a1 = h == 0 ? -J.w1i : J.w1i;
+        h1 = 1 - h;
+    }
+    if (J.w2i != 0.0f) {
+        a2 = h == 0 ? -J.w2i : J.w2i;
+        h2 = 1 - h;
+    }
+    if (J.ci != 0.0f) {
+        ac = h == 0 ? -J.ci : J.ci;
+        hc = 1 - h;
+    }
+    if (src1 == SRC1_PSI) a1 *= ps;
+    if (src2 == SRC2_PSI) a2 *= ps;
+    ac *= ps;
+
+    float o[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) o[q] = 0.0f;
+    if (src1 == SRC1_PSI && psi_const) {
+        if (h1 == 0) {
+            float e[R];
+            vload_nt<float, R>(J.E + off, e);
+            const float base = J.beta * (float)n * J.psi0;
+#pragma unroll
+            for (int q = 0; q < R; q++) o[q] = a1 * (J.alpha * e[q] * J.psi0 - base);
+        }
+    } else if (src1 != SRC1_ZERO) {
+        const float* src = ((src1 == SRC1_PSI) ? J.psi : J.b1) + (uint64_t)h1 * N;
+        float x[R], acc[R];
+        vload<float, R>(src + off, x);
+        lds_store<float, R>(s_x + t * R, x);
+#pragma unroll
+        for (int q = 0; q < R; q++) acc[q] = 0.0f;
+#pragma unroll
+        for (int b = 0; b < RB; b++)
+#pragma unroll
+            for (int q = 0; q < R; q++) acc[q] += x[q ^ (1 << b)];
+        // partner tiles, UNR loads in flight per thread
+        int j = 0;
+        for (; j + UNR <= ob; j += UNR) {
+            float y[UNR][R];
+#pragma unroll
+            for (int u = 0; u < UNR; u++)
+                vload<float, R>(src + ((tile ^ (1ull << (j + u))) << HB) + (uint64_t)t * R, y[u]);
+#pragma unroll
+            for (int u = 0; u < UNR; u++)
+#pragma unroll
+                for (int q = 0; q < R; q++) acc[q] += y[u][q];
+        }
+        for (; j < ob; j++) {
+            float y[R];
+            vload<float, R>(src + ((tile ^ (1ull << j)) << HB) + (uint64_t)t * R, y);
+#pragma unroll
+            for (int q = 0; q < R; q++) acc[q] += y[q];
+        }
+        float e[R];
+        vload_nt<float, R>(J.E + off, e);
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < HB - RB; b++) {
+            float y[R];
+            lds_load<float, R>(s_x + (t ^ (1 << b)) * R, y);
+#pragma unroll
+            for (int q = 0; q < R; q++) acc[q] += y[q];
+        }
+#pragma unroll
+        for (int q = 0; q < R; q++) o[q] = a1 * (J.alpha * e[q] * x[q] - J.beta * acc[q]);
+    }
+    if (src2 != SRC2_ZERO) {
+        float y[R];
+        if (src2 == SRC2_PSI && psi_const) {
+#pragma unroll
+            for (int q = 0; q < R; q++) y[q] = h2 == 0 ? J.psi0 : 0.0f;
+        } else {
+            const float* s2 = ((src2 == SRC2_PSI) ? J.psi : J.b2) + (uint64_t)h2 * N;
+            vload_nt<float, R>(s2 + off, y);
+        }
+#pragma unroll
+        for (int q = 0; q < R; q++) o[q] -= a2 * y[q];
+    }
+    if (ac != 0.0f) {
+        float y[R];
+        if (psi_const) {
+#pragma unroll
+            for (int q = 0; q < R; q++) y[q] = hc == 0 ? J.psi0 : 0.0f;
+        } else {
+            vload_nt<float, R>(J.psi + (uint64_t)hc * N + off, y);
+        }
+#pragma unroll
+        for (int q = 0; q < R; q++) o[q] += ac * y[q];
+    }
+    vstore<float, R>(J.out + (uint64_t)h * N + off, o);
+    if (flags & F_FINAL) {
+        double nrm = 0.0;
+#pragma unroll
+        for (int q = 0; q < R; q++) nrm += (double)o[q] * (double)o[q];
+        const double sum = wg_sum_t0<NT>(nrm, s_red);
+        if (t == 0) J.norm_part[((uint64_t)h << ob) + tile] = sum;
+    }
+}
+
+int half_tile_bits_supported(int hb) { return hb == 13 || hb == 14; }
+
+// variant: 0 = default for the tile size; 1.. = the alternatives the tuner tries
+int half_variants(int hb) { return hb == 14 ? 3 : 3; }
+
+template <int HB, int UNR, int OCC>
+static void launch_half_v(uint32_t grid, const void* jobs, uint32_t njobs, int n, int lc, hipStream_t st)
+{
+    hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC>), dim3(grid), dim3(1024), 0, st,
+                       (const TermJob<float>*)jobs, njobs, n, lc);
+}
+
+int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    const int ob = n - hb;
+    if (ob < 0 || !half_tile_bits_supported(hb)) return -2;
+    if (lc > ob + 17) lc = ob + 17;
+    const uint64_t units = (uint64_t)njobs << (ob + 1);
+    const uint64_t per = 8ull << lc;
+    const uint64_t grid64 = (units + per - 1) / per * per;
+    if (grid64 > 0x7fffffffull) return -3;
+    const uint32_t grid = (uint32_t)grid64;
+    if (hb == 14) {
+        switch (variant) {
+        case 1: launch_half_v<14, 2, 4>(grid, jobs_dev, njobs, n, lc, st); break;
+        case 2: launch_half_v<14, 4, 4>(grid, jobs_dev, njobs, n, lc, st); break;
+        default: launch_half_v<14, 1, 8>(grid, jobs_dev, njobs, n, lc, st); break;
+        }
+    } else {
+        switch (variant) {
+        case 1: launch_half_v<13, 1, 8>(grid, jobs_dev, njobs, n, lc, st); break;
+        case 2: launch_half_v<13, 2, 8>(grid, jobs_dev, njobs, n, lc, st); break;
+        default: launch_half_v<13, 4, 8>(grid, jobs_dev, njobs, n, lc, st); break;
+        }
+    }
+    return check_launch("half_term");
+}
+
+// ===========================================================================
 // Stage reduction: norm, next-stage scale, probability readout
 // ===========================================================================
 template <typename real>
@@ -627,7 +899,7 @@ __global__ void __launch_bounds__(256) reduce_kernel(const ReduceJob* __restrict
     }
     if (t == 0) {
         const double tot = red[0];
-        *J.psi_scale = 1.0 / sqrt(tot);
+        if (J.psi_scale) *J.psi_scale = 1.0 / sqrt(tot);
         if (J.prob) {
             const real* o = (const real*)J.out;
             const double re = (double)o[J.eloc], im = (double)o[J.N + J.eloc];

@@ -103,15 +103,28 @@ def test_solve_c1_vs_oracle(ctx, oracle, c1_recs):
 SOLVE_CASES = [(4, 2, 3, 20), (8, 1, 6, 100), (11, 3, 4, 100), (13, 2, 2, 30), (14, 5, 2, 12), (17, 2, 1, 6)]
 
 
-@pytest.mark.parametrize("path", ["default", "global"])
+PATHS = {
+    "default": {},                                        # n <= 13: LDS-resident; else tuned sweeps
+    "global": dict(small_n_max=0),                        # sweeps for every n
+    "complex": dict(small_n_max=0, tile_bits=12),         # complex-tile sweep kernel
+    "half13": dict(small_n_max=0, tile_bits=13, variant=1),   # real-half sweep kernel, 2^13 tiles
+    "half14": dict(small_n_max=0, tile_bits=14),          # real-half sweep kernel, 2^14 tiles
+    "halfdirect": dict(small_n_max=0, tile_bits=13, use_graph=False, group_bytes=48 << 20),  # small groups
+}
+
+
+@pytest.mark.parametrize("path", list(PATHS))
 @pytest.mark.parametrize("n,m,count,samples", SOLVE_CASES)
 def test_solve_vs_oracle(ctx, oracle, n, m, count, samples, path):
-    """default: n <= 13 runs in the LDS-resident kernel; global: forced through the sweep kernel."""
     if path == "global" and n > 13:
         pytest.skip("same as default above n = 13")
+    if path.startswith("half") and n < 13:
+        pytest.skip("real-half tiles need n >= 13")
+    if path == "half14" and n < 14:
+        pytest.skip("2^14 real-half tiles need n >= 14")
     _say("solve n", n, "m", m, "count", count, "samples", samples, "path", path)
     recs = M.generate_sk(n, count, 1000 + 10 * n + m)
-    kw = dict(small_n_max=0) if path == "global" else {}
+    kw = PATHS[path]
     p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True, **kw)
     po, tto, spo = oracle.solve(n, m, recs, samples=samples, threads=8, with_samples=True)
     assert np.array_equal(tt, tto)
