@@ -122,6 +122,16 @@ mqw_status mqw_chebyshev_coeffs(double s, double tol, double* c, int cap, int* K
  * n(n+1)/2 N(0,1) doubles (SplitMix64-seeded xoshiro256**, Marsaglia polar). */
 mqw_status mqw_generate_sk(int n, int count, uint64_t seed, double* out);
 
+/* ---- reference file surface ---------------------------------------------- */
+/* Number of complete n(n+1)/2-double records in a file. */
+mqw_status mqw_count_instances(const char* path, int n, long* count);
+/* Read count records starting at record `start` (MultiQW.cpp:268-271,303-304);
+ * MQW_EIO if the file is missing or holds fewer records. */
+mqw_status mqw_read_instances(const char* path, int n, long start, long count, double* out);
+/* Append count float32 results to <dir>/output_<n>_<m> (MultiQW.cpp:264-266,457);
+ * dir NULL or "" = the working directory. */
+mqw_status mqw_append_output(const char* dir, int n, int m, const float* probs, long count);
+
 /* ---- device pieces -------------------------------------------------------- */
 /* Energy build + ground-state search on the GPU (MultiQW.cpp:325-362).
  * hp (nullable, 2^n floats) receives the shifted float diagonal H_P;
@@ -150,9 +160,12 @@ mqw_status mqw_solve(mqw_ctx* ctx, int n, int m, const double* recs, int count, 
 mqw_status mqw_solve_batch(mqw_ctx* ctx, const mqw_instance* inst, int count, const mqw_opts* opts,
                            float* prob, float* total_t);
 
-/* Autotuner: times sweep configurations for n on this device, stores the
- * best in the context (and in the cache file when path != NULL). */
-mqw_status mqw_tune(mqw_ctx* ctx, int n, const char* cache_path, int* tile_bits, int* chunk_bits);
+/* Autotuner: times the sweep-kernel configurations (kernel kind and tile
+ * bits, loads-in-flight variant, XCD chunk bits) for n on this device and
+ * keeps the fastest for later solves on the context.  With cache_path, a
+ * recorded choice for this GPU architecture and n is reused, and a new one is
+ * appended (set MQW_RETUNE=1 to measure again).  Outputs may be NULL. */
+mqw_status mqw_tune(mqw_ctx* ctx, int n, const char* cache_path, int* tile_bits, int* chunk_bits, int* variant);
 
 /* Kernel statistics of the last solve on this context. */
 typedef struct {

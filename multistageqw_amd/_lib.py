@@ -27,7 +27,7 @@ EXPORTS = [
     "mqw_create", "mqw_destroy", "mqw_last_error", "mqw_status_string", "mqw_abi_version", "mqw_opts_default",
     "mqw_unpack", "mqw_schedule_heuristic", "mqw_sample_times", "mqw_chebyshev_coeffs", "mqw_generate_sk",
     "mqw_energy", "mqw_clenshaw_step_dev", "mqw_clenshaw", "mqw_solve", "mqw_solve_batch", "mqw_tune",
-    "mqw_last_stats",
+    "mqw_last_stats", "mqw_count_instances", "mqw_read_instances", "mqw_append_output",
 ]
 
 
@@ -95,8 +95,11 @@ def load():
         "mqw_clenshaw": ([vp, C.c_int, fp, C.c_int, fp, fp, C.c_float, C.c_float, C.c_int], C.c_int),
         "mqw_solve": ([vp, C.c_int, C.c_int, dp, C.c_int, C.POINTER(Opts), fp, fp], C.c_int),
         "mqw_solve_batch": ([vp, C.POINTER(Instance), C.c_int, C.POINTER(Opts), fp, fp], C.c_int),
-        "mqw_tune": ([vp, C.c_int, C.c_char_p, ip, ip], C.c_int),
+        "mqw_tune": ([vp, C.c_int, C.c_char_p, ip, ip, ip], C.c_int),
         "mqw_last_stats": ([vp, C.POINTER(Stats)], C.c_int),
+        "mqw_count_instances": ([C.c_char_p, C.c_int, C.POINTER(C.c_long)], C.c_int),
+        "mqw_read_instances": ([C.c_char_p, C.c_int, C.c_long, C.c_long, dp], C.c_int),
+        "mqw_append_output": ([C.c_char_p, C.c_int, C.c_int, fp, C.c_long], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -29,23 +29,23 @@ def params_per_instance(n: int) -> int:
     return n * (n + 1) // 2
 
 
+def count_instances(path: str, n: int) -> int:
+    c = C.c_long(0)
+    L.check(L.load().mqw_count_instances(path.encode(), n, C.byref(c)))
+    return c.value
+
+
 def read_instances(path: str, n: int, start: int = 0, count: Optional[int] = None) -> np.ndarray:
     """Read records of n(n+1)/2 little-endian float64 (MultiQW.cpp:268-271,303-304).
 
     Raises on a missing file or when fewer than ``count`` records remain, where
     the reference would silently repeat its last record.
     """
-    P = params_per_instance(n)
-    size = os.path.getsize(path)
-    have = size // (8 * P) - start
     if count is None:
-        count = have
-    if start < 0 or count < 0 or have < count:
-        raise ValueError(f"{path}: {max(have, 0)} records from {start}, {count} requested")
-    with open(path, "rb") as f:
-        f.seek(8 * P * start)
-        data = np.frombuffer(f.read(8 * P * count), dtype="<f8")
-    return data.reshape(count, P).astype(np.float64)
+        count = count_instances(path, n) - start
+    out = np.zeros(max(count, 0) * params_per_instance(n), np.float64)
+    L.check(L.load().mqw_read_instances(path.encode(), n, start, count, L.f64p(out)))
+    return out.reshape(count, params_per_instance(n))
 
 
 def unpack(n: int, rec) -> tuple[np.ndarray, np.ndarray]:
@@ -101,10 +101,9 @@ def format_line(total_t: float, prob: float) -> str:
 
 def write_output(n: int, m: int, probs, directory: str = ".") -> str:
     """Append float32 results to output_<n>_<m> (MultiQW.cpp:264-266,457)."""
-    path = os.path.join(directory, f"output_{n}_{m}")
-    with open(path, "ab") as f:
-        f.write(np.ascontiguousarray(probs, "<f4").tobytes())
-    return path
+    p = np.ascontiguousarray(probs, np.float32)
+    L.check(L.load().mqw_append_output(directory.encode(), n, m, L.f32p(p), p.size))
+    return os.path.join(directory, f"output_{n}_{m}")
 
 
 def make_opts(samples=L.DEFAULT_SAMPLES, tol=L.DEFAULT_TOL, precision=L.FP32, energy_mode=L.ENERGY_COMPAT,
@@ -225,11 +224,12 @@ class Context:
             return prob, tt, sp.reshape(cnt, o.samples)
         return prob, tt
 
-    def tune(self, n: int, cache_path: Optional[str] = None):
-        tb, lc = C.c_int(0), C.c_int(0)
+    def tune(self, n: int, cache_path: Optional[str] = None) -> dict:
+        """Autotune the sweep kernel for n (mqw_tune); returns the chosen configuration."""
+        tb, lc, var = C.c_int(0), C.c_int(0), C.c_int(0)
         self._check(self._lib.mqw_tune(self._h, n, cache_path.encode() if cache_path else None, C.byref(tb),
-                                       C.byref(lc)))
-        return tb.value, lc.value
+                                       C.byref(lc), C.byref(var)))
+        return dict(tile_bits=tb.value, chunk_bits=lc.value, variant=var.value)
 
     def last_stats(self) -> dict:
         s = L.Stats()

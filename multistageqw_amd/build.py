@@ -24,7 +24,9 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 # Packed FP32 VALU ops are kept out of every kernel (they are disabled for the
 # device pass; hipcc's host pass warns that it does not know the feature).
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", "-Wno-unused-value", f"-I{INCLUDE}", f"-I{CSRC}"]
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", "-Wno-unused-value", f"-I{INCLUDE}", f"-I{CSRC}",
+          f"-I{ROCM}/include"]
 
 KERNEL_SOURCES = ["mqw_kernels.hip", "mqw_lds.hip", "mqw_mlp.hip"]
 HOST_SOURCES = ["mqw_engine.cpp", "mqw_host.cpp", "mqw_tune.cpp", "mqw_io.cpp"]
@@ -53,7 +55,8 @@ def build(verbose: bool = False) -> str:
     for src in HOST_SOURCES:
         path = os.path.join(CSRC, src)
         if os.path.exists(path):
-            steps.append([HIPCC, *COMMON, "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-c", path,
+            # host-only C++ (no device pass) against the HIP runtime API
+            steps.append([HIPCC, "-x", "c++", *COMMON, "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-c", path,
                           "-o", _obj(src)])
     with ThreadPoolExecutor(max_workers=min(8, len(steps))) as ex:
         outs = list(ex.map(_run, steps))
@@ -70,7 +73,7 @@ def build(verbose: bool = False) -> str:
           copy_src])
     cli_src = os.path.join(CSRC, "mqw_cli.cpp")
     if os.path.exists(cli_src):
-        _run([HIPCC, *COMMON, "-D__HIP_PLATFORM_AMD__", "-o", CLI, cli_src, f"-L{HERE}", "-lmqw",
+        _run([HIPCC, "-x", "c++", *COMMON, "-D__HIP_PLATFORM_AMD__", "-o", CLI, cli_src, f"-L{HERE}", "-lmqw",
               f"-Wl,-rpath,$ORIGIN"])
     return LIB
 

@@ -11,7 +11,7 @@
 // sweeps (the reference's injection-only top term is folded into the first
 // sweep) run as term-kernel launches in lockstep across the trajectories of a
 // batch, aligned so every trajectory ends its stage on the same launch.
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 
 #include <math.h>
 #include <stdarg.h>
@@ -29,25 +29,7 @@
 
 using namespace mqw;
 
-struct DevBuf {
-    void* ptr = nullptr;
-    size_t cap = 0;
-};
-
-struct mqw_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    std::map<std::string, DevBuf> bufs;
-    mqw_stats stats{};
-    int tuned_tb[MQW_MAX_N + 1];
-    int tuned_lc[MQW_MAX_N + 1];
-    int tuned_var[MQW_MAX_N + 1];
-};
-
-namespace {
-
-mqw_status fail(mqw_ctx* c, mqw_status s, const char* fmt, ...)
+mqw_status mqw::fail(mqw_ctx* c, mqw_status s, const char* fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -58,14 +40,8 @@ mqw_status fail(mqw_ctx* c, mqw_status s, const char* fmt, ...)
     return s;
 }
 
-#define HIPCHK(ctx, expr)                                                                                   \
-    do {                                                                                                    \
-        hipError_t _e = (expr);                                                                             \
-        if (_e != hipSuccess) return fail((ctx), MQW_EDEVICE, "%s: %s", #expr, hipGetErrorString(_e));      \
-    } while (0)
-
 // Grow-only named device buffer.  Growing synchronises the stream first.
-void* dev_get(mqw_ctx* c, const std::string& name, size_t bytes)
+void* mqw::dev_get(mqw_ctx* c, const std::string& name, size_t bytes)
 {
     if (bytes == 0) bytes = 256;
     DevBuf& b = c->bufs[name];
@@ -84,7 +60,7 @@ void* dev_get(mqw_ctx* c, const std::string& name, size_t bytes)
     return b.ptr;
 }
 
-void release_bufs(mqw_ctx* c)
+void mqw::release_bufs(mqw_ctx* c)
 {
     hipStreamSynchronize(c->stream);
     for (auto& kv : c->bufs)
@@ -92,21 +68,13 @@ void release_bufs(mqw_ctx* c)
     c->bufs.clear();
 }
 
-bool finite_rec(int n, const double* rec)
-{
-    const int P = n * (n + 1) / 2;
-    for (int k = 0; k < P; k++)
-        if (!isfinite(rec[k])) return false;
-    return true;
-}
-
-int default_tile_bits(int precision, int n)
+static int default_tile_bits(int precision, int n)
 {
     const int tb = precision == MQW_FP32 ? 12 : 11;
     return std::min(n, tb);
 }
 
-int default_chunk_bits(int n, int tb)
+static int default_chunk_bits(int n, int tb)
 {
     // runs of 64 tiles (a sub-cube of the outer bits) per XCD measured best
     return std::min(n - tb, 6);
@@ -115,15 +83,7 @@ int default_chunk_bits(int n, int tb)
 // Which sweep kernel runs a group: the real-half kernel (FP32, n >= its tile
 // bits) or the complex-tile kernel.  `tb` is the tile bits of that kernel and
 // `parts` the number of partial norm sums one output leaves.
-struct SweepCfg {
-    bool half = false;
-    int tb = 12;
-    int lc = 0;
-    int variant = 0;
-    uint32_t parts = 1;
-};
-
-SweepCfg choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits, int chunk_bits, int variant = -1)
+SweepCfg mqw::choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits, int chunk_bits, int variant)
 {
     SweepCfg k;
     k.variant = variant >= 0 ? variant : c->tuned_var[n];
@@ -148,6 +108,16 @@ SweepCfg choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits, int
     }
     k.lc = std::max(0, std::min(k.lc, 24));
     return k;
+}
+
+namespace {
+
+bool finite_rec(int n, const double* rec)
+{
+    const int P = n * (n + 1) / 2;
+    for (int k = 0; k < P; k++)
+        if (!isfinite(rec[k])) return false;
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -991,17 +961,6 @@ mqw_status mqw_solve_batch(mqw_ctx* c, const mqw_instance* inst, int count, cons
     }
     if (count == 0) return MQW_OK;
     return solve_problems(c, ps, o, prob, total_t);
-}
-
-mqw_status mqw_tune(mqw_ctx* c, int n, const char* cache_path, int* tile_bits, int* chunk_bits)
-{
-    (void)cache_path;
-    if (!c) return MQW_EINVAL;
-    if (n < MQW_MIN_N || n > MQW_MAX_N) return fail(c, MQW_EINVAL, "n out of range");
-    const SweepCfg k = choose_sweep(c, MQW_FP32, n, 0, -1);
-    if (tile_bits) *tile_bits = k.tb;
-    if (chunk_bits) *chunk_bits = k.lc;
-    return MQW_OK;
 }
 
 } // extern "C"

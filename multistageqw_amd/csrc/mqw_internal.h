@@ -5,8 +5,11 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
 #include <vector>
+
+#include <hip/hip_runtime_api.h>
 
 #include "../../include/mqw.h"
 
@@ -142,4 +145,45 @@ int lds_grid(int n, int device);
 
 const char* last_kernel_error();
 
+// ---------------------------------------------------------------------------
+// Context (opaque in the C ABI) and helpers shared by the host translation units.
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+};
+
+struct SweepCfg {
+    bool half = false; // real-half kernel (FP32) instead of complex tiles
+    int tb = 12;       // tile bits of that kernel
+    int lc = 0;        // XCD chunk bits
+    int variant = 0;   // loads-in-flight variant of the real-half kernel
+    uint32_t parts = 1; // partial norm sums per output
+};
+
 } // namespace mqw
+
+struct mqw_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<std::string, mqw::DevBuf> bufs;
+    mqw_stats stats{};
+    int tuned_tb[MQW_MAX_N + 1];
+    int tuned_lc[MQW_MAX_N + 1];
+    int tuned_var[MQW_MAX_N + 1];
+    std::string arch;
+};
+
+namespace mqw {
+mqw_status fail(mqw_ctx* c, mqw_status s, const char* fmt, ...);
+void* dev_get(mqw_ctx* c, const std::string& name, size_t bytes);
+void release_bufs(mqw_ctx* c);
+SweepCfg choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits, int chunk_bits, int variant = -1);
+} // namespace mqw
+
+#define HIPCHK(ctx, expr)                                                                                   \
+    do {                                                                                                    \
+        hipError_t _e = (expr);                                                                             \
+        if (_e != hipSuccess) return mqw::fail((ctx), MQW_EDEVICE, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)

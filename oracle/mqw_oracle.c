@@ -515,7 +515,9 @@ typedef struct {
     const double* rec;
     float* prob;
     float* total_t;
-    float* sample_probs; /* nullable: samples floats */
+    float* sample_probs;    /* nullable: samples floats */
+    const float* gamma_ov;  /* nullable: m per-stage gammas replacing the heuristic */
+    const float* short_t_ov;/* nullable: m per-stage times replacing the heuristic */
 } oracle_job;
 
 static void solve_one(const oracle_job* jb)
@@ -529,6 +531,15 @@ static void solve_one(const oracle_job* jb)
     oracle_energy_compat(n, J, h, hp, NULL, &info);
     oracle_schedule sch;
     oracle_schedule_heuristic(n, m, J, h, &sch);
+    if (jb->gamma_ov && jb->short_t_ov) {
+        float tot = 0;
+        for (int i = 0; i < m; i++) {
+            sch.gamma[i] = jb->gamma_ov[i];
+            sch.short_t[i] = jb->short_t_ov[i];
+            tot += sch.short_t[i];
+        }
+        sch.total_t = tot;
+    }
     float* times = (float*)malloc(sizeof(float) * (size_t)m * samples);
     oracle_sample_times(m, samples, sch.short_t, times);
     float* sp = (float*)malloc(sizeof(float) * samples);
@@ -637,9 +648,12 @@ static void* worker(void* arg)
 
 /* Solve count problems; threads > 1 runs problems in parallel (the run.sh
  * fan-out of MultiQW.cpp, in one process).  sample_probs (nullable) receives
- * count*samples per-sample probabilities.  Returns 0, or -1 on bad args. */
-int oracle_solve(int n, int m, const double* recs, int count, int samples, double tol, int precision,
-                 int threads, float* prob, float* total_t, float* sample_probs)
+ * count*samples per-sample probabilities.  gamma/short_t (nullable) hold
+ * count*m per-stage values replacing the heuristic schedule.  Returns 0, or
+ * -1 on bad args. */
+int oracle_solve_sched(int n, int m, const double* recs, int count, int samples, double tol, int precision,
+                       int threads, float* prob, float* total_t, float* sample_probs, const float* gamma,
+                       const float* short_t)
 {
     if (n < 4 || n > 26 || m < 1 || m > ORACLE_MAX_M || count < 0 || samples < 1) return -1;
     const int P = n * (n + 1) / 2;
@@ -654,6 +668,8 @@ int oracle_solve(int n, int m, const double* recs, int count, int samples, doubl
         jobs[c].prob = prob + c;
         jobs[c].total_t = total_t + c;
         jobs[c].sample_probs = sample_probs ? sample_probs + (size_t)c * samples : NULL;
+        jobs[c].gamma_ov = gamma ? gamma + (size_t)c * m : NULL;
+        jobs[c].short_t_ov = short_t ? short_t + (size_t)c * m : NULL;
     }
     if (threads <= 1) {
         for (int c = 0; c < count; c++) solve_one(&jobs[c]);
@@ -671,4 +687,11 @@ int oracle_solve(int n, int m, const double* recs, int count, int samples, doubl
     }
     free(jobs);
     return 0;
+}
+
+int oracle_solve(int n, int m, const double* recs, int count, int samples, double tol, int precision, int threads,
+                 float* prob, float* total_t, float* sample_probs)
+{
+    return oracle_solve_sched(n, m, recs, count, samples, tol, precision, threads, prob, total_t, sample_probs, NULL,
+                              NULL);
 }

@@ -72,6 +72,9 @@ def lib():
         L.oracle_solve.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
                                    fp, fp, fp]
         L.oracle_solve.restype = C.c_int
+        L.oracle_solve_sched.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                         fp, fp, fp, fp, fp]
+        L.oracle_solve_sched.restype = C.c_int
         _lib = L
     return _lib
 
@@ -186,15 +189,19 @@ def clenshaw_f64(coeffs, psi, hp, n, gamma, scale):
 
 
 def solve(n: int, m: int, recs, samples: int = 100, tol: float = 1e-6, precision: int = 0,
-          threads: int = 1, with_samples: bool = False):
-    """Per-problem loop of MultiQW.cpp:296-455.  recs: (count, n(n+1)/2) float64."""
+          threads: int = 1, with_samples: bool = False, gamma=None, short_t=None):
+    """Per-problem loop of MultiQW.cpp:296-455.  recs: (count, n(n+1)/2) float64.
+    gamma/short_t: optional (count, m) schedules replacing the heuristic."""
     recs = np.ascontiguousarray(recs, np.float64).reshape(-1, n * (n + 1) // 2)
     cnt = recs.shape[0]
     prob = np.zeros(cnt, np.float32)
     tt = np.zeros(cnt, np.float32)
     sp = np.zeros(cnt * samples, np.float32) if with_samples else None
-    rc = lib().oracle_solve(n, m, _d(recs), cnt, samples, tol, precision, threads, _f(prob), _f(tt),
-                            _f(sp) if sp is not None else None)
+    g = np.ascontiguousarray(gamma, np.float32).reshape(cnt, m) if gamma is not None else None
+    st = np.ascontiguousarray(short_t, np.float32).reshape(cnt, m) if short_t is not None else None
+    rc = lib().oracle_solve_sched(n, m, _d(recs), cnt, samples, tol, precision, threads, _f(prob), _f(tt),
+                                  _f(sp) if sp is not None else None, _f(g) if g is not None else None,
+                                  _f(st) if st is not None else None)
     if rc != 0:
         raise ValueError("oracle_solve rejected its arguments")
     if with_samples:
