@@ -167,6 +167,32 @@ mqw_status mqw_solve_batch(mqw_ctx* ctx, const mqw_instance* inst, int count, co
  * appended (set MQW_RETUNE=1 to measure again).  Outputs may be NULL. */
 mqw_status mqw_tune(mqw_ctx* ctx, int n, const char* cache_path, int* tile_bits, int* chunk_bits, int* variant);
 
+/* ---- learned schedule ------------------------------------------------------ */
+/* Instance features (32 floats each): n/20, m/5, mean, variance and max |.| of
+ * the couplings and of the fields, then a 24-bin histogram (fractions) of the
+ * shifted diagonal H_P over [-E_abs, E_abs], computed on the GPU. */
+#define MQW_FEATURES 32
+mqw_status mqw_features(mqw_ctx* ctx, const mqw_instance* inst, int count, float* out);
+
+/* MLP 32 -> 128 -> 128 -> 10 with GELU, trained with hand-written HIP kernels.
+ * Outputs: per stage i < 5, (log gamma_i, log t_i) offsets from the heuristic. */
+#define MQW_MLP_IN 32
+#define MQW_MLP_HIDDEN 128
+#define MQW_MLP_OUT 10
+#define MQW_MLP_PARAMS 22026 /* W1 b1 W2 b2 W3 b3, row-major W[in][out] */
+typedef struct mqw_mlp mqw_mlp;
+mqw_status mqw_mlp_create(mqw_ctx* ctx, uint64_t seed, mqw_mlp** out);
+void mqw_mlp_destroy(mqw_mlp* mlp);
+mqw_status mqw_mlp_get_params(mqw_mlp* mlp, float* out, size_t count);
+mqw_status mqw_mlp_set_params(mqw_mlp* mlp, const float* in, size_t count); /* also resets Adam state */
+/* Y[B x 10] = f(X[B x 32]). */
+mqw_status mqw_mlp_forward(mqw_mlp* mlp, const float* X, int B, float* Y);
+/* Masked MSE  L = sum M (Y - T)^2 / sum M  and its gradient (kept on the
+ * device for the next Adam step; copied to grads when non-NULL). */
+mqw_status mqw_mlp_loss_grad(mqw_mlp* mlp, const float* X, const float* T, const float* M, int B, float* loss,
+                             float* grads);
+mqw_status mqw_mlp_adam_step(mqw_mlp* mlp, float lr, float beta1, float beta2, float eps, int step);
+
 /* Kernel statistics of the last solve on this context. */
 typedef struct {
     uint64_t sweeps;         /* term-kernel launches (global-memory path) */

@@ -27,8 +27,12 @@ EXPORTS = [
     "mqw_create", "mqw_destroy", "mqw_last_error", "mqw_status_string", "mqw_abi_version", "mqw_opts_default",
     "mqw_unpack", "mqw_schedule_heuristic", "mqw_sample_times", "mqw_chebyshev_coeffs", "mqw_generate_sk",
     "mqw_energy", "mqw_clenshaw_step_dev", "mqw_clenshaw", "mqw_solve", "mqw_solve_batch", "mqw_tune",
-    "mqw_last_stats", "mqw_count_instances", "mqw_read_instances", "mqw_append_output",
+    "mqw_last_stats", "mqw_count_instances", "mqw_read_instances", "mqw_append_output", "mqw_features",
+    "mqw_mlp_create", "mqw_mlp_destroy", "mqw_mlp_get_params", "mqw_mlp_set_params", "mqw_mlp_forward",
+    "mqw_mlp_loss_grad", "mqw_mlp_adam_step",
 ]
+FEATURES = 32
+MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS = 32, 128, 10, 22026
 
 
 class MQWError(RuntimeError):
@@ -100,6 +104,14 @@ def load():
         "mqw_count_instances": ([C.c_char_p, C.c_int, C.POINTER(C.c_long)], C.c_int),
         "mqw_read_instances": ([C.c_char_p, C.c_int, C.c_long, C.c_long, dp], C.c_int),
         "mqw_append_output": ([C.c_char_p, C.c_int, C.c_int, fp, C.c_long], C.c_int),
+        "mqw_features": ([vp, C.POINTER(Instance), C.c_int, fp], C.c_int),
+        "mqw_mlp_create": ([vp, C.c_uint64, C.POINTER(vp)], C.c_int),
+        "mqw_mlp_destroy": ([vp], None),
+        "mqw_mlp_get_params": ([vp, fp, C.c_size_t], C.c_int),
+        "mqw_mlp_set_params": ([vp, fp, C.c_size_t], C.c_int),
+        "mqw_mlp_forward": ([vp, fp, C.c_int, fp], C.c_int),
+        "mqw_mlp_loss_grad": ([vp, fp, fp, fp, C.c_int, fp, fp], C.c_int),
+        "mqw_mlp_adam_step": ([vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

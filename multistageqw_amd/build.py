@@ -29,7 +29,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", "-Wno-unu
           f"-I{ROCM}/include"]
 
 KERNEL_SOURCES = ["mqw_kernels.hip", "mqw_lds.hip", "mqw_mlp.hip"]
-HOST_SOURCES = ["mqw_engine.cpp", "mqw_host.cpp", "mqw_tune.cpp", "mqw_io.cpp"]
+HOST_SOURCES = ["mqw_engine.cpp", "mqw_host.cpp", "mqw_tune.cpp", "mqw_io.cpp", "mqw_learn.cpp"]
 
 
 def _run(cmd):

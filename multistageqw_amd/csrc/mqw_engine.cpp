@@ -853,6 +853,81 @@ mqw_status mqw_energy(mqw_ctx* c, int n, const double* rec, int mode, float* hp,
     return MQW_OK;
 }
 
+mqw_status mqw_features(mqw_ctx* c, const mqw_instance* inst, int count, float* out)
+{
+    if (!c) return MQW_EINVAL;
+    if (count < 0 || (count > 0 && (!inst || !out))) return fail(c, MQW_EINVAL, "bad count or null output");
+    hipSetDevice(c->device);
+    constexpr int kBins = 24;
+    std::vector<Problem> ps((size_t)count);
+    std::map<int, std::vector<Problem*>> groups;
+    for (int k = 0; k < count; k++) {
+        const mqw_instance& in = inst[k];
+        if (in.n < MQW_MIN_N || in.n > MQW_MAX_N || in.m < 1 || in.m > MQW_MAX_STAGES || !in.rec)
+            return fail(c, MQW_EINVAL, "instance %d: bad n, m or record", k);
+        if (!finite_rec(in.n, in.rec)) return fail(c, MQW_EINVAL, "instance %d holds NaN or Inf", k);
+        ps[k].n = in.n;
+        ps[k].m = in.m;
+        ps[k].rec = in.rec;
+        ps[k].out = k;
+        unpack(in.n, in.rec, ps[k].J, ps[k].h);
+        groups[in.n].push_back(&ps[k]);
+    }
+    std::vector<uint32_t> counts((size_t)count * kBins);
+    for (auto& kv : groups) {
+        const int n = kv.first;
+        auto& g = kv.second;
+        const size_t N = (size_t)1 << n;
+        float* E = (float*)dev_get(c, "feat_E", sizeof(float) * N * g.size());
+        uint32_t* cnt = (uint32_t*)dev_get(c, "feat_counts", sizeof(uint32_t) * kBins * g.size());
+        if (!E || !cnt) return fail(c, MQW_ENOMEM, "device allocation failed for features at n=%d", n);
+        mqw_status st = group_energies(c, n, g, MQW_ENERGY_COMPAT, MQW_FP32, E);
+        if (st) return st;
+        HIPCHK(c, hipMemsetAsync(cnt, 0, sizeof(uint32_t) * kBins * g.size(), c->stream));
+        for (size_t i = 0; i < g.size(); i++)
+            if (diag_histogram(E + i * N, N, g[i]->info.e_abs, kBins, cnt + i * kBins, c->stream))
+                return fail(c, MQW_EDEVICE, "histogram launch failed");
+        std::vector<uint32_t> hc(kBins * g.size());
+        HIPCHK(c, hipMemcpyAsync(hc.data(), cnt, sizeof(uint32_t) * hc.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < g.size(); i++)
+            std::copy(hc.begin() + i * kBins, hc.begin() + (i + 1) * kBins, counts.begin() + (size_t)g[i]->out * kBins);
+    }
+    for (int k = 0; k < count; k++) {
+        const Problem& p = ps[k];
+        const int n = p.n;
+        float* f = out + (size_t)k * MQW_FEATURES;
+        double js = 0, jq = 0, jm = 0, hs = 0, hq = 0, hm = 0;
+        int np = 0;
+        for (int i = 1; i < n; i++)
+            for (int j = 0; j < i; j++) {
+                const double t = 2.0 * (double)p.J[i * n + j]; // the file's coupling (MultiQW.cpp:318-319 halves it)
+                js += t;
+                jq += t * t;
+                jm = std::max(jm, fabs(t));
+                np++;
+            }
+        for (int i = 0; i < n; i++) {
+            const double v = p.h[i];
+            hs += v;
+            hq += v * v;
+            hm = std::max(hm, fabs(v));
+        }
+        const double jmean = js / np, hmean = hs / n;
+        f[0] = (float)(n / 20.0);
+        f[1] = (float)(p.m / 5.0);
+        f[2] = (float)jmean;
+        f[3] = (float)(jq / np - jmean * jmean);
+        f[4] = (float)jm;
+        f[5] = (float)hmean;
+        f[6] = (float)(hq / n - hmean * hmean);
+        f[7] = (float)hm;
+        const double N = (double)((size_t)1 << n);
+        for (int b = 0; b < kBins; b++) f[8 + b] = (float)(counts[(size_t)k * kBins + b] / N);
+    }
+    return MQW_OK;
+}
+
 mqw_status mqw_clenshaw_step_dev(mqw_ctx* c, const float* b1, float* b2, const float* hp, const float* psi,
                                  unsigned n, float scale, float gamma, float coef)
 {

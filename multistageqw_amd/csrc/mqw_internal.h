@@ -144,6 +144,8 @@ int launch_lds(int n, const LdsTraj* trajs, uint32_t ntraj, const LdsSeg* segs, 
 int lds_grid(int n, int device);
 
 const char* last_kernel_error();
+// Counts of the shifted diagonal in `bins` equal bins over [-r, r] (counts zeroed by the caller).
+int diag_histogram(const float* hp, uint64_t N, float r, int bins, uint32_t* counts, void* stream);
 
 // ---------------------------------------------------------------------------
 // Context (opaque in the C ABI) and helpers shared by the host translation units.
