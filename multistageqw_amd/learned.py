@@ -127,7 +127,7 @@ class Dataset:
 
 
 def make_dataset(ctx: Context, count: int, seed: int, n_range=(12, 18), m: int = 3, candidates: int = 10,
-                 sigma: float = 0.35, samples: int = 20, chunk: int = 256) -> Dataset:
+                 sigma: float = 0.35, samples: int = 20, chunk: int = 256, log=None) -> Dataset:
     rng = np.random.default_rng(seed)
     X, T, P0, PB = [], [], [], []
     scored = 0
@@ -153,6 +153,8 @@ def make_dataset(ctx: Context, count: int, seed: int, n_range=(12, 18), m: int =
         T.append(offs[np.arange(cnt), best])
         P0.append(p[:, 0])
         PB.append(p[np.arange(cnt), best])
+        if log:
+            log(dict(stage="dataset", done=c0 + cnt, of=count, scored=scored))
     X = np.concatenate(X)
     return Dataset(X, np.concatenate(T), loss_mask([m] * X.shape[0]), np.concatenate(P0), np.concatenate(PB), scored)
 
@@ -178,7 +180,7 @@ def train(mlp: MLP, data: Dataset, epochs: int = 200, batch: int = 256, lr: floa
 
 
 def evaluate(ctx: Context, mlp: MLP, count: int, seed: int, n: int = 20, m: int = 3, samples: int = 100,
-             chunk: int = 64) -> dict:
+             chunk: int = 64, log=None) -> dict:
     """Mean ground-state probability: heuristic vs MLP proposal vs proposal-with-fallback."""
     ph, pm = [], []
     t0 = time.time()
@@ -192,8 +194,67 @@ def evaluate(ctx: Context, mlp: MLP, count: int, seed: int, n: int = 20, m: int 
         p_m = score(ctx, inst, prop, samples)
         ph.append(p_h)
         pm.append(p_m)
+        if log:
+            log(dict(stage="evaluate", done=c0 + cnt, of=count, p_heuristic=float(np.concatenate(ph).mean()),
+                     p_mlp=float(np.concatenate(pm).mean())))
     ph, pm = np.concatenate(ph), np.concatenate(pm)
     pf = np.maximum(ph, pm)
     return dict(count=count, n=n, m=m, samples=samples, p_heuristic=float(ph.mean()), p_mlp=float(pm.mean()),
                 p_fallback=float(pf.mean()), gain_fallback=float(pf.mean() - ph.mean()),
                 gain_raw=float(pm.mean() - ph.mean()), win_rate=float(np.mean(pm > ph)), seconds=time.time() - t0)
+
+
+def main(argv=None) -> int:
+    """Train on simulator-scored schedules and evaluate against the heuristic.
+
+    python -m multistageqw_amd.learned --train 2000 --candidates 10 --eval 2048 --weights w.bin
+    prints one JSON line per progress step and a final summary line.
+    """
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=2000, help="training instances (n uniform in n-min..n-max)")
+    ap.add_argument("--candidates", type=int, default=10, help="scored schedules per training instance")
+    ap.add_argument("--n-min", type=int, default=12)
+    ap.add_argument("--n-max", type=int, default=18)
+    ap.add_argument("--m", type=int, default=3)
+    ap.add_argument("--train-samples", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--eval", type=int, default=2048, help="fresh evaluation instances")
+    ap.add_argument("--eval-n", type=int, default=20)
+    ap.add_argument("--eval-samples", type=int, default=100)
+    ap.add_argument("--weights", default=None, help="save (and with --load, read) the parameters here")
+    ap.add_argument("--load", action="store_true")
+    a = ap.parse_args(argv)
+
+    def log(d):
+        print(json.dumps(d), flush=True)
+
+    ctx = Context(0)
+    mlp = MLP(ctx, seed=1)
+    summary = dict(m=a.m)
+    if a.load and a.weights:
+        mlp.set_params(np.fromfile(a.weights, "<f4"))
+    else:
+        t0 = time.time()
+        data = make_dataset(ctx, a.train, seed=1, n_range=(a.n_min, a.n_max), m=a.m, candidates=a.candidates,
+                            samples=a.train_samples, log=log)
+        t1 = time.time()
+        info = train(mlp, data, epochs=a.epochs, batch=256, lr=1e-3)
+        summary.update(scored_schedules=data.scored, dataset_s=t1 - t0, train_s=time.time() - t1,
+                       loss_history=info["history"][-3:], train_p_heuristic=float(data.p_heur.mean()),
+                       train_p_best=float(data.p_best.mean()))
+        if a.weights:
+            mlp.params().astype("<f4").tofile(a.weights)
+    if a.eval > 0:
+        summary["evaluation"] = evaluate(ctx, mlp, a.eval, seed=2, n=a.eval_n, m=a.m, samples=a.eval_samples,
+                                         log=log)
+    log(dict(stage="summary", **summary))
+    mlp.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())
