@@ -88,18 +88,19 @@ SweepCfg mqw::choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits
     SweepCfg k;
     k.variant = variant >= 0 ? variant : c->tuned_var[n];
     int tb = tile_bits > 0 ? tile_bits : c->tuned_tb[n];
-    // Untuned default, from measurements on MI355X: complex 2^12 tiles up to
-    // n = 21, real-half 2^13 tiles (fewer partner-tile loads) above.
-    if (tb <= 0 && precision == MQW_FP32 && n >= 22) {
+    // Untuned default, from measurements on MI355X: from n = 14 the real-half
+    // kernel on 2^13-float tiles, two partner loads in flight, non-temporal
+    // output stores (variant 4); complex 2^12 tiles below.
+    if (tb <= 0 && precision == MQW_FP32 && n >= 14) {
         tb = 13;
-        if (variant < 0 && c->tuned_var[n] == 0) k.variant = 1;
+        if (variant < 0 && c->tuned_var[n] == 0) k.variant = 4;
     }
     if (precision == MQW_FP32 && n >= 13 && tb > 0 && half_tile_bits_supported(tb)) {
         if (tb > n) tb = 13;
         k.half = true;
         k.tb = tb;
         k.parts = 2u << (n - tb);
-        k.lc = chunk_bits >= 0 ? chunk_bits : (c->tuned_lc[n] >= 0 ? c->tuned_lc[n] : std::min(n - tb, 6));
+        k.lc = chunk_bits >= 0 ? chunk_bits : (c->tuned_lc[n] >= 0 ? c->tuned_lc[n] : std::min(n - tb + 1, 6));
     } else {
         if (tb <= 0 || tb > n || !term_tile_bits_supported(precision, tb)) tb = default_tile_bits(precision, n);
         k.tb = tb;
@@ -386,7 +387,7 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
             else if (k.half)
                 rc = launch_half_term(k.tb, k.variant, jd + op.off, op.count, n, k.lc, c->stream);
             else
-                rc = launch_term(precision, k.tb, jd + op.off, op.count, n, k.lc, c->stream);
+                rc = launch_term(precision, k.tb, k.variant, jd + op.off, op.count, n, k.lc, c->stream);
             if (rc) return rc;
         }
         return 0;

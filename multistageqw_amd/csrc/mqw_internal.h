@@ -113,7 +113,9 @@ int launch_energy(int n, const float* J_dev, const float* h_dev, double e_init, 
                   double* e64, int e64_shift, const EnergyScratch& scr, EnergyDev* info_dev, void* stream);
 
 int term_tile_bits_supported(int precision, int tb);
-int launch_term(int precision, int tb, const void* jobs_dev, uint32_t njobs, int n, int lc, void* stream);
+int launch_term(int precision, int tb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc,
+                void* stream);
+int term_variants();
 // Real-half sweep (FP32): hb bits of one real half per workgroup; partial
 // sums are laid out [half][tile], 2 * 2^(n-hb) per output.
 int half_tile_bits_supported(int hb);

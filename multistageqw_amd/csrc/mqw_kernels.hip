@@ -337,20 +337,20 @@ struct NVec16<double> {
 };
 
 template <typename real, int R>
-__device__ __forceinline__ void vload(const real* __restrict__ p, real (&v)[R])
+__device__ __forceinline__ void vload(const real* __restrict__ p, real (&v)[R], int stride = NVec16<real>::W)
 {
     using vt = typename NVec16<real>::type;
     constexpr int W = NVec16<real>::W;
 #pragma unroll
     for (int q = 0; q < R / W; q++) {
-        const vt x = *gview(reinterpret_cast<const vt*>(p + q * W));
+        const vt x = *gview(reinterpret_cast<const vt*>(p + q * stride));
 #pragma unroll
         for (int w = 0; w < W; w++) v[q * W + w] = x[w];
     }
 }
 
 template <typename real, int R>
-__device__ __forceinline__ void vstore(real* p, const real (&v)[R])
+__device__ __forceinline__ void vstore(real* p, const real (&v)[R], int stride = NVec16<real>::W)
 {
     using vt = typename NVec16<real>::type;
     constexpr int W = NVec16<real>::W;
@@ -359,7 +359,7 @@ __device__ __forceinline__ void vstore(real* p, const real (&v)[R])
         vt x;
 #pragma unroll
         for (int w = 0; w < W; w++) x[w] = v[q * W + w];
-        *gview(reinterpret_cast<vt*>(p + q * W)) = x;
+        *gview(reinterpret_cast<vt*>(p + q * stride)) = x;
     }
 }
 
@@ -367,20 +367,20 @@ __device__ __forceinline__ void vstore(real* p, const real (&v)[R])
 // non-temporal hint so L2 keeps the b_{k+1} tiles that neighbouring
 // workgroups re-read as partner tiles.
 template <typename real, int R>
-__device__ __forceinline__ void vload_nt(const real* __restrict__ p, real (&v)[R])
+__device__ __forceinline__ void vload_nt(const real* __restrict__ p, real (&v)[R], int stride = NVec16<real>::W)
 {
     using vt = typename NVec16<real>::type;
     constexpr int W = NVec16<real>::W;
 #pragma unroll
     for (int q = 0; q < R / W; q++) {
-        const vt x = __builtin_nontemporal_load(gview(reinterpret_cast<const vt*>(p + q * W)));
+        const vt x = __builtin_nontemporal_load(gview(reinterpret_cast<const vt*>(p + q * stride)));
 #pragma unroll
         for (int w = 0; w < W; w++) v[q * W + w] = x[w];
     }
 }
 
 template <typename real, int R>
-__device__ __forceinline__ void vstore_nt(real* p, const real (&v)[R])
+__device__ __forceinline__ void vstore_nt(real* p, const real (&v)[R], int stride = NVec16<real>::W)
 {
     using vt = typename NVec16<real>::type;
     constexpr int W = NVec16<real>::W;
@@ -389,26 +389,26 @@ __device__ __forceinline__ void vstore_nt(real* p, const real (&v)[R])
         vt x;
 #pragma unroll
         for (int w = 0; w < W; w++) x[w] = v[q * W + w];
-        __builtin_nontemporal_store(x, gview(reinterpret_cast<vt*>(p + q * W)));
+        __builtin_nontemporal_store(x, gview(reinterpret_cast<vt*>(p + q * stride)));
     }
 }
 
 // LDS accesses of the tile exchange buffer (generic pointers into __shared__).
 template <typename real, int R>
-__device__ __forceinline__ void lds_load(const real* p, real (&v)[R])
+__device__ __forceinline__ void lds_load(const real* p, real (&v)[R], int stride = NVec16<real>::W)
 {
     using vt = typename NVec16<real>::type;
     constexpr int W = NVec16<real>::W;
 #pragma unroll
     for (int q = 0; q < R / W; q++) {
-        const vt x = *reinterpret_cast<const vt*>(p + q * W);
+        const vt x = *reinterpret_cast<const vt*>(p + q * stride);
 #pragma unroll
         for (int w = 0; w < W; w++) v[q * W + w] = x[w];
     }
 }
 
 template <typename real, int R>
-__device__ __forceinline__ void lds_store(real* p, const real (&v)[R])
+__device__ __forceinline__ void lds_store(real* p, const real (&v)[R], int stride = NVec16<real>::W)
 {
     using vt = typename NVec16<real>::type;
     constexpr int W = NVec16<real>::W;
@@ -417,7 +417,7 @@ __device__ __forceinline__ void lds_store(real* p, const real (&v)[R])
         vt x;
 #pragma unroll
         for (int w = 0; w < W; w++) x[w] = v[q * W + w];
-        *reinterpret_cast<vt*>(p + q * W) = x;
+        *reinterpret_cast<vt*>(p + q * stride) = x;
     }
 }
 
@@ -457,8 +457,8 @@ __device__ __forceinline__ uint64_t xcd_unit(uint32_t bid, int lc)
     return (chunk << lc) | (uint64_t)(slot & ((1u << lc) - 1u));
 }
 
-template <typename real, int TB, int RB>
-__global__ void __launch_bounds__(1 << (TB - RB))
+template <typename real, int TB, int RB, bool NTS>
+__global__ void __launch_bounds__(1 << (TB - RB), 8)
 term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int lc)
 {
     constexpr int NT = 1 << (TB - RB);
@@ -481,7 +481,10 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
     const bool psi_const = (flags & F_PSI_CONST) != 0;
     const uint64_t N = 1ull << n;
     const int t = threadIdx.x;
-    const uint64_t off = (tile << TB) + (uint64_t)t * R;
+    // plane layout: chunk c of thread t holds elements c*(NT*W) + t*W + [0, W)
+    constexpr int W = NVec16<real>::W;
+    constexpr int PS = NT * W;
+    const uint64_t off = (tile << TB) + (uint64_t)t * W;
     real ps = real(1);
     if (flags & F_NORM_IN) {
         // 1/||Psi|| from the previous stage's partial sums, same order in every workgroup
@@ -501,8 +504,8 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
 
     real pr[R], pi[R], e[R];
     if (!psi_const) {
-        vload_nt<real, R>(J.psi + off, pr);
-        vload_nt<real, R>(J.psi + N + off, pi);
+        vload_nt<real, R>(J.psi + off, pr, PS);
+        vload_nt<real, R>(J.psi + N + off, pi, PS);
     } else {
 #pragma unroll
         for (int q = 0; q < R; q++) {
@@ -510,11 +513,11 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
             pi[q] = real(0);
         }
     }
-    vload_nt<real, R>(J.E + off, e);
+    vload_nt<real, R>(J.E + off, e, PS);
     real yr[R], yi[R];
     if (src2 == SRC2_ARRAY) {
-        vload_nt<real, R>(J.b2 + off, yr);
-        vload_nt<real, R>(J.b2 + N + off, yi);
+        vload_nt<real, R>(J.b2 + off, yr, PS);
+        vload_nt<real, R>(J.b2 + N + off, yi, PS);
     }
 
     real ur[R], ui[R];
@@ -538,11 +541,11 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
                 xi[q] = pi[q];
             }
         } else {
-            vload<real, R>(src + off, xr);
-            vload<real, R>(src + N + off, xi);
+            vload<real, R>(src + off, xr, PS);
+            vload<real, R>(src + N + off, xi, PS);
         }
-        lds_store<real, R>(s_re + t * R, xr);
-        lds_store<real, R>(s_im + t * R, xi);
+        lds_store<real, R>(s_re + t * W, xr, PS);
+        lds_store<real, R>(s_im + t * W, xi, PS);
         real ar[R], ai[R];
 #pragma unroll
         for (int q = 0; q < R; q++) {
@@ -560,10 +563,10 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
         // flips of the outer bits: partner tiles, coalesced loads
 #pragma unroll 4
         for (int j = 0; j < ob; j++) {
-            const uint64_t poff = ((tile ^ (1ull << j)) << TB) + (uint64_t)t * R;
+            const uint64_t poff = ((tile ^ (1ull << j)) << TB) + (uint64_t)t * W;
             real yr[R], yi[R];
-            vload<real, R>(src + poff, yr);
-            vload<real, R>(src + N + poff, yi);
+            vload<real, R>(src + poff, yr, PS);
+            vload<real, R>(src + N + poff, yi, PS);
 #pragma unroll
             for (int q = 0; q < R; q++) {
                 ar[q] += yr[q];
@@ -576,8 +579,8 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
         for (int b = 0; b < TB - RB; b++) {
             const int pt = t ^ (1 << b);
             real yr[R], yi[R];
-            lds_load<real, R>(s_re + pt * R, yr);
-            lds_load<real, R>(s_im + pt * R, yi);
+            lds_load<real, R>(s_re + pt * W, yr, PS);
+            lds_load<real, R>(s_im + pt * W, yi, PS);
 #pragma unroll
             for (int q = 0; q < R; q++) {
                 ar[q] += yr[q];
@@ -617,8 +620,13 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
         oi[q] = (w1r * ui[q] + w1i * ur[q]) - (w2r * yi[q] + w2i * yr[q]) + (cr * pi[q] + ci * pr[q]);
         nrm += (double)orr[q] * (double)orr[q] + (double)oi[q] * (double)oi[q];
     }
-    vstore<real, R>(J.out + off, orr);
-    vstore<real, R>(J.out + N + off, oi);
+    if (NTS) {
+        vstore_nt<real, R>(J.out + off, orr, PS);
+        vstore_nt<real, R>(J.out + N + off, oi, PS);
+    } else {
+        vstore<real, R>(J.out + off, orr, PS);
+        vstore<real, R>(J.out + N + off, oi, PS);
+    }
 
     if (flags & F_FINAL) {
         const double s = wg_sum_t0<NT>(nrm, s_red);
@@ -627,7 +635,7 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
 }
 
 template <typename real, int TB>
-static int launch_term_tb(const void* jobs, uint32_t njobs, int n, int lc, hipStream_t st)
+static int launch_term_tb(const void* jobs, uint32_t njobs, int n, int lc, int variant, hipStream_t st)
 {
     // 16-byte vectors per thread at least; at most 1024 threads per workgroup
     constexpr int RB0 = (sizeof(real) == 4) ? 2 : 1;
@@ -641,44 +649,48 @@ static int launch_term_tb(const void* jobs, uint32_t njobs, int n, int lc, hipSt
     const uint64_t per = 8ull << lc;
     const uint64_t grid = (units + per - 1) / per * per;
     if (grid > 0x7fffffffull) return -3;
-    hipLaunchKernelGGL((term_kernel<real, TB, RB>), dim3((uint32_t)grid), dim3(NT), 0, st,
-                       (const TermJob<real>*)jobs, njobs, n, lc);
+    if (variant == 1)
+        hipLaunchKernelGGL((term_kernel<real, TB, RB, true>), dim3((uint32_t)grid), dim3(NT), 0, st,
+                           (const TermJob<real>*)jobs, njobs, n, lc);
+    else
+        hipLaunchKernelGGL((term_kernel<real, TB, RB, false>), dim3((uint32_t)grid), dim3(NT), 0, st,
+                           (const TermJob<real>*)jobs, njobs, n, lc);
     return check_launch("term");
 }
 
 int term_tile_bits_supported(int precision, int tb)
 {
-    if (precision == MQW_FP32) return tb >= 4 && tb <= 13;
+    (void)precision;
     return tb >= 4 && tb <= 12;
 }
 
-int launch_term(int precision, int tb, const void* jobs_dev, uint32_t njobs, int n, int lc, void* stream)
+int launch_term(int precision, int tb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc,
+                void* stream)
 {
     hipStream_t st = (hipStream_t)stream;
     if (precision == MQW_FP32) {
         switch (tb) {
-        case 4: return launch_term_tb<float, 4>(jobs_dev, njobs, n, lc, st);
-        case 5: return launch_term_tb<float, 5>(jobs_dev, njobs, n, lc, st);
-        case 6: return launch_term_tb<float, 6>(jobs_dev, njobs, n, lc, st);
-        case 7: return launch_term_tb<float, 7>(jobs_dev, njobs, n, lc, st);
-        case 8: return launch_term_tb<float, 8>(jobs_dev, njobs, n, lc, st);
-        case 9: return launch_term_tb<float, 9>(jobs_dev, njobs, n, lc, st);
-        case 10: return launch_term_tb<float, 10>(jobs_dev, njobs, n, lc, st);
-        case 11: return launch_term_tb<float, 11>(jobs_dev, njobs, n, lc, st);
-        case 12: return launch_term_tb<float, 12>(jobs_dev, njobs, n, lc, st);
-        case 13: return launch_term_tb<float, 13>(jobs_dev, njobs, n, lc, st);
+        case 4: return launch_term_tb<float, 4>(jobs_dev, njobs, n, lc, variant, st);
+        case 5: return launch_term_tb<float, 5>(jobs_dev, njobs, n, lc, variant, st);
+        case 6: return launch_term_tb<float, 6>(jobs_dev, njobs, n, lc, variant, st);
+        case 7: return launch_term_tb<float, 7>(jobs_dev, njobs, n, lc, variant, st);
+        case 8: return launch_term_tb<float, 8>(jobs_dev, njobs, n, lc, variant, st);
+        case 9: return launch_term_tb<float, 9>(jobs_dev, njobs, n, lc, variant, st);
+        case 10: return launch_term_tb<float, 10>(jobs_dev, njobs, n, lc, variant, st);
+        case 11: return launch_term_tb<float, 11>(jobs_dev, njobs, n, lc, variant, st);
+        case 12: return launch_term_tb<float, 12>(jobs_dev, njobs, n, lc, variant, st);
         }
     } else {
         switch (tb) {
-        case 4: return launch_term_tb<double, 4>(jobs_dev, njobs, n, lc, st);
-        case 5: return launch_term_tb<double, 5>(jobs_dev, njobs, n, lc, st);
-        case 6: return launch_term_tb<double, 6>(jobs_dev, njobs, n, lc, st);
-        case 7: return launch_term_tb<double, 7>(jobs_dev, njobs, n, lc, st);
-        case 8: return launch_term_tb<double, 8>(jobs_dev, njobs, n, lc, st);
-        case 9: return launch_term_tb<double, 9>(jobs_dev, njobs, n, lc, st);
-        case 10: return launch_term_tb<double, 10>(jobs_dev, njobs, n, lc, st);
-        case 11: return launch_term_tb<double, 11>(jobs_dev, njobs, n, lc, st);
-        case 12: return launch_term_tb<double, 12>(jobs_dev, njobs, n, lc, st);
+        case 4: return launch_term_tb<double, 4>(jobs_dev, njobs, n, lc, variant, st);
+        case 5: return launch_term_tb<double, 5>(jobs_dev, njobs, n, lc, variant, st);
+        case 6: return launch_term_tb<double, 6>(jobs_dev, njobs, n, lc, variant, st);
+        case 7: return launch_term_tb<double, 7>(jobs_dev, njobs, n, lc, variant, st);
+        case 8: return launch_term_tb<double, 8>(jobs_dev, njobs, n, lc, variant, st);
+        case 9: return launch_term_tb<double, 9>(jobs_dev, njobs, n, lc, variant, st);
+        case 10: return launch_term_tb<double, 10>(jobs_dev, njobs, n, lc, variant, st);
+        case 11: return launch_term_tb<double, 11>(jobs_dev, njobs, n, lc, variant, st);
+        case 12: return launch_term_tb<double, 12>(jobs_dev, njobs, n, lc, variant, st);
         }
     }
     return -1;
@@ -693,7 +705,7 @@ int launch_term(int precision, int tb, const void* jobs_dev, uint32_t njobs, int
 // complex tile of 2^(HB-1), two more tile bits are exchanged on chip and
 // fewer flips need partner-tile loads.  The diagonal is read by both halves.
 // ===========================================================================
-template <int HB, int UNR, int OCC>
+template <int HB, int UNR, int OCC, bool NTS>
 __global__ void __launch_bounds__(1024, OCC)
 half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n, int lc)
 {
@@ -718,7 +730,9 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
     const bool psi_const = (flags & F_PSI_CONST) != 0;
     const uint64_t N = 1ull << n;
     const int t = threadIdx.x;
-    const uint64_t off = (tile << HB) + (uint64_t)t * R;
+    // plane layout: chunk c of thread t holds elements c*4096 + 4t + [0, 4)
+    constexpr int PS = NT * 4;
+    const uint64_t off = (tile << HB) + (uint64_t)t * 4;
 
     float ps = 1.0f;
     if (flags & F_NORM_IN) {
@@ -761,7 +775,7 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
     if (src1 == SRC1_PSI && psi_const) {
         if (h1 == 0) {
             float e[R];
-            vload_nt<float, R>(J.E + off, e);
+            vload_nt<float, R>(J.E + off, e, PS);
             const float base = J.beta * (float)n * J.psi0;
 #pragma unroll
             for (int q = 0; q < R; q++) o[q] = a1 * (J.alpha * e[q] * J.psi0 - base);
@@ -769,8 +783,8 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
     } else if (src1 != SRC1_ZERO) {
         const float* src = ((src1 == SRC1_PSI) ? J.psi : J.b1) + (uint64_t)h1 * N;
         float x[R], acc[R];
-        vload<float, R>(src + off, x);
-        lds_store<float, R>(s_x + t * R, x);
+        vload<float, R>(src + off, x, PS);
+        lds_store<float, R>(s_x + t * 4, x, PS);
 #pragma unroll
         for (int q = 0; q < R; q++) acc[q] = 0.0f;
 #pragma unroll
@@ -783,7 +797,7 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
             float y[UNR][R];
 #pragma unroll
             for (int u = 0; u < UNR; u++)
-                vload<float, R>(src + ((tile ^ (1ull << (j + u))) << HB) + (uint64_t)t * R, y[u]);
+                vload<float, R>(src + ((tile ^ (1ull << (j + u))) << HB) + (uint64_t)t * 4, y[u], PS);
 #pragma unroll
             for (int u = 0; u < UNR; u++)
 #pragma unroll
@@ -791,17 +805,17 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
         }
         for (; j < ob; j++) {
             float y[R];
-            vload<float, R>(src + ((tile ^ (1ull << j)) << HB) + (uint64_t)t * R, y);
+            vload<float, R>(src + ((tile ^ (1ull << j)) << HB) + (uint64_t)t * 4, y, PS);
 #pragma unroll
             for (int q = 0; q < R; q++) acc[q] += y[q];
         }
         float e[R];
-        vload_nt<float, R>(J.E + off, e);
+        vload_nt<float, R>(J.E + off, e, PS);
         __syncthreads();
 #pragma unroll
         for (int b = 0; b < HB - RB; b++) {
             float y[R];
-            lds_load<float, R>(s_x + (t ^ (1 << b)) * R, y);
+            lds_load<float, R>(s_x + (t ^ (1 << b)) * 4, y, PS);
 #pragma unroll
             for (int q = 0; q < R; q++) acc[q] += y[q];
         }
@@ -815,7 +829,7 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
             for (int q = 0; q < R; q++) y[q] = h2 == 0 ? J.psi0 : 0.0f;
         } else {
             const float* s2 = ((src2 == SRC2_PSI) ? J.psi : J.b2) + (uint64_t)h2 * N;
-            vload_nt<float, R>(s2 + off, y);
+            vload_nt<float, R>(s2 + off, y, PS);
         }
 #pragma unroll
         for (int q = 0; q < R; q++) o[q] -= a2 * y[q];
@@ -826,12 +840,15 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
 #pragma unroll
             for (int q = 0; q < R; q++) y[q] = hc == 0 ? J.psi0 : 0.0f;
         } else {
-            vload_nt<float, R>(J.psi + (uint64_t)hc * N + off, y);
+            vload_nt<float, R>(J.psi + (uint64_t)hc * N + off, y, PS);
         }
 #pragma unroll
         for (int q = 0; q < R; q++) o[q] += ac * y[q];
     }
-    vstore<float, R>(J.out + (uint64_t)h * N + off, o);
+    if (NTS)
+        vstore_nt<float, R>(J.out + (uint64_t)h * N + off, o, PS);
+    else
+        vstore<float, R>(J.out + (uint64_t)h * N + off, o, PS);
     if (flags & F_FINAL) {
         double nrm = 0.0;
 #pragma unroll
@@ -843,14 +860,20 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
 
 int half_tile_bits_supported(int hb) { return hb == 13 || hb == 14; }
 
-// variant: 0 = default for the tile size; 1.. = the alternatives the tuner tries
-int half_variants(int hb) { return hb == 14 ? 3 : 3; }
+// variant: 0 = default for the tile size; 1.. = the alternatives the tuner
+// tries (loads in flight per thread; +3 = non-temporal output stores)
+int half_variants(int hb) { return 6; }
+int term_variants() { return 2; }
 
 template <int HB, int UNR, int OCC>
-static void launch_half_v(uint32_t grid, const void* jobs, uint32_t njobs, int n, int lc, hipStream_t st)
+static void launch_half_v(uint32_t grid, const void* jobs, uint32_t njobs, int n, int lc, bool nts, hipStream_t st)
 {
-    hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC>), dim3(grid), dim3(1024), 0, st,
-                       (const TermJob<float>*)jobs, njobs, n, lc);
+    if (nts)
+        hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, true>), dim3(grid), dim3(1024), 0, st,
+                           (const TermJob<float>*)jobs, njobs, n, lc);
+    else
+        hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, false>), dim3(grid), dim3(1024), 0, st,
+                           (const TermJob<float>*)jobs, njobs, n, lc);
 }
 
 int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc, void* stream)
@@ -864,17 +887,19 @@ int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, 
     const uint64_t grid64 = (units + per - 1) / per * per;
     if (grid64 > 0x7fffffffull) return -3;
     const uint32_t grid = (uint32_t)grid64;
+    const bool nts = variant >= 3;
+    const int v = variant % 3;
     if (hb == 14) {
-        switch (variant) {
-        case 1: launch_half_v<14, 2, 4>(grid, jobs_dev, njobs, n, lc, st); break;
-        case 2: launch_half_v<14, 4, 4>(grid, jobs_dev, njobs, n, lc, st); break;
-        default: launch_half_v<14, 1, 8>(grid, jobs_dev, njobs, n, lc, st); break;
+        switch (v) {
+        case 1: launch_half_v<14, 2, 4>(grid, jobs_dev, njobs, n, lc, nts, st); break;
+        case 2: launch_half_v<14, 4, 4>(grid, jobs_dev, njobs, n, lc, nts, st); break;
+        default: launch_half_v<14, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, st); break;
         }
     } else {
-        switch (variant) {
-        case 1: launch_half_v<13, 1, 8>(grid, jobs_dev, njobs, n, lc, st); break;
-        case 2: launch_half_v<13, 2, 8>(grid, jobs_dev, njobs, n, lc, st); break;
-        default: launch_half_v<13, 4, 8>(grid, jobs_dev, njobs, n, lc, st); break;
+        switch (v) {
+        case 1: launch_half_v<13, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, st); break;
+        case 2: launch_half_v<13, 2, 8>(grid, jobs_dev, njobs, n, lc, nts, st); break;
+        default: launch_half_v<13, 4, 8>(grid, jobs_dev, njobs, n, lc, nts, st); break;
         }
     }
     return check_launch("half_term");

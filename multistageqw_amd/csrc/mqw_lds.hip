@@ -7,6 +7,8 @@
 // is published to LDS (double-buffered, one barrier per term) so the other
 // threads can read its flips along the bits RB..n-1.  Global memory is touched
 // for E once per trajectory, for the stage coefficients, and for the readouts.
+// Chunk c of thread t holds elements c*(4 NT) + 4t + [0, 4) ("plane layout"),
+// so every wave-wide LDS or global access is contiguous.
 //
 // Term recurrence (MultiQW.cpp:88-173,177-236), complex, planar per thread:
 //   b_{K-1} = c_{K-1} i^{(K-1)&1} psi
@@ -64,6 +66,8 @@ lds_kernel(const LdsTraj* __restrict__ trajs, uint32_t ntraj, const LdsSeg* __re
     using S = LdsShape<NB>;
     constexpr int RB = S::RB, R = S::R, NT = S::NT, NRED = S::NRED;
     constexpr int N = 1 << NB;
+    // plane layout: chunk c of thread t holds elements c*(4 NT) + 4t + [0, 4)
+    constexpr int PS = 4 * NT;
     __shared__ float lre[2][N];
     __shared__ float lim[2][N];
     __shared__ double red[NRED];
@@ -77,7 +81,7 @@ lds_kernel(const LdsTraj* __restrict__ trajs, uint32_t ntraj, const LdsSeg* __re
         float e[R];
 #pragma unroll
         for (int q = 0; q < R; q += 4) {
-            const float4 v = *reinterpret_cast<const float4*>(T.E + t * R + q);
+            const float4 v = *reinterpret_cast<const float4*>(T.E + (q / 4) * PS + t * 4);
             e[q] = v.x;
             e[q + 1] = v.y;
             e[q + 2] = v.z;
@@ -112,8 +116,9 @@ lds_kernel(const LdsTraj* __restrict__ trajs, uint32_t ntraj, const LdsSeg* __re
                 float* wi = lim[buf];
 #pragma unroll
                 for (int q = 0; q < R; q += 4) {
-                    *reinterpret_cast<float4*>(wr + t * R + q) = make_float4(xr[q], xr[q + 1], xr[q + 2], xr[q + 3]);
-                    *reinterpret_cast<float4*>(wi + t * R + q) = make_float4(xi[q], xi[q + 1], xi[q + 2], xi[q + 3]);
+                    const int a = (q / 4) * PS + t * 4;
+                    *reinterpret_cast<float4*>(wr + a) = make_float4(xr[q], xr[q + 1], xr[q + 2], xr[q + 3]);
+                    *reinterpret_cast<float4*>(wi + a) = make_float4(xi[q], xi[q + 1], xi[q + 2], xi[q + 3]);
                 }
                 __syncthreads();
                 float ar[R], ai[R];
@@ -134,8 +139,9 @@ lds_kernel(const LdsTraj* __restrict__ trajs, uint32_t ntraj, const LdsSeg* __re
                     const int pt = t ^ (1 << b);
 #pragma unroll
                     for (int q = 0; q < R; q += 4) {
-                        const float4 u = *reinterpret_cast<const float4*>(wr + pt * R + q);
-                        const float4 v = *reinterpret_cast<const float4*>(wi + pt * R + q);
+                        const int a = (q / 4) * PS + pt * 4;
+                        const float4 u = *reinterpret_cast<const float4*>(wr + a);
+                        const float4 v = *reinterpret_cast<const float4*>(wi + a);
                         ar[q] += u.x;
                         ar[q + 1] += u.y;
                         ar[q + 2] += u.z;
@@ -179,8 +185,10 @@ lds_kernel(const LdsTraj* __restrict__ trajs, uint32_t ntraj, const LdsSeg* __re
             __syncthreads();
             const double tot = bcast;
             if (G.prob_slot >= 0) {
-                const uint32_t own = T.eloc - (uint32_t)(t * R);
-                if (own < (uint32_t)R) {
+                // owner of eloc under the plane layout: element w of chunk c of thread tt
+                const uint32_t w = T.eloc & 3u, tt = (T.eloc >> 2) & (uint32_t)(NT - 1);
+                const uint32_t own = (T.eloc / (uint32_t)PS) * 4u + w;
+                if (tt == (uint32_t)t) {
                     double re = 0.0, im = 0.0;
 #pragma unroll
                     for (int q = 0; q < R; q++)

@@ -29,8 +29,9 @@ std::vector<Cand> candidates(int n)
     std::vector<Cand> v;
     for (int tb : {11, 12}) {
         if (tb > n) continue;
-        for (int lc : {0, 3, 6, 8})
-            if (lc <= n - tb) v.push_back({tb, 0, lc});
+        for (int var = 0; var < term_variants(); var++)
+            for (int lc : {0, 3, 6, 8})
+                if (lc <= n - tb) v.push_back({tb, var, lc});
     }
     for (int hb : {13, 14}) {
         if (hb > n) continue;
@@ -123,7 +124,7 @@ extern "C" mqw_status mqw_tune(mqw_ctx* c, int n, const char* cache_path, int* t
             auto launch = [&](int q) {
                 const void* jobs = jd + (size_t)(q & 1) * T;
                 return half ? launch_half_term(cd.tb, cd.variant, jobs, (uint32_t)T, n, cd.lc, c->stream)
-                            : launch_term(MQW_FP32, cd.tb, jobs, (uint32_t)T, n, cd.lc, c->stream);
+                            : launch_term(MQW_FP32, cd.tb, cd.variant, jobs, (uint32_t)T, n, cd.lc, c->stream);
             };
             if (launch(0) || launch(1)) continue;
             HIPCHK(c, hipEventRecord(e0, c->stream));
