@@ -78,12 +78,21 @@ def main() -> int:
         probs.append(p)
     wall = time.perf_counter() - t0
     inst = args.steps * args.per_step
-    value = inst / wall
+    value = inst / wall if wall > 0 else 0.0
     # Roofline of the fused term kernel: algorithmic bytes (36 B per amplitude
     # per complex H-apply) over the event-timed device time of the sweeps.
     achieved = agg["alg"] / (agg["gpu_ms"] * 1e-3) / 1e9 if agg["gpu_ms"] > 0 else 0.0
     per_launch_bytes = agg["alg"] / max(1, agg["sweeps"])
     per_launch_ms = agg["gpu_ms"] / max(1, agg["sweeps"])
+    # HBM traffic per launch: the calibrated FETCH_SIZE + WRITE_SIZE ratio to
+    # algorithmic bytes measured on this workload (profiles/, rocprofv3 --pmc).
+    traffic, traffic_src = None, None
+    prof = os.path.join(ROOT, "profiles", "half_term_kernel_n20_m3_traffic.md")
+    if os.path.exists(prof) and n == 20 and m == 3:
+        for line in open(prof):
+            if line.startswith("traffic_per_algorithmic_byte:"):
+                traffic = float(line.split(":")[1]) * per_launch_bytes
+                traffic_src = "profiles/half_term_kernel_n20_m3_traffic.md"
 
     apply = None
     if args.apply_n > 0:
@@ -119,7 +128,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": wall / max(1, args.steps) * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -129,13 +138,13 @@ def main() -> int:
                                f"{args.per_step} instances per step", "n": n, "m": m,
                    "instances_per_step": args.per_step, "samples": 100},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "term_kernel (fused Clenshaw sweep)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": "fused Clenshaw sweep (half_term_kernel / term_kernel)",
                      "alg_bytes_per_launch": per_launch_bytes, "avg_launch_ms": per_launch_ms,
                      "launches": agg["sweeps"]},
         "cpu_baseline": cpu,
         "apply_rate": apply,
-        "mean_prob": float(np.mean(np.concatenate(probs))),
+        "mean_prob": float(np.mean(np.concatenate(probs))) if probs else None,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
