@@ -1,0 +1,100 @@
+"""Measurement studies beyond the bench line.
+
+  python -m multistageqw_amd.studies precision --n 24 --m 3 --samples 8
+      FP32 state/recurrence against an all-FP64 GPU run of the same instance:
+      per-sample ground-state probabilities, max |difference|.
+  python -m multistageqw_amd.studies large --n 28 --m 5 --samples 2
+      one large instance: complex H|psi> applies/s and algorithmic GB/s.
+  python -m multistageqw_amd.studies sweep --ns 14 16 18 20 22 24 26
+      apply rate of the sweep kernel per n (one instance, a few samples).
+  python -m multistageqw_amd.studies uneven --count 256
+      the uneven workload (n uniform in 12..22, m uniform in 1..5), instances/s.
+
+Each prints JSON lines.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import FP32, FP64, Context, generate_sk
+
+PEAK = 8000.0
+
+
+def _apply_line(ctx, n, wall, **extra):
+    st = ctx.last_stats()
+    applies = st["term_units"] / (st["gpu_ms"] * 1e-3)
+    bpa = 36.0 if extra.get("precision", "fp32") == "fp32" else 72.0
+    gbs = applies * bpa * (1 << n) / 1e9
+    return dict(n=n, applies_per_s=applies, algorithmic_GBps=gbs, frac=gbs / PEAK, device_ms=st["gpu_ms"],
+                wall_s=wall, applies=st["term_units"], **extra)
+
+
+def precision(ctx, n, m, samples, seed):
+    rec = generate_sk(n, 1, seed)
+    t0 = time.time()
+    p32, _, s32 = ctx.solve(n, m, rec, samples=samples, precision=FP32, with_samples=True)
+    r32 = _apply_line(ctx, n, time.time() - t0, precision="fp32")
+    t0 = time.time()
+    p64, _, s64 = ctx.solve(n, m, rec, samples=samples, precision=FP64, with_samples=True)
+    r64 = _apply_line(ctx, n, time.time() - t0, precision="fp64")
+    return dict(study="precision", n=n, m=m, samples=samples, seed=seed, p_fp32=float(p32[0]), p_fp64=float(p64[0]),
+                max_abs_diff_per_sample=float(np.max(np.abs(s32 - s64))), abs_diff_mean=float(abs(p32[0] - p64[0])),
+                fp32=r32, fp64=r64)
+
+
+def large(ctx, n, m, samples, seed):
+    rec = generate_sk(n, 1, seed)
+    ctx.solve(n, m, rec, samples=1)
+    t0 = time.time()
+    p, _ = ctx.solve(n, m, rec, samples=samples)
+    return dict(study="large", m=m, samples=samples, p=float(p[0]), **_apply_line(ctx, n, time.time() - t0))
+
+
+def uneven(ctx, count, samples, seed):
+    rng = np.random.default_rng(seed)
+    inst = []
+    for k in range(count):
+        n = int(rng.integers(12, 23))
+        m = int(rng.integers(1, 6))
+        inst.append((n, m, generate_sk(n, 1, seed * 100003 + k)[0]))
+    t0 = time.time()
+    p, _ = ctx.solve_batch(inst, samples=samples)
+    wall = time.time() - t0
+    st = ctx.last_stats()
+    return dict(study="uneven", count=count, samples=samples, instances_per_s=count / wall, wall_s=wall,
+                device_ms=st["gpu_ms"], algorithmic_GBps=st["algorithmic_bytes"] / (st["gpu_ms"] * 1e-3) / 1e9,
+                lds_trajectories=st["lds_trajectories"], mean_p=float(np.mean(p)))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("study", choices=["precision", "large", "sweep", "uneven"])
+    ap.add_argument("--n", type=int, default=24)
+    ap.add_argument("--ns", type=int, nargs="*", default=[14, 16, 18, 20, 22, 24, 26])
+    ap.add_argument("--m", type=int, default=3)
+    ap.add_argument("--samples", type=int, default=8)
+    ap.add_argument("--count", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=5)
+    a = ap.parse_args(argv)
+    ctx = Context(0)
+    if a.study == "precision":
+        print(json.dumps(precision(ctx, a.n, a.m, a.samples, a.seed)), flush=True)
+    elif a.study == "large":
+        print(json.dumps(large(ctx, a.n, a.m, a.samples, a.seed)), flush=True)
+    elif a.study == "sweep":
+        for n in a.ns:
+            print(json.dumps(large(ctx, n, a.m, a.samples, a.seed)), flush=True)
+    else:
+        print(json.dumps(uneven(ctx, a.count, a.samples, a.seed)), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
