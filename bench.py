@@ -64,7 +64,7 @@ def main() -> int:
 
     for k in range(args.warmup):
         step(k)
-    agg = dict(amp_terms=0, gpu_ms=0.0, sweeps=0, term_units=0, alg=0.0)
+    agg = dict(amp_terms=0, gpu_ms=0.0, sweeps=0, term_units=0, alg=0.0, req=0.0)
     t0 = time.perf_counter()
     probs = []
     for k in range(args.steps):
@@ -75,6 +75,7 @@ def main() -> int:
         agg["sweeps"] += st["sweeps"]
         agg["term_units"] += st["term_units"]
         agg["alg"] += st["algorithmic_bytes"]
+        agg["req"] += st["required_bytes"]
         probs.append(p)
     wall = time.perf_counter() - t0
     inst = args.steps * args.per_step
@@ -82,6 +83,8 @@ def main() -> int:
     # Roofline of the fused term kernel: algorithmic bytes (36 B per amplitude
     # per complex H-apply) over the event-timed device time of the sweeps.
     achieved = agg["alg"] / (agg["gpu_ms"] * 1e-3) / 1e9 if agg["gpu_ms"] > 0 else 0.0
+    # The same without the psi reads that first-stage terms skip (psi = |+> is a constant there).
+    achieved_req = agg["req"] / (agg["gpu_ms"] * 1e-3) / 1e9 if agg["gpu_ms"] > 0 else 0.0
     per_launch_bytes = agg["alg"] / max(1, agg["sweeps"])
     per_launch_ms = agg["gpu_ms"] / max(1, agg["sweeps"])
     # HBM traffic per launch: the calibrated FETCH_SIZE + WRITE_SIZE ratio to
@@ -139,6 +142,7 @@ def main() -> int:
                    "instances_per_step": args.per_step, "samples": 100},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "achieved_required": achieved_req, "frac_required": achieved_req / HBM_PEAK_GBS,
                      "kernel": "fused Clenshaw sweep (half_term_kernel / term_kernel)",
                      "alg_bytes_per_launch": per_launch_bytes, "avg_launch_ms": per_launch_ms,
                      "launches": agg["sweeps"]},

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 1
+#define MQW_ABI_VERSION 2
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -77,7 +77,7 @@ typedef struct {
     double tol;        /* Chebyshev tail tolerance (1e-6) */
     int precision;     /* mqw_precision: state/recurrence in FP32 (default) or FP64 */
     int energy_mode;   /* mqw_energy_mode for H_P and E_loc (COMPAT = the reference walk) */
-    int tile_bits;     /* 0 = tuned/default; else bits of a sweep tile (10..13) */
+    int tile_bits;     /* 0 = tuned/default; else tile bits: complex tiles 4..12, real-half 13..14 (FP32) */
     int chunk_bits;    /* -1 = tuned/default; XCD chunk of consecutive tiles, log2 */
     int small_n_max;   /* -1 = default; largest n run by the LDS-resident kernel (0 = never) */
     int use_graph;     /* 1 = capture each stage plan in a hipGraph (default 1) */
@@ -202,6 +202,8 @@ typedef struct {
     double gpu_ms;           /* device time of the propagation (events) */
     double sweep_ms;         /* device time inside term kernels only (events) */
     uint64_t lds_trajectories; /* trajectories run by the LDS-resident kernel */
+    uint64_t psi_const_amp_terms; /* amplitude-terms whose psi is the constant |+> (no psi read) */
+    double required_bytes;   /* algorithmic_bytes less the psi reads the |+> terms skip */
 } mqw_stats;
 mqw_status mqw_last_stats(const mqw_ctx* ctx, mqw_stats* out);
 

@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmqw.so")
 CLI_PATH = os.path.join(HERE, "mqw")
 
+ABI_VERSION = 2  # MQW_ABI_VERSION of include/mqw.h this binding matches
 MAX_STAGES = 64
 DEFAULT_SAMPLES = 100
 DEFAULT_TOL = 1e-6
@@ -66,7 +67,8 @@ class Instance(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("sweeps", C.c_uint64), ("term_units", C.c_uint64), ("amp_terms", C.c_uint64),
                 ("algorithmic_bytes", C.c_double), ("gpu_ms", C.c_double), ("sweep_ms", C.c_double),
-                ("lds_trajectories", C.c_uint64)]
+                ("lds_trajectories", C.c_uint64), ("psi_const_amp_terms", C.c_uint64),
+                ("required_bytes", C.c_double)]
 
 
 _lib = None
@@ -80,6 +82,10 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m multistageqw_amd.build`")
     L = C.CDLL(LIB_PATH)
+    L.mqw_abi_version.restype = C.c_int
+    if L.mqw_abi_version() != ABI_VERSION:
+        raise ImportError(f"{LIB_PATH} has ABI version {L.mqw_abi_version()}, this binding needs {ABI_VERSION}: "
+                          "rebuild it with `python -m multistageqw_amd.build`")
     fp, dp, ip = C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_int)
     vp = C.c_void_p
     sig = {

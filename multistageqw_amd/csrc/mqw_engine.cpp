@@ -221,6 +221,7 @@ struct Plan {
     std::vector<ReduceJob> rjobs;
     std::vector<Op> ops;
     uint64_t amp_terms = 0;
+    uint64_t psi_const_amp_terms = 0;
     uint64_t terms = 0;
     uint64_t sweeps = 0;
 };
@@ -354,6 +355,7 @@ void build_plan(int n, uint32_t parts, const std::vector<Traj>& trajs, const std
                     P.jobs.push_back(seq[i][q]);
                     op.count++;
                     P.amp_terms += N;
+                    if (seq[i][q].flags & F_PSI_CONST) P.psi_const_amp_terms += N;
                 }
             P.ops.push_back(op);
             P.sweeps++;
@@ -615,6 +617,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         c->stats.sweeps += plan.sweeps;
         c->stats.term_units += plan.terms;
         c->stats.amp_terms += plan.amp_terms;
+        c->stats.psi_const_amp_terms += plan.psi_const_amp_terms;
         g0 = g1;
     }
     HIPCHK(c, hipMemcpyAsync(probs.data(), probs_d, sizeof(double) * probs.size(), hipMemcpyDeviceToHost, c->stream));
@@ -645,6 +648,8 @@ mqw_status solve_problems(mqw_ctx* c, std::vector<Problem>& probs_in, const mqw_
         if (st) return st;
     }
     c->stats.algorithmic_bytes = (double)c->stats.amp_terms * (o.precision == MQW_FP32 ? 36.0 : 72.0);
+    c->stats.required_bytes = c->stats.algorithmic_bytes -
+                              (double)c->stats.psi_const_amp_terms * (o.precision == MQW_FP32 ? 8.0 : 16.0);
     for (Problem& p : probs_in) {
         double s = 0.0;
         for (int j = 0; j < o.samples; j++) s += sp[p.prob_base + j];

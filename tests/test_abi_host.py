@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_defaults():
     lib = M.load()
-    assert lib.mqw_abi_version() == 1
+    assert lib.mqw_abi_version() == L.ABI_VERSION
     o = M.make_opts()
     assert (o.samples, o.tol, o.precision, o.energy_mode, o.use_graph) == (100, 1e-6, 0, 0, 1)
 
