@@ -2,7 +2,8 @@
 // plain copy of a known number of bytes in a few launch shapes.
 // rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this program gives the counter
 // reading that corresponds to exactly `bytes` read and `bytes` written per
-// dispatch, the scale for the term kernel's counter readings.
+// dispatch, the scale for the term kernel's counter readings.  A last
+// kernel reads four arrays and writes a fifth, the sweep's stream mix.
 // Usage: hbm_copy [MiB] [reps]
 #include <hip/hip_runtime.h>
 
@@ -37,6 +38,21 @@ __global__ void __launch_bounds__(256) copy_flat(const f4* __restrict__ src, f4*
             else
                 dst[i] = v[u];
         }
+    }
+}
+
+// The sweep kernel's stream mix without its partner loads: four streams read
+// and one written per element (b_{k+1}, b_{k+2}, psi, E -> b_k), one float4
+// per thread per stream, non-temporal like the sweep's streams.
+__global__ void __launch_bounds__(256)
+mix4r1w(const f4* __restrict__ a, const f4* __restrict__ b, const f4* __restrict__ c, const f4* __restrict__ d,
+        f4* __restrict__ o, size_t n4)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n4) {
+        const f4 v = a[i] + __builtin_nontemporal_load(b + i) + __builtin_nontemporal_load(c + i) +
+                     __builtin_nontemporal_load(d + i);
+        __builtin_nontemporal_store(v, o + i);
     }
 }
 
@@ -95,6 +111,33 @@ int main(int argc, char** argv)
         printf("copy %-9s %zu MiB: %.3f ms, %.1f GB/s (read+write)\n", v.name, mib, best,
                2.0 * bytes / (best * 1e-3) / 1e9);
     }
+    // 4-read / 1-write mix over five separate arrays of `bytes` each
+    f4 *c = nullptr, *d = nullptr, *o = nullptr;
+    if (hipMalloc(&c, bytes) == hipSuccess && hipMalloc(&d, bytes) == hipSuccess && hipMalloc(&o, bytes) == hipSuccess) {
+        hipMemset(c, 0, bytes);
+        hipMemset(d, 0, bytes);
+        hipMemset(o, 0, bytes);
+        double best = 1e30;
+        for (int r = 0; r < reps; r++) {
+            hipEvent_t e0, e1;
+            hipEventCreate(&e0);
+            hipEventCreate(&e1);
+            hipEventRecord(e0, 0);
+            hipLaunchKernelGGL(mix4r1w, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, 0, a, b, c, d, o, n4);
+            hipEventRecord(e1, 0);
+            hipEventSynchronize(e1);
+            float ms = 0;
+            hipEventElapsedTime(&ms, e0, e1);
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+            if (ms < best) best = ms;
+        }
+        printf("mix4r1w   %zu MiB x5: %.3f ms, %.1f GB/s (4 read + 1 write)\n", mib, best,
+               5.0 * bytes / (best * 1e-3) / 1e9);
+    }
+    hipFree(c);
+    hipFree(d);
+    hipFree(o);
     hipFree(a);
     hipFree(b);
     return 0;

@@ -705,7 +705,7 @@ int launch_term(int precision, int tb, int variant, const void* jobs_dev, uint32
 // complex tile of 2^(HB-1), two more tile bits are exchanged on chip and
 // fewer flips need partner-tile loads.  The diagonal is read by both halves.
 // ===========================================================================
-template <int HB, int UNR, int OCC, bool NTS>
+template <int HB, int UNR, int OCC, bool NTS, bool PAIR>
 __global__ void __launch_bounds__(1024, OCC)
 half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n, int lc)
 {
@@ -713,17 +713,21 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
     constexpr int RB = HB - 10;
     constexpr int R = 1 << RB;
     constexpr int TS = 1 << HB;
+    // PAIR: one workgroup produces the two tiles that differ in the top tile
+    // bit, so that flip is an exchange of registers instead of a partner load
+    constexpr int NS = PAIR ? 2 : 1;
     __shared__ float s_x[TS];
     __shared__ double s_red[NT / 64];
     __shared__ double s_bcast;
 
     const int ob = n - HB;
+    const int pb = ob - (PAIR ? 1 : 0); // tile bits that are partner loads
     const uint64_t unit = xcd_unit(blockIdx.x, lc);
-    const uint64_t jh = unit >> ob;
-    const uint64_t job = jh >> 1;
+    // unit = (job, half, tile); with PAIR the tile index drops its top bit
+    const uint64_t job = unit >> (pb + 1);
     if (job >= njobs) return;
-    const int h = (int)(jh & 1);
-    const uint64_t tile = unit & ((1ull << ob) - 1ull);
+    const int h = (int)((unit >> pb) & 1u);
+    const uint64_t tile0 = unit & ((1ull << pb) - 1ull);
     const TermJob<float>& J = jobs[job];
     const uint32_t flags = J.flags;
     const uint32_t src1 = flags & 3u, src2 = flags & 12u;
@@ -732,7 +736,6 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
     const int t = threadIdx.x;
     // plane layout: chunk c of thread t holds elements c*4096 + 4t + [0, 4)
     constexpr int PS = NT * 4;
-    const uint64_t off = (tile << HB) + (uint64_t)t * 4;
 
     float ps = 1.0f;
     if (flags & F_NORM_IN) {
@@ -744,7 +747,7 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
         __syncthreads();
         const double tot = s_bcast;
         ps = (float)(1.0 / sqrt(tot));
-        if ((flags & F_READOUT) && tile == 0 && h == 0 && t == 0) {
+        if ((flags & F_READOUT) && tile0 == 0 && h == 0 && t == 0) {
             const double re = (double)J.psi[J.eloc], im = (double)J.psi[N + J.eloc];
             *J.prob = (re * re + im * im) / tot;
         }
@@ -769,111 +772,144 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
     if (src2 == SRC2_PSI) a2 *= ps;
     ac *= ps;
 
-    float o[R];
+    const bool src1_arr = src1 != SRC1_ZERO && !(src1 == SRC1_PSI && psi_const);
+    const float* src = ((src1 == SRC1_PSI) ? J.psi : J.b1) + (uint64_t)h1 * N;
+    // own b_{k+1} tiles, both of a pair loaded up front
+    float xs[NS][R];
+    if (src1_arr) {
 #pragma unroll
-    for (int q = 0; q < R; q++) o[q] = 0.0f;
-    if (src1 == SRC1_PSI && psi_const) {
-        if (h1 == 0) {
+        for (int sd = 0; sd < NS; sd++)
+            vload<float, R>(src + ((tile0 | ((uint64_t)sd << pb)) << HB) + (uint64_t)t * 4, xs[sd], PS);
+    }
+
+#pragma unroll
+    for (int sd = 0; sd < NS; sd++) {
+        const uint64_t tile = tile0 | ((uint64_t)sd << pb);
+        const uint64_t off = (tile << HB) + (uint64_t)t * 4;
+        // the previous tile's LDS reads and norm sum are complete
+        if (sd > 0) __syncthreads();
+        float o[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) o[q] = 0.0f;
+        if (src1 == SRC1_PSI && psi_const) {
+            if (h1 == 0) {
+                float e[R];
+                vload_nt<float, R>(J.E + off, e, PS);
+                const float base = J.beta * (float)n * J.psi0;
+#pragma unroll
+                for (int q = 0; q < R; q++) o[q] = a1 * (J.alpha * e[q] * J.psi0 - base);
+            }
+        } else if (src1_arr) {
+            const float(&x)[R] = xs[sd];
+            float acc[R];
+            lds_store<float, R>(s_x + t * 4, x, PS);
+#pragma unroll
+            for (int q = 0; q < R; q++) acc[q] = 0.0f;
+#pragma unroll
+            for (int b = 0; b < RB; b++)
+#pragma unroll
+                for (int q = 0; q < R; q++) acc[q] += x[q ^ (1 << b)];
+            // partner tiles in bit order, UNR loads in flight per thread
+            int j = 0;
+            for (; j + UNR <= pb; j += UNR) {
+                float y[UNR][R];
+#pragma unroll
+                for (int u = 0; u < UNR; u++)
+                    vload<float, R>(src + ((tile ^ (1ull << (j + u))) << HB) + (uint64_t)t * 4, y[u], PS);
+#pragma unroll
+                for (int u = 0; u < UNR; u++)
+#pragma unroll
+                    for (int q = 0; q < R; q++) acc[q] += y[u][q];
+            }
+            for (; j < pb; j++) {
+                float y[R];
+                vload<float, R>(src + ((tile ^ (1ull << j)) << HB) + (uint64_t)t * 4, y, PS);
+#pragma unroll
+                for (int q = 0; q < R; q++) acc[q] += y[q];
+            }
+            if (PAIR) {
+                // the top tile bit, last in the same order
+#pragma unroll
+                for (int q = 0; q < R; q++) acc[q] += xs[NS - 1 - sd][q];
+            }
             float e[R];
             vload_nt<float, R>(J.E + off, e, PS);
-            const float base = J.beta * (float)n * J.psi0;
+            __syncthreads();
 #pragma unroll
-            for (int q = 0; q < R; q++) o[q] = a1 * (J.alpha * e[q] * J.psi0 - base);
+            for (int b = 0; b < HB - RB; b++) {
+                float y[R];
+                lds_load<float, R>(s_x + (t ^ (1 << b)) * 4, y, PS);
+#pragma unroll
+                for (int q = 0; q < R; q++) acc[q] += y[q];
+            }
+#pragma unroll
+            for (int q = 0; q < R; q++) o[q] = a1 * (J.alpha * e[q] * x[q] - J.beta * acc[q]);
         }
-    } else if (src1 != SRC1_ZERO) {
-        const float* src = ((src1 == SRC1_PSI) ? J.psi : J.b1) + (uint64_t)h1 * N;
-        float x[R], acc[R];
-        vload<float, R>(src + off, x, PS);
-        lds_store<float, R>(s_x + t * 4, x, PS);
-#pragma unroll
-        for (int q = 0; q < R; q++) acc[q] = 0.0f;
-#pragma unroll
-        for (int b = 0; b < RB; b++)
-#pragma unroll
-            for (int q = 0; q < R; q++) acc[q] += x[q ^ (1 << b)];
-        // partner tiles, UNR loads in flight per thread
-        int j = 0;
-        for (; j + UNR <= ob; j += UNR) {
-            float y[UNR][R];
-#pragma unroll
-            for (int u = 0; u < UNR; u++)
-                vload<float, R>(src + ((tile ^ (1ull << (j + u))) << HB) + (uint64_t)t * 4, y[u], PS);
-#pragma unroll
-            for (int u = 0; u < UNR; u++)
-#pragma unroll
-                for (int q = 0; q < R; q++) acc[q] += y[u][q];
-        }
-        for (; j < ob; j++) {
+        if (src2 != SRC2_ZERO) {
             float y[R];
-            vload<float, R>(src + ((tile ^ (1ull << j)) << HB) + (uint64_t)t * 4, y, PS);
+            if (src2 == SRC2_PSI && psi_const) {
 #pragma unroll
-            for (int q = 0; q < R; q++) acc[q] += y[q];
+                for (int q = 0; q < R; q++) y[q] = h2 == 0 ? J.psi0 : 0.0f;
+            } else {
+                const float* s2 = ((src2 == SRC2_PSI) ? J.psi : J.b2) + (uint64_t)h2 * N;
+                vload_nt<float, R>(s2 + off, y, PS);
+            }
+#pragma unroll
+            for (int q = 0; q < R; q++) o[q] -= a2 * y[q];
         }
-        float e[R];
-        vload_nt<float, R>(J.E + off, e, PS);
-        __syncthreads();
-#pragma unroll
-        for (int b = 0; b < HB - RB; b++) {
+        if (ac != 0.0f) {
             float y[R];
-            lds_load<float, R>(s_x + (t ^ (1 << b)) * 4, y, PS);
+            if (psi_const) {
 #pragma unroll
-            for (int q = 0; q < R; q++) acc[q] += y[q];
+                for (int q = 0; q < R; q++) y[q] = hc == 0 ? J.psi0 : 0.0f;
+            } else {
+                vload_nt<float, R>(J.psi + (uint64_t)hc * N + off, y, PS);
+            }
+#pragma unroll
+            for (int q = 0; q < R; q++) o[q] += ac * y[q];
         }
+        if (NTS)
+            vstore_nt<float, R>(J.out + (uint64_t)h * N + off, o, PS);
+        else
+            vstore<float, R>(J.out + (uint64_t)h * N + off, o, PS);
+        if (flags & F_FINAL) {
+            double nrm = 0.0;
 #pragma unroll
-        for (int q = 0; q < R; q++) o[q] = a1 * (J.alpha * e[q] * x[q] - J.beta * acc[q]);
-    }
-    if (src2 != SRC2_ZERO) {
-        float y[R];
-        if (src2 == SRC2_PSI && psi_const) {
-#pragma unroll
-            for (int q = 0; q < R; q++) y[q] = h2 == 0 ? J.psi0 : 0.0f;
-        } else {
-            const float* s2 = ((src2 == SRC2_PSI) ? J.psi : J.b2) + (uint64_t)h2 * N;
-            vload_nt<float, R>(s2 + off, y, PS);
+            for (int q = 0; q < R; q++) nrm += (double)o[q] * (double)o[q];
+            const double sum = wg_sum_t0<NT>(nrm, s_red);
+            if (t == 0) J.norm_part[((uint64_t)h << ob) + tile] = sum;
         }
-#pragma unroll
-        for (int q = 0; q < R; q++) o[q] -= a2 * y[q];
-    }
-    if (ac != 0.0f) {
-        float y[R];
-        if (psi_const) {
-#pragma unroll
-            for (int q = 0; q < R; q++) y[q] = hc == 0 ? J.psi0 : 0.0f;
-        } else {
-            vload_nt<float, R>(J.psi + (uint64_t)hc * N + off, y, PS);
-        }
-#pragma unroll
-        for (int q = 0; q < R; q++) o[q] += ac * y[q];
-    }
-    if (NTS)
-        vstore_nt<float, R>(J.out + (uint64_t)h * N + off, o, PS);
-    else
-        vstore<float, R>(J.out + (uint64_t)h * N + off, o, PS);
-    if (flags & F_FINAL) {
-        double nrm = 0.0;
-#pragma unroll
-        for (int q = 0; q < R; q++) nrm += (double)o[q] * (double)o[q];
-        const double sum = wg_sum_t0<NT>(nrm, s_red);
-        if (t == 0) J.norm_part[((uint64_t)h << ob) + tile] = sum;
     }
 }
 
 int half_tile_bits_supported(int hb) { return hb == 13 || hb == 14; }
 
 // variant: 0 = default for the tile size; 1.. = the alternatives the tuner
-// tries (loads in flight per thread; +3 = non-temporal output stores)
-int half_variants(int hb) { return 6; }
+// tries (loads in flight per thread; +3 = non-temporal output stores; +6 =
+// tile pairs across the top tile bit)
+int half_variants(int hb) { return 12; }
 int term_variants() { return 2; }
 
 template <int HB, int UNR, int OCC>
-static void launch_half_v(uint32_t grid, const void* jobs, uint32_t njobs, int n, int lc, bool nts, hipStream_t st)
+static void launch_half_v(uint32_t grid, const void* jobs, uint32_t njobs, int n, int lc, bool nts, bool pair,
+                          hipStream_t st)
 {
-    if (nts)
-        hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, true>), dim3(grid), dim3(1024), 0, st,
-                           (const TermJob<float>*)jobs, njobs, n, lc);
-    else
-        hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, false>), dim3(grid), dim3(1024), 0, st,
-                           (const TermJob<float>*)jobs, njobs, n, lc);
+    const TermJob<float>* j = (const TermJob<float>*)jobs;
+    if (pair) {
+        if (nts)
+            hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, true, true>), dim3(grid), dim3(1024), 0, st, j, njobs,
+                               n, lc);
+        else
+            hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, false, true>), dim3(grid), dim3(1024), 0, st, j, njobs,
+                               n, lc);
+    } else {
+        if (nts)
+            hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, true, false>), dim3(grid), dim3(1024), 0, st, j, njobs,
+                               n, lc);
+        else
+            hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, false, false>), dim3(grid), dim3(1024), 0, st, j, njobs,
+                               n, lc);
+    }
 }
 
 int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc, void* stream)
@@ -881,25 +917,29 @@ int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, 
     hipStream_t st = (hipStream_t)stream;
     const int ob = n - hb;
     if (ob < 0 || !half_tile_bits_supported(hb)) return -2;
-    if (lc > ob + 17) lc = ob + 17;
-    const uint64_t units = (uint64_t)njobs << (ob + 1);
+    if (variant < 0 || variant >= half_variants(hb)) variant = 0;
+    // a pair needs a top tile bit to pair across
+    const bool pair = variant >= 6 && ob >= 1;
+    const bool nts = (variant % 6) >= 3;
+    const int v = variant % 3;
+    const int pb = ob - (pair ? 1 : 0);
+    if (lc > pb + 17) lc = pb + 17;
+    const uint64_t units = (uint64_t)njobs << (pb + 1);
     const uint64_t per = 8ull << lc;
     const uint64_t grid64 = (units + per - 1) / per * per;
     if (grid64 > 0x7fffffffull) return -3;
     const uint32_t grid = (uint32_t)grid64;
-    const bool nts = variant >= 3;
-    const int v = variant % 3;
     if (hb == 14) {
         switch (v) {
-        case 1: launch_half_v<14, 2, 4>(grid, jobs_dev, njobs, n, lc, nts, st); break;
-        case 2: launch_half_v<14, 4, 4>(grid, jobs_dev, njobs, n, lc, nts, st); break;
-        default: launch_half_v<14, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, st); break;
+        case 1: launch_half_v<14, 2, 4>(grid, jobs_dev, njobs, n, lc, nts, pair, st); break;
+        case 2: launch_half_v<14, 4, 4>(grid, jobs_dev, njobs, n, lc, nts, pair, st); break;
+        default: launch_half_v<14, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, pair, st); break;
         }
     } else {
         switch (v) {
-        case 1: launch_half_v<13, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, st); break;
-        case 2: launch_half_v<13, 2, 8>(grid, jobs_dev, njobs, n, lc, nts, st); break;
-        default: launch_half_v<13, 4, 8>(grid, jobs_dev, njobs, n, lc, nts, st); break;
+        case 1: launch_half_v<13, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, pair, st); break;
+        case 2: launch_half_v<13, 2, 8>(grid, jobs_dev, njobs, n, lc, nts, pair, st); break;
+        default: launch_half_v<13, 4, 8>(grid, jobs_dev, njobs, n, lc, nts, pair, st); break;
         }
     }
     return check_launch("half_term");

@@ -3,7 +3,8 @@
 // bits) on synthetic term jobs shaped like a batched solve, keeps the fastest
 // in the context and, given a cache path, records it as a line
 //   <gfx arch> <n> <tile bits> <variant> <chunk bits> <GB/s>
-// so later processes reuse the choice without measuring.
+// so later processes reuse the choice without measuring.  MQW_TUNE_VERBOSE=1
+// prints every candidate's rate to stderr.
 #include <hip/hip_runtime_api.h>
 
 #include <stdio.h>
@@ -36,7 +37,7 @@ std::vector<Cand> candidates(int n)
     for (int hb : {13, 14}) {
         if (hb > n) continue;
         for (int var = 0; var < half_variants(hb); var++)
-            for (int lc : {3, 6, 8})
+            for (int lc : {3, 4, 5, 6, 7, 8})
                 if (lc <= n - hb + 1) v.push_back({hb, var, lc});
     }
     if (v.empty()) v.push_back({std::min(n, 12), 0, 0});
@@ -88,12 +89,14 @@ extern "C" mqw_status mqw_tune(mqw_ctx* c, int n, const char* cache_path, int* t
         // about 2^26 amplitudes per launch, like a batch of trajectories
         const size_t T = std::max<size_t>(1, ((size_t)1 << 26) >> n);
         float* st = (float*)dev_get(c, "tune_state", sizeof(float) * 6 * N * T);
-        float* E = (float*)dev_get(c, "tune_E", sizeof(float) * N);
+        // a diagonal per instance, several trajectories per instance
+        const size_t NE = std::min<size_t>(T, 8);
+        float* E = (float*)dev_get(c, "tune_E", sizeof(float) * N * NE);
         double* part = (double*)dev_get(c, "tune_part", sizeof(double) * 2 * N);
         TermJob<float>* jd = (TermJob<float>*)dev_get(c, "tune_jobs", sizeof(TermJob<float>) * 2 * T);
         if (!st || !E || !part || !jd) return fail(c, MQW_ENOMEM, "device allocation failed while tuning n=%d", n);
         HIPCHK(c, hipMemsetAsync(st, 0, sizeof(float) * 6 * N * T, c->stream));
-        HIPCHK(c, hipMemsetAsync(E, 0, sizeof(float) * N, c->stream));
+        HIPCHK(c, hipMemsetAsync(E, 0, sizeof(float) * N * NE, c->stream));
         // two job tables: b1 = A, b2 = out = B, psi = C, then A and B swapped
         std::vector<TermJob<float>> hj(2 * T);
         for (int ph = 0; ph < 2; ph++)
@@ -105,7 +108,7 @@ extern "C" mqw_status mqw_tune(mqw_ctx* c, int n, const char* cache_path, int* t
                 j.b1 = ph ? B : A;
                 j.b2 = j.out = ph ? A : B;
                 j.psi = st + (t * 3 + 2) * 2 * N;
-                j.E = E;
+                j.E = E + (t % NE) * N;
                 j.norm_part = part;
                 j.alpha = 0.1f;
                 j.beta = 0.01f;
@@ -119,6 +122,8 @@ extern "C" mqw_status mqw_tune(mqw_ctx* c, int n, const char* cache_path, int* t
         HIPCHK(c, hipEventCreate(&e0));
         HIPCHK(c, hipEventCreate(&e1));
         const int reps = 8;
+        const char* vb = getenv("MQW_TUNE_VERBOSE");
+        const bool verbose = vb && *vb == '1';
         for (const Cand& cd : candidates(n)) {
             const bool half = cd.tb >= 13;
             auto launch = [&](int q) {
@@ -136,6 +141,7 @@ extern "C" mqw_status mqw_tune(mqw_ctx* c, int n, const char* cache_path, int* t
             float ms = 0.0f;
             hipEventElapsedTime(&ms, e0, e1);
             const double gbs = 36.0 * (double)N * (double)T * reps / (ms * 1e-3) / 1e9;
+            if (verbose) fprintf(stderr, "mqw_tune n=%d tb=%d variant=%d lc=%d %.1f GB/s\n", n, cd.tb, cd.variant, cd.lc, gbs);
             if (gbs > best_gbs) {
                 best_gbs = gbs;
                 best = cd;

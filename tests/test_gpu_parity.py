@@ -110,6 +110,7 @@ PATHS = {
     "half13": dict(small_n_max=0, tile_bits=13, variant=1),   # real-half sweep kernel, 2^13 tiles
     "half14": dict(small_n_max=0, tile_bits=14),          # real-half sweep kernel, 2^14 tiles
     "halfdirect": dict(small_n_max=0, tile_bits=13, use_graph=False, group_bytes=48 << 20),  # small groups
+    "halfpaired": dict(small_n_max=0, tile_bits=13, variant=10),  # tile pairs across the top tile bit
 }
 
 
@@ -147,3 +148,12 @@ def test_solve_fp64_vs_dense(ctx, oracle):
             ref = D.walk_probability(e - (info["emax"] + info["e0"]) / 2, sch["gamma"].astype(np.float64), times,
                                      info["eloc"], reset=(m != 1))
             assert np.max(np.abs(sp[k] - ref)) < 5e-6
+
+
+def test_tile_pairs_are_bit_identical(ctx):
+    """Pairing tiles across the top tile bit keeps every sum in the same order."""
+    _say("tile pairs vs single tiles n=17 m=2")
+    recs = M.generate_sk(17, 2, 77)
+    _, _, a = ctx.solve(17, 2, recs, samples=6, with_samples=True, small_n_max=0, tile_bits=13, variant=4)
+    _, _, b = ctx.solve(17, 2, recs, samples=6, with_samples=True, small_n_max=0, tile_bits=13, variant=10)
+    assert np.array_equal(a, b)
