@@ -10,7 +10,10 @@
 // n outside [4, 30], a missing or short file, problems > the file holds.
 //
 // Environment (optional): MQW_DEVICE (0), MQW_SAMPLES (100), MQW_PRECISION
-// (fp32|fp64), MQW_ENERGY (compat|exact), MQW_BATCH (problems per GPU call).
+// (fp32|fp64), MQW_ENERGY (compat|exact), MQW_BATCH (problems per GPU call),
+// MQW_FLUSH (1 = append each batch's results as soon as it finishes, so an
+// interrupted run keeps them and can be resumed with `start`; the default
+// writes one block at exit like MultiQW.cpp:457).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -61,7 +64,9 @@ int main(int argc, char** argv)
     if (prec && strcmp(prec, "fp64") == 0) o.precision = MQW_FP64;
     const char* en = getenv("MQW_ENERGY");
     if (en && strcmp(en, "exact") == 0) o.energy_mode = MQW_ENERGY_EXACT;
-    const long batch = env_int("MQW_BATCH", n <= 14 ? 2000 : (n <= 20 ? 32 : 4));
+    long batch = env_int("MQW_BATCH", n <= 14 ? 2000 : (n <= 20 ? 32 : 4));
+    if (batch < 1) batch = 1;
+    const bool flush = env_int("MQW_FLUSH", 0) != 0;
 
     mqw_ctx* ctx = nullptr;
     mqw_status st = mqw_create(env_int("MQW_DEVICE", 0), &ctx);
@@ -89,9 +94,14 @@ int main(int argc, char** argv)
         }
         for (long k = 0; k < cnt; k++) printf("%g %g\n", (double)tt[k], (double)results[b0 + k]);
         fflush(stdout);
+        if (flush && mqw_append_output(nullptr, n, m, results.data() + b0, cnt) != MQW_OK) {
+            fprintf(stderr, "mqw: cannot append to output_%d_%d\n", n, m);
+            mqw_destroy(ctx);
+            return 1;
+        }
     }
     mqw_destroy(ctx);
-    if (mqw_append_output(nullptr, n, m, results.data(), problems) != MQW_OK) {
+    if (!flush && mqw_append_output(nullptr, n, m, results.data(), problems) != MQW_OK) {
         fprintf(stderr, "mqw: cannot append to output_%d_%d\n", n, m);
         return 1;
     }

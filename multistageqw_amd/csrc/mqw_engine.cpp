@@ -88,19 +88,33 @@ SweepCfg mqw::choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits
     SweepCfg k;
     k.variant = variant >= 0 ? variant : c->tuned_var[n];
     int tb = tile_bits > 0 ? tile_bits : c->tuned_tb[n];
-    // Untuned default, from measurements on MI355X: from n = 14 the real-half
-    // kernel on 2^13-float tiles, two partner loads in flight, non-temporal
-    // output stores (variant 4); complex 2^12 tiles below.
+    int dflt_lc = -1;
+    // Untuned FP32 default from n = 14, measured on MI355X over batches of
+    // 2^31/2^n trajectories (at most 1600): real-half tiles, non-temporal
+    // output stores; 2^14-float tiles with one partner load in flight while
+    // the outer tile bits fit a 2^5..2^6-tile XCD chunk (n <= 21), 2^13
+    // tiles with two loads in flight above.
     if (tb <= 0 && precision == MQW_FP32 && n >= 14) {
-        tb = 13;
-        if (variant < 0 && c->tuned_var[n] == 0) k.variant = 4;
+        int dv;
+        if (n == 14) {
+            tb = 13, dv = 4, dflt_lc = 3;
+        } else if (n <= 19) {
+            tb = 14, dv = 3, dflt_lc = 5;
+        } else if (n <= 21) {
+            tb = 14, dv = 3, dflt_lc = 6;
+        } else {
+            tb = 13, dv = 4, dflt_lc = 5;
+        }
+        if (variant < 0) k.variant = dv;
     }
     if (precision == MQW_FP32 && n >= 13 && tb > 0 && half_tile_bits_supported(tb)) {
         if (tb > n) tb = 13;
         k.half = true;
         k.tb = tb;
         k.parts = 2u << (n - tb);
-        k.lc = chunk_bits >= 0 ? chunk_bits : (c->tuned_lc[n] >= 0 ? c->tuned_lc[n] : std::min(n - tb + 1, 6));
+        k.lc = chunk_bits >= 0 ? chunk_bits
+                               : (c->tuned_lc[n] >= 0 ? c->tuned_lc[n]
+                                                      : (dflt_lc >= 0 ? dflt_lc : std::min(n - tb + 1, 6)));
     } else {
         if (tb <= 0 || tb > n || !term_tile_bits_supported(precision, tb)) tb = default_tile_bits(precision, n);
         k.tb = tb;

@@ -705,7 +705,7 @@ int launch_term(int precision, int tb, int variant, const void* jobs_dev, uint32
 // complex tile of 2^(HB-1), two more tile bits are exchanged on chip and
 // fewer flips need partner-tile loads.  The diagonal is read by both halves.
 // ===========================================================================
-template <int HB, int UNR, int OCC, bool NTS, bool PAIR>
+template <int HB, int UNR, int OCC, bool NTS, int MODE>
 __global__ void __launch_bounds__(1024, OCC)
 half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n, int lc)
 {
@@ -713,20 +713,24 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
     constexpr int RB = HB - 10;
     constexpr int R = 1 << RB;
     constexpr int TS = 1 << HB;
-    // PAIR: one workgroup produces the two tiles that differ in the top tile
-    // bit, so that flip is an exchange of registers instead of a partner load
-    constexpr int NS = PAIR ? 2 : 1;
+    // MODE 0: one workgroup per (half, tile).
+    // MODE 1: per (half, tile pair): the two tiles that differ in the top tile
+    //         bit, so that flip is an exchange of registers, not a load.
+    // MODE 2: per tile: both halves, one read of the diagonal.
+    constexpr int NS = MODE == 0 ? 1 : 2;
     __shared__ float s_x[TS];
     __shared__ double s_red[NT / 64];
     __shared__ double s_bcast;
 
     const int ob = n - HB;
-    const int pb = ob - (PAIR ? 1 : 0); // tile bits that are partner loads
+    const int pb = ob - (MODE == 1 ? 1 : 0); // tile bits that are partner loads
     const uint64_t unit = xcd_unit(blockIdx.x, lc);
-    // unit = (job, half, tile); with PAIR the tile index drops its top bit
-    const uint64_t job = unit >> (pb + 1);
+    // unit = (job, half, tile) in modes 0 and 1 (mode 1 drops the top tile
+    // bit), (job, tile) in mode 2
+    const int ub = MODE == 2 ? pb : pb + 1;
+    const uint64_t job = unit >> ub;
     if (job >= njobs) return;
-    const int h = (int)((unit >> pb) & 1u);
+    const int h0 = MODE == 2 ? 0 : (int)((unit >> pb) & 1u);
     const uint64_t tile0 = unit & ((1ull << pb) - 1ull);
     const TermJob<float>& J = jobs[job];
     const uint32_t flags = J.flags;
@@ -747,46 +751,57 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
         __syncthreads();
         const double tot = s_bcast;
         ps = (float)(1.0 / sqrt(tot));
-        if ((flags & F_READOUT) && tile0 == 0 && h == 0 && t == 0) {
+        if ((flags & F_READOUT) && tile0 == 0 && h0 == 0 && t == 0) {
             const double re = (double)J.psi[J.eloc], im = (double)J.psi[N + J.eloc];
             *J.prob = (re * re + im * im) / tot;
         }
     }
-    // out_h = a1 u_{h1} - a2 s2_{h2} + ac Psi_{hc}: each complex scalar is real
-    // or imaginary, so it maps one source half onto this output half.
-    float a1 = J.w1r, a2 = J.w2r, ac = J.cr;
-    int h1 = h, h2 = h, hc = h;
-    if (J.w1i != 0.0f) {
-        a1 = h == 0 ? -J.w1i : J.w1i;
-        h1 = 1 - h;
-    }
-    if (J.w2i != 0.0f) {
-        a2 = h == 0 ? -J.w2i : J.w2i;
-        h2 = 1 - h;
-    }
-    if (J.ci != 0.0f) {
-        ac = h == 0 ? -J.ci : J.ci;
-        hc = 1 - h;
-    }
-    if (src1 == SRC1_PSI) a1 *= ps;
-    if (src2 == SRC2_PSI) a2 *= ps;
-    ac *= ps;
 
     const bool src1_arr = src1 != SRC1_ZERO && !(src1 == SRC1_PSI && psi_const);
-    const float* src = ((src1 == SRC1_PSI) ? J.psi : J.b1) + (uint64_t)h1 * N;
-    // own b_{k+1} tiles, both of a pair loaded up front
+    const float* src_base = (src1 == SRC1_PSI) ? J.psi : J.b1;
+    const bool swap1 = J.w1i != 0.0f; // the first scalar is imaginary: halves swap
+
+    // own b_{k+1} tiles of every side, loaded up front
     float xs[NS][R];
     if (src1_arr) {
 #pragma unroll
-        for (int sd = 0; sd < NS; sd++)
-            vload<float, R>(src + ((tile0 | ((uint64_t)sd << pb)) << HB) + (uint64_t)t * 4, xs[sd], PS);
+        for (int sd = 0; sd < NS; sd++) {
+            const int hs = MODE == 2 ? sd : h0;
+            const uint64_t ts = MODE == 1 ? (tile0 | ((uint64_t)sd << pb)) : tile0;
+            vload<float, R>(src_base + (uint64_t)(swap1 ? 1 - hs : hs) * N + (ts << HB) + (uint64_t)t * 4, xs[sd], PS);
+        }
     }
+    // mode 2: the diagonal of the tile, shared by both halves
+    float e2[R];
+    if (MODE == 2) vload_nt<float, R>(J.E + (tile0 << HB) + (uint64_t)t * 4, e2, PS);
 
 #pragma unroll
     for (int sd = 0; sd < NS; sd++) {
-        const uint64_t tile = tile0 | ((uint64_t)sd << pb);
+        const int h = MODE == 2 ? sd : h0;
+        const uint64_t tile = MODE == 1 ? (tile0 | ((uint64_t)sd << pb)) : tile0;
         const uint64_t off = (tile << HB) + (uint64_t)t * 4;
-        // the previous tile's LDS reads and norm sum are complete
+        // out_h = a1 u_{h1} - a2 s2_{h2} + ac Psi_{hc}: each complex scalar is
+        // real or imaginary, so it maps one source half onto this output half.
+        float a1 = J.w1r, a2 = J.w2r, ac = J.cr;
+        int h1 = h, h2 = h, hc = h;
+        if (swap1) {
+            a1 = h == 0 ? -J.w1i : J.w1i;
+            h1 = 1 - h;
+        }
+        if (J.w2i != 0.0f) {
+            a2 = h == 0 ? -J.w2i : J.w2i;
+            h2 = 1 - h;
+        }
+        if (J.ci != 0.0f) {
+            ac = h == 0 ? -J.ci : J.ci;
+            hc = 1 - h;
+        }
+        if (src1 == SRC1_PSI) a1 *= ps;
+        if (src2 == SRC2_PSI) a2 *= ps;
+        ac *= ps;
+        const float* src = src_base + (uint64_t)h1 * N;
+
+        // the previous side's LDS reads and norm sum are complete
         if (sd > 0) __syncthreads();
         float o[R];
 #pragma unroll
@@ -794,7 +809,12 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
         if (src1 == SRC1_PSI && psi_const) {
             if (h1 == 0) {
                 float e[R];
-                vload_nt<float, R>(J.E + off, e, PS);
+                if (MODE == 2) {
+#pragma unroll
+                    for (int q = 0; q < R; q++) e[q] = e2[q];
+                } else {
+                    vload_nt<float, R>(J.E + off, e, PS);
+                }
                 const float base = J.beta * (float)n * J.psi0;
 #pragma unroll
                 for (int q = 0; q < R; q++) o[q] = a1 * (J.alpha * e[q] * J.psi0 - base);
@@ -827,13 +847,18 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
 #pragma unroll
                 for (int q = 0; q < R; q++) acc[q] += y[q];
             }
-            if (PAIR) {
+            if (MODE == 1) {
                 // the top tile bit, last in the same order
 #pragma unroll
                 for (int q = 0; q < R; q++) acc[q] += xs[NS - 1 - sd][q];
             }
             float e[R];
-            vload_nt<float, R>(J.E + off, e, PS);
+            if (MODE == 2) {
+#pragma unroll
+                for (int q = 0; q < R; q++) e[q] = e2[q];
+            } else {
+                vload_nt<float, R>(J.E + off, e, PS);
+            }
             __syncthreads();
 #pragma unroll
             for (int b = 0; b < HB - RB; b++) {
@@ -884,31 +909,34 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
 
 int half_tile_bits_supported(int hb) { return hb == 13 || hb == 14; }
 
-// variant: 0 = default for the tile size; 1.. = the alternatives the tuner
-// tries (loads in flight per thread; +3 = non-temporal output stores; +6 =
-// tile pairs across the top tile bit)
-int half_variants(int hb) { return 12; }
+// variant = v + 3 nts + 6 mode: v picks the loads in flight per thread and
+// the occupancy, nts = non-temporal output stores, mode = one tile per
+// workgroup (0), tile pairs across the top tile bit (1), both halves of a
+// tile (2).
+int half_variants(int hb) { return 18; }
 int term_variants() { return 2; }
 
+template <int HB, int UNR, int OCC, int MODE>
+static void launch_half_m(uint32_t grid, const TermJob<float>* j, uint32_t njobs, int n, int lc, bool nts,
+                          hipStream_t st)
+{
+    if (nts)
+        hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, true, MODE>), dim3(grid), dim3(1024), 0, st, j, njobs, n,
+                           lc);
+    else
+        hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, false, MODE>), dim3(grid), dim3(1024), 0, st, j, njobs, n,
+                           lc);
+}
+
 template <int HB, int UNR, int OCC>
-static void launch_half_v(uint32_t grid, const void* jobs, uint32_t njobs, int n, int lc, bool nts, bool pair,
+static void launch_half_v(uint32_t grid, const void* jobs, uint32_t njobs, int n, int lc, bool nts, int mode,
                           hipStream_t st)
 {
     const TermJob<float>* j = (const TermJob<float>*)jobs;
-    if (pair) {
-        if (nts)
-            hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, true, true>), dim3(grid), dim3(1024), 0, st, j, njobs,
-                               n, lc);
-        else
-            hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, false, true>), dim3(grid), dim3(1024), 0, st, j, njobs,
-                               n, lc);
-    } else {
-        if (nts)
-            hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, true, false>), dim3(grid), dim3(1024), 0, st, j, njobs,
-                               n, lc);
-        else
-            hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, false, false>), dim3(grid), dim3(1024), 0, st, j, njobs,
-                               n, lc);
+    switch (mode) {
+    case 1: launch_half_m<HB, UNR, OCC, 1>(grid, j, njobs, n, lc, nts, st); break;
+    case 2: launch_half_m<HB, UNR, OCC, 2>(grid, j, njobs, n, lc, nts, st); break;
+    default: launch_half_m<HB, UNR, OCC, 0>(grid, j, njobs, n, lc, nts, st); break;
     }
 }
 
@@ -918,28 +946,29 @@ int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, 
     const int ob = n - hb;
     if (ob < 0 || !half_tile_bits_supported(hb)) return -2;
     if (variant < 0 || variant >= half_variants(hb)) variant = 0;
-    // a pair needs a top tile bit to pair across
-    const bool pair = variant >= 6 && ob >= 1;
+    int mode = variant / 6;
+    if (mode == 1 && ob < 1) mode = 0; // a pair needs a top tile bit to pair across
     const bool nts = (variant % 6) >= 3;
     const int v = variant % 3;
-    const int pb = ob - (pair ? 1 : 0);
-    if (lc > pb + 17) lc = pb + 17;
-    const uint64_t units = (uint64_t)njobs << (pb + 1);
+    const int pb = ob - (mode == 1 ? 1 : 0);
+    const int ub = mode == 2 ? pb : pb + 1; // unit bits per job
+    if (lc > ub + 16) lc = ub + 16;
+    const uint64_t units = (uint64_t)njobs << ub;
     const uint64_t per = 8ull << lc;
     const uint64_t grid64 = (units + per - 1) / per * per;
     if (grid64 > 0x7fffffffull) return -3;
     const uint32_t grid = (uint32_t)grid64;
     if (hb == 14) {
         switch (v) {
-        case 1: launch_half_v<14, 2, 4>(grid, jobs_dev, njobs, n, lc, nts, pair, st); break;
-        case 2: launch_half_v<14, 4, 4>(grid, jobs_dev, njobs, n, lc, nts, pair, st); break;
-        default: launch_half_v<14, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, pair, st); break;
+        case 1: launch_half_v<14, 2, 4>(grid, jobs_dev, njobs, n, lc, nts, mode, st); break;
+        case 2: launch_half_v<14, 4, 4>(grid, jobs_dev, njobs, n, lc, nts, mode, st); break;
+        default: launch_half_v<14, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, mode, st); break;
         }
     } else {
         switch (v) {
-        case 1: launch_half_v<13, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, pair, st); break;
-        case 2: launch_half_v<13, 2, 8>(grid, jobs_dev, njobs, n, lc, nts, pair, st); break;
-        default: launch_half_v<13, 4, 8>(grid, jobs_dev, njobs, n, lc, nts, pair, st); break;
+        case 1: launch_half_v<13, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, mode, st); break;
+        case 2: launch_half_v<13, 2, 8>(grid, jobs_dev, njobs, n, lc, nts, mode, st); break;
+        default: launch_half_v<13, 4, 8>(grid, jobs_dev, njobs, n, lc, nts, mode, st); break;
         }
     }
     return check_launch("half_term");

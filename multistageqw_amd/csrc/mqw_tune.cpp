@@ -37,7 +37,7 @@ std::vector<Cand> candidates(int n)
     for (int hb : {13, 14}) {
         if (hb > n) continue;
         for (int var = 0; var < half_variants(hb); var++)
-            for (int lc : {3, 4, 5, 6, 7, 8})
+            for (int lc : {3, 4, 5, 6, 7})
                 if (lc <= n - hb + 1) v.push_back({hb, var, lc});
     }
     if (v.empty()) v.push_back({std::min(n, 12), 0, 0});
@@ -86,11 +86,11 @@ extern "C" mqw_status mqw_tune(mqw_ctx* c, int n, const char* cache_path, int* t
     double best_gbs = 0.0;
     if (!have) {
         const size_t N = (size_t)1 << n;
-        // about 2^26 amplitudes per launch, like a batch of trajectories
-        const size_t T = std::max<size_t>(1, ((size_t)1 << 26) >> n);
+        // about 2^27 amplitudes per launch, like a batch of trajectories
+        const size_t T = std::max<size_t>(1, ((size_t)1 << 27) >> n);
         float* st = (float*)dev_get(c, "tune_state", sizeof(float) * 6 * N * T);
-        // a diagonal per instance, several trajectories per instance
-        const size_t NE = std::min<size_t>(T, 8);
+        // a diagonal per instance, dozens of trajectories (samples) per instance
+        const size_t NE = std::max<size_t>(1, std::min<size_t>(T / 32, 64));
         float* E = (float*)dev_get(c, "tune_E", sizeof(float) * N * NE);
         double* part = (double*)dev_get(c, "tune_part", sizeof(double) * 2 * N);
         TermJob<float>* jd = (TermJob<float>*)dev_get(c, "tune_jobs", sizeof(TermJob<float>) * 2 * T);

@@ -7,8 +7,9 @@
       one large instance: complex H|psi> applies/s and algorithmic GB/s.
   python -m multistageqw_amd.studies sweep --ns 14 16 18 20 22 24 26
       apply rate of the sweep kernel per n (one instance, a few samples).
-  python -m multistageqw_amd.studies uneven --count 256
-      the uneven workload (n uniform in 12..22, m uniform in 1..5), instances/s.
+  python -m multistageqw_amd.studies uneven --count 4096 --seed 3 --fp64-every 16 --tune-cache tune.txt
+      the uneven workload (n uniform in 12..22, m uniform in 1..5): instances/s,
+      bytes-weighted GB/s, per-instance error against FP64 runs of a subset.
 
 Each prints JSON lines.
 """
@@ -56,20 +57,47 @@ def large(ctx, n, m, samples, seed):
     return dict(study="large", m=m, samples=samples, p=float(p[0]), **_apply_line(ctx, n, time.time() - t0))
 
 
-def uneven(ctx, count, samples, seed):
+def uneven(ctx, count, samples, seed, chunk=256, fp64_every=0, tune_cache=None):
+    """The uneven workload: n ~ U{12..22}, m ~ U{1..5}, solved chunk by chunk
+    (each chunk one mqw_solve_batch call); every fp64_every-th instance is
+    also run with an FP64 state for the per-instance error."""
     rng = np.random.default_rng(seed)
     inst = []
     for k in range(count):
         n = int(rng.integers(12, 23))
         m = int(rng.integers(1, 6))
         inst.append((n, m, generate_sk(n, 1, seed * 100003 + k)[0]))
-    t0 = time.time()
-    p, _ = ctx.solve_batch(inst, samples=samples)
-    wall = time.time() - t0
-    st = ctx.last_stats()
-    return dict(study="uneven", count=count, samples=samples, instances_per_s=count / wall, wall_s=wall,
-                device_ms=st["gpu_ms"], algorithmic_GBps=st["algorithmic_bytes"] / (st["gpu_ms"] * 1e-3) / 1e9,
-                lds_trajectories=st["lds_trajectories"], mean_p=float(np.mean(p)))
+    tuned = {}
+    if tune_cache is not None:
+        for n in sorted({i[0] for i in inst}):
+            tuned[n] = ctx.tune(n, tune_cache)
+        print(json.dumps(dict(study="uneven", tuned=tuned)), flush=True)
+    probs = np.zeros(count, np.float32)
+    dev_ms = alg = 0.0
+    lds = 0
+    wall = 0.0
+    for c0 in range(0, count, chunk):
+        part = inst[c0:c0 + chunk]
+        t0 = time.time()
+        p, _ = ctx.solve_batch(part, samples=samples)
+        wall += time.time() - t0
+        st = ctx.last_stats()
+        dev_ms += st["gpu_ms"]
+        alg += st["algorithmic_bytes"]
+        lds += st["lds_trajectories"]
+        probs[c0:c0 + len(part)] = p
+        print(json.dumps(dict(study="uneven", done=c0 + len(part), wall_s=round(wall, 2))), flush=True)
+    out = dict(study="uneven", count=count, samples=samples, instances_per_s=count / wall, wall_s=wall,
+               device_ms=dev_ms, algorithmic_GBps=alg / (dev_ms * 1e-3) / 1e9, lds_trajectories=lds,
+               mean_p=float(np.mean(probs)))
+    if fp64_every:
+        idx = list(range(0, count, fp64_every))
+        t0 = time.time()
+        p64, _ = ctx.solve_batch([inst[i] for i in idx], samples=samples, precision=FP64)
+        d = np.abs(probs[idx] - p64)
+        out.update(fp64_instances=len(idx), fp64_wall_s=time.time() - t0, max_abs_err_vs_fp64=float(d.max()),
+                   mean_abs_err_vs_fp64=float(d.mean()))
+    return out
 
 
 def main(argv=None):
@@ -81,6 +109,9 @@ def main(argv=None):
     ap.add_argument("--samples", type=int, default=8)
     ap.add_argument("--count", type=int, default=256)
     ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--chunk", type=int, default=256)
+    ap.add_argument("--fp64-every", type=int, default=0)
+    ap.add_argument("--tune-cache", default=None)
     a = ap.parse_args(argv)
     ctx = Context(0)
     if a.study == "precision":
@@ -91,7 +122,7 @@ def main(argv=None):
         for n in a.ns:
             print(json.dumps(large(ctx, n, a.m, a.samples, a.seed)), flush=True)
     else:
-        print(json.dumps(uneven(ctx, a.count, a.samples, a.seed)), flush=True)
+        print(json.dumps(uneven(ctx, a.count, a.samples, a.seed, a.chunk, a.fp64_every, a.tune_cache)), flush=True)
     ctx.close()
     return 0
 

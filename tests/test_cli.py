@@ -11,8 +11,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FIX = os.path.join(ROOT, "tests", "golden", "sk_n10_m2_seed1.bin")
 
 
-def _run(args, cwd):
-    return subprocess.run([M.CLI_PATH, *map(str, args)], cwd=cwd, capture_output=True, text=True, timeout=300)
+def _run(args, cwd, env=None):
+    e = dict(os.environ, **(env or {}))
+    return subprocess.run([M.CLI_PATH, *map(str, args)], cwd=cwd, capture_output=True, text=True, timeout=300, env=e)
 
 
 def test_cli_usage_error(tmp_path):
@@ -51,3 +52,17 @@ def test_cli_end_to_end_matches_oracle(tmp_path, oracle):
     r2 = _run([10, 2, FIX, 0, 2], tmp_path)
     assert r2.returncode == 0
     assert np.fromfile(str(tmp_path / "output_10_2"), "<f4").size == 14
+
+
+@pytest.mark.gpu
+def test_cli_incremental_flush(tmp_path):
+    """MQW_FLUSH=1 appends per batch; the file holds the same floats as one block at exit."""
+    print("[case] cli flush n=10 m=2 slice 0..12", flush=True)
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    r1 = _run([10, 2, FIX, 0, 12], a)
+    r2 = _run([10, 2, FIX, 0, 12], b, env={"MQW_FLUSH": "1", "MQW_BATCH": "5"})
+    assert r1.returncode == 0 and r2.returncode == 0, r2.stderr
+    assert r1.stdout == r2.stdout
+    assert (a / "output_10_2").read_bytes() == (b / "output_10_2").read_bytes()

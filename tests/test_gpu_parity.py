@@ -111,6 +111,7 @@ PATHS = {
     "half14": dict(small_n_max=0, tile_bits=14),          # real-half sweep kernel, 2^14 tiles
     "halfdirect": dict(small_n_max=0, tile_bits=13, use_graph=False, group_bytes=48 << 20),  # small groups
     "halfpaired": dict(small_n_max=0, tile_bits=13, variant=10),  # tile pairs across the top tile bit
+    "halfboth": dict(small_n_max=0, tile_bits=13, variant=16),    # both halves of a tile per workgroup
 }
 
 
@@ -150,10 +151,11 @@ def test_solve_fp64_vs_dense(ctx, oracle):
             assert np.max(np.abs(sp[k] - ref)) < 5e-6
 
 
-def test_tile_pairs_are_bit_identical(ctx):
-    """Pairing tiles across the top tile bit keeps every sum in the same order."""
-    _say("tile pairs vs single tiles n=17 m=2")
+@pytest.mark.parametrize("variant", [10, 16, 7, 13])
+def test_workgroup_modes_are_bit_identical(ctx, variant):
+    """Tile pairs (variants 6-11) and both-halves workgroups (12-17) keep every sum in the same order."""
+    _say("sweep variant", variant, "vs variant 4, n=17 m=2")
     recs = M.generate_sk(17, 2, 77)
     _, _, a = ctx.solve(17, 2, recs, samples=6, with_samples=True, small_n_max=0, tile_bits=13, variant=4)
-    _, _, b = ctx.solve(17, 2, recs, samples=6, with_samples=True, small_n_max=0, tile_bits=13, variant=10)
+    _, _, b = ctx.solve(17, 2, recs, samples=6, with_samples=True, small_n_max=0, tile_bits=13, variant=variant)
     assert np.array_equal(a, b)
