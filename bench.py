@@ -42,7 +42,7 @@ def main() -> int:
     ap.add_argument("--apply-n", type=int, default=24, help="n of the apply-rate measurement (0 = skip)")
     ap.add_argument("--apply-samples", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores); -1 = skip")
-    ap.add_argument("--cpu-samples", type=int, default=4)
+    ap.add_argument("--cpu-samples", type=int, default=8)
     args = ap.parse_args()
 
     if args.gpus != 1:
