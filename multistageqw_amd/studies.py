@@ -49,9 +49,10 @@ def precision(ctx, n, m, samples, seed):
                 fp32=r32, fp64=r64)
 
 
-def large(ctx, n, m, samples, seed):
+def large(ctx, n, m, samples, seed, warm=True):
     rec = generate_sk(n, 1, seed)
-    ctx.solve(n, m, rec, samples=1)
+    if warm:
+        ctx.solve(n, m, rec, samples=1)
     t0 = time.time()
     p, _ = ctx.solve(n, m, rec, samples=samples)
     return dict(study="large", m=m, samples=samples, p=float(p[0]), **_apply_line(ctx, n, time.time() - t0))
@@ -112,12 +113,13 @@ def main(argv=None):
     ap.add_argument("--chunk", type=int, default=256)
     ap.add_argument("--fp64-every", type=int, default=0)
     ap.add_argument("--tune-cache", default=None)
+    ap.add_argument("--no-warm", action="store_true", help="large: skip the warm-up solve (one solve to profile)")
     a = ap.parse_args(argv)
     ctx = Context(0)
     if a.study == "precision":
         print(json.dumps(precision(ctx, a.n, a.m, a.samples, a.seed)), flush=True)
     elif a.study == "large":
-        print(json.dumps(large(ctx, a.n, a.m, a.samples, a.seed)), flush=True)
+        print(json.dumps(large(ctx, a.n, a.m, a.samples, a.seed, warm=not a.no_warm)), flush=True)
     elif a.study == "sweep":
         for n in a.ns:
             print(json.dumps(large(ctx, n, a.m, a.samples, a.seed)), flush=True)
