@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 2
+#define MQW_ABI_VERSION 3
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -86,6 +86,10 @@ typedef struct {
                                 trajectories share every sweep launch */
     int variant;             /* -1 = tuned/default; else a sweep-kernel variant (loads in flight) */
     float* sample_probs;     /* optional output: count*samples per-sample probabilities */
+    int high_pass;           /* -1 = by n (default); 0 = off; 1, 2, 3 = on with pencil rows of 64, 128,
+                                256 B: before each sweep a pass sums the flips over the top 10, 9, 8
+                                bits in on-chip pencils, so the sweep loads no partner tiles for them
+                                (FP32 real-half sweeps, n >= 17) */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are

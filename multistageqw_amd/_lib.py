@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmqw.so")
 CLI_PATH = os.path.join(HERE, "mqw")
 
-ABI_VERSION = 2  # MQW_ABI_VERSION of include/mqw.h this binding matches
+ABI_VERSION = 3  # MQW_ABI_VERSION of include/mqw.h this binding matches
 MAX_STAGES = 64
 DEFAULT_SAMPLES = 100
 DEFAULT_TOL = 1e-6
@@ -56,7 +56,7 @@ class Opts(C.Structure):
     _fields_ = [("samples", C.c_int), ("tol", C.c_double), ("precision", C.c_int), ("energy_mode", C.c_int),
                 ("tile_bits", C.c_int), ("chunk_bits", C.c_int), ("small_n_max", C.c_int),
                 ("use_graph", C.c_int), ("max_device_bytes", C.c_size_t), ("group_bytes", C.c_size_t), ("variant", C.c_int),
-                ("sample_probs", C.POINTER(C.c_float))]
+                ("sample_probs", C.POINTER(C.c_float)), ("high_pass", C.c_int)]
 
 
 class Instance(C.Structure):

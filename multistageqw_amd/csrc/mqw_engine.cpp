@@ -68,6 +68,11 @@ void mqw::release_bufs(mqw_ctx* c)
     c->bufs.clear();
 }
 
+// Smallest n at which the high-bit pass is on by default, and its row width
+// (128 B rows: measured best of 64, 128 and 256 B at n = 23..28).
+static constexpr int kHighPassMinN = 25;
+static constexpr int kHighPassWb = 1;
+
 static int default_tile_bits(int precision, int n)
 {
     const int tb = precision == MQW_FP32 ? 12 : 11;
@@ -83,7 +88,8 @@ static int default_chunk_bits(int n, int tb)
 // Which sweep kernel runs a group: the real-half kernel (FP32, n >= its tile
 // bits) or the complex-tile kernel.  `tb` is the tile bits of that kernel and
 // `parts` the number of partial norm sums one output leaves.
-SweepCfg mqw::choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits, int chunk_bits, int variant)
+SweepCfg mqw::choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits, int chunk_bits, int variant,
+                           int high_pass)
 {
     SweepCfg k;
     k.variant = variant >= 0 ? variant : c->tuned_var[n];
@@ -93,7 +99,8 @@ SweepCfg mqw::choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits
     // 2^31/2^n trajectories (at most 1600): real-half tiles, non-temporal
     // output stores; 2^14-float tiles with one partner load in flight while
     // the outer tile bits fit a 2^5..2^6-tile XCD chunk (n <= 21), 2^13
-    // tiles with two loads in flight above.
+    // tiles with two loads in flight above, and 2^14 tiles again once the
+    // high-bit pass takes the top bits (n >= kHighPassMinN).
     if (tb <= 0 && precision == MQW_FP32 && n >= 14) {
         int dv;
         if (n == 14) {
@@ -102,8 +109,10 @@ SweepCfg mqw::choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits
             tb = 14, dv = 3, dflt_lc = 5;
         } else if (n <= 21) {
             tb = 14, dv = 3, dflt_lc = 6;
-        } else {
+        } else if (n < kHighPassMinN || high_pass == 0) {
             tb = 13, dv = 4, dflt_lc = 5;
+        } else {
+            tb = 14, dv = 4, dflt_lc = 5;
         }
         if (variant < 0) k.variant = dv;
     }
@@ -122,6 +131,17 @@ SweepCfg mqw::choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits
         k.lc = chunk_bits >= 0 ? chunk_bits : (c->tuned_lc[n] >= 0 ? c->tuned_lc[n] : default_chunk_bits(n, tb));
     }
     k.lc = std::max(0, std::min(k.lc, 24));
+    // High-bit pass: from n = kHighPassMinN by default (measured), or as asked
+    // (high_pass = 1, 2, 3: pencil rows of 64, 128, 256 B).  It takes up to
+    // 10 - wb of the outer bits; the sweep keeps partner loads for the rest.
+    if (k.half && high_pass != 0) {
+        const int wb = high_pass > 0 ? std::min(high_pass, 3) - 1 : kHighPassWb;
+        const int hi = std::min(10 - wb, n - k.tb);
+        if ((high_pass > 0 || n >= kHighPassMinN) && high_pass_supported(n, hi, wb)) {
+            k.hi = hi;
+            k.hwb = wb;
+        }
+    }
     return k;
 }
 
@@ -224,7 +244,7 @@ mqw_status group_energies(mqw_ctx* c, int n, std::vector<Problem*>& ps, int mode
 // (F_READOUT); a reduce launch per group reads out the last stage.
 // ---------------------------------------------------------------------------
 struct Op {
-    int kind; // 0 = term launch, 1 = reduce launch
+    int kind; // 0 = term launch, 1 = reduce launch, 2 = high-bit pass over the jobs of the next term launch
     size_t off;
     uint32_t count;
 };
@@ -245,11 +265,12 @@ template <typename real>
 struct Slot {
     real* st[3];
     double* part[2];
+    real* hs = nullptr; // high-bit sums of the current term's s1 (planar), when the pass is on
 };
 
 template <typename real>
 void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, const real* E, uint32_t eloc,
-               double* probs, std::vector<TermJob<real>>& out, ReduceJob& fin, bool& has_fin)
+               double* probs, std::vector<TermJob<real>>& out, ReduceJob& fin, bool& has_fin, int hi = 0)
 {
     const size_t N = (size_t)1 << n;
     const real psi0 = (real)(float)(1 / sqrt((double)N));
@@ -314,6 +335,11 @@ void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, cons
                 j.alpha = (real)(1.0 / sc);
                 j.beta = (real)((double)sg.gamma / sc);
                 if (r == 0) flags |= F_FINAL;
+                // s1 is an array (not |+>): its high-bit sums come from the pass
+                if (hi > 0 && !(d == 0 && pconst)) {
+                    flags |= F_HSUM;
+                    j.hsum = sl.hs;
+                }
             }
             if (flags & F_FINAL) j.norm_part = sl.part[par];
             j.flags = flags;
@@ -345,7 +371,7 @@ void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, cons
 template <typename real>
 void build_plan(int n, uint32_t parts, const std::vector<Traj>& trajs, const std::vector<std::vector<size_t>>& groups,
                 const std::vector<Problem*>& ps, const real* E, const std::vector<Slot<real>>& slots, double* probs,
-                Plan<real>& P)
+                Plan<real>& P, int hi = 0)
 {
     const size_t N = (size_t)1 << n;
     for (const auto& grp : groups) {
@@ -357,20 +383,23 @@ void build_plan(int n, uint32_t parts, const std::vector<Traj>& trajs, const std
             ReduceJob fin;
             bool has = false;
             traj_jobs<real>(n, parts, tr, slots[i], E + (size_t)tr.prob * N, ps[tr.prob]->info.eloc, probs, seq[i],
-                            fin, has);
+                            fin, has, hi);
             if (has) fins.push_back(fin);
             L = std::max(L, seq[i].size());
             P.terms += seq[i].size();
         }
         for (size_t q = 0; q < L; q++) {
             Op op{0, P.jobs.size(), 0};
+            bool any_hsum = false;
             for (size_t i = 0; i < grp.size(); i++)
                 if (q < seq[i].size()) {
                     P.jobs.push_back(seq[i][q]);
                     op.count++;
                     P.amp_terms += N;
                     if (seq[i][q].flags & F_PSI_CONST) P.psi_const_amp_terms += N;
+                    any_hsum |= (seq[i][q].flags & F_HSUM) != 0;
                 }
+            if (any_hsum) P.ops.push_back(Op{2, op.off, op.count});
             P.ops.push_back(op);
             P.sweeps++;
         }
@@ -400,8 +429,10 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
             int rc;
             if (op.kind == 1)
                 rc = launch_reduce(precision, rd + op.off, op.count, c->stream);
+            else if (op.kind == 2)
+                rc = launch_high_pass(k.hi, k.hwb, jd + op.off, op.count, n, c->stream);
             else if (k.half)
-                rc = launch_half_term(k.tb, k.variant, jd + op.off, op.count, n, k.lc, c->stream);
+                rc = launch_half_term(k.tb, k.variant, jd + op.off, op.count, n, k.lc, k.hi, c->stream);
             else
                 rc = launch_term(precision, k.tb, k.variant, jd + op.off, op.count, n, k.lc, c->stream);
             if (rc) return rc;
@@ -533,7 +564,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     const size_t N = (size_t)1 << n;
     const int precision = o.precision;
     const int samples = o.samples;
-    const SweepCfg k = choose_sweep(c, precision, n, o.tile_bits, o.chunk_bits, o.variant);
+    const SweepCfg k = choose_sweep(c, precision, n, o.tile_bits, o.chunk_bits, o.variant, o.high_pass);
     real* E = (real*)dev_get(c, "E_group", sizeof(real) * N * ps.size());
     if (!E) return fail(c, MQW_ENOMEM, "device allocation failed for %zu diagonals at n=%d", ps.size(), n);
     mqw_status st = group_energies(c, n, ps, o.energy_mode, precision, E);
@@ -588,7 +619,8 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
 
     // groups sized to the memory-side cache, longest trajectories first
     const uint32_t nparts = k.parts;
-    const size_t per_traj = sizeof(real) * 6 * N + sizeof(double) * 2 * nparts;
+    const size_t nst = k.hi > 0 ? 8 : 6; // planar states per slot (+ the high-bit sums)
+    const size_t per_traj = sizeof(real) * nst * N + sizeof(double) * 2 * nparts;
     const size_t budget = o.group_bytes ? o.group_bytes : ((size_t)32 << 30);
     size_t G = std::max<size_t>(1, budget / per_traj);
     if (o.max_device_bytes) G = std::min(G, std::max<size_t>(1, o.max_device_bytes / per_traj));
@@ -603,12 +635,13 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     std::vector<std::vector<size_t>> groups;
     for (size_t i = 0; i < order.size(); i += G)
         groups.emplace_back(order.begin() + i, order.begin() + std::min(order.size(), i + G));
-    real* state = (real*)dev_get(c, "state", sizeof(real) * 6 * N * G);
+    real* state = (real*)dev_get(c, "state", sizeof(real) * nst * N * G);
     double* parts = (double*)dev_get(c, "norm_part", sizeof(double) * 2 * nparts * G);
     if (!state || !parts) return fail(c, MQW_ENOMEM, "device allocation failed for %zu trajectory slots at n=%d", G, n);
     std::vector<Slot<real>> slots(G);
     for (size_t i = 0; i < G; i++) {
         for (int q = 0; q < 3; q++) slots[i].st[q] = state + (i * 3 + (size_t)q) * 2 * N;
+        if (k.hi > 0) slots[i].hs = state + (3 * G + i) * 2 * N;
         for (int q = 0; q < 2; q++) slots[i].part[q] = parts + (i * 2 + (size_t)q) * nparts;
     }
     // bound the host job table: run the groups in batches
@@ -625,7 +658,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         }
         std::vector<std::vector<size_t>> batch(groups.begin() + g0, groups.begin() + g1);
         Plan<real> plan;
-        build_plan<real>(n, nparts, trajs, batch, ps, E, slots, probs_d, plan);
+        build_plan<real>(n, nparts, trajs, batch, ps, E, slots, probs_d, plan, k.hi);
         st = run_plan<real>(c, precision, n, k, plan, o.use_graph != 0, &c->stats.gpu_ms);
         if (st) return st;
         c->stats.sweeps += plan.sweeps;
@@ -740,6 +773,7 @@ void mqw_opts_default(mqw_opts* o)
     o->max_device_bytes = 0;
     o->group_bytes = 0;
     o->variant = -1;
+    o->high_pass = -1;
     o->sample_probs = nullptr;
 }
 
@@ -974,7 +1008,7 @@ mqw_status mqw_clenshaw(mqw_ctx* c, int n, const float* coeffs, int K, float* ps
     if (!(scale > 0.0f) || !isfinite(scale) || !isfinite(gamma)) return fail(c, MQW_EINVAL, "bad scale");
     hipSetDevice(c->device);
     const size_t N = (size_t)1 << n;
-    const SweepCfg k = choose_sweep(c, MQW_FP32, n, 0, -1);
+    const SweepCfg k = choose_sweep(c, MQW_FP32, n, 0, -1, -1, 0);
     const uint32_t ntiles = k.parts;
     float* E = (float*)dev_get(c, "cl_E", sizeof(float) * N);
     float* state = (float*)dev_get(c, "cl_state", sizeof(float) * 8 * N);

@@ -60,6 +60,7 @@ enum : uint32_t {
     F_FINAL = 1u << 5,     // write the tile's sum |out|^2 to norm_part[tile]
     F_NORM_IN = 1u << 6,   // Psi's norm from norm_in (ntiles partial sums)
     F_READOUT = 1u << 7,   // tile 0 writes |Psi[eloc]|^2 / ||Psi||^2 to *prob
+    F_HSUM = 1u << 8,      // s1's flips over its top `hi` bits are read from hsum
 };
 
 template <typename real>
@@ -72,6 +73,7 @@ struct TermJob {
     const double* norm_in;   // F_NORM_IN: partial sums of |psi|^2, one per tile
     double* norm_part;       // F_FINAL: one double per tile of out
     double* prob;            // F_READOUT
+    real* hsum;              // F_HSUM: planar sums of s1 over flips of the top hi bits
     real alpha, beta;
     real w1r, w1i, w2r, w2i;
     real cr, ci;
@@ -120,7 +122,14 @@ int term_variants();
 // sums are laid out [half][tile], 2 * 2^(n-hb) per output.
 int half_tile_bits_supported(int hb);
 int half_variants(int hb);
-int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc, void* stream);
+// hi > 0: jobs flagged F_HSUM take the flips of the top hi bits from hsum
+// (written by launch_high_pass) instead of partner-tile loads.
+int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc, int hi, void* stream);
+// High-bit pass (FP32): for every job flagged F_HSUM, hsum = sum over the top
+// hi bits b of s1[k ^ 2^b], both real halves, in pencils of 64 * 2^wb B rows;
+// 4 <= hi, hi + wb <= 10, wb <= 2, n - hi >= 4 + wb.
+int high_pass_supported(int n, int hi, int wb);
+int launch_high_pass(int hi, int wb, const void* jobs_dev, uint32_t njobs, int n, void* stream);
 int launch_reduce(int precision, const ReduceJob* jobs_dev, uint32_t njobs, void* stream);
 int launch_half_step(const float* b1, float* b2, const float* hp, const float* psi, int n, float alpha,
                      float beta, float coef, void* stream);
@@ -163,6 +172,8 @@ struct SweepCfg {
     int lc = 0;        // XCD chunk bits
     int variant = 0;   // loads-in-flight variant of the real-half kernel
     uint32_t parts = 1; // partial norm sums per output
+    int hi = 0;        // top bits summed by the high-bit pass (0 = none)
+    int hwb = 0;       // its pencil rows are 64 * 2^hwb bytes
 };
 
 } // namespace mqw
@@ -183,7 +194,8 @@ namespace mqw {
 mqw_status fail(mqw_ctx* c, mqw_status s, const char* fmt, ...);
 void* dev_get(mqw_ctx* c, const std::string& name, size_t bytes);
 void release_bufs(mqw_ctx* c);
-SweepCfg choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits, int chunk_bits, int variant = -1);
+SweepCfg choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits, int chunk_bits, int variant = -1,
+                      int high_pass = 0);
 } // namespace mqw
 
 #define HIPCHK(ctx, expr)                                                                                   \

@@ -707,7 +707,7 @@ int launch_term(int precision, int tb, int variant, const void* jobs_dev, uint32
 // ===========================================================================
 template <int HB, int UNR, int OCC, bool NTS, int MODE>
 __global__ void __launch_bounds__(1024, OCC)
-half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n, int lc)
+half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n, int lc, int hi)
 {
     constexpr int NT = 1024;
     constexpr int RB = HB - 10;
@@ -829,9 +829,11 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
             for (int b = 0; b < RB; b++)
 #pragma unroll
                 for (int q = 0; q < R; q++) acc[q] += x[q ^ (1 << b)];
-            // partner tiles in bit order, UNR loads in flight per thread
+            // partner tiles in bit order, UNR loads in flight per thread; with
+            // F_HSUM the top hi bits come from the high-bit pass instead
+            const int pl = (flags & F_HSUM) ? pb - hi : pb;
             int j = 0;
-            for (; j + UNR <= pb; j += UNR) {
+            for (; j + UNR <= pl; j += UNR) {
                 float y[UNR][R];
 #pragma unroll
                 for (int u = 0; u < UNR; u++)
@@ -841,9 +843,15 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
 #pragma unroll
                     for (int q = 0; q < R; q++) acc[q] += y[u][q];
             }
-            for (; j < pb; j++) {
+            for (; j < pl; j++) {
                 float y[R];
                 vload<float, R>(src + ((tile ^ (1ull << j)) << HB) + (uint64_t)t * 4, y, PS);
+#pragma unroll
+                for (int q = 0; q < R; q++) acc[q] += y[q];
+            }
+            if (flags & F_HSUM) {
+                float y[R];
+                vload_nt<float, R>(J.hsum + (uint64_t)h1 * N + off, y, PS);
 #pragma unroll
                 for (int q = 0; q < R; q++) acc[q] += y[q];
             }
@@ -917,37 +925,39 @@ int half_variants(int hb) { return 18; }
 int term_variants() { return 2; }
 
 template <int HB, int UNR, int OCC, int MODE>
-static void launch_half_m(uint32_t grid, const TermJob<float>* j, uint32_t njobs, int n, int lc, bool nts,
+static void launch_half_m(uint32_t grid, const TermJob<float>* j, uint32_t njobs, int n, int lc, bool nts, int hi,
                           hipStream_t st)
 {
     if (nts)
         hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, true, MODE>), dim3(grid), dim3(1024), 0, st, j, njobs, n,
-                           lc);
+                           lc, hi);
     else
         hipLaunchKernelGGL((half_term_kernel<HB, UNR, OCC, false, MODE>), dim3(grid), dim3(1024), 0, st, j, njobs, n,
-                           lc);
+                           lc, hi);
 }
 
 template <int HB, int UNR, int OCC>
 static void launch_half_v(uint32_t grid, const void* jobs, uint32_t njobs, int n, int lc, bool nts, int mode,
-                          hipStream_t st)
+                          int hi, hipStream_t st)
 {
     const TermJob<float>* j = (const TermJob<float>*)jobs;
     switch (mode) {
-    case 1: launch_half_m<HB, UNR, OCC, 1>(grid, j, njobs, n, lc, nts, st); break;
-    case 2: launch_half_m<HB, UNR, OCC, 2>(grid, j, njobs, n, lc, nts, st); break;
-    default: launch_half_m<HB, UNR, OCC, 0>(grid, j, njobs, n, lc, nts, st); break;
+    case 1: launch_half_m<HB, UNR, OCC, 1>(grid, j, njobs, n, lc, nts, hi, st); break;
+    case 2: launch_half_m<HB, UNR, OCC, 2>(grid, j, njobs, n, lc, nts, hi, st); break;
+    default: launch_half_m<HB, UNR, OCC, 0>(grid, j, njobs, n, lc, nts, hi, st); break;
     }
 }
 
-int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc, void* stream)
+int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc, int hi, void* stream)
 {
     hipStream_t st = (hipStream_t)stream;
     const int ob = n - hb;
     if (ob < 0 || !half_tile_bits_supported(hb)) return -2;
+    if (hi < 0 || hi > ob) return -2;
     if (variant < 0 || variant >= half_variants(hb)) variant = 0;
     int mode = variant / 6;
-    if (mode == 1 && ob < 1) mode = 0; // a pair needs a top tile bit to pair across
+    // a pair needs a top tile bit to pair across, and one the high-bit pass leaves
+    if (mode == 1 && (ob < 1 || hi > 0)) mode = 0;
     const bool nts = (variant % 6) >= 3;
     const int v = variant % 3;
     const int pb = ob - (mode == 1 ? 1 : 0);
@@ -960,18 +970,123 @@ int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, 
     const uint32_t grid = (uint32_t)grid64;
     if (hb == 14) {
         switch (v) {
-        case 1: launch_half_v<14, 2, 4>(grid, jobs_dev, njobs, n, lc, nts, mode, st); break;
-        case 2: launch_half_v<14, 4, 4>(grid, jobs_dev, njobs, n, lc, nts, mode, st); break;
-        default: launch_half_v<14, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, mode, st); break;
+        case 1: launch_half_v<14, 2, 4>(grid, jobs_dev, njobs, n, lc, nts, mode, hi, st); break;
+        case 2: launch_half_v<14, 4, 4>(grid, jobs_dev, njobs, n, lc, nts, mode, hi, st); break;
+        default: launch_half_v<14, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, mode, hi, st); break;
         }
     } else {
         switch (v) {
-        case 1: launch_half_v<13, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, mode, st); break;
-        case 2: launch_half_v<13, 2, 8>(grid, jobs_dev, njobs, n, lc, nts, mode, st); break;
-        default: launch_half_v<13, 4, 8>(grid, jobs_dev, njobs, n, lc, nts, mode, st); break;
+        case 1: launch_half_v<13, 1, 8>(grid, jobs_dev, njobs, n, lc, nts, mode, hi, st); break;
+        case 2: launch_half_v<13, 2, 8>(grid, jobs_dev, njobs, n, lc, nts, mode, hi, st); break;
+        default: launch_half_v<13, 4, 8>(grid, jobs_dev, njobs, n, lc, nts, mode, hi, st); break;
         }
     }
     return check_launch("half_term");
+}
+
+// ===========================================================================
+// High-bit pass (FP32).  For every job flagged F_HSUM, hsum[k] = sum over the
+// top H bits b of s1[k ^ 2^b], both real halves.  One workgroup takes a pencil
+// of 16 * 2^WB consecutive amplitudes (a row of 64 * 2^WB bytes) by all 2^H
+// values of the top bits: the 2^H rows are staged in 64 KiB of LDS, two row
+// bits are flipped in registers and the other H - 2 through LDS.  The sweep
+// that follows reads hsum instead of loading the partner tiles of those bits,
+// which at large n miss L2.
+// ===========================================================================
+template <int H, int WB>
+__global__ void __launch_bounds__(1 << (H + WB))
+high_pass_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n)
+{
+    using f4 = NVec16<float>::type;
+    constexpr int ROWS = 1 << H;
+    constexpr int RG = ROWS / 4;      // row groups; thread t holds rows q*RG + rg, q < 4
+    constexpr int CW = 4 << WB;       // float4 columns per row
+    constexpr int CB = 4 + WB;        // amplitude bits inside a row
+    __shared__ f4 s_rows[ROWS * CW];
+
+    const int p0 = n - H; // the first of the top bits
+    const uint64_t npen = 1ull << (p0 - CB);
+    // (job, half, pencil); runs of 8 consecutive pencils on one XCD, so rows
+    // that share a 128 B line are fetched into the same L2
+    const uint64_t unit = xcd_unit(blockIdx.x, 3);
+    const uint64_t job = unit / (2 * npen);
+    if (job >= njobs) return;
+    const TermJob<float>& J = jobs[job];
+    if (!(J.flags & F_HSUM)) return;
+    const uint64_t rem = unit - job * 2 * npen;
+    const uint64_t h = rem / npen, pen = rem - h * npen;
+    const uint64_t N = 1ull << n;
+    const float* src = (((J.flags & 3u) == SRC1_PSI) ? J.psi : J.b1) + h * N + (pen << CB);
+    float* dst = J.hsum + h * N + (pen << CB);
+    const int t = threadIdx.x;
+    const int col = t & (CW - 1), rg = t >> (2 + WB);
+
+    f4 x[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        x[q] = *gview(reinterpret_cast<const f4*>(src + ((uint64_t)(q * RG + rg) << p0)) + col);
+#pragma unroll
+    for (int q = 0; q < 4; q++) s_rows[(q * RG + rg) * CW + col] = x[q];
+    // row bits H-2 and H-1 (the q bits) in registers
+    f4 acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[q] = x[q ^ 1] + x[q ^ 2];
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < H - 2; b++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[q] += s_rows[(q * RG + (rg ^ (1 << b))) * CW + col];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        __builtin_nontemporal_store(acc[q], gview(reinterpret_cast<f4*>(dst + ((uint64_t)(q * RG + rg) << p0)) + col));
+}
+
+// wb: row of 64 * 2^wb bytes; hi + wb <= 10 keeps the rows in 64 KiB of LDS
+int high_pass_supported(int n, int hi, int wb)
+{
+    return wb >= 0 && wb <= 2 && hi >= 4 && hi + wb <= 10 && n - hi >= 4 + wb;
+}
+
+template <int H, int WB>
+static void launch_hp(uint64_t grid, const void* jobs, uint32_t njobs, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL((high_pass_kernel<H, WB>), dim3((uint32_t)grid), dim3(1 << (H + WB)), 0, st,
+                       (const TermJob<float>*)jobs, njobs, n);
+}
+
+template <int WB>
+static void launch_hp_w(int hi, uint64_t grid, const void* jobs, uint32_t njobs, int n, hipStream_t st)
+{
+    switch (hi) {
+    case 4: launch_hp<4, WB>(grid, jobs, njobs, n, st); break;
+    case 5: launch_hp<5, WB>(grid, jobs, njobs, n, st); break;
+    case 6: launch_hp<6, WB>(grid, jobs, njobs, n, st); break;
+    case 7: launch_hp<7, WB>(grid, jobs, njobs, n, st); break;
+    case 8: launch_hp<8, WB>(grid, jobs, njobs, n, st); break;
+    case 9:
+        if constexpr (WB <= 1) launch_hp<9, WB>(grid, jobs, njobs, n, st);
+        break;
+    default:
+        if constexpr (WB == 0) launch_hp<10, WB>(grid, jobs, njobs, n, st);
+        break;
+    }
+}
+
+int launch_high_pass(int hi, int wb, const void* jobs_dev, uint32_t njobs, int n, void* stream)
+{
+    if (!high_pass_supported(n, hi, wb)) return -2;
+    hipStream_t st = (hipStream_t)stream;
+    const uint64_t units = (uint64_t)njobs * 2ull * (1ull << (n - hi - 4 - wb));
+    if (units == 0) return 0;
+    const uint64_t grid = (units + 63) / 64 * 64; // whole XCD runs of 8 pencils
+    if (grid > 0x7fffffffull) return -3;
+    if (wb == 2)
+        launch_hp_w<2>(hi, grid, jobs_dev, njobs, n, st);
+    else if (wb == 1)
+        launch_hp_w<1>(hi, grid, jobs_dev, njobs, n, st);
+    else
+        launch_hp_w<0>(hi, grid, jobs_dev, njobs, n, st);
+    return check_launch("high_pass");
 }
 
 // ===========================================================================

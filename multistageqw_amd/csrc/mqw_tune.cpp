@@ -128,7 +128,7 @@ extern "C" mqw_status mqw_tune(mqw_ctx* c, int n, const char* cache_path, int* t
             const bool half = cd.tb >= 13;
             auto launch = [&](int q) {
                 const void* jobs = jd + (size_t)(q & 1) * T;
-                return half ? launch_half_term(cd.tb, cd.variant, jobs, (uint32_t)T, n, cd.lc, c->stream)
+                return half ? launch_half_term(cd.tb, cd.variant, jobs, (uint32_t)T, n, cd.lc, 0, c->stream)
                             : launch_term(MQW_FP32, cd.tb, cd.variant, jobs, (uint32_t)T, n, cd.lc, c->stream);
             };
             if (launch(0) || launch(1)) continue;

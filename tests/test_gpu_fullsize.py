@@ -39,10 +39,12 @@ def _j0_record(n, seed):
     return rec
 
 
+@pytest.mark.parametrize("high_pass", [-1, 0])
 @pytest.mark.parametrize("n,m,samples", [(24, 2, 2), (28, 2, 1)])
-def test_fullsize_j0_closed_form(ctx, oracle, n, m, samples):
-    """n = 24 and n = 28 (the large single-instance size): GPU walk vs the product formula."""
-    _say("J=0 closed form n", n, "m", m, "samples", samples)
+def test_fullsize_j0_closed_form(ctx, oracle, n, m, samples, high_pass):
+    """n = 24 and n = 28 (the large single-instance size): GPU walk vs the product formula,
+    with the default sweep (the high-bit pass from n = 25) and with partner loads only."""
+    _say("J=0 closed form n", n, "m", m, "samples", samples, "high_pass", high_pass)
     rec = _j0_record(n, 40 + n)
     J, h = oracle.unpack(n, rec)
     sch = oracle.schedule(n, m, J, h)
@@ -52,7 +54,7 @@ def test_fullsize_j0_closed_form(ctx, oracle, n, m, samples):
     eloc = sum(b << i for i, b in enumerate(bits))
     _, info = ctx.energy(n, rec, M.ENERGY_COMPAT)
     assert info["eloc"] == eloc
-    _, tt, sp = ctx.solve(n, m, rec[None, :], samples=samples, with_samples=True)
+    _, tt, sp = ctx.solve(n, m, rec[None, :], samples=samples, with_samples=True, high_pass=high_pass)
     assert tt[0] == sch["total_t"]
     for j in range(samples):
         ref = D.product_probability(h.astype(np.float64), sch["gamma"].astype(np.float64), times[:, j], bits)

@@ -159,3 +159,20 @@ def test_workgroup_modes_are_bit_identical(ctx, variant):
     _, _, a = ctx.solve(17, 2, recs, samples=6, with_samples=True, small_n_max=0, tile_bits=13, variant=4)
     _, _, b = ctx.solve(17, 2, recs, samples=6, with_samples=True, small_n_max=0, tile_bits=13, variant=variant)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,m,tb,samples,hp", [(17, 2, 13, 6, 1), (19, 3, 13, 3, 1), (21, 2, 14, 2, 1),
+                                                (18, 1, 14, 6, 1), (19, 2, 13, 3, 2), (21, 3, 13, 2, 3)])
+def test_high_bit_pass_vs_oracle(ctx, oracle, n, m, tb, samples, hp):
+    """The sweep with the high-bit pass: the top min(11 - hp, n - tb) bits summed in LDS pencils
+    with rows of 32 * 2^hp bytes."""
+    _say("high-bit pass n", n, "m", m, "tb", tb, "samples", samples, "rows", 32 << hp)
+    recs = M.generate_sk(n, 2, 500 + n)
+    p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, tile_bits=tb,
+                          high_pass=hp)
+    po, tto, spo = oracle.solve(n, m, recs, samples=samples, threads=2, with_samples=True)
+    assert np.array_equal(tt, tto)
+    assert np.max(np.abs(sp - spo)) < PROB_TOL
+    # the same sums in another order: close to the sweep without the pass
+    _, _, sq = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, tile_bits=tb, high_pass=0)
+    assert np.max(np.abs(sp - sq)) < 1e-6
