@@ -993,7 +993,7 @@ int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, 
 // that follows reads hsum instead of loading the partner tiles of those bits,
 // which at large n miss L2.
 // ===========================================================================
-template <int H, int WB>
+template <int H, int WB, int PPW>
 __global__ void __launch_bounds__(1 << (H + WB))
 high_pass_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n)
 {
@@ -1006,39 +1006,68 @@ high_pass_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n)
 
     const int p0 = n - H; // the first of the top bits
     const uint64_t npen = 1ull << (p0 - CB);
-    // (job, half, pencil); runs of 8 consecutive pencils on one XCD, so rows
-    // that share a 128 B line are fetched into the same L2
-    const uint64_t unit = xcd_unit(blockIdx.x, 3);
-    const uint64_t job = unit / (2 * npen);
-    if (job >= njobs) return;
-    const TermJob<float>& J = jobs[job];
-    if (!(J.flags & F_HSUM)) return;
-    const uint64_t rem = unit - job * 2 * npen;
-    const uint64_t h = rem / npen, pen = rem - h * npen;
     const uint64_t N = 1ull << n;
-    const float* src = (((J.flags & 3u) == SRC1_PSI) ? J.psi : J.b1) + h * N + (pen << CB);
-    float* dst = J.hsum + h * N + (pen << CB);
     const int t = threadIdx.x;
     const int col = t & (CW - 1), rg = t >> (2 + WB);
+    // PPW consecutive (job, half, pencil) units per workgroup; runs of 8
+    // workgroups on one XCD, so rows that share a 128 B line meet in one L2.
+    // The next unit's rows are loaded while the current one is summed.
+    const uint64_t u0 = xcd_unit(blockIdx.x, 3) * PPW;
+    auto source = [&](uint64_t unit, const float*& src, float*& dst) -> bool {
+        const uint64_t job = unit / (2 * npen);
+        if (job >= njobs) return false;
+        const TermJob<float>& J = jobs[job];
+        if (!(J.flags & F_HSUM)) return false;
+        const uint64_t rem = unit - job * 2 * npen;
+        const uint64_t h = rem / npen, pen = rem - h * npen;
+        src = (((J.flags & 3u) == SRC1_PSI) ? J.psi : J.b1) + h * N + (pen << CB);
+        dst = J.hsum + h * N + (pen << CB);
+        return true;
+    };
+    auto load = [&](const float* src, f4 (&x)[4]) {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            x[q] = *gview(reinterpret_cast<const f4*>(src + ((uint64_t)(q * RG + rg) << p0)) + col);
+    };
 
+    const float* src;
+    float* dst;
+    bool have = source(u0, src, dst);
     f4 x[4];
+    if (have) load(src, x);
 #pragma unroll
-    for (int q = 0; q < 4; q++)
-        x[q] = *gview(reinterpret_cast<const f4*>(src + ((uint64_t)(q * RG + rg) << p0)) + col);
+    for (int i = 0; i < PPW; i++) {
+        const float* nsrc = nullptr;
+        float* ndst = nullptr;
+        f4 nx[4];
+        const bool nhave = i + 1 < PPW && source(u0 + i + 1, nsrc, ndst);
+        if (nhave) load(nsrc, nx);
+        if (have) {
+            if (i > 0) __syncthreads(); // the previous unit's LDS reads are done
 #pragma unroll
-    for (int q = 0; q < 4; q++) s_rows[(q * RG + rg) * CW + col] = x[q];
-    // row bits H-2 and H-1 (the q bits) in registers
-    f4 acc[4];
+            for (int q = 0; q < 4; q++) s_rows[(q * RG + rg) * CW + col] = x[q];
+            // row bits H-2 and H-1 (the q bits) in registers
+            f4 acc[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) acc[q] = x[q ^ 1] + x[q ^ 2];
-    __syncthreads();
+            for (int q = 0; q < 4; q++) acc[q] = x[q ^ 1] + x[q ^ 2];
+            __syncthreads();
 #pragma unroll
-    for (int b = 0; b < H - 2; b++)
+            for (int b = 0; b < H - 2; b++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) acc[q] += s_rows[(q * RG + (rg ^ (1 << b))) * CW + col];
+                for (int q = 0; q < 4; q++) acc[q] += s_rows[(q * RG + (rg ^ (1 << b))) * CW + col];
 #pragma unroll
-    for (int q = 0; q < 4; q++)
-        __builtin_nontemporal_store(acc[q], gview(reinterpret_cast<f4*>(dst + ((uint64_t)(q * RG + rg) << p0)) + col));
+            for (int q = 0; q < 4; q++)
+                __builtin_nontemporal_store(acc[q],
+                                            gview(reinterpret_cast<f4*>(dst + ((uint64_t)(q * RG + rg) << p0)) + col));
+        }
+        if (nhave) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) x[q] = nx[q];
+            src = nsrc;
+            dst = ndst;
+        }
+        have = nhave;
+    }
 }
 
 // wb: row of 64 * 2^wb bytes; hi + wb <= 10 keeps the rows in 64 KiB of LDS
@@ -1047,10 +1076,14 @@ int high_pass_supported(int n, int hi, int wb)
     return wb >= 0 && wb <= 2 && hi >= 4 && hi + wb <= 10 && n - hi >= 4 + wb;
 }
 
+// pencils per workgroup (1 measured best: a second, prefetched pencil was
+// 8 % slower at n = 24..26)
+constexpr int kHighPassPPW = 1;
+
 template <int H, int WB>
 static void launch_hp(uint64_t grid, const void* jobs, uint32_t njobs, int n, hipStream_t st)
 {
-    hipLaunchKernelGGL((high_pass_kernel<H, WB>), dim3((uint32_t)grid), dim3(1 << (H + WB)), 0, st,
+    hipLaunchKernelGGL((high_pass_kernel<H, WB, kHighPassPPW>), dim3((uint32_t)grid), dim3(1 << (H + WB)), 0, st,
                        (const TermJob<float>*)jobs, njobs, n);
 }
 
@@ -1078,7 +1111,8 @@ int launch_high_pass(int hi, int wb, const void* jobs_dev, uint32_t njobs, int n
     hipStream_t st = (hipStream_t)stream;
     const uint64_t units = (uint64_t)njobs * 2ull * (1ull << (n - hi - 4 - wb));
     if (units == 0) return 0;
-    const uint64_t grid = (units + 63) / 64 * 64; // whole XCD runs of 8 pencils
+    const uint64_t wgs = (units + kHighPassPPW - 1) / kHighPassPPW;
+    const uint64_t grid = (wgs + 63) / 64 * 64; // whole XCD runs of 8 workgroups
     if (grid > 0x7fffffffull) return -3;
     if (wb == 2)
         launch_hp_w<2>(hi, grid, jobs_dev, njobs, n, st);
