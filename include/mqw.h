@@ -164,12 +164,16 @@ mqw_status mqw_solve(mqw_ctx* ctx, int n, int m, const double* recs, int count, 
 mqw_status mqw_solve_batch(mqw_ctx* ctx, const mqw_instance* inst, int count, const mqw_opts* opts,
                            float* prob, float* total_t);
 
-/* Autotuner: times the sweep-kernel configurations (kernel kind and tile
- * bits, loads-in-flight variant, XCD chunk bits) for n on this device and
- * keeps the fastest for later solves on the context.  With cache_path, a
- * recorded choice for this GPU architecture and n is reused, and a new one is
- * appended (set MQW_RETUNE=1 to measure again).  Outputs may be NULL. */
-mqw_status mqw_tune(mqw_ctx* ctx, int n, const char* cache_path, int* tile_bits, int* chunk_bits, int* variant);
+/* Autotuner: for n on this device, times every sweep configuration (kernel
+ * kind and tile bits, loads-in-flight variant, XCD chunk bits) on synthetic
+ * jobs, then runs real solves of a generated instance with the fastest few,
+ * the default and (from n = 22) the high-bit pass, and keeps the fastest for
+ * later solves on the context.  With cache_path, a recorded choice for this
+ * GPU architecture and n is reused, and a new one is appended (set
+ * MQW_RETUNE=1 to measure again).  Outputs may be NULL; *high_pass receives the
+ * mqw_opts.high_pass value chosen (0 = off, 1..3 = on). */
+mqw_status mqw_tune(mqw_ctx* ctx, int n, const char* cache_path, int* tile_bits, int* chunk_bits, int* variant,
+                    int* high_pass);
 
 /* ---- learned schedule ------------------------------------------------------ */
 /* Instance features (32 floats each): n/20, m/5, mean, variance and max |.| of

@@ -105,7 +105,7 @@ def load():
         "mqw_clenshaw": ([vp, C.c_int, fp, C.c_int, fp, fp, C.c_float, C.c_float, C.c_int], C.c_int),
         "mqw_solve": ([vp, C.c_int, C.c_int, dp, C.c_int, C.POINTER(Opts), fp, fp], C.c_int),
         "mqw_solve_batch": ([vp, C.POINTER(Instance), C.c_int, C.POINTER(Opts), fp, fp], C.c_int),
-        "mqw_tune": ([vp, C.c_int, C.c_char_p, ip, ip, ip], C.c_int),
+        "mqw_tune": ([vp, C.c_int, C.c_char_p, ip, ip, ip, ip], C.c_int),
         "mqw_last_stats": ([vp, C.POINTER(Stats)], C.c_int),
         "mqw_count_instances": ([C.c_char_p, C.c_int, C.POINTER(C.c_long)], C.c_int),
         "mqw_read_instances": ([C.c_char_p, C.c_int, C.c_long, C.c_long, dp], C.c_int),

@@ -226,11 +226,11 @@ class Context:
         return prob, tt
 
     def tune(self, n: int, cache_path: Optional[str] = None) -> dict:
-        """Autotune the sweep kernel for n (mqw_tune); returns the chosen configuration."""
-        tb, lc, var = C.c_int(0), C.c_int(0), C.c_int(0)
+        """Autotune the sweep for n (mqw_tune); returns the chosen configuration."""
+        tb, lc, var, hp = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
         self._check(self._lib.mqw_tune(self._h, n, cache_path.encode() if cache_path else None, C.byref(tb),
-                                       C.byref(lc), C.byref(var)))
-        return dict(tile_bits=tb.value, chunk_bits=lc.value, variant=var.value)
+                                       C.byref(lc), C.byref(var), C.byref(hp)))
+        return dict(tile_bits=tb.value, chunk_bits=lc.value, variant=var.value, high_pass=hp.value)
 
     def last_stats(self) -> dict:
         s = L.Stats()

@@ -92,6 +92,7 @@ SweepCfg mqw::choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits
                            int high_pass)
 {
     SweepCfg k;
+    if (high_pass < 0 && c->tuned_hp[n] >= 0) high_pass = c->tuned_hp[n];
     k.variant = variant >= 0 ? variant : c->tuned_var[n];
     int tb = tile_bits > 0 ? tile_bits : c->tuned_tb[n];
     int dflt_lc = -1;
@@ -794,6 +795,7 @@ mqw_status mqw_create(int device, mqw_ctx** out)
         c->tuned_tb[i] = 0;
         c->tuned_lc[i] = -1;
         c->tuned_var[i] = 0;
+        c->tuned_hp[i] = -1;
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return MQW_EDEVICE;
     *out = c.release();

@@ -187,6 +187,7 @@ struct mqw_ctx {
     int tuned_tb[MQW_MAX_N + 1];
     int tuned_lc[MQW_MAX_N + 1];
     int tuned_var[MQW_MAX_N + 1];
+    int tuned_hp[MQW_MAX_N + 1]; // high_pass option chosen by the tuner, -1 = none
     std::string arch;
 };
 

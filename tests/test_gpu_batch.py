@@ -52,9 +52,9 @@ def test_autotuner_cache_and_results(ctx, oracle, tmp_path):
     cache = str(tmp_path / "tune.txt")
     _say("tune n=16")
     cfg = ctx.tune(n, cache)
-    assert cfg["tile_bits"] in (11, 12, 13, 14)
+    assert cfg["tile_bits"] in (11, 12, 13, 14) and cfg["high_pass"] in (0, 1, 2, 3)
     lines = open(cache).read().split("\n")
-    assert lines[0].split()[1] == "16"
+    assert lines[0].split()[1] == "16" and len(lines[0].split()) == 7
     again = ctx.tune(n, cache)  # reuses the recorded choice
     assert again == cfg and len(open(cache).read().strip().split("\n")) == 1
     recs = M.generate_sk(n, 1, 9)
@@ -82,3 +82,14 @@ def test_exact_energy_mode(ctx, oracle):
     p_e, _ = ctx.solve(n, 2, recs, samples=10, energy_mode=M.ENERGY_EXACT)
     # the diagonal differs only by the reference's float-delta drift
     assert np.max(np.abs(p_c - p_e)) < 1e-4
+
+
+def test_autotuner_weighs_the_high_bit_pass(ctx, tmp_path):
+    """From n = 22 the real-solve stage also times the sweep with the high-bit pass."""
+    n = 23
+    _say("tune n=23")
+    cache = str(tmp_path / "tune23.txt")
+    cfg = ctx.tune(n, cache)
+    assert cfg["high_pass"] in (0, 2)
+    f = open(cache).read().split()
+    assert int(f[1]) == n and int(f[5]) == cfg["high_pass"] and float(f[6]) > 0
