@@ -176,3 +176,20 @@ def test_high_bit_pass_vs_oracle(ctx, oracle, n, m, tb, samples, hp):
     # the same sums in another order: close to the sweep without the pass
     _, _, sq = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, tile_bits=tb, high_pass=0)
     assert np.max(np.abs(sp - sq)) < 1e-6
+
+
+def test_clenshaw_seam_is_unitary_n18(ctx, oracle):
+    """exp(iHt) on the GPU keeps ||psi|| = 1 at n = 18 (FP32 state)."""
+    n = 18
+    N = 1 << n
+    _say("unitarity n=18")
+    rec = M.generate_sk(n, 1, 61)[0]
+    hp, info = ctx.energy(n, rec, M.ENERGY_COMPAT)
+    gamma, t = 1.2, 1.7
+    scale = float(np.float32(info["e_abs"] + np.float32(gamma) * n))
+    c = M.chebyshev_coeffs(scale * t).astype(np.float32)
+    rng = np.random.default_rng(62)
+    z = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    z /= np.linalg.norm(z)
+    out = ctx.clenshaw(c, np.concatenate([z.real, z.imag]).astype(np.float32), hp, gamma, scale)
+    assert abs(np.linalg.norm(out.astype(np.float64)) - 1.0) < 2e-5

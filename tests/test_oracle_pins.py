@@ -231,6 +231,30 @@ def test_clenshaw_vs_dense_expm(oracle, n):
     assert np.max(np.abs(o64[:N] + 1j * o64[N:] - wrong)) > 1e-2
 
 
+def test_clenshaw_is_unitary_beyond_dense_sizes(oracle):
+    """exp(iHt) preserves the norm: at n = 14 (no dense check) the FP64 and FP32
+    restatements keep ||psi|| = 1, and a wrong odd-term phase would not."""
+    n = 14
+    N = 1 << n
+    rec = _rng(90).standard_normal(n * (n + 1) // 2)
+    J, h = oracle.unpack(n, rec)
+    hp, _, info = oracle.energy_compat(n, J, h)
+    gamma, t = 0.9, 2.1
+    scale = float(np.float32(info["e_abs"] + np.float32(gamma) * n))
+    c = oracle.cheb_coeffs(scale * t)
+    z = _rng(91).standard_normal(N) + 1j * _rng(92).standard_normal(N)
+    z /= np.linalg.norm(z)
+    psi = np.concatenate([z.real, z.imag])
+    o64 = oracle.clenshaw_f64(c, psi, hp.astype(np.float64), n, gamma, scale)
+    o32 = oracle.clenshaw_f32(c.astype(np.float32), psi.astype(np.float32), hp, n, gamma, scale, False)
+    assert abs(np.linalg.norm(o64) - 1.0) < 1e-9
+    assert abs(np.linalg.norm(o32.astype(np.float64)) - 1.0) < 2e-5
+    # real coefficients only (cos(sx) + sin(sx) without the i on odd terms) is not unitary
+    bad = oracle.clenshaw_f64(c * np.where(np.arange(c.size) % 2 == 1, 0.0, 1.0), psi, hp.astype(np.float64), n,
+                              gamma, scale)
+    assert abs(np.linalg.norm(bad) - 1.0) > 1e-3
+
+
 def test_group_property_pins_m1_chain(oracle):
     """e^{iHt2} e^{iHt1} = e^{iH(t1+t2)}: chained stages equal one long stage (the m == 1 path)."""
     n = 8
