@@ -422,9 +422,8 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
     if (!P.rjobs.empty())
         HIPCHK(c, hipMemcpyAsync(rd, P.rjobs.data(), sizeof(ReduceJob) * P.rjobs.size(), hipMemcpyHostToDevice,
                                  c->stream));
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
+    EventPair ev;
+    HIPCHK(c, ev.create());
     auto issue = [&]() -> int {
         for (const Op& op : P.ops) {
             int rc;
@@ -441,39 +440,29 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
         return 0;
     };
     mqw_status st = MQW_OK;
-    HIPCHK(c, hipEventRecord(e0, c->stream));
+    HIPCHK(c, hipEventRecord(ev.a, c->stream));
     if (use_graph) {
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
+        GraphGuard gr;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         const int rc = issue();
-        hipError_t ce = hipStreamEndCapture(c->stream, &g);
-        if (rc || ce != hipSuccess) {
-            if (g) hipGraphDestroy(g);
+        hipError_t ce = hipStreamEndCapture(c->stream, &gr.g);
+        if (rc || ce != hipSuccess)
             return fail(c, MQW_EDEVICE, "graph capture failed: %s / %s", last_kernel_error(),
                         hipGetErrorString(ce));
-        }
-        hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        if (ie != hipSuccess) {
-            hipGraphDestroy(g);
-            return fail(c, MQW_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ie));
-        }
-        hipError_t le = hipGraphLaunch(ge, c->stream);
+        hipError_t ie = hipGraphInstantiate(&gr.ge, gr.g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) return fail(c, MQW_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+        hipError_t le = hipGraphLaunch(gr.ge, c->stream);
         if (le != hipSuccess) st = fail(c, MQW_EDEVICE, "hipGraphLaunch: %s", hipGetErrorString(le));
-        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipEventRecord(ev.b, c->stream));
         hipError_t se = hipStreamSynchronize(c->stream);
-        hipGraphExecDestroy(ge);
-        hipGraphDestroy(g);
         if (se != hipSuccess) return fail(c, MQW_EDEVICE, "stream sync: %s", hipGetErrorString(se));
     } else {
         if (issue()) st = fail(c, MQW_EDEVICE, "term launch: %s", last_kernel_error());
-        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipEventRecord(ev.b, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     float t = 0.0f;
-    hipEventElapsedTime(&t, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    hipEventElapsedTime(&t, ev.a, ev.b);
     *ms += t;
     return st;
 }
@@ -517,39 +506,31 @@ mqw_status run_lds(mqw_ctx* c, int n, const std::vector<Traj>& trajs, const std:
     HIPCHK(c, hipMemcpyAsync(ds, hs.data(), sizeof(LdsSeg) * hs.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(dc, hc.data(), sizeof(float) * hc.size(), hipMemcpyHostToDevice, c->stream));
     const int grid = lds_grid(n, c->device);
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
-    HIPCHK(c, hipEventRecord(e0, c->stream));
+    EventPair ev;
+    HIPCHK(c, ev.create());
+    HIPCHK(c, hipEventRecord(ev.a, c->stream));
     if (use_graph) {
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
+        GraphGuard gr;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         const int rc = launch_lds(n, dt, (uint32_t)ht.size(), ds, dc, probs_d, nullptr, grid, c->stream);
-        hipError_t ce = hipStreamEndCapture(c->stream, &g);
-        if (rc || ce != hipSuccess) {
-            if (g) hipGraphDestroy(g);
+        hipError_t ce = hipStreamEndCapture(c->stream, &gr.g);
+        if (rc || ce != hipSuccess)
             return fail(c, MQW_EDEVICE, "graph capture of the LDS kernel failed (%d): %s", rc,
                         hipGetErrorString(ce));
-        }
-        HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-        hipError_t le = hipGraphLaunch(ge, c->stream);
-        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipGraphInstantiate(&gr.ge, gr.g, nullptr, nullptr, 0));
+        hipError_t le = hipGraphLaunch(gr.ge, c->stream);
+        HIPCHK(c, hipEventRecord(ev.b, c->stream));
         hipError_t se = hipStreamSynchronize(c->stream);
-        hipGraphExecDestroy(ge);
-        hipGraphDestroy(g);
         if (le != hipSuccess) return fail(c, MQW_EDEVICE, "hipGraphLaunch: %s", hipGetErrorString(le));
         if (se != hipSuccess) return fail(c, MQW_EDEVICE, "stream sync: %s", hipGetErrorString(se));
     } else {
         if (launch_lds(n, dt, (uint32_t)ht.size(), ds, dc, probs_d, nullptr, grid, c->stream))
             return fail(c, MQW_EDEVICE, "LDS kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
-        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipEventRecord(ev.b, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     float ms = 0.0f;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    hipEventElapsedTime(&ms, ev.a, ev.b);
     c->stats.gpu_ms += ms;
     c->stats.lds_trajectories += trajs.size();
     c->stats.amp_terms += amp_terms;

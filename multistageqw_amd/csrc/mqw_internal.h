@@ -192,6 +192,30 @@ struct mqw_ctx {
 };
 
 namespace mqw {
+// Timing events and captured graphs, released on every return path.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t create()
+    {
+        hipError_t e = hipEventCreate(&a);
+        return e != hipSuccess ? e : hipEventCreate(&b);
+    }
+    ~EventPair()
+    {
+        if (a) hipEventDestroy(a);
+        if (b) hipEventDestroy(b);
+    }
+};
+struct GraphGuard {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    ~GraphGuard()
+    {
+        if (ge) hipGraphExecDestroy(ge);
+        if (g) hipGraphDestroy(g);
+    }
+};
+
 mqw_status fail(mqw_ctx* c, mqw_status s, const char* fmt, ...);
 void* dev_get(mqw_ctx* c, const std::string& name, size_t bytes);
 void release_bufs(mqw_ctx* c);

@@ -124,9 +124,8 @@ mqw_status synthetic_rates(mqw_ctx* c, int n, const std::vector<Cand>& cands, st
             j.flags = SRC1_ARRAY | SRC2_ARRAY;
         }
     HIPCHK(c, hipMemcpyAsync(jd, hj.data(), sizeof(TermJob<float>) * hj.size(), hipMemcpyHostToDevice, c->stream));
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
+    EventPair ev;
+    HIPCHK(c, ev.create());
     const int reps = 8;
     gbs.assign(cands.size(), 0.0);
     for (size_t i = 0; i < cands.size(); i++) {
@@ -138,21 +137,19 @@ mqw_status synthetic_rates(mqw_ctx* c, int n, const std::vector<Cand>& cands, st
                         : launch_term(MQW_FP32, cd.tb, cd.variant, jobs, (uint32_t)T, n, cd.lc, c->stream);
         };
         if (launch(0) || launch(1)) continue;
-        HIPCHK(c, hipEventRecord(e0, c->stream));
+        HIPCHK(c, hipEventRecord(ev.a, c->stream));
         int bad = 0;
         for (int q = 0; q < reps; q++) bad |= launch(q);
-        HIPCHK(c, hipEventRecord(e1, c->stream));
-        HIPCHK(c, hipEventSynchronize(e1));
+        HIPCHK(c, hipEventRecord(ev.b, c->stream));
+        HIPCHK(c, hipEventSynchronize(ev.b));
         if (bad) continue;
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, e0, e1);
+        hipEventElapsedTime(&ms, ev.a, ev.b);
         gbs[i] = 36.0 * (double)N * (double)T * reps / (ms * 1e-3) / 1e9;
         if (verbose)
             fprintf(stderr, "mqw_tune n=%d synthetic tb=%d variant=%d lc=%d %.1f GB/s\n", n, cd.tb, cd.variant, cd.lc,
                     gbs[i]);
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     return MQW_OK;
 }
 
