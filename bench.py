@@ -46,9 +46,9 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part of "
-              "this build: trajectories are independent, see DESIGN.md for how the path would shard.",
-              file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}). The (instance, sample) "
+              "trajectories are independent and would shard across GPUs with no exchange; this benchmark "
+              "measures one GPU's throughput.", file=sys.stderr)
         return 2
 
     import multistageqw_amd as M

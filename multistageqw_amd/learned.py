@@ -1,4 +1,4 @@
-"""Learned per-stage schedule: an MLP proposes (gamma_i, t_i), the simulator judges.
+"""Learned per-stage schedule: an MLP proposes (gamma_i, t_i), the simulator scores it.
 
 The heuristic of MultiQW.cpp:386-423 picks gamma_i and the stage times from
 closed-form moments.  Here a 32 -> 128 -> 128 -> 10 GELU network (trained on
