@@ -395,7 +395,7 @@ __device__ __forceinline__ void vstore_nt(real* p, const real (&v)[R], int strid
 
 // LDS accesses of the tile exchange buffer (generic pointers into __shared__).
 template <typename real, int R>
-__device__ __forceinline__ void lds_load(const real* p, real (&v)[R], int stride = NVec16<real>::W)
+__device__ __forceinline__ void lds_read(const real* p, real (&v)[R], int stride = NVec16<real>::W)
 {
     using vt = typename NVec16<real>::type;
     constexpr int W = NVec16<real>::W;
@@ -408,7 +408,7 @@ __device__ __forceinline__ void lds_load(const real* p, real (&v)[R], int stride
 }
 
 template <typename real, int R>
-__device__ __forceinline__ void lds_store(real* p, const real (&v)[R], int stride = NVec16<real>::W)
+__device__ __forceinline__ void lds_write(real* p, const real (&v)[R], int stride = NVec16<real>::W)
 {
     using vt = typename NVec16<real>::type;
     constexpr int W = NVec16<real>::W;
@@ -544,8 +544,8 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
             vload<real, R>(src + off, xr, PS);
             vload<real, R>(src + N + off, xi, PS);
         }
-        lds_store<real, R>(s_re + t * W, xr, PS);
-        lds_store<real, R>(s_im + t * W, xi, PS);
+        lds_write<real, R>(s_re + t * W, xr, PS);
+        lds_write<real, R>(s_im + t * W, xi, PS);
         real ar[R], ai[R];
 #pragma unroll
         for (int q = 0; q < R; q++) {
@@ -579,8 +579,8 @@ term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, int l
         for (int b = 0; b < TB - RB; b++) {
             const int pt = t ^ (1 << b);
             real yr[R], yi[R];
-            lds_load<real, R>(s_re + pt * W, yr, PS);
-            lds_load<real, R>(s_im + pt * W, yi, PS);
+            lds_read<real, R>(s_re + pt * W, yr, PS);
+            lds_read<real, R>(s_im + pt * W, yi, PS);
 #pragma unroll
             for (int q = 0; q < R; q++) {
                 ar[q] += yr[q];
@@ -822,7 +822,7 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
         } else if (src1_arr) {
             const float(&x)[R] = xs[sd];
             float acc[R];
-            lds_store<float, R>(s_x + t * 4, x, PS);
+            lds_write<float, R>(s_x + t * 4, x, PS);
 #pragma unroll
             for (int q = 0; q < R; q++) acc[q] = 0.0f;
 #pragma unroll
@@ -871,7 +871,7 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
 #pragma unroll
             for (int b = 0; b < HB - RB; b++) {
                 float y[R];
-                lds_load<float, R>(s_x + (t ^ (1 << b)) * 4, y, PS);
+                lds_read<float, R>(s_x + (t ^ (1 << b)) * 4, y, PS);
 #pragma unroll
                 for (int q = 0; q < R; q++) acc[q] += y[q];
             }
