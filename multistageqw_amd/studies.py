@@ -7,6 +7,8 @@
       one large instance: complex H|psi> applies/s and algorithmic GB/s.
   python -m multistageqw_amd.studies sweep --ns 14 16 18 20 22 24 26
       apply rate of the sweep kernel per n (one instance, a few samples).
+  python -m multistageqw_amd.studies batch --n 10 --m 2 --count 64 --samples 100 --seed 1
+      one (n, m) batch (the n=10 case runs on the LDS-resident kernel), instances/s.
   python -m multistageqw_amd.studies uneven --count 4096 --seed 3 --fp64-every 16 --tune-cache tune.txt
       the uneven workload (n uniform in 12..22, m uniform in 1..5): instances/s,
       bytes-weighted GB/s, per-instance error against FP64 runs of a subset.
@@ -58,6 +60,23 @@ def large(ctx, n, m, samples, seed, warm=True):
     return dict(study="large", m=m, samples=samples, p=float(p[0]), **_apply_line(ctx, n, time.time() - t0))
 
 
+def batch(ctx, n, m, count, samples, seed, reps=3):
+    """count seeded instances of one (n, m): instances/s over reps solves after a warm-up."""
+    recs = generate_sk(n, count, seed)
+    ctx.solve(n, m, recs, samples=samples)
+    t0 = time.time()
+    dev = alg = 0.0
+    for _ in range(reps):
+        p, _ = ctx.solve(n, m, recs, samples=samples)
+        st = ctx.last_stats()
+        dev += st["gpu_ms"]
+        alg += st["algorithmic_bytes"]
+    wall = time.time() - t0
+    return dict(study="batch", n=n, m=m, count=count, samples=samples, seed=seed, instances_per_s=reps * count / wall,
+                wall_s=wall / reps, device_ms=dev / reps, algorithmic_GBps=alg / (dev * 1e-3) / 1e9,
+                lds_trajectories=st["lds_trajectories"], mean_p=float(np.mean(p)))
+
+
 def uneven(ctx, count, samples, seed, chunk=256, fp64_every=0, tune_cache=None):
     """The uneven workload: n ~ U{12..22}, m ~ U{1..5}, solved chunk by chunk
     (each chunk one mqw_solve_batch call); every fp64_every-th instance is
@@ -103,7 +122,7 @@ def uneven(ctx, count, samples, seed, chunk=256, fp64_every=0, tune_cache=None):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["precision", "large", "sweep", "uneven"])
+    ap.add_argument("study", choices=["precision", "large", "sweep", "uneven", "batch"])
     ap.add_argument("--n", type=int, default=24)
     ap.add_argument("--ns", type=int, nargs="*", default=[14, 16, 18, 20, 22, 24, 26])
     ap.add_argument("--m", type=int, default=3)
@@ -123,6 +142,8 @@ def main(argv=None):
     elif a.study == "sweep":
         for n in a.ns:
             print(json.dumps(large(ctx, n, a.m, a.samples, a.seed)), flush=True)
+    elif a.study == "batch":
+        print(json.dumps(batch(ctx, a.n, a.m, a.count, a.samples, a.seed)), flush=True)
     else:
         print(json.dumps(uneven(ctx, a.count, a.samples, a.seed, a.chunk, a.fp64_every, a.tune_cache)), flush=True)
     ctx.close()
