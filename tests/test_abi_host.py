@@ -131,3 +131,23 @@ def test_invalid_arguments_are_errors():
 def test_format_line_matches_iostream_defaults():
     assert M.format_line(np.float32(0.55484647), np.float32(0.25346103)) == "0.554846 0.253461"
     assert M.format_line(np.float32(12.0), np.float32(1.5e-5)) == "12 1.5e-05"
+
+
+def _append_block(args):
+    d, k, size = args
+    vals = np.full(size, float(k), np.float32)
+    return M.write_output(10, 2, vals, directory=d)
+
+
+def test_concurrent_appends_keep_blocks_whole(tmp_path):
+    """Slices of one run append their blocks concurrently (run.sh:27-38); each lands whole."""
+    import multiprocessing as mp
+    size, procs = 1 << 20, 6  # 4 MiB per block: many write() calls' worth
+    with mp.get_context("spawn").Pool(procs) as pool:
+        pool.map(_append_block, [(str(tmp_path), k, size) for k in range(procs)])
+    out = np.fromfile(str(tmp_path / "output_10_2"), "<f4")
+    assert out.size == procs * size
+    blocks = out.reshape(procs, size)
+    assert sorted(int(b[0]) for b in blocks) == list(range(procs))
+    for b in blocks:
+        assert np.all(b == b[0])
