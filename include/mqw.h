@@ -90,6 +90,12 @@ typedef struct {
                                 256 B: before each sweep a pass sums the flips over the top 10, 9, 8
                                 bits in on-chip pencils, so the sweep loads no partner tiles for them
                                 (FP32 real-half sweeps, n >= 17) */
+    const uint32_t* mixer_masks; /* NULL = the reference's mixer sum_i X_i; else mixer_count flip masks:
+                                    H = H_P - gamma * sum_k prod_{i in m_k} X_i (multi-X mixers, the
+                                    reference's README.md:41 to-do); spectral bound E_abs + gamma * count.
+                                    The heuristic schedule still assumes sum_i X_i: pass schedules
+                                    through mqw_instance.gamma/short_t. */
+    int mixer_count;         /* 1..64 masks, each nonzero and < 2^n */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are

@@ -431,6 +431,8 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
                 rc = launch_reduce(precision, rd + op.off, op.count, c->stream);
             else if (op.kind == 2)
                 rc = launch_high_pass(k.hi, k.hwb, jd + op.off, op.count, n, c->stream);
+            else if (k.nmask > 0)
+                rc = launch_mask_term(precision, jd + op.off, op.count, n, k.masks, k.nmask, c->stream);
             else if (k.half)
                 rc = launch_half_term(k.tb, k.variant, jd + op.off, op.count, n, k.lc, k.hi, c->stream);
             else
@@ -546,7 +548,25 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     const size_t N = (size_t)1 << n;
     const int precision = o.precision;
     const int samples = o.samples;
-    const SweepCfg k = choose_sweep(c, precision, n, o.tile_bits, o.chunk_bits, o.variant, o.high_pass);
+    SweepCfg k = choose_sweep(c, precision, n, o.tile_bits, o.chunk_bits, o.variant, o.high_pass);
+    const bool mixer = o.mixer_count > 0;
+    if (mixer) {
+        // X-string mixer: its own sweep kernel, masks on the device
+        for (int q = 0; q < o.mixer_count; q++)
+            if ((uint64_t)o.mixer_masks[q] >= (1ull << n))
+                return fail(c, MQW_EINVAL, "mixer mask %u has bits above n=%d", o.mixer_masks[q], n);
+        uint32_t* md = (uint32_t*)dev_get(c, "mixer_masks", sizeof(uint32_t) * o.mixer_count);
+        if (!md) return fail(c, MQW_ENOMEM, "device allocation failed for mixer masks");
+        HIPCHK(c, hipMemcpyAsync(md, o.mixer_masks, sizeof(uint32_t) * o.mixer_count, hipMemcpyHostToDevice,
+                                 c->stream));
+        k = SweepCfg{};
+        k.tb = mask_tile_bits(n);
+        k.parts = 1u << (n - k.tb);
+        k.masks = md;
+        k.nmask = o.mixer_count;
+    }
+    // spectral bound of the mixer: n for sum_i X_i, the mask count for X-strings
+    const float mix_norm = mixer ? (float)o.mixer_count : (float)(unsigned)n;
     real* E = (real*)dev_get(c, "E_group", sizeof(real) * N * ps.size());
     if (!E) return fail(c, MQW_ENOMEM, "device allocation failed for %zu diagonals at n=%d", ps.size(), n);
     mqw_status st = group_energies(c, n, ps, o.energy_mode, precision, E);
@@ -560,7 +580,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         auto seg = [&](int i, int j, int64_t slot) {
             Segment sg;
             sg.gamma = pr.sch.gamma[i];
-            sg.onenorm = fmaf(sg.gamma, (float)(unsigned)n, pr.info.e_abs);
+            sg.onenorm = fmaf(sg.gamma, mix_norm, pr.info.e_abs);
             const double s = (double)(sg.onenorm * pr.times[(size_t)i * samples + j]);
             std::vector<double> cc;
             cheb_coeffs_vec(s, o.tol, cc);
@@ -590,7 +610,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     if (!probs_d) return fail(c, MQW_ENOMEM, "device allocation failed for probabilities");
     HIPCHK(c, hipMemcpyAsync(probs_d, probs.data(), sizeof(double) * probs.size(), hipMemcpyHostToDevice, c->stream));
     const int small_max = o.small_n_max >= 0 ? o.small_n_max : 13;
-    if (precision == MQW_FP32 && n <= small_max && lds_supported(n)) {
+    if (precision == MQW_FP32 && n <= small_max && lds_supported(n) && !mixer) {
         st = run_lds(c, n, trajs, ps, (const float*)E, probs_d, o.use_graph != 0);
         if (st) return st;
         HIPCHK(c, hipMemcpyAsync(probs.data(), probs_d, sizeof(double) * probs.size(), hipMemcpyDeviceToHost,
@@ -663,6 +683,10 @@ mqw_status solve_problems(mqw_ctx* c, std::vector<Problem>& probs_in, const mqw_
     if (o.precision != MQW_FP32 && o.precision != MQW_FP64) return fail(c, MQW_EINVAL, "bad precision");
     if (o.energy_mode != MQW_ENERGY_COMPAT && o.energy_mode != MQW_ENERGY_EXACT)
         return fail(c, MQW_EINVAL, "bad energy mode");
+    if (o.mixer_count < 0 || o.mixer_count > 64 || (o.mixer_count > 0 && !o.mixer_masks))
+        return fail(c, MQW_EINVAL, "mixer_count must be 0..64 with masks given");
+    for (int q = 0; q < o.mixer_count; q++)
+        if (o.mixer_masks[q] == 0u) return fail(c, MQW_EINVAL, "mixer mask %d is empty", q);
     std::map<int, std::vector<Problem*>> groups;
     size_t slots = 0;
     for (Problem& p : probs_in) {

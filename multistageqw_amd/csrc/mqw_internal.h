@@ -130,6 +130,11 @@ int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, 
 // 4 <= hi, hi + wb <= 10, wb <= 2, n - hi >= 4 + wb.
 int high_pass_supported(int n, int hi, int wb);
 int launch_high_pass(int hi, int wb, const void* jobs_dev, uint32_t njobs, int n, void* stream);
+// X-string mixer sweep (FP32/FP64): sum over nmask flip masks instead of the
+// hypercube; 2^mask_tile_bits(n) amplitudes (one partial norm) per workgroup.
+int mask_tile_bits(int n);
+int launch_mask_term(int precision, const void* jobs_dev, uint32_t njobs, int n, const uint32_t* masks_dev, int nmask,
+                     void* stream);
 int launch_reduce(int precision, const ReduceJob* jobs_dev, uint32_t njobs, void* stream);
 int launch_half_step(const float* b1, float* b2, const float* hp, const float* psi, int n, float alpha,
                      float beta, float coef, void* stream);
@@ -174,6 +179,8 @@ struct SweepCfg {
     uint32_t parts = 1; // partial norm sums per output
     int hi = 0;        // top bits summed by the high-bit pass (0 = none)
     int hwb = 0;       // its pencil rows are 64 * 2^hwb bytes
+    const uint32_t* masks = nullptr; // X-string mixer on the device (nmask > 0), else the hypercube
+    int nmask = 0;
 };
 
 } // namespace mqw

@@ -985,6 +985,178 @@ int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, 
 }
 
 // ===========================================================================
+// X-string mixers (the reference's README.md:41 lists multi-X mixers as a
+// to-do): the term of term_kernel with H_G = sum_i X_i replaced by
+// H_M = sum_k X^{m_k}, X^m flipping every bit of the mask m.  Four consecutive
+// amplitudes per thread; each mask's partners are one aligned 16-element-byte
+// group whose lanes are permuted by the mask's two low bits, so every load of
+// a wave stays inside the same cache lines.  FP32 and FP64.
+// ===========================================================================
+constexpr int kMaskTileBits = 10;
+
+// Deterministic sum over the nt threads of a workgroup (tree in LDS), valid in every thread.
+__device__ __forceinline__ double block_sum(double v, double* red, int nt)
+{
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int w = nt >> 1; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+template <typename real>
+__global__ void __launch_bounds__(256)
+mask_term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, const uint32_t* __restrict__ masks,
+                 int nmask)
+{
+    __shared__ double red[256];
+    const int tmb = n < kMaskTileBits ? n : kMaskTileBits;
+    const int nt = 1 << (tmb - 2);
+    const uint64_t tiles = 1ull << (n - tmb);
+    const uint64_t job = blockIdx.x / tiles;
+    if (job >= njobs) return;
+    const uint64_t tile = blockIdx.x - job * tiles;
+    const TermJob<real>& J = jobs[job];
+    const uint32_t flags = J.flags;
+    const uint32_t src1 = flags & 3u, src2 = flags & 12u;
+    const bool psi_const = (flags & F_PSI_CONST) != 0;
+    const uint64_t N = 1ull << n;
+    const int t = threadIdx.x;
+    const uint64_t x0 = (tile << tmb) + 4 * (uint64_t)t;
+
+    real ps = real(1);
+    if (flags & F_NORM_IN) {
+        double s = 0.0;
+        for (uint64_t k = t; k < tiles; k += nt) s += J.norm_in[k];
+        const double tot = block_sum(s, red, nt);
+        ps = (real)(1.0 / sqrt(tot));
+        if ((flags & F_READOUT) && tile == 0 && t == 0) {
+            const double re = (double)J.psi[J.eloc], im = (double)J.psi[N + J.eloc];
+            *J.prob = (re * re + im * im) / tot;
+        }
+    }
+
+    real pr[4], pi[4], e[4], yr[4], yi[4], ur[4], ui[4];
+    if (!psi_const) {
+        vload<real, 4>(J.psi + x0, pr);
+        vload<real, 4>(J.psi + N + x0, pi);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            pr[q] = J.psi0;
+            pi[q] = real(0);
+        }
+    }
+    vload<real, 4>(J.E + x0, e);
+    if (src2 == SRC2_ARRAY) {
+        vload<real, 4>(J.b2 + x0, yr);
+        vload<real, 4>(J.b2 + N + x0, yi);
+    }
+    if (src1 == SRC1_ZERO) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) ur[q] = ui[q] = real(0);
+    } else if (src1 == SRC1_PSI && psi_const) {
+        // |+> is an eigenvector of every X-string, eigenvalue 1 each
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            ur[q] = J.alpha * e[q] * J.psi0 - J.beta * (real)nmask * J.psi0;
+            ui[q] = real(0);
+        }
+    } else {
+        const real* src = (src1 == SRC1_PSI) ? J.psi : J.b1;
+        real xr[4], xi[4], ar[4], ai[4];
+        if (src1 == SRC1_PSI) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                xr[q] = pr[q];
+                xi[q] = pi[q];
+            }
+        } else {
+            vload<real, 4>(src + x0, xr);
+            vload<real, 4>(src + N + x0, xi);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) ar[q] = ai[q] = real(0);
+        for (int k = 0; k < nmask; k++) {
+            const uint64_t m = masks[k];
+            const uint64_t base = (x0 ^ m) & ~(uint64_t)3;
+            const int lo = (int)(m & 3u);
+            real tr[4], ti[4];
+            vload<real, 4>(src + base, tr);
+            vload<real, 4>(src + N + base, ti);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                ar[q] += tr[q ^ lo];
+                ai[q] += ti[q ^ lo];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            ur[q] = J.alpha * e[q] * xr[q] - J.beta * ar[q];
+            ui[q] = J.alpha * e[q] * xi[q] - J.beta * ai[q];
+        }
+    }
+    real w1r = J.w1r, w1i = J.w1i, w2r = J.w2r, w2i = J.w2i;
+    if (src1 == SRC1_PSI) {
+        w1r *= ps;
+        w1i *= ps;
+    }
+    if (src2 == SRC2_ZERO) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) yr[q] = yi[q] = real(0);
+    } else if (src2 == SRC2_PSI) {
+        w2r *= ps;
+        w2i *= ps;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            yr[q] = pr[q];
+            yi[q] = pi[q];
+        }
+    }
+    const real cr = J.cr * ps, ci = J.ci * ps;
+    real orr[4], oi[4];
+    double nrm = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        orr[q] = (w1r * ur[q] - w1i * ui[q]) - (w2r * yr[q] - w2i * yi[q]) + (cr * pr[q] - ci * pi[q]);
+        oi[q] = (w1r * ui[q] + w1i * ur[q]) - (w2r * yi[q] + w2i * yr[q]) + (cr * pi[q] + ci * pr[q]);
+        nrm += (double)orr[q] * (double)orr[q] + (double)oi[q] * (double)oi[q];
+    }
+    vstore<real, 4>(J.out + x0, orr);
+    vstore<real, 4>(J.out + N + x0, oi);
+    if (flags & F_FINAL) {
+        const double s = block_sum(nrm, red, nt);
+        if (t == 0) J.norm_part[tile] = s;
+    }
+}
+
+int mask_tile_bits(int n) { return n < kMaskTileBits ? n : kMaskTileBits; }
+
+int launch_mask_term(int precision, const void* jobs_dev, uint32_t njobs, int n, const uint32_t* masks_dev, int nmask,
+                     void* stream)
+{
+    if (n < 4 || nmask < 1) return -2;
+    const int tmb = mask_tile_bits(n);
+    const uint64_t grid = (uint64_t)njobs << (n - tmb);
+    if (grid == 0) return 0;
+    if (grid > 0x7fffffffull) return -3;
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 block(1u << (tmb - 2));
+    if (precision == MQW_FP32)
+        hipLaunchKernelGGL(mask_term_kernel<float>, dim3((uint32_t)grid), block, 0, st,
+                           (const TermJob<float>*)jobs_dev, njobs, n, masks_dev, nmask);
+    else
+        hipLaunchKernelGGL(mask_term_kernel<double>, dim3((uint32_t)grid), block, 0, st,
+                           (const TermJob<double>*)jobs_dev, njobs, n, masks_dev, nmask);
+    return check_launch("mask_term");
+}
+
+// ===========================================================================
 // High-bit pass (FP32).  For every job flagged F_HSUM, hsum[k] = sum over the
 // top H bits b of s1[k ^ 2^b], both real halves.  One workgroup takes a pencil
 // of 16 * 2^WB consecutive amplitudes (a row of 64 * 2^WB bytes) by all 2^H

@@ -16,19 +16,20 @@ from __future__ import annotations
 import numpy as np
 
 
-def hypercube(n: int) -> np.ndarray:
+def hypercube(n: int, masks=None) -> np.ndarray:
+    """sum_i X_i; with masks, the X-string mixer sum_k X^{m_k} (X^m flips the bits of m)."""
     N = 1 << n
     H = np.zeros((N, N))
     k = np.arange(N)
-    for i in range(n):
-        H[k, k ^ (1 << i)] += 1.0
+    for m in (masks if masks is not None else [1 << i for i in range(n)]):
+        H[k, k ^ int(m)] += 1.0
     return H
 
 
-def hamiltonian(E, gamma: float) -> np.ndarray:
+def hamiltonian(E, gamma: float, masks=None) -> np.ndarray:
     E = np.asarray(E, np.float64)
     n = int(np.log2(E.size))
-    return np.diag(E) - gamma * hypercube(n)
+    return np.diag(E) - gamma * hypercube(n, masks)
 
 
 def evolve(E, gamma: float, t: float, psi) -> np.ndarray:
@@ -37,18 +38,19 @@ def evolve(E, gamma: float, t: float, psi) -> np.ndarray:
     return V @ (np.exp(1j * w * t) * (V.T @ np.asarray(psi, np.complex128)))
 
 
-def walk_probability(E, gammas, times, eloc: int, reset: bool = True):
+def walk_probability(E, gammas, times, eloc: int, reset: bool = True, masks=None):
     """Multi-stage walk from |+>, renormalised after each stage (MultiQW.cpp:432-451).
 
     times: [m, samples].  reset=True resets to |+> per sample (m != 1);
     reset=False chains the samples (the m == 1 path of MultiQW.cpp:426-430).
-    Returns per-sample probabilities of state ``eloc``.
+    Returns per-sample probabilities of state ``eloc``.  masks: an X-string
+    mixer in place of sum_i X_i.
     """
     E = np.asarray(E, np.float64)
     N = E.size
     n = int(np.log2(N))
     m, samples = np.asarray(times).shape
-    eig = [np.linalg.eigh(np.diag(E) - g * hypercube(n)) for g in gammas]
+    eig = [np.linalg.eigh(np.diag(E) - g * hypercube(n, masks)) for g in gammas]
     plus = np.full(N, 1 / np.sqrt(N), np.complex128)
     psi = plus.copy()
     out = np.zeros(samples)
