@@ -149,6 +149,13 @@ mqw_status mqw_append_output(const char* dir, int n, int m, const float* probs, 
 mqw_status mqw_energy(mqw_ctx* ctx, int n, const double* rec, int mode, float* hp, double* e,
                       mqw_energy_info* info);
 
+/* Moments of the diagonal energies the walk uses (H_P + shift, float):
+ * out[0..3] = mean, variance, skewness, excess kurtosis, from FP64 sums on
+ * the device.  The reference computes the variance in closed form (HP2,
+ * MultiQW.cpp:375) and leaves skewness and kurtosis commented out
+ * (MultiQW.cpp:379-380). */
+mqw_status mqw_diag_moments(mqw_ctx* ctx, int n, const double* rec, int mode, double* out);
+
 /* One Clenshaw step on one real half, device pointers (the seam at
  * MultiQW.cpp:88-173): b2 <- (2/scale)(H_P - gamma H_G) b1 - b2 + coef psi. */
 mqw_status mqw_clenshaw_step_dev(mqw_ctx* ctx, const float* b1_dev, float* b2_dev, const float* hp_dev,

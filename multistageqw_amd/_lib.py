@@ -30,7 +30,7 @@ EXPORTS = [
     "mqw_energy", "mqw_clenshaw_step_dev", "mqw_clenshaw", "mqw_solve", "mqw_solve_batch", "mqw_tune",
     "mqw_last_stats", "mqw_count_instances", "mqw_read_instances", "mqw_append_output", "mqw_features",
     "mqw_mlp_create", "mqw_mlp_destroy", "mqw_mlp_get_params", "mqw_mlp_set_params", "mqw_mlp_forward",
-    "mqw_mlp_loss_grad", "mqw_mlp_adam_step",
+    "mqw_mlp_loss_grad", "mqw_mlp_adam_step", "mqw_diag_moments",
 ]
 FEATURES = 32
 MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS = 32, 128, 10, 22026
@@ -102,6 +102,7 @@ def load():
         "mqw_chebyshev_coeffs": ([C.c_double, C.c_double, dp, C.c_int, ip], C.c_int),
         "mqw_generate_sk": ([C.c_int, C.c_int, C.c_uint64, dp], C.c_int),
         "mqw_energy": ([vp, C.c_int, dp, C.c_int, fp, dp, C.POINTER(EnergyInfo)], C.c_int),
+        "mqw_diag_moments": ([vp, C.c_int, dp, C.c_int, dp], C.c_int),
         "mqw_clenshaw_step_dev": ([vp, vp, vp, vp, vp, C.c_uint, C.c_float, C.c_float, C.c_float], C.c_int),
         "mqw_clenshaw": ([vp, C.c_int, fp, C.c_int, fp, fp, C.c_float, C.c_float, C.c_int], C.c_int),
         "mqw_solve": ([vp, C.c_int, C.c_int, dp, C.c_int, C.POINTER(Opts), fp, fp], C.c_int),

@@ -172,6 +172,14 @@ class Context:
                  e_init=info.e_init)
         return (hp, e, d) if want_e64 else (hp, d)
 
+    def diag_moments(self, n: int, rec, mode: int = L.ENERGY_COMPAT) -> dict:
+        """Mean, variance, skewness and excess kurtosis of the diagonal energies
+        (the moments MultiQW.cpp:375,379-380 has in closed form or commented out)."""
+        rec = np.ascontiguousarray(rec, np.float64)
+        out = np.zeros(4, np.float64)
+        self._check(self._lib.mqw_diag_moments(self._h, n, L.f64p(rec), mode, L.f64p(out)))
+        return dict(mean=out[0], variance=out[1], skewness=out[2], excess_kurtosis=out[3])
+
     # -- Clenshaw_step on device buffers (MultiQW.cpp:88-173) ---------------
     def clenshaw_step_ptr(self, b1: int, b2: int, hp: int, psi: int, n: int, scale: float, gamma: float,
                           coef: float):

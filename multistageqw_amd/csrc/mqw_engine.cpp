@@ -914,6 +914,44 @@ mqw_status mqw_energy(mqw_ctx* c, int n, const double* rec, int mode, float* hp,
     return MQW_OK;
 }
 
+// Moments of the walk's diagonal (the float energies H_P + shift): the
+// skewness and kurtosis MultiQW.cpp:379-380 leaves commented out, from FP64
+// power sums on the device in two passes (mean, then central sums).
+mqw_status mqw_diag_moments(mqw_ctx* c, int n, const double* rec, int mode, double* out)
+{
+    if (!c) return MQW_EINVAL;
+    if (!out) return fail(c, MQW_EINVAL, "null output");
+    mqw_energy_info info;
+    mqw_status st = mqw_energy(c, n, rec, mode, nullptr, nullptr, &info);
+    if (st) return st;
+    const size_t N = (size_t)1 << n;
+    const float* hpd = (const float*)dev_get(c, "energy_hp", sizeof(float) * N);
+    const int nb = diag_moment_blocks(N);
+    double* part = (double*)dev_get(c, "moments_part", sizeof(double) * 4 * (size_t)nb);
+    if (!hpd || !part) return fail(c, MQW_ENOMEM, "device allocation failed for moments");
+    std::vector<double> hpart((size_t)4 * nb);
+    auto sums = [&](double mu, double s[4]) -> mqw_status {
+        if (diag_power_sums(hpd, N, (double)info.shift, mu, part, c->stream))
+            return fail(c, MQW_EDEVICE, "moments kernel: %s", last_kernel_error());
+        HIPCHK(c, hipMemcpyAsync(hpart.data(), part, sizeof(double) * hpart.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int k = 0; k < 4; k++) s[k] = 0.0;
+        for (int b = 0; b < nb; b++)
+            for (int k = 0; k < 4; k++) s[k] += hpart[(size_t)b * 4 + k];
+        return MQW_OK;
+    };
+    double s[4];
+    if ((st = sums(0.0, s))) return st;
+    const double mean = s[0] / (double)N;
+    if ((st = sums(mean, s))) return st;
+    const double var = s[1] / (double)N;
+    out[0] = mean;
+    out[1] = var;
+    out[2] = var > 0 ? (s[2] / (double)N) / (var * sqrt(var)) : 0.0;
+    out[3] = var > 0 ? (s[3] / (double)N) / (var * var) - 3.0 : 0.0;
+    return MQW_OK;
+}
+
 mqw_status mqw_features(mqw_ctx* c, const mqw_instance* inst, int count, float* out)
 {
     if (!c) return MQW_EINVAL;

@@ -162,6 +162,9 @@ int lds_grid(int n, int device);
 const char* last_kernel_error();
 // Counts of the shifted diagonal in `bins` equal bins over [-r, r] (counts zeroed by the caller).
 int diag_histogram(const float* hp, uint64_t N, float r, int bins, uint32_t* counts, void* stream);
+// Power sums k = 1..4 of (hp + shift - mu): 4 doubles per block into part.
+int diag_moment_blocks(uint64_t N);
+int diag_power_sums(const float* hp, uint64_t N, double shift, double mu, double* part, void* stream);
 
 // ---------------------------------------------------------------------------
 // Context (opaque in the C ABI) and helpers shared by the host translation units.

@@ -133,6 +133,35 @@ hist_kernel(const float* __restrict__ hp, uint64_t N, float r, int bins, uint32_
         if (loc[k]) atomicAdd(&counts[k], loc[k]);
 }
 
+// Power sums of (x + shift - mu) for k = 1..4 over the diagonal, one
+// deterministic partial per workgroup (grid-stride, fixed tree reduction).
+__global__ void __launch_bounds__(256)
+moments_kernel(const float* __restrict__ hp, uint64_t N, double shift, double mu, double* __restrict__ part)
+{
+    __shared__ double red[4][256];
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < N; i += (uint64_t)gridDim.x * 256) {
+        const double d = (double)hp[i] + shift - mu;
+        const double d2 = d * d;
+        s1 += d;
+        s2 += d2;
+        s3 += d2 * d;
+        s4 += d2 * d2;
+    }
+    const int t = threadIdx.x;
+    red[0][t] = s1;
+    red[1][t] = s2;
+    red[2][t] = s3;
+    red[3][t] = s4;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 4; k++) red[k][t] += red[k][t + w];
+        __syncthreads();
+    }
+    if (t < 4) part[(uint64_t)blockIdx.x * 4 + t] = red[t][0];
+}
+
 static int ok() { return hipGetLastError() == hipSuccess ? 0 : -1; }
 static unsigned blocks(int n) { return (unsigned)((n + 255) / 256); }
 
@@ -179,6 +208,23 @@ int diag_histogram(const float* hp, uint64_t N, float r, int bins, uint32_t* cou
     uint64_t g = (N + 255) / 256;
     if (g > 4096) g = 4096;
     hipLaunchKernelGGL(hist_kernel, dim3((unsigned)g), dim3(256), 0, (hipStream_t)stream, hp, N, r, bins, counts);
+    return ok();
+}
+
+} // namespace mqw
+
+namespace mqw {
+
+int diag_moment_blocks(uint64_t N)
+{
+    uint64_t g = (N + 255) / 256;
+    return (int)(g > 1024 ? 1024 : g);
+}
+
+int diag_power_sums(const float* hp, uint64_t N, double shift, double mu, double* part, void* stream)
+{
+    hipLaunchKernelGGL(moments_kernel, dim3((unsigned)diag_moment_blocks(N)), dim3(256), 0, (hipStream_t)stream, hp,
+                       N, shift, mu, part);
     return ok();
 }
 
