@@ -25,7 +25,7 @@ MAX_M = 64
 
 def build() -> str:
     """Compile the oracle with its Makefile (gcc)."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
     return _LIB_PATH
 
 
