@@ -86,10 +86,11 @@ typedef struct {
                                 trajectories share every sweep launch */
     int variant;             /* -1 = tuned/default; else a sweep-kernel variant (loads in flight) */
     float* sample_probs;     /* optional output: count*samples per-sample probabilities */
-    int high_pass;           /* -1 = by n (default); 0 = off; 1, 2, 3 = on with pencil rows of 64, 128,
-                                256 B: before each sweep a pass sums the flips over the top 10, 9, 8
-                                bits in on-chip pencils, so the sweep loads no partner tiles for them
-                                (FP32 real-half sweeps, n >= 17) */
+    int high_pass;           /* -1 = by n (default); 0 = off; 1, 2, 3 = on with LDS pencil rows of 64,
+                                128, 256 B: before each sweep a pass sums the flips over the top 10, 9,
+                                8 bits in on-chip pencils, so the sweep loads no partner tiles for them;
+                                4 = on, register pencils (no LDS) for the 4..7 bits beyond the sweep's
+                                XCD chunk (FP32 real-half sweeps, n >= 17) */
     const uint32_t* mixer_masks; /* NULL = the reference's mixer sum_i X_i; else mixer_count flip masks:
                                     H = H_P - gamma * sum_k prod_{i in m_k} X_i (multi-X mixers, the
                                     reference's README.md:41 to-do); spectral bound E_abs + gamma * count.

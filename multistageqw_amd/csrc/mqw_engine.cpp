@@ -133,12 +133,25 @@ SweepCfg mqw::choose_sweep(const mqw_ctx* c, int precision, int n, int tile_bits
     }
     k.lc = std::max(0, std::min(k.lc, 24));
     // High-bit pass: from n = kHighPassMinN by default (measured), or as asked
-    // (high_pass = 1, 2, 3: pencil rows of 64, 128, 256 B).  It takes up to
-    // 10 - wb of the outer bits; the sweep keeps partner loads for the rest.
-    if (k.half && high_pass != 0) {
-        const int wb = high_pass > 0 ? std::min(high_pass, 3) - 1 : kHighPassWb;
-        const int hi = std::min(10 - wb, n - k.tb);
-        if ((high_pass > 0 || n >= kHighPassMinN) && high_pass_supported(n, hi, wb)) {
+    // (high_pass = 1, 2, 3: LDS pencil rows of 64, 128, 256 B, up to 10 - wb
+    // of the outer bits; 4: register pencils for the 4..7 bits beyond the XCD
+    // chunk).  The sweep keeps partner loads for the rest.
+    const bool pass_on = high_pass > 0 || (high_pass < 0 && n >= kHighPassMinN);
+    if (k.half && pass_on) {
+        int wb, hi;
+        // by n: the register variant while the bits beyond the chunk are at
+        // most 7 (measured 3 % faster at n = 25, 26), else 128 B LDS pencils
+        if (high_pass < 0 && n - k.tb - k.lc <= 7) high_pass = 4;
+        if (high_pass == 4) {
+            // register variant: just the outer bits beyond the XCD chunk, 4..7 of them
+            wb = -1;
+            hi = std::max(4, std::min(7, n - k.tb - k.lc));
+        } else {
+            wb = high_pass > 0 ? std::min(high_pass, 3) - 1 : kHighPassWb;
+            hi = std::min(10 - wb, n - k.tb);
+        }
+        hi = std::min(hi, n - k.tb);
+        if (high_pass_supported(n, hi, wb)) {
             k.hi = hi;
             k.hwb = wb;
         }

@@ -1242,10 +1242,64 @@ high_pass_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n)
     }
 }
 
+// Register variant of the high-bit pass for up to 7 top bits, without LDS or
+// barriers: one wave per pencil of 32 amplitudes (128 B rows) by 2^H rows,
+// H = 3 + RB.  Lane l holds columns (l & 7) of rows (q << 3) | (l >> 3), so
+// each load instruction reads 8 whole 128 B lines; the three lane row bits
+// are exchanged by cross-lane shuffles, the RB others in registers.
+template <int RB>
+__global__ void __launch_bounds__(256)
+high_pass_reg_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n)
+{
+    using f4 = NVec16<float>::type;
+    constexpr int H = 3 + RB;
+    constexpr int Q = 1 << RB;
+    const int p0 = n - H;
+    const uint64_t npen = 1ull << (p0 - 5);
+    const int lane = threadIdx.x & 63;
+    const uint64_t unit = ((uint64_t)blockIdx.x << 2) + (threadIdx.x >> 6); // one pencil per wave
+    const uint64_t job = unit / (2 * npen);
+    if (job >= njobs) return; // uniform per wave; the kernel has no barriers
+    const TermJob<float>& J = jobs[job];
+    if (!(J.flags & F_HSUM)) return;
+    const uint64_t rem = unit - job * 2 * npen;
+    const uint64_t h = rem / npen, pen = rem - h * npen;
+    const uint64_t N = 1ull << n;
+    const float* src = (((J.flags & 3u) == SRC1_PSI) ? J.psi : J.b1) + h * N + (pen << 5);
+    float* dst = J.hsum + h * N + (pen << 5);
+    const int col = lane & 7, rl = lane >> 3;
+
+    f4 x[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        x[q] = *gview(reinterpret_cast<const f4*>(src + ((uint64_t)((q << 3) | rl) << p0)) + col);
+    f4 acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        acc[q] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int b = 0; b < RB; b++) acc[q] += x[q ^ (1 << b)];
+    }
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            f4 y;
+#pragma unroll
+            for (int w = 0; w < 4; w++) y[w] = __shfl_xor(x[q][w], 8 << b, 64);
+            acc[q] += y;
+        }
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        __builtin_nontemporal_store(acc[q],
+                                    gview(reinterpret_cast<f4*>(dst + ((uint64_t)((q << 3) | rl) << p0)) + col));
+}
+
 // wb: row of 64 * 2^wb bytes; hi + wb <= 10 keeps the rows in 64 KiB of LDS
 int high_pass_supported(int n, int hi, int wb)
 {
-    return wb >= 0 && wb <= 2 && hi >= 4 && hi + wb <= 10 && n - hi >= 4 + wb;
+    if (wb < 0) return hi >= 4 && hi <= 7 && n - hi >= 5; // register variant, 128 B rows
+    return wb <= 2 && hi >= 4 && hi + wb <= 10 && n - hi >= 4 + wb;
 }
 
 // pencils per workgroup (1 measured best: a second, prefetched pencil was
@@ -1281,6 +1335,20 @@ int launch_high_pass(int hi, int wb, const void* jobs_dev, uint32_t njobs, int n
 {
     if (!high_pass_supported(n, hi, wb)) return -2;
     hipStream_t st = (hipStream_t)stream;
+    if (wb < 0) {
+        const uint64_t units = (uint64_t)njobs * 2ull * (1ull << (n - hi - 5));
+        const uint64_t grid = (units + 3) / 4; // 4 waves, one pencil each
+        if (grid == 0) return 0;
+        if (grid > 0x7fffffffull) return -3;
+        const TermJob<float>* j = (const TermJob<float>*)jobs_dev;
+        switch (hi) {
+        case 4: hipLaunchKernelGGL(high_pass_reg_kernel<1>, dim3((uint32_t)grid), dim3(256), 0, st, j, njobs, n); break;
+        case 5: hipLaunchKernelGGL(high_pass_reg_kernel<2>, dim3((uint32_t)grid), dim3(256), 0, st, j, njobs, n); break;
+        case 6: hipLaunchKernelGGL(high_pass_reg_kernel<3>, dim3((uint32_t)grid), dim3(256), 0, st, j, njobs, n); break;
+        default: hipLaunchKernelGGL(high_pass_reg_kernel<4>, dim3((uint32_t)grid), dim3(256), 0, st, j, njobs, n); break;
+        }
+        return check_launch("high_pass_reg");
+    }
     const uint64_t units = (uint64_t)njobs * 2ull * (1ull << (n - hi - 4 - wb));
     if (units == 0) return 0;
     const uint64_t wgs = (units + kHighPassPPW - 1) / kHighPassPPW;

@@ -3,7 +3,7 @@
 //      XCD chunk bits) times a few launches on synthetic term jobs shaped
 //      like a batched solve;
 //   2. the fastest few of those, the measured default configuration and,
-//      from n = 22, the same with the high-bit pass, each run a real solve of
+//      from n = 22, the high-bit pass (LDS and register pencils), each run a real solve of
 //      a generated SK instance (m = 3, enough samples for about 2^26
 //      amplitudes in flight); the fastest by device time wins.
 // The choice is kept in the context and, given a cache path, recorded as a line
@@ -195,15 +195,16 @@ extern "C" mqw_status mqw_tune(mqw_ctx* c, int n, const char* cache_path, int* t
         for (size_t i = 0; i < order.size(); i++) order[i] = i;
         std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return gbs[a] > gbs[b]; });
         // stage 2 candidates: the 5 fastest synthetic ones, the default, and
-        // from n = 22 the high-bit pass with 2^14 tiles and with the 2
-        // fastest real-half candidates
+        // from n = 22 the high-bit pass (LDS and register pencils) with 2^14
+        // tiles and the LDS pass with the 2 fastest real-half candidates
         std::vector<Cand> real;
         for (size_t i = 0; i < order.size() && real.size() < 5; i++)
             if (gbs[order[i]] > 0) real.push_back(cands[order[i]]);
         const SweepCfg d = choose_sweep(c, MQW_FP32, n, 0, -1, -1, -1);
-        real.push_back({d.tb, d.variant, d.lc, d.hi > 0 ? d.hwb + 1 : 0});
+        real.push_back({d.tb, d.variant, d.lc, d.hi > 0 ? (d.hwb < 0 ? 4 : d.hwb + 1) : 0});
         if (n >= 22 && d.half) {
             real.push_back({14, 4, 5, 2});
+            real.push_back({14, 4, 5, 4});
             int added = 0;
             for (size_t i = 0; i < order.size() && added < 2; i++)
                 if (cands[order[i]].tb >= 13 && gbs[order[i]] > 0) {

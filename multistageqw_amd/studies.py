@@ -51,12 +51,12 @@ def precision(ctx, n, m, samples, seed):
                 fp32=r32, fp64=r64)
 
 
-def large(ctx, n, m, samples, seed, warm=True):
+def large(ctx, n, m, samples, seed, warm=True, **opts):
     rec = generate_sk(n, 1, seed)
     if warm:
-        ctx.solve(n, m, rec, samples=1)
+        ctx.solve(n, m, rec, samples=1, **opts)
     t0 = time.time()
-    p, _ = ctx.solve(n, m, rec, samples=samples)
+    p, _ = ctx.solve(n, m, rec, samples=samples, **opts)
     return dict(study="large", m=m, samples=samples, p=float(p[0]), **_apply_line(ctx, n, time.time() - t0))
 
 
@@ -133,15 +133,17 @@ def main(argv=None):
     ap.add_argument("--fp64-every", type=int, default=0)
     ap.add_argument("--tune-cache", default=None)
     ap.add_argument("--no-warm", action="store_true", help="large: skip the warm-up solve (one solve to profile)")
+    ap.add_argument("--high-pass", type=int, default=-1, help="large/sweep: mqw_opts.high_pass (-1 by n)")
     a = ap.parse_args(argv)
     ctx = Context(0)
     if a.study == "precision":
         print(json.dumps(precision(ctx, a.n, a.m, a.samples, a.seed)), flush=True)
     elif a.study == "large":
-        print(json.dumps(large(ctx, a.n, a.m, a.samples, a.seed, warm=not a.no_warm)), flush=True)
+        print(json.dumps(large(ctx, a.n, a.m, a.samples, a.seed, warm=not a.no_warm, high_pass=a.high_pass)),
+              flush=True)
     elif a.study == "sweep":
         for n in a.ns:
-            print(json.dumps(large(ctx, n, a.m, a.samples, a.seed)), flush=True)
+            print(json.dumps(large(ctx, n, a.m, a.samples, a.seed, high_pass=a.high_pass)), flush=True)
     elif a.study == "batch":
         print(json.dumps(batch(ctx, a.n, a.m, a.count, a.samples, a.seed)), flush=True)
     else:

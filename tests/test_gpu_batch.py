@@ -90,6 +90,6 @@ def test_autotuner_weighs_the_high_bit_pass(ctx, tmp_path):
     _say("tune n=23")
     cache = str(tmp_path / "tune23.txt")
     cfg = ctx.tune(n, cache)
-    assert cfg["high_pass"] in (0, 2)
+    assert cfg["high_pass"] in (0, 2, 4)
     f = open(cache).read().split()
     assert int(f[1]) == n and int(f[5]) == cfg["high_pass"] and float(f[6]) > 0

@@ -162,10 +162,11 @@ def test_workgroup_modes_are_bit_identical(ctx, variant):
 
 
 @pytest.mark.parametrize("n,m,tb,samples,hp", [(17, 2, 13, 6, 1), (19, 3, 13, 3, 1), (21, 2, 14, 2, 1),
-                                                (18, 1, 14, 6, 1), (19, 2, 13, 3, 2), (21, 3, 13, 2, 3)])
+                                                (18, 1, 14, 6, 1), (19, 2, 13, 3, 2), (21, 3, 13, 2, 3),
+                                                (19, 2, 13, 3, 4), (21, 3, 14, 2, 4)])
 def test_high_bit_pass_vs_oracle(ctx, oracle, n, m, tb, samples, hp):
-    """The sweep with the high-bit pass: the top min(11 - hp, n - tb) bits summed in LDS pencils
-    with rows of 32 * 2^hp bytes."""
+    """The sweep with the high-bit pass: hp 1..3, the top min(11 - hp, n - tb) bits summed in LDS
+    pencils with rows of 32 * 2^hp bytes; hp 4, the register variant (4..7 bits, 128 B rows)."""
     _say("high-bit pass n", n, "m", m, "tb", tb, "samples", samples, "rows", 32 << hp)
     recs = M.generate_sk(n, 2, 500 + n)
     p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, tile_bits=tb,
