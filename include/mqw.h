@@ -97,10 +97,14 @@ typedef struct {
                                     The heuristic schedule still assumes sum_i X_i: pass schedules
                                     through mqw_instance.gamma/short_t. */
     int mixer_count;         /* 1..64 masks, each nonzero and < 2^n */
+    int fixed_times;         /* 0 = Monte-Carlo stage times U(t, 2t) as MultiQW.cpp:417-430; 1 = every
+                                sample runs each stage for exactly short_t (deterministic schedules,
+                                e.g. a time-dependent anneal split into stages) */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are
- * non-NULL (length m): the learned-schedule path. */
+ * non-NULL (length m): the learned-schedule path.  Override gammas may be any
+ * finite value (the spectral bound uses |gamma|); times must be positive. */
 typedef struct {
     int n, m;
     const double* rec;    /* n(n+1)/2 doubles in the reference file layout */

@@ -57,7 +57,7 @@ class Opts(C.Structure):
                 ("tile_bits", C.c_int), ("chunk_bits", C.c_int), ("small_n_max", C.c_int),
                 ("use_graph", C.c_int), ("max_device_bytes", C.c_size_t), ("group_bytes", C.c_size_t), ("variant", C.c_int),
                 ("sample_probs", C.POINTER(C.c_float)), ("high_pass", C.c_int),
-                ("mixer_masks", C.POINTER(C.c_uint32)), ("mixer_count", C.c_int)]
+                ("mixer_masks", C.POINTER(C.c_uint32)), ("mixer_count", C.c_int), ("fixed_times", C.c_int)]
 
 
 class Instance(C.Structure):

@@ -593,7 +593,8 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         auto seg = [&](int i, int j, int64_t slot) {
             Segment sg;
             sg.gamma = pr.sch.gamma[i];
-            sg.onenorm = fmaf(sg.gamma, mix_norm, pr.info.e_abs);
+            // spectral bound; |gamma| so schedules that cross zero keep a valid bound
+            sg.onenorm = fmaf(fabsf(sg.gamma), mix_norm, pr.info.e_abs);
             const double s = (double)(sg.onenorm * pr.times[(size_t)i * samples + j]);
             std::vector<double> cc;
             cheb_coeffs_vec(s, o.tol, cc);
@@ -729,7 +730,7 @@ mqw_status solve_problems(mqw_ctx* c, std::vector<Problem>& probs_in, const mqw_
 }
 
 mqw_status prepare_problem(mqw_ctx* c, Problem& p, int n, int m, const double* rec, const float* gamma,
-                           const float* short_t, int samples)
+                           const float* short_t, int samples, int fixed_times)
 {
     if (n < MQW_MIN_N || n > MQW_MAX_N) return fail(c, MQW_EINVAL, "n=%d outside [%d, %d]", n, MQW_MIN_N, MQW_MAX_N);
     if (m < 1 || m > MQW_MAX_STAGES) return fail(c, MQW_EINVAL, "m=%d outside [1, %d]", m, MQW_MAX_STAGES);
@@ -743,8 +744,8 @@ mqw_status prepare_problem(mqw_ctx* c, Problem& p, int n, int m, const double* r
     if (gamma && short_t) {
         float tot = 0;
         for (int i = 0; i < m; i++) {
-            if (!(gamma[i] > 0) || !(short_t[i] > 0) || !isfinite(gamma[i]) || !isfinite(short_t[i]))
-                return fail(c, MQW_EINVAL, "schedule override must be positive and finite");
+            if (!isfinite(gamma[i]) || !(short_t[i] > 0) || !isfinite(short_t[i]))
+                return fail(c, MQW_EINVAL, "schedule override: gamma finite, times positive and finite");
             p.sch.gamma[i] = gamma[i];
             p.sch.short_t[i] = short_t[i];
             tot += short_t[i];
@@ -752,7 +753,13 @@ mqw_status prepare_problem(mqw_ctx* c, Problem& p, int n, int m, const double* r
         p.sch.total_t = tot;
     }
     p.times.assign((size_t)m * samples, 0.0f);
-    sample_times(m, samples, p.sch.short_t, p.times.data());
+    if (fixed_times) {
+        // every sample runs each stage for exactly short_t (a deterministic schedule)
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < samples; j++) p.times[(size_t)i * samples + j] = p.sch.short_t[i];
+    } else {
+        sample_times(m, samples, p.sch.short_t, p.times.data());
+    }
     return MQW_OK;
 }
 
@@ -1120,7 +1127,7 @@ mqw_status mqw_solve(mqw_ctx* c, int n, int m, const double* recs, int count, co
     const int P = n * (n + 1) / 2;
     std::vector<Problem> ps((size_t)count);
     for (int k = 0; k < count; k++) {
-        mqw_status st = prepare_problem(c, ps[k], n, m, recs + (size_t)k * P, nullptr, nullptr, o.samples);
+        mqw_status st = prepare_problem(c, ps[k], n, m, recs + (size_t)k * P, nullptr, nullptr, o.samples, o.fixed_times);
         if (st) return st;
         ps[k].out = k;
     }
@@ -1142,7 +1149,7 @@ mqw_status mqw_solve_batch(mqw_ctx* c, const mqw_instance* inst, int count, cons
     std::vector<Problem> ps((size_t)count);
     for (int k = 0; k < count; k++) {
         mqw_status st = prepare_problem(c, ps[k], inst[k].n, inst[k].m, inst[k].rec, inst[k].gamma, inst[k].short_t,
-                                        o.samples);
+                                        o.samples, o.fixed_times);
         if (st) return st;
         ps[k].out = k;
     }
