@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmqw.so")
 CLI_PATH = os.path.join(HERE, "mqw")
 
-ABI_VERSION = 3  # MQW_ABI_VERSION of include/mqw.h this binding matches
+ABI_VERSION = 4  # MQW_ABI_VERSION of include/mqw.h this binding matches
 MAX_STAGES = 64
 DEFAULT_SAMPLES = 100
 DEFAULT_TOL = 1e-6
@@ -57,7 +57,8 @@ class Opts(C.Structure):
                 ("tile_bits", C.c_int), ("chunk_bits", C.c_int), ("small_n_max", C.c_int),
                 ("use_graph", C.c_int), ("max_device_bytes", C.c_size_t), ("group_bytes", C.c_size_t), ("variant", C.c_int),
                 ("sample_probs", C.POINTER(C.c_float)), ("high_pass", C.c_int),
-                ("mixer_masks", C.POINTER(C.c_uint32)), ("mixer_count", C.c_int), ("fixed_times", C.c_int)]
+                ("mixer_masks", C.POINTER(C.c_uint32)), ("mixer_count", C.c_int), ("fixed_times", C.c_int),
+                ("shared_first_stage", C.c_int)]
 
 
 class Instance(C.Structure):
@@ -69,7 +70,8 @@ class Stats(C.Structure):
     _fields_ = [("sweeps", C.c_uint64), ("term_units", C.c_uint64), ("amp_terms", C.c_uint64),
                 ("algorithmic_bytes", C.c_double), ("gpu_ms", C.c_double), ("sweep_ms", C.c_double),
                 ("lds_trajectories", C.c_uint64), ("psi_const_amp_terms", C.c_uint64),
-                ("required_bytes", C.c_double)]
+                ("required_bytes", C.c_double), ("aux_ms", C.c_double), ("aux_bytes", C.c_double),
+                ("aux_launches", C.c_uint64)]
 
 
 _lib = None

@@ -56,8 +56,8 @@ def build(verbose: bool = False) -> str:
         path = os.path.join(CSRC, src)
         if os.path.exists(path):
             # host-only C++ (no device pass) against the HIP runtime API
-            steps.append([HIPCC, "-x", "c++", *COMMON, "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-c", path,
-                          "-o", _obj(src)])
+            steps.append([HIPCC, "-x", "c++", *COMMON, "-ffp-contract=off", "-pthread", "-D__HIP_PLATFORM_AMD__", "-c",
+                          path, "-o", _obj(src)])
     with ThreadPoolExecutor(max_workers=min(8, len(steps))) as ex:
         outs = list(ex.map(_run, steps))
     if verbose:
@@ -66,7 +66,7 @@ def build(verbose: bool = False) -> str:
                 print(o)
     objs = [s[-1] for s in steps]
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     copy_src = os.path.join(CSRC, "hbm_copy.hip")
     _run([HIPCC, "-x", "hip", f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, "-o", os.path.join(HERE, "hbm_copy"),

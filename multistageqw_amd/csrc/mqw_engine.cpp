@@ -23,6 +23,8 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "mqw_internal.h"
@@ -186,6 +188,7 @@ struct Problem {
 
 struct Segment {
     float gamma, onenorm;
+    double s = 0.0;        // onenorm * t: the argument of the coefficients
     std::vector<double> c; // FP64 coefficients, K = c.size()
     int64_t prob_slot;     // -1 = no readout
 };
@@ -194,7 +197,37 @@ struct Traj {
     int prob;                     // index into the group's problem list
     std::vector<Segment> segs;
     const void* psi_in = nullptr; // input of the first stage; nullptr = |+>
+    bool first_done = false;      // the first stage comes from the shared basis (a combine op)
 };
+
+// Chebyshev coefficients of every segment.  Segments are independent (the
+// Miller scratch of cheb_coeffs_vec is per thread), so large batches spread
+// over up to 16 host threads; the values do not depend on the split.
+void fill_coeffs(std::vector<Traj>& trajs, double tol, bool to_float)
+{
+    std::vector<Segment*> segs;
+    for (Traj& tr : trajs)
+        for (Segment& sg : tr.segs) segs.push_back(&sg);
+    auto work = [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; i++) {
+            Segment& sg = *segs[i];
+            cheb_coeffs_vec(sg.s, tol, sg.c);
+            if (to_float)
+                for (double& v : sg.c) v = (double)(float)v; // the reference casts to float (:440)
+        }
+    };
+    const size_t total = segs.size();
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t nt = std::min<size_t>({(size_t)(hw ? hw : 1), 16, total / 1024 + 1});
+    if (nt <= 1) {
+        work(0, total);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; t++) th.emplace_back(work, total * t / nt, total * (t + 1) / nt);
+    work(0, total / nt);
+    for (std::thread& x : th) x.join();
+}
 
 // ---------------------------------------------------------------------------
 // Energies for a group of problems of one n: E (float) or E64 (double) per
@@ -258,15 +291,22 @@ mqw_status group_energies(mqw_ctx* c, int n, std::vector<Problem*>& ps, int mode
 // (F_READOUT); a reduce launch per group reads out the last stage.
 // ---------------------------------------------------------------------------
 struct Op {
-    int kind; // 0 = term launch, 1 = reduce launch, 2 = high-bit pass over the jobs of the next term launch
+    int kind; // 0 = term launch, 1 = reduce launch, 2 = high-bit pass over the jobs of the next term launch,
+              // 3 = combine launch (shared first stage), 4 = real-half term launch (first-stage basis)
     size_t off;
     uint32_t count;
+    uint32_t aux = 0; // kind 3: largest basis size of its jobs
 };
 
 template <typename real>
 struct Plan {
     std::vector<TermJob<real>> jobs;
     std::vector<ReduceJob> rjobs;
+    std::vector<CombineJob> cjobs;
+    std::vector<CombineItem> citems;
+    std::vector<float> ccoef;
+    double aux_bytes = 0.0;   // algorithmic bytes of the combine launches
+    uint64_t aux_launches = 0;
     std::vector<Op> ops;
     uint64_t amp_terms = 0;
     uint64_t psi_const_amp_terms = 0;
@@ -290,8 +330,18 @@ void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, cons
     const real psi0 = (real)(float)(1 / sqrt((double)N));
     int p = -1, a = 0, b = 1, par = 0;
     int64_t prev_slot = -1;
+    size_t s0 = 0;
+    if (tr.first_done) {
+        // a combine op left the first stage's output in st[0] and its partial norms in part[0]
+        p = 0;
+        a = 1;
+        b = 2;
+        par = 1;
+        prev_slot = tr.segs[0].prob_slot;
+        s0 = 1;
+    }
     auto rc = [](double v) { return (real)v; };
-    for (size_t s = 0; s < tr.segs.size(); s++) {
+    for (size_t s = s0; s < tr.segs.size(); s++) {
         const Segment& sg = tr.segs[s];
         const std::vector<double>& cc = sg.c;
         const int K = (int)cc.size();
@@ -381,14 +431,69 @@ void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, cons
     }
 }
 
+// Per-problem shared first-stage bases (FP32): V (nullptr = none) and kc.
+struct Bases {
+    std::vector<const float*> V;
+    std::vector<int> kc;
+};
+
+// Combine jobs for the first_done members of one group: per problem, batches
+// of up to kCombineBatch samples, each writing its slot's st[0] and part[0].
+template <typename real>
+void combine_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_t>& grp, const Bases& B,
+                 const std::vector<Slot<real>>& slots, Plan<real>& P)
+{
+    if constexpr (std::is_same<real, float>::value) {
+        const size_t N = (size_t)1 << n;
+        std::map<int, std::vector<size_t>> byprob; // problem -> member indices
+        for (size_t i = 0; i < grp.size(); i++)
+            if (trajs[grp[i]].first_done) byprob[trajs[grp[i]].prob].push_back(i);
+        if (byprob.empty()) return;
+        Op op{3, P.cjobs.size(), 0, 0};
+        for (const auto& kv : byprob) {
+            const int pr = kv.first, kc = B.kc[pr];
+            const std::vector<size_t>& mem = kv.second;
+            for (size_t i0 = 0; i0 < mem.size(); i0 += kCombineBatch) {
+                const size_t ns = std::min<size_t>(kCombineBatch, mem.size() - i0);
+                CombineJob cj{};
+                cj.V = B.V[pr];
+                cj.kc = (uint32_t)kc;
+                cj.ns = (uint32_t)ns;
+                cj.item_off = (uint32_t)P.citems.size();
+                cj.psi0 = (float)(1 / sqrt((double)N));
+                for (size_t q = 0; q < ns; q++) {
+                    const size_t i = mem[i0 + q];
+                    const std::vector<double>& cc = trajs[grp[i]].segs[0].c;
+                    CombineItem it{};
+                    it.out = slots[i].st[0];
+                    it.norm_part = slots[i].part[0];
+                    it.coef_off = (uint32_t)P.ccoef.size();
+                    for (int k = 0; k < kc; k++) P.ccoef.push_back(k < (int)cc.size() ? (float)cc[k] : 0.0f);
+                    P.citems.push_back(it);
+                }
+                P.cjobs.push_back(cj);
+                op.count++;
+                op.aux = std::max<uint32_t>(op.aux, (uint32_t)kc);
+                // reads the kc - 1 stored vectors once, writes both halves of every sample
+                P.aux_bytes += (double)N * (4.0 * (kc - 1) + 8.0 * (double)ns);
+            }
+        }
+        P.ops.push_back(op);
+        P.aux_launches++;
+    } else {
+        (void)n, (void)trajs, (void)grp, (void)B, (void)slots, (void)P;
+    }
+}
+
 // Groups of trajectories (indices into trajs), each run to completion before the next.
 template <typename real>
 void build_plan(int n, uint32_t parts, const std::vector<Traj>& trajs, const std::vector<std::vector<size_t>>& groups,
                 const std::vector<Problem*>& ps, const real* E, const std::vector<Slot<real>>& slots, double* probs,
-                Plan<real>& P, int hi = 0)
+                Plan<real>& P, int hi = 0, const Bases* B = nullptr)
 {
     const size_t N = (size_t)1 << n;
     for (const auto& grp : groups) {
+        if (B) combine_ops<real>(n, trajs, grp, *B, slots, P);
         std::vector<std::vector<TermJob<real>>> seq(grp.size());
         std::vector<ReduceJob> fins;
         size_t L = 0;
@@ -430,6 +535,21 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
     TermJob<real>* jd = (TermJob<real>*)dev_get(c, "plan_jobs", sizeof(TermJob<real>) * P.jobs.size());
     ReduceJob* rd = (ReduceJob*)dev_get(c, "plan_rjobs", sizeof(ReduceJob) * std::max<size_t>(1, P.rjobs.size()));
     if (!jd || !rd) return fail(c, MQW_ENOMEM, "device allocation failed for %zu sweep jobs", P.jobs.size());
+    CombineJob* cd = nullptr;
+    CombineItem* ci = nullptr;
+    float* cf = nullptr;
+    if (!P.cjobs.empty()) {
+        cd = (CombineJob*)dev_get(c, "plan_cjobs", sizeof(CombineJob) * P.cjobs.size());
+        ci = (CombineItem*)dev_get(c, "plan_citems", sizeof(CombineItem) * P.citems.size());
+        cf = (float*)dev_get(c, "plan_ccoef", sizeof(float) * P.ccoef.size());
+        if (!cd || !ci || !cf) return fail(c, MQW_ENOMEM, "device allocation failed for combine jobs");
+        HIPCHK(c, hipMemcpyAsync(cd, P.cjobs.data(), sizeof(CombineJob) * P.cjobs.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(ci, P.citems.data(), sizeof(CombineItem) * P.citems.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(cf, P.ccoef.data(), sizeof(float) * P.ccoef.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+    }
     HIPCHK(c, hipMemcpyAsync(jd, P.jobs.data(), sizeof(TermJob<real>) * P.jobs.size(), hipMemcpyHostToDevice,
                              c->stream));
     if (!P.rjobs.empty())
@@ -442,6 +562,10 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
             int rc;
             if (op.kind == 1)
                 rc = launch_reduce(precision, rd + op.off, op.count, c->stream);
+            else if (op.kind == 3)
+                rc = launch_combine(cd + op.off, op.count, ci, cf, n, k.parts, op.aux, c->stream);
+            else if (op.kind == 4)
+                rc = launch_half_real(k.tb, k.variant, jd + op.off, op.count, n, k.lc, c->stream);
             else if (op.kind == 2)
                 rc = launch_high_pass(k.hi, k.hwb, jd + op.off, op.count, n, c->stream);
             else if (k.nmask > 0)
@@ -553,6 +677,72 @@ mqw_status run_lds(mqw_ctx* c, int n, const std::vector<Traj>& trajs, const std:
 }
 
 // ---------------------------------------------------------------------------
+// Shared first-stage bases: V_{q+1} = 2 Ht V_q - V_{q-1} (V_1 = Ht |+>) on the
+// real half only, for every problem with bases.kc > 1; launch q holds step q
+// of every problem still recurring.  Ht = (H_P - gamma_1 H_G) / onenorm_1 with
+// the first stage's gamma and bound, as that stage's Clenshaw terms use.
+// ---------------------------------------------------------------------------
+mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Problem*>& ps,
+                     const std::vector<Traj>& trajs, const float* E, const Bases& B, bool use_graph)
+{
+    const size_t N = (size_t)1 << n;
+    const float psi0 = (float)(1 / sqrt((double)N));
+    std::vector<const Segment*> seg0(ps.size(), nullptr);
+    for (const Traj& tr : trajs)
+        if (!seg0[tr.prob]) seg0[tr.prob] = &tr.segs[0];
+    int steps = 0;
+    for (size_t p = 0; p < ps.size(); p++) steps = std::max(steps, B.kc[p] - 1);
+    Plan<float> P;
+    double bytes = 0.0;
+    for (int q = 0; q < steps; q++) {
+        Op op{4, P.jobs.size(), 0};
+        for (size_t p = 0; p < ps.size(); p++) {
+            if (q >= B.kc[p] - 1) continue;
+            const Segment& sg = *seg0[p];
+            float* V = const_cast<float*>(B.V[p]);
+            TermJob<float> j{};
+            j.E = E + p * N;
+            j.psi0 = psi0;
+            j.w1r = 1.0f;
+            const double sc = (double)sg.onenorm * (q == 0 ? 1.0 : 0.5);
+            j.alpha = (float)(1.0 / sc);
+            j.beta = (float)((double)sg.gamma / sc);
+            j.out = V + (size_t)q * N;
+            uint32_t flags = F_PSI_CONST;
+            if (q == 0) {
+                flags |= SRC1_PSI | SRC2_ZERO;
+                bytes += 8.0 * N;
+            } else {
+                flags |= SRC1_ARRAY;
+                j.b1 = V + (size_t)(q - 1) * N;
+                j.w2r = 1.0f;
+                if (q == 1) {
+                    flags |= SRC2_PSI;
+                    bytes += 12.0 * N;
+                } else {
+                    flags |= SRC2_ARRAY;
+                    j.b2 = V + (size_t)(q - 2) * N;
+                    bytes += 16.0 * N;
+                }
+            }
+            j.flags = flags;
+            P.jobs.push_back(j);
+            op.count++;
+        }
+        P.ops.push_back(op);
+    }
+    if (P.jobs.empty()) return MQW_OK;
+    double ms = 0.0;
+    mqw_status st = run_plan<float>(c, MQW_FP32, n, k, P, use_graph, &ms);
+    if (st) return st;
+    c->stats.gpu_ms += ms;
+    c->stats.aux_ms += ms;
+    c->stats.aux_bytes += bytes;
+    c->stats.aux_launches += P.ops.size();
+    return MQW_OK;
+}
+
+// ---------------------------------------------------------------------------
 // One group of problems of the same n.
 // ---------------------------------------------------------------------------
 template <typename real>
@@ -595,12 +785,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             sg.gamma = pr.sch.gamma[i];
             // spectral bound; |gamma| so schedules that cross zero keep a valid bound
             sg.onenorm = fmaf(fabsf(sg.gamma), mix_norm, pr.info.e_abs);
-            const double s = (double)(sg.onenorm * pr.times[(size_t)i * samples + j]);
-            std::vector<double> cc;
-            cheb_coeffs_vec(s, o.tol, cc);
-            if (precision == MQW_FP32)
-                for (double& v : cc) v = (double)(float)v; // the reference casts to float (:440)
-            sg.c = std::move(cc);
+            sg.s = (double)(sg.onenorm * pr.times[(size_t)i * samples + j]);
             sg.prob_slot = slot;
             return sg;
         };
@@ -619,6 +804,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             }
         }
     }
+    fill_coeffs(trajs, o.tol, precision == MQW_FP32);
 
     double* probs_d = (double*)dev_get(c, "probs", sizeof(double) * probs.size());
     if (!probs_d) return fail(c, MQW_ENOMEM, "device allocation failed for probabilities");
@@ -633,19 +819,59 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         return MQW_OK;
     }
 
-    // groups sized to the memory-side cache, longest trajectories first
     const uint32_t nparts = k.parts;
+    // shared first stage: one real Chebyshev basis per instance with m > 1
+    Bases bases;
+    size_t basis_bytes = 0;
+    if (std::is_same<real, float>::value && o.shared_first_stage && k.half && !mixer && n >= k.tb &&
+        (N / nparts) % 256 == 0) {
+        const size_t cap = o.max_device_bytes ? o.max_device_bytes / 4 : ((size_t)16 << 30);
+        bases.V.assign(ps.size(), nullptr);
+        bases.kc.assign(ps.size(), 0);
+        for (const Traj& tr : trajs)
+            if (ps[tr.prob]->m > 1) bases.kc[tr.prob] = std::max(bases.kc[tr.prob], (int)tr.segs[0].c.size());
+        std::vector<size_t> off(ps.size(), 0);
+        for (size_t p = 0; p < ps.size(); p++) {
+            const int kc = bases.kc[p];
+            const size_t b = sizeof(float) * N * (size_t)std::max(kc - 1, 0);
+            if (kc < 1 || kc > kCombineMaxK || basis_bytes + b > cap) {
+                bases.kc[p] = 0;
+                continue;
+            }
+            off[p] = basis_bytes;
+            basis_bytes += b;
+        }
+        float* V = basis_bytes ? (float*)dev_get(c, "basis", basis_bytes) : nullptr;
+        if (basis_bytes && !V) return fail(c, MQW_ENOMEM, "device allocation failed for first-stage bases");
+        bool any = false;
+        for (size_t p = 0; p < ps.size(); p++)
+            if (bases.kc[p] > 0) {
+                bases.V[p] = (const float*)((const char*)V + off[p]);
+                any = true;
+            }
+        if (any) {
+            st = run_bases(c, n, k, ps, trajs, (const float*)E, bases, o.use_graph != 0);
+            if (st) return st;
+            for (Traj& tr : trajs)
+                if (bases.kc[tr.prob] > 0) tr.first_done = true;
+        }
+    }
+    const bool use_bases = basis_bytes > 0;
+
+    // groups sized to the memory-side cache, longest trajectories first
     const size_t nst = k.hi > 0 ? 8 : 6; // planar states per slot (+ the high-bit sums)
     const size_t per_traj = sizeof(real) * nst * N + sizeof(double) * 2 * nparts;
     const size_t budget = o.group_bytes ? o.group_bytes : ((size_t)32 << 30);
     size_t G = std::max<size_t>(1, budget / per_traj);
-    if (o.max_device_bytes) G = std::min(G, std::max<size_t>(1, o.max_device_bytes / per_traj));
+    if (o.max_device_bytes)
+        G = std::min(G, std::max<size_t>(1, (o.max_device_bytes - std::min(o.max_device_bytes, basis_bytes)) / per_traj));
     G = std::min(G, trajs.size());
     std::vector<size_t> order(trajs.size());
     std::vector<size_t> cost(trajs.size(), 0);
     for (size_t t = 0; t < trajs.size(); t++) {
         order[t] = t;
-        for (const Segment& sg : trajs[t].segs) cost[t] += std::max<size_t>(sg.c.size() - 1, 1);
+        for (size_t s = trajs[t].first_done ? 1 : 0; s < trajs[t].segs.size(); s++)
+            cost[t] += std::max<size_t>(trajs[t].segs[s].c.size() - 1, 1);
     }
     std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
     std::vector<std::vector<size_t>> groups;
@@ -674,9 +900,14 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         }
         std::vector<std::vector<size_t>> batch(groups.begin() + g0, groups.begin() + g1);
         Plan<real> plan;
-        build_plan<real>(n, nparts, trajs, batch, ps, E, slots, probs_d, plan, k.hi);
-        st = run_plan<real>(c, precision, n, k, plan, o.use_graph != 0, &c->stats.gpu_ms);
+        build_plan<real>(n, nparts, trajs, batch, ps, E, slots, probs_d, plan, k.hi, use_bases ? &bases : nullptr);
+        double ms = 0.0;
+        st = run_plan<real>(c, precision, n, k, plan, o.use_graph != 0, &ms);
         if (st) return st;
+        c->stats.gpu_ms += ms;
+        c->stats.sweep_ms += ms;
+        c->stats.aux_bytes += plan.aux_bytes;
+        c->stats.aux_launches += plan.aux_launches;
         c->stats.sweeps += plan.sweeps;
         c->stats.term_units += plan.terms;
         c->stats.amp_terms += plan.amp_terms;
@@ -800,6 +1031,7 @@ void mqw_opts_default(mqw_opts* o)
     o->group_bytes = 0;
     o->variant = -1;
     o->high_pass = -1;
+    o->shared_first_stage = 1;
     o->sample_probs = nullptr;
 }
 

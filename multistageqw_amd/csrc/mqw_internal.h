@@ -92,6 +92,31 @@ struct ReduceJob {
     double* prob;            // nullable: receives |out[eloc]|^2 / ||out||^2
 };
 
+// Shared first stage.  Every m > 1 sample of an instance starts its first
+// stage from |+> under the same H, so the Chebyshev vectors V_k = T_k(Ht)|+>
+// (real, Ht = H / onenorm) are computed once per instance (real-half term
+// jobs, launch_half_real) and each sample's stage output is the combination
+//     psi_j = sum_k c_j[k] i^(k&1) V_k
+// of its own Jacobi-Anger coefficients: the quantity the Clenshaw terms of
+// that stage evaluate, by the forward recurrence instead of the backward one.
+constexpr int kCombineMaxK = 64;   // basis vectors per instance, V_0 included
+constexpr int kCombineBatch = 64;  // samples per combine job
+struct CombineItem {
+    float* out;          // planar output (2N floats)
+    double* norm_part;   // nparts partial sums of |out|^2, one per workgroup
+    uint32_t coef_off;   // kc coefficients (zero padded) in the coefficient table
+    uint32_t pad;
+};
+struct CombineJob {
+    const float* V;      // V_1 .. V_{kc-1}, N floats each, contiguous
+    uint32_t kc;         // basis vectors including V_0 = psi0 * ones (<= kCombineMaxK)
+    uint32_t ns;         // samples (items) of this job (<= kCombineBatch)
+    uint32_t item_off;   // first item
+    float psi0;
+};
+int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem* items_dev, const float* coefs_dev,
+                   int n, uint32_t nparts, uint32_t kc_max, void* stream);
+
 struct EnergyDev {
     double e0, emax;
     uint32_t eloc;
@@ -125,6 +150,9 @@ int half_variants(int hb);
 // hi > 0: jobs flagged F_HSUM take the flips of the top hi bits from hsum
 // (written by launch_high_pass) instead of partner-tile loads.
 int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc, int hi, void* stream);
+// The real half only, one workgroup per (job, tile): real recurrences whose
+// arrays are N floats (jobs with real scalars and no F_HSUM).
+int launch_half_real(int hb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc, void* stream);
 // High-bit pass (FP32): for every job flagged F_HSUM, hsum = sum over the top
 // hi bits b of s1[k ^ 2^b], both real halves, in pencils of 64 * 2^wb B rows;
 // 4 <= hi, hi + wb <= 10, wb <= 2, n - hi >= 4 + wb.

@@ -717,7 +717,9 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
     // MODE 1: per (half, tile pair): the two tiles that differ in the top tile
     //         bit, so that flip is an exchange of registers, not a load.
     // MODE 2: per tile: both halves, one read of the diagonal.
-    constexpr int NS = MODE == 0 ? 1 : 2;
+    // MODE 3: per tile, the real half only (real recurrences: the imaginary
+    //         halves of the arrays are never touched and may be absent).
+    constexpr int NS = (MODE == 0 || MODE == 3) ? 1 : 2;
     __shared__ float s_x[TS];
     __shared__ double s_red[NT / 64];
     __shared__ double s_bcast;
@@ -726,11 +728,11 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
     const int pb = ob - (MODE == 1 ? 1 : 0); // tile bits that are partner loads
     const uint64_t unit = xcd_unit(blockIdx.x, lc);
     // unit = (job, half, tile) in modes 0 and 1 (mode 1 drops the top tile
-    // bit), (job, tile) in mode 2
-    const int ub = MODE == 2 ? pb : pb + 1;
+    // bit), (job, tile) in modes 2 and 3
+    const int ub = (MODE == 2 || MODE == 3) ? pb : pb + 1;
     const uint64_t job = unit >> ub;
     if (job >= njobs) return;
-    const int h0 = MODE == 2 ? 0 : (int)((unit >> pb) & 1u);
+    const int h0 = (MODE == 2 || MODE == 3) ? 0 : (int)((unit >> pb) & 1u);
     const uint64_t tile0 = unit & ((1ull << pb) - 1ull);
     const TermJob<float>& J = jobs[job];
     const uint32_t flags = J.flags;
@@ -944,24 +946,38 @@ static void launch_half_v(uint32_t grid, const void* jobs, uint32_t njobs, int n
     switch (mode) {
     case 1: launch_half_m<HB, UNR, OCC, 1>(grid, j, njobs, n, lc, nts, hi, st); break;
     case 2: launch_half_m<HB, UNR, OCC, 2>(grid, j, njobs, n, lc, nts, hi, st); break;
+    case 3: launch_half_m<HB, UNR, OCC, 3>(grid, j, njobs, n, lc, nts, hi, st); break;
     default: launch_half_m<HB, UNR, OCC, 0>(grid, j, njobs, n, lc, nts, hi, st); break;
     }
 }
 
+static int launch_half_mode(int hb, int variant, int mode, const void* jobs_dev, uint32_t njobs, int n, int lc, int hi,
+                            hipStream_t st);
+
 int launch_half_term(int hb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc, int hi, void* stream)
 {
-    hipStream_t st = (hipStream_t)stream;
+    if (variant < 0 || variant >= half_variants(hb)) variant = 0;
+    return launch_half_mode(hb, variant, variant / 6, jobs_dev, njobs, n, lc, hi, (hipStream_t)stream);
+}
+
+int launch_half_real(int hb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc, void* stream)
+{
+    if (variant < 0 || variant >= half_variants(hb)) variant = 0;
+    return launch_half_mode(hb, variant, 3, jobs_dev, njobs, n, lc, 0, (hipStream_t)stream);
+}
+
+static int launch_half_mode(int hb, int variant, int mode, const void* jobs_dev, uint32_t njobs, int n, int lc, int hi,
+                            hipStream_t st)
+{
     const int ob = n - hb;
     if (ob < 0 || !half_tile_bits_supported(hb)) return -2;
     if (hi < 0 || hi > ob) return -2;
-    if (variant < 0 || variant >= half_variants(hb)) variant = 0;
-    int mode = variant / 6;
     // a pair needs a top tile bit to pair across, and one the high-bit pass leaves
     if (mode == 1 && (ob < 1 || hi > 0)) mode = 0;
     const bool nts = (variant % 6) >= 3;
     const int v = variant % 3;
     const int pb = ob - (mode == 1 ? 1 : 0);
-    const int ub = mode == 2 ? pb : pb + 1; // unit bits per job
+    const int ub = (mode == 2 || mode == 3) ? pb : pb + 1; // unit bits per job
     if (lc > ub + 16) lc = ub + 16;
     const uint64_t units = (uint64_t)njobs << ub;
     const uint64_t per = 8ull << lc;
@@ -1361,6 +1377,90 @@ int launch_high_pass(int hi, int wb, const void* jobs_dev, uint32_t njobs, int n
     else
         launch_hp_w<0>(hi, grid, jobs_dev, njobs, n, st);
     return check_launch("high_pass");
+}
+
+// ===========================================================================
+// Shared first stage: each sample's output psi_j = sum_k c_j[k] i^(k&1) V_k
+// from one instance's real Chebyshev vectors.  A workgroup takes one
+// contiguous chunk of N / nparts amplitudes for up to kCombineBatch samples:
+// every thread keeps the KR basis values of its amplitude in registers, reads
+// each sample's coefficients from LDS (broadcast) and writes both real halves
+// of that sample's output.  The |psi_j|^2 of the chunk goes to
+// norm_part[chunk] of the sample: a fixed-order sum (cross-lane tree, then
+// the 4 waves in order), so the norm is the same on every run.
+// ===========================================================================
+template <int KR>
+__global__ void __launch_bounds__(256)
+combine_kernel(const CombineJob* __restrict__ jobs, uint32_t njobs, const CombineItem* __restrict__ items,
+               const float* __restrict__ coefs, int n, uint32_t nparts)
+{
+    __shared__ float s_c[kCombineBatch * KR];
+    __shared__ double s_w[4][kCombineBatch];
+    const uint32_t job = blockIdx.x / nparts, part = blockIdx.x % nparts;
+    if (job >= njobs) return;
+    const CombineJob& J = jobs[job];
+    const int kc = (int)J.kc, ns = (int)J.ns;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int i = t; i < ns * KR; i += 256) {
+        const int j = i / KR, k = i % KR;
+        s_c[i] = k < kc ? coefs[items[J.item_off + j].coef_off + k] : 0.0f;
+    }
+    for (int i = t; i < 4 * kCombineBatch; i += 256) (&s_w[0][0])[i] = 0.0;
+    __syncthreads();
+    const uint64_t N = 1ull << n;
+    const uint64_t C = N / nparts;
+    const uint64_t base = (uint64_t)part * C;
+    for (uint64_t a0 = 0; a0 < C; a0 += 256) {
+        const uint64_t a = base + a0 + (uint64_t)t;
+        float v[KR];
+        v[0] = J.psi0;
+#pragma unroll
+        for (int k = 1; k < KR; k++) v[k] = k < kc ? J.V[(uint64_t)(k - 1) * N + a] : 0.0f;
+        for (int j = 0; j < ns; j++) {
+            const float* c = s_c + j * KR;
+            float re = 0.0f, im = 0.0f;
+#pragma unroll
+            for (int k = 0; k < KR; k += 2) {
+                re += c[k] * v[k];
+                im += c[k + 1] * v[k + 1];
+            }
+            float* o = items[J.item_off + j].out;
+            o[a] = re;
+            o[N + a] = im;
+            double s = (double)re * (double)re + (double)im * (double)im;
+#pragma unroll
+            for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+            if (lane == 0) s_w[w][j] += s;
+        }
+    }
+    __syncthreads();
+    if (t < ns) items[J.item_off + t].norm_part[part] = ((s_w[0][t] + s_w[1][t]) + s_w[2][t]) + s_w[3][t];
+}
+
+int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem* items_dev, const float* coefs_dev,
+                   int n, uint32_t nparts, uint32_t kc_max, void* stream)
+{
+    if (njobs == 0) return 0;
+    const uint64_t N = 1ull << n;
+    // whole 256-amplitude rows per workgroup
+    if (nparts == 0 || N % nparts != 0 || (N / nparts) % 256 != 0) return -2;
+    if (kc_max < 1 || kc_max > (uint32_t)kCombineMaxK) return -2;
+    const uint64_t grid = (uint64_t)njobs * nparts;
+    if (grid > 0x7fffffffull) return -3;
+    hipStream_t st = (hipStream_t)stream;
+    if (kc_max <= 16)
+        hipLaunchKernelGGL(combine_kernel<16>, dim3((uint32_t)grid), dim3(256), 0, st, jobs_dev, njobs, items_dev,
+                           coefs_dev, n, nparts);
+    else if (kc_max <= 32)
+        hipLaunchKernelGGL(combine_kernel<32>, dim3((uint32_t)grid), dim3(256), 0, st, jobs_dev, njobs, items_dev,
+                           coefs_dev, n, nparts);
+    else if (kc_max <= 48)
+        hipLaunchKernelGGL(combine_kernel<48>, dim3((uint32_t)grid), dim3(256), 0, st, jobs_dev, njobs, items_dev,
+                           coefs_dev, n, nparts);
+    else
+        hipLaunchKernelGGL(combine_kernel<64>, dim3((uint32_t)grid), dim3(256), 0, st, jobs_dev, njobs, items_dev,
+                           coefs_dev, n, nparts);
+    return check_launch("combine");
 }
 
 // ===========================================================================
