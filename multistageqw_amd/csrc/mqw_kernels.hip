@@ -922,7 +922,7 @@ int half_tile_bits_supported(int hb) { return hb == 13 || hb == 14; }
 // variant = v + 3 nts + 6 mode: v picks the loads in flight per thread and
 // the occupancy, nts = non-temporal output stores, mode = one tile per
 // workgroup (0), tile pairs across the top tile bit (1), both halves of a
-// tile (2).
+// tile (2).  Mode 3 (real halves only) is launch_half_real's.
 int half_variants(int hb) { return 18; }
 int term_variants() { return 2; }
 
