@@ -84,15 +84,15 @@ def main() -> int:
     inst = args.steps * args.per_step
     value = inst / wall if wall > 0 else 0.0
     # Roofline of the fused term kernel: algorithmic bytes (36 B per amplitude
-    # per complex H-apply) over the event-timed device time of the sweep plans
-    # (term launches, plus the small reduce and combine launches among them).
+    # per complex H-apply) over the event-timed device time of the term sweeps
+    # (term launches and the small per-group reduce launches).
     achieved = agg["alg"] / (agg["sweep_ms"] * 1e-3) / 1e9 if agg["sweep_ms"] > 0 else 0.0
     # The same without the psi reads of terms whose psi is the constant |+>.
     achieved_req = agg["req"] / (agg["sweep_ms"] * 1e-3) / 1e9 if agg["sweep_ms"] > 0 else 0.0
     per_launch_bytes = agg["alg"] / max(1, agg["sweeps"])
     per_launch_ms = agg["sweep_ms"] / max(1, agg["sweeps"])
-    # Every propagation kernel together: the shared first-stage basis recursions
-    # and combines (aux) with the term sweeps, over all event-timed device time.
+    # Every propagation kernel together: the shared bases, combines and adjoint
+    # readouts (aux) with the term sweeps, over all event-timed device time.
     achieved_all = (agg["alg"] + agg["aux_bytes"]) / (agg["gpu_ms"] * 1e-3) / 1e9 if agg["gpu_ms"] > 0 else 0.0
     # HBM traffic per launch: the calibrated FETCH_SIZE + WRITE_SIZE ratio to
     # algorithmic bytes measured on this workload (profiles/, rocprofv3 --pmc).
@@ -114,11 +114,11 @@ def main() -> int:
         wa = time.perf_counter() - ta
         sa = ctx.last_stats()
         # complex H-applies of the term sweeps over their device time (the shared
-        # first-stage basis, real-only, is timed apart: basis_ms)
+        # bases, combines and readouts are timed apart: aux_ms)
         applies = sa["term_units"] / (sa["sweep_ms"] * 1e-3)
         gbs = applies * 36.0 * (1 << na) / 1e9
         apply = dict(n=na, m=m, samples=args.apply_samples, applies_per_s=applies, gbs=gbs,
-                     frac=gbs / HBM_PEAK_GBS, device_ms=sa["gpu_ms"], basis_ms=sa["aux_ms"], wall_s=wa)
+                     frac=gbs / HBM_PEAK_GBS, device_ms=sa["gpu_ms"], aux_ms=sa["aux_ms"], wall_s=wa)
 
     cpu = None
     if args.cpu_threads >= 0:
@@ -156,7 +156,7 @@ def main() -> int:
                      "alg_bytes_per_launch": per_launch_bytes, "avg_launch_ms": per_launch_ms,
                      "launches": agg["sweeps"], "achieved_all_kernels": achieved_all,
                      "frac_all_kernels": achieved_all / HBM_PEAK_GBS, "device_ms": agg["gpu_ms"],
-                     "sweep_ms": agg["sweep_ms"], "basis_ms": agg["aux_ms"], "aux_bytes": agg["aux_bytes"],
+                     "sweep_ms": agg["sweep_ms"], "aux_ms": agg["aux_ms"], "aux_bytes": agg["aux_bytes"],
                      "aux_launches": agg["aux_launches"]},
         "cpu_baseline": cpu,
         "apply_rate": apply,

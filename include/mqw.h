@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 4
+#define MQW_ABI_VERSION 5
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -104,6 +104,13 @@ typedef struct {
                                 vectors T_k(H/onenorm)|+> of their common first stage, each sample's
                                 output is its own coefficient combination of them (FP32 real-half
                                 sweep, hypercube mixer); 0 = every sample runs its own Clenshaw terms */
+    int final_readout;       /* 1 (default) = the last stage of an m > 1 sample is read out without
+                                propagating it: psi_m[E_loc] = sum_k a_k <T_k(H/onenorm) e_Eloc, psi>
+                                from one real basis per instance, and ||psi_m|| = ||psi|| (the truncated
+                                series is unitary to within its coefficient tail, <= tol), so the
+                                probability moves by at most 2 tol relative (MultiQW.cpp:441-446
+                                renormalises the propagated state); 0 = the last stage runs as
+                                Clenshaw terms and its output is renormalised, as the reference */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are
@@ -230,13 +237,13 @@ typedef struct {
     uint64_t amp_terms;      /* sum over applies of 2^n (amplitude-terms) */
     double algorithmic_bytes;/* 36 B per amplitude-term (FP32) or 72 (FP64), executed */
     double gpu_ms;           /* device time of the propagation (events) */
-    double sweep_ms;         /* device time of the sweep plans: term, high-bit pass, combine and reduce launches */
+    double sweep_ms;         /* device time of the term sweeps: term, high-bit pass and reduce launches */
     uint64_t lds_trajectories; /* trajectories run by the LDS-resident kernel */
     uint64_t psi_const_amp_terms; /* amplitude-terms whose psi is the constant |+> (no psi read) */
     double required_bytes;   /* algorithmic_bytes less the psi reads the |+> terms skip */
-    double aux_ms;           /* device time of the shared first-stage basis recursions (in gpu_ms) */
-    double aux_bytes;        /* algorithmic bytes of the basis recursions and combine launches */
-    uint64_t aux_launches;   /* basis and combine launches (not in sweeps) */
+    double aux_ms;           /* device time of the shared bases, combine and readout launches (in gpu_ms) */
+    double aux_bytes;        /* algorithmic bytes of the basis recursions, combine and readout launches */
+    uint64_t aux_launches;   /* basis, combine and readout launches (not in sweeps) */
 } mqw_stats;
 mqw_status mqw_last_stats(const mqw_ctx* ctx, mqw_stats* out);
 

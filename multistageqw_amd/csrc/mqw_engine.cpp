@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <deque>
 #include <map>
 #include <memory>
 #include <string>
@@ -198,6 +199,7 @@ struct Traj {
     std::vector<Segment> segs;
     const void* psi_in = nullptr; // input of the first stage; nullptr = |+>
     bool first_done = false;      // the first stage comes from the shared basis (a combine op)
+    bool adjoint = false;         // the last stage is read out by the adjoint dots (no terms)
 };
 
 // Chebyshev coefficients of every segment.  Segments are independent (the
@@ -292,10 +294,12 @@ mqw_status group_energies(mqw_ctx* c, int n, std::vector<Problem*>& ps, int mode
 // ---------------------------------------------------------------------------
 struct Op {
     int kind; // 0 = term launch, 1 = reduce launch, 2 = high-bit pass over the jobs of the next term launch,
-              // 3 = combine launch (shared first stage), 4 = real-half term launch (first-stage basis)
+              // 3 = combine launch (shared first stage), 4 = real-half term launch (basis recurrences),
+              // 5 = adjoint readout dots, 6 = adjoint readout finalize
     size_t off;
     uint32_t count;
-    uint32_t aux = 0; // kind 3: largest basis size of its jobs
+    uint32_t aux = 0;  // kinds 3, 5: largest basis size of its jobs / tiles
+    uint32_t aux2 = 0; // kinds 5, 6: amplitude chunks of the dots
 };
 
 template <typename real>
@@ -305,7 +309,10 @@ struct Plan {
     std::vector<CombineJob> cjobs;
     std::vector<CombineItem> citems;
     std::vector<float> ccoef;
-    double aux_bytes = 0.0;   // algorithmic bytes of the combine launches
+    std::vector<ReadTile> rtiles;
+    std::vector<ReadItem> ritems;
+    size_t dots_doubles = 0;  // largest readout-dot scratch of one op
+    double aux_bytes = 0.0;   // algorithmic bytes of the combine and readout launches
     uint64_t aux_launches = 0;
     std::vector<Op> ops;
     uint64_t amp_terms = 0;
@@ -322,9 +329,16 @@ struct Slot {
     real* hs = nullptr; // high-bit sums of the current term's s1 (planar), when the pass is on
 };
 
+// The last stage's input of an adjoint trajectory: its state and partial norms.
+struct AdjointIn {
+    const void* psi = nullptr;
+    const double* norm_part = nullptr;
+};
+
 template <typename real>
 void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, const real* E, uint32_t eloc,
-               double* probs, std::vector<TermJob<real>>& out, ReduceJob& fin, bool& has_fin, int hi = 0)
+               double* probs, std::vector<TermJob<real>>& out, ReduceJob& fin, bool& has_fin, int hi = 0,
+               AdjointIn* adj = nullptr)
 {
     const size_t N = (size_t)1 << n;
     const real psi0 = (real)(float)(1 / sqrt((double)N));
@@ -341,7 +355,9 @@ void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, cons
         s0 = 1;
     }
     auto rc = [](double v) { return (real)v; };
-    for (size_t s = s0; s < tr.segs.size(); s++) {
+    // an adjoint trajectory stops before its last stage
+    const size_t s1 = tr.adjoint ? tr.segs.size() - 1 : tr.segs.size();
+    for (size_t s = s0; s < s1; s++) {
         const Segment& sg = tr.segs[s];
         const std::vector<double>& cc = sg.c;
         const int K = (int)cc.size();
@@ -419,6 +435,14 @@ void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, cons
         par ^= 1;
         prev_slot = sg.prob_slot;
     }
+    if (tr.adjoint) {
+        has_fin = false;
+        if (adj) {
+            adj->psi = sl.st[p];
+            adj->norm_part = sl.part[par ^ 1];
+        }
+        return;
+    }
     has_fin = prev_slot >= 0;
     if (has_fin) {
         fin = ReduceJob{};
@@ -431,11 +455,70 @@ void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, cons
     }
 }
 
-// Per-problem shared first-stage bases (FP32): V (nullptr = none) and kc.
+// Per-problem shared bases (FP32): the first stage's V (nullptr = none, kc
+// vectors with V_0 = |+> implicit) and the last stage's adjoint X (kx vectors
+// from X_0 = e_eloc).
 struct Bases {
     std::vector<const float*> V;
     std::vector<int> kc;
+    std::vector<const float*> X;
+    std::vector<int> kx;
 };
+
+// Readout-dot tiles for the adjoint members of one group: per problem, tiles
+// of up to kReadTile samples; the dots are split over amplitude chunks so a
+// launch has about 2048 workgroups.
+template <typename real>
+void readout_ops(int n, uint32_t parts, const std::vector<Traj>& trajs, const std::vector<size_t>& grp,
+                 const std::vector<AdjointIn>& adj, const Bases& B, double* probs, Plan<real>& P)
+{
+    const size_t N = (size_t)1 << n;
+    std::map<int, std::vector<size_t>> byprob;
+    for (size_t i = 0; i < grp.size(); i++)
+        if (trajs[grp[i]].adjoint) byprob[trajs[grp[i]].prob].push_back(i);
+    if (byprob.empty()) return;
+    const size_t t0 = P.rtiles.size();
+    uint32_t kx_max = 1;
+    for (const auto& kv : byprob) {
+        const int pr = kv.first, kx = B.kx[pr];
+        const std::vector<size_t>& mem = kv.second;
+        for (size_t i0 = 0; i0 < mem.size(); i0 += kReadTile) {
+            const size_t ns = std::min<size_t>(kReadTile, mem.size() - i0);
+            ReadTile rt{};
+            rt.X = B.X[pr];
+            rt.kx = (uint32_t)kx;
+            rt.ns = (uint32_t)ns;
+            rt.item_off = (uint32_t)P.ritems.size();
+            for (size_t q = 0; q < ns; q++) {
+                const size_t i = mem[i0 + q];
+                const Traj& tr = trajs[grp[i]];
+                const std::vector<double>& cc = tr.segs.back().c;
+                ReadItem it{};
+                it.psi = (const float*)adj[i].psi;
+                it.norm_part = adj[i].norm_part;
+                it.prob = probs + tr.segs.back().prob_slot;
+                it.coef_off = (uint32_t)P.ccoef.size();
+                for (int k = 0; k < kx; k++) P.ccoef.push_back(k < (int)cc.size() ? (float)cc[k] : 0.0f);
+                P.ritems.push_back(it);
+            }
+            P.rtiles.push_back(rt);
+            kx_max = std::max<uint32_t>(kx_max, (uint32_t)kx);
+            // reads the kx basis vectors and the ns states once
+            P.aux_bytes += (double)N * (4.0 * kx + 8.0 * (double)ns);
+        }
+    }
+    const uint32_t ntiles = (uint32_t)(P.rtiles.size() - t0);
+    // chunks of at least 4096 amplitudes; about 2048 workgroups per launch
+    uint32_t nchunk = 1;
+    while ((uint64_t)nchunk * ntiles < 2048 && (N / (nchunk * 2)) >= 4096) nchunk *= 2;
+    P.dots_doubles = std::max<size_t>(P.dots_doubles, (size_t)ntiles * nchunk * kReadTile * 2);
+    Op d{5, t0, ntiles, kx_max, nchunk};
+    Op f{6, t0, ntiles, kx_max, nchunk};
+    P.ops.push_back(d);
+    P.ops.push_back(f);
+    P.aux_launches += 2;
+    (void)parts;
+}
 
 // Combine jobs for the first_done members of one group: per problem, batches
 // of up to kCombineBatch samples, each writing its slot's st[0] and part[0].
@@ -496,13 +579,14 @@ void build_plan(int n, uint32_t parts, const std::vector<Traj>& trajs, const std
         if (B) combine_ops<real>(n, trajs, grp, *B, slots, P);
         std::vector<std::vector<TermJob<real>>> seq(grp.size());
         std::vector<ReduceJob> fins;
+        std::vector<AdjointIn> adj(grp.size());
         size_t L = 0;
         for (size_t i = 0; i < grp.size(); i++) {
             const Traj& tr = trajs[grp[i]];
             ReduceJob fin;
             bool has = false;
             traj_jobs<real>(n, parts, tr, slots[i], E + (size_t)tr.prob * N, ps[tr.prob]->info.eloc, probs, seq[i],
-                            fin, has, hi);
+                            fin, has, hi, &adj[i]);
             if (has) fins.push_back(fin);
             L = std::max(L, seq[i].size());
             P.terms += seq[i].size();
@@ -522,6 +606,7 @@ void build_plan(int n, uint32_t parts, const std::vector<Traj>& trajs, const std
             P.ops.push_back(op);
             P.sweeps++;
         }
+        if (B && !B->X.empty()) readout_ops<real>(n, parts, trajs, grp, adj, *B, probs, P);
         if (!fins.empty()) {
             P.ops.push_back(Op{1, P.rjobs.size(), (uint32_t)fins.size()});
             P.rjobs.insert(P.rjobs.end(), fins.begin(), fins.end());
@@ -530,35 +615,56 @@ void build_plan(int n, uint32_t parts, const std::vector<Traj>& trajs, const std
 }
 
 template <typename real>
-mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<real>& P, bool use_graph, double* ms)
+// Runs the plan's ops in order.  Consecutive ops of one class (sweep: term,
+// reduce and high-bit pass launches; aux: basis, combine and readout
+// launches) form one hipGraph (or direct launches) between two events, so
+// the sweep's device time is measured apart from the rest.
+mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<real>& P, bool use_graph,
+                    double* ms_sweep, double* ms_aux)
 {
-    TermJob<real>* jd = (TermJob<real>*)dev_get(c, "plan_jobs", sizeof(TermJob<real>) * P.jobs.size());
+    TermJob<real>* jd = (TermJob<real>*)dev_get(c, "plan_jobs", sizeof(TermJob<real>) * std::max<size_t>(1, P.jobs.size()));
     ReduceJob* rd = (ReduceJob*)dev_get(c, "plan_rjobs", sizeof(ReduceJob) * std::max<size_t>(1, P.rjobs.size()));
     if (!jd || !rd) return fail(c, MQW_ENOMEM, "device allocation failed for %zu sweep jobs", P.jobs.size());
     CombineJob* cd = nullptr;
     CombineItem* ci = nullptr;
     float* cf = nullptr;
+    ReadTile* rtd = nullptr;
+    ReadItem* rid = nullptr;
+    double* dots = nullptr;
+    if (!P.ccoef.empty()) {
+        cf = (float*)dev_get(c, "plan_ccoef", sizeof(float) * P.ccoef.size());
+        if (!cf) return fail(c, MQW_ENOMEM, "device allocation failed for coefficient rows");
+        HIPCHK(c, hipMemcpyAsync(cf, P.ccoef.data(), sizeof(float) * P.ccoef.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+    }
     if (!P.cjobs.empty()) {
         cd = (CombineJob*)dev_get(c, "plan_cjobs", sizeof(CombineJob) * P.cjobs.size());
         ci = (CombineItem*)dev_get(c, "plan_citems", sizeof(CombineItem) * P.citems.size());
-        cf = (float*)dev_get(c, "plan_ccoef", sizeof(float) * P.ccoef.size());
-        if (!cd || !ci || !cf) return fail(c, MQW_ENOMEM, "device allocation failed for combine jobs");
+        if (!cd || !ci) return fail(c, MQW_ENOMEM, "device allocation failed for combine jobs");
         HIPCHK(c, hipMemcpyAsync(cd, P.cjobs.data(), sizeof(CombineJob) * P.cjobs.size(), hipMemcpyHostToDevice,
                                  c->stream));
         HIPCHK(c, hipMemcpyAsync(ci, P.citems.data(), sizeof(CombineItem) * P.citems.size(), hipMemcpyHostToDevice,
                                  c->stream));
-        HIPCHK(c, hipMemcpyAsync(cf, P.ccoef.data(), sizeof(float) * P.ccoef.size(), hipMemcpyHostToDevice,
+    }
+    if (!P.rtiles.empty()) {
+        rtd = (ReadTile*)dev_get(c, "plan_rtiles", sizeof(ReadTile) * P.rtiles.size());
+        rid = (ReadItem*)dev_get(c, "plan_ritems", sizeof(ReadItem) * P.ritems.size());
+        dots = (double*)dev_get(c, "plan_dots", sizeof(double) * P.dots_doubles);
+        if (!rtd || !rid || !dots) return fail(c, MQW_ENOMEM, "device allocation failed for readout tiles");
+        HIPCHK(c, hipMemcpyAsync(rtd, P.rtiles.data(), sizeof(ReadTile) * P.rtiles.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(rid, P.ritems.data(), sizeof(ReadItem) * P.ritems.size(), hipMemcpyHostToDevice,
                                  c->stream));
     }
-    HIPCHK(c, hipMemcpyAsync(jd, P.jobs.data(), sizeof(TermJob<real>) * P.jobs.size(), hipMemcpyHostToDevice,
-                             c->stream));
+    if (!P.jobs.empty())
+        HIPCHK(c, hipMemcpyAsync(jd, P.jobs.data(), sizeof(TermJob<real>) * P.jobs.size(), hipMemcpyHostToDevice,
+                                 c->stream));
     if (!P.rjobs.empty())
         HIPCHK(c, hipMemcpyAsync(rd, P.rjobs.data(), sizeof(ReduceJob) * P.rjobs.size(), hipMemcpyHostToDevice,
                                  c->stream));
-    EventPair ev;
-    HIPCHK(c, ev.create());
-    auto issue = [&]() -> int {
-        for (const Op& op : P.ops) {
+    auto issue = [&](size_t b, size_t e) -> int {
+        for (size_t q = b; q < e; q++) {
+            const Op& op = P.ops[q];
             int rc;
             if (op.kind == 1)
                 rc = launch_reduce(precision, rd + op.off, op.count, c->stream);
@@ -566,6 +672,10 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
                 rc = launch_combine(cd + op.off, op.count, ci, cf, n, k.parts, op.aux, c->stream);
             else if (op.kind == 4)
                 rc = launch_half_real(k.tb, k.variant, jd + op.off, op.count, n, k.lc, c->stream);
+            else if (op.kind == 5)
+                rc = launch_readout_dots(rtd + op.off, op.count, rid, cf, n, op.aux2, op.aux, dots, c->stream);
+            else if (op.kind == 6)
+                rc = launch_readout_finalize(rtd + op.off, op.count, rid, op.aux2, k.parts, dots, c->stream);
             else if (op.kind == 2)
                 rc = launch_high_pass(k.hi, k.hwb, jd + op.off, op.count, n, c->stream);
             else if (k.nmask > 0)
@@ -578,31 +688,55 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
         }
         return 0;
     };
-    mqw_status st = MQW_OK;
-    HIPCHK(c, hipEventRecord(ev.a, c->stream));
-    if (use_graph) {
-        GraphGuard gr;
-        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = issue();
-        hipError_t ce = hipStreamEndCapture(c->stream, &gr.g);
-        if (rc || ce != hipSuccess)
-            return fail(c, MQW_EDEVICE, "graph capture failed: %s / %s", last_kernel_error(),
-                        hipGetErrorString(ce));
-        hipError_t ie = hipGraphInstantiate(&gr.ge, gr.g, nullptr, nullptr, 0);
-        if (ie != hipSuccess) return fail(c, MQW_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ie));
-        hipError_t le = hipGraphLaunch(gr.ge, c->stream);
-        if (le != hipSuccess) st = fail(c, MQW_EDEVICE, "hipGraphLaunch: %s", hipGetErrorString(le));
-        HIPCHK(c, hipEventRecord(ev.b, c->stream));
-        hipError_t se = hipStreamSynchronize(c->stream);
-        if (se != hipSuccess) return fail(c, MQW_EDEVICE, "stream sync: %s", hipGetErrorString(se));
-    } else {
-        if (issue()) st = fail(c, MQW_EDEVICE, "term launch: %s", last_kernel_error());
-        HIPCHK(c, hipEventRecord(ev.b, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+    auto aux_kind = [](int kind) { return kind >= 3; };
+    struct Run {
+        size_t b, e;
+        bool aux;
+    };
+    std::vector<Run> runs;
+    for (size_t q = 0; q < P.ops.size(); q++) {
+        const bool a = aux_kind(P.ops[q].kind);
+        if (runs.empty() || runs.back().aux != a) runs.push_back(Run{q, q, a});
+        runs.back().e = q + 1;
     }
-    float t = 0.0f;
-    hipEventElapsedTime(&t, ev.a, ev.b);
-    *ms += t;
+    std::deque<EventPair> evs;
+    std::deque<GraphGuard> graphs;
+    mqw_status st = MQW_OK;
+    for (const Run& r : runs) {
+        evs.emplace_back();
+        EventPair& ev = evs.back();
+        HIPCHK(c, ev.create());
+        HIPCHK(c, hipEventRecord(ev.a, c->stream));
+        if (use_graph) {
+            graphs.emplace_back();
+            GraphGuard& gr = graphs.back();
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            const int rc = issue(r.b, r.e);
+            hipError_t ce = hipStreamEndCapture(c->stream, &gr.g);
+            if (rc || ce != hipSuccess)
+                return fail(c, MQW_EDEVICE, "graph capture failed: %s / %s", last_kernel_error(),
+                            hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&gr.ge, gr.g, nullptr, nullptr, 0);
+            if (ie != hipSuccess) return fail(c, MQW_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+            hipError_t le = hipGraphLaunch(gr.ge, c->stream);
+            if (le != hipSuccess) {
+                st = fail(c, MQW_EDEVICE, "hipGraphLaunch: %s", hipGetErrorString(le));
+                break;
+            }
+        } else if (issue(r.b, r.e)) {
+            st = fail(c, MQW_EDEVICE, "launch: %s", last_kernel_error());
+            break;
+        }
+        HIPCHK(c, hipEventRecord(ev.b, c->stream));
+    }
+    hipError_t se = hipStreamSynchronize(c->stream);
+    if (se != hipSuccess) return fail(c, MQW_EDEVICE, "stream sync: %s", hipGetErrorString(se));
+    if (st) return st;
+    for (size_t q = 0; q < runs.size(); q++) {
+        float t = 0.0f;
+        hipEventElapsedTime(&t, evs[q].a, evs[q].b);
+        *(runs[q].aux ? ms_aux : ms_sweep) += t;
+    }
     return st;
 }
 
@@ -677,67 +811,103 @@ mqw_status run_lds(mqw_ctx* c, int n, const std::vector<Traj>& trajs, const std:
 }
 
 // ---------------------------------------------------------------------------
-// Shared first-stage bases: V_{q+1} = 2 Ht V_q - V_{q-1} (V_1 = Ht |+>) on the
-// real half only, for every problem with bases.kc > 1; launch q holds step q
-// of every problem still recurring.  Ht = (H_P - gamma_1 H_G) / onenorm_1 with
-// the first stage's gamma and bound, as that stage's Clenshaw terms use.
+// Shared bases, real halves only (launch_half_real), for every problem with
+// bases.kc > 1 or bases.kx > 1; launch q holds step q of every recurrence
+// still running:
+//   first stage  V_{q+1} = 2 Ht_1 V_q - V_{q-1},  V_1 = Ht_1 |+>
+//   last stage   X_{q+1} = 2 Ht_m X_q - X_{q-1},  X_1 = Ht_m e_eloc
+// with Ht_i = (H_P - gamma_i H_G) / onenorm_i, the stage's gamma and bound as
+// its Clenshaw terms use.
 // ---------------------------------------------------------------------------
 mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Problem*>& ps,
                      const std::vector<Traj>& trajs, const float* E, const Bases& B, bool use_graph)
 {
     const size_t N = (size_t)1 << n;
     const float psi0 = (float)(1 / sqrt((double)N));
-    std::vector<const Segment*> seg0(ps.size(), nullptr);
+    std::vector<const Segment*> seg0(ps.size(), nullptr), segl(ps.size(), nullptr);
     for (const Traj& tr : trajs)
-        if (!seg0[tr.prob]) seg0[tr.prob] = &tr.segs[0];
-    int steps = 0;
-    for (size_t p = 0; p < ps.size(); p++) steps = std::max(steps, B.kc[p] - 1);
-    Plan<float> P;
+        if (!seg0[tr.prob]) {
+            seg0[tr.prob] = &tr.segs.front();
+            segl[tr.prob] = &tr.segs.back();
+        }
     double bytes = 0.0;
+    // X_0 = e_eloc, ahead of the recurrence on the same stream
+    for (size_t p = 0; p < ps.size(); p++)
+        if (B.kx[p] > 0) {
+            if (launch_unit_vector(const_cast<float*>(B.X[p]), N, ps[p]->info.eloc, c->stream))
+                return fail(c, MQW_EDEVICE, "unit vector: %s", last_kernel_error());
+            bytes += 4.0 * N;
+        }
+    int steps = 0;
+    for (size_t p = 0; p < ps.size(); p++) steps = std::max({steps, B.kc[p] - 1, B.kx[p] - 1});
+    Plan<float> P;
+    auto job = [&](size_t p, const Segment& sg, int q) {
+        TermJob<float> j{};
+        j.E = E + p * N;
+        j.psi0 = psi0;
+        j.w1r = 1.0f;
+        const double sc = (double)sg.onenorm * (q == 0 ? 1.0 : 0.5);
+        j.alpha = (float)(1.0 / sc);
+        j.beta = (float)((double)sg.gamma / sc);
+        return j;
+    };
     for (int q = 0; q < steps; q++) {
         Op op{4, P.jobs.size(), 0};
         for (size_t p = 0; p < ps.size(); p++) {
-            if (q >= B.kc[p] - 1) continue;
-            const Segment& sg = *seg0[p];
-            float* V = const_cast<float*>(B.V[p]);
-            TermJob<float> j{};
-            j.E = E + p * N;
-            j.psi0 = psi0;
-            j.w1r = 1.0f;
-            const double sc = (double)sg.onenorm * (q == 0 ? 1.0 : 0.5);
-            j.alpha = (float)(1.0 / sc);
-            j.beta = (float)((double)sg.gamma / sc);
-            j.out = V + (size_t)q * N;
-            uint32_t flags = F_PSI_CONST;
-            if (q == 0) {
-                flags |= SRC1_PSI | SRC2_ZERO;
-                bytes += 8.0 * N;
-            } else {
-                flags |= SRC1_ARRAY;
-                j.b1 = V + (size_t)(q - 1) * N;
-                j.w2r = 1.0f;
-                if (q == 1) {
-                    flags |= SRC2_PSI;
+            if (q < B.kc[p] - 1) {
+                float* V = const_cast<float*>(B.V[p]); // V_1 .. V_{kc-1}
+                TermJob<float> j = job(p, *seg0[p], q);
+                j.out = V + (size_t)q * N;
+                uint32_t flags = F_PSI_CONST;
+                if (q == 0) {
+                    flags |= SRC1_PSI | SRC2_ZERO;
+                    bytes += 8.0 * N;
+                } else {
+                    flags |= SRC1_ARRAY;
+                    j.b1 = V + (size_t)(q - 1) * N;
+                    j.w2r = 1.0f;
+                    if (q == 1) {
+                        flags |= SRC2_PSI;
+                        bytes += 12.0 * N;
+                    } else {
+                        flags |= SRC2_ARRAY;
+                        j.b2 = V + (size_t)(q - 2) * N;
+                        bytes += 16.0 * N;
+                    }
+                }
+                j.flags = flags;
+                P.jobs.push_back(j);
+                op.count++;
+            }
+            if (q < B.kx[p] - 1) {
+                float* X = const_cast<float*>(B.X[p]); // X_0 .. X_{kx-1}
+                TermJob<float> j = job(p, *segl[p], q);
+                j.out = X + (size_t)(q + 1) * N;
+                j.b1 = X + (size_t)q * N;
+                uint32_t flags = SRC1_ARRAY;
+                if (q == 0) {
+                    flags |= SRC2_ZERO;
                     bytes += 12.0 * N;
                 } else {
                     flags |= SRC2_ARRAY;
-                    j.b2 = V + (size_t)(q - 2) * N;
+                    j.b2 = X + (size_t)(q - 1) * N;
+                    j.w2r = 1.0f;
                     bytes += 16.0 * N;
                 }
+                j.flags = flags;
+                P.jobs.push_back(j);
+                op.count++;
             }
-            j.flags = flags;
-            P.jobs.push_back(j);
-            op.count++;
         }
         P.ops.push_back(op);
     }
-    if (P.jobs.empty()) return MQW_OK;
-    double ms = 0.0;
-    mqw_status st = run_plan<float>(c, MQW_FP32, n, k, P, use_graph, &ms);
-    if (st) return st;
-    c->stats.gpu_ms += ms;
-    c->stats.aux_ms += ms;
     c->stats.aux_bytes += bytes;
+    if (P.jobs.empty()) return MQW_OK;
+    double ms_sweep = 0.0, ms_aux = 0.0; // every op here is a basis launch (aux)
+    mqw_status st = run_plan<float>(c, MQW_FP32, n, k, P, use_graph, &ms_sweep, &ms_aux);
+    if (st) return st;
+    c->stats.gpu_ms += ms_sweep + ms_aux;
+    c->stats.aux_ms += ms_sweep + ms_aux;
     c->stats.aux_launches += P.ops.size();
     return MQW_OK;
 }
@@ -820,43 +990,64 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     }
 
     const uint32_t nparts = k.parts;
-    // shared first stage: one real Chebyshev basis per instance with m > 1
+    // shared bases, one per instance with m > 1 (samples restart from |+>):
+    // the first stage's V (combined per sample) and the last stage's adjoint X
     Bases bases;
     size_t basis_bytes = 0;
-    if (std::is_same<real, float>::value && o.shared_first_stage && k.half && !mixer && n >= k.tb &&
-        (N / nparts) % 256 == 0) {
+    bool use_bases = false;
+    const bool basis_path =
+        std::is_same<real, float>::value && k.half && !mixer && n >= k.tb && (N / nparts) % 256 == 0;
+    if (basis_path && (o.shared_first_stage || o.final_readout)) {
         const size_t cap = o.max_device_bytes ? o.max_device_bytes / 4 : ((size_t)16 << 30);
         bases.V.assign(ps.size(), nullptr);
         bases.kc.assign(ps.size(), 0);
-        for (const Traj& tr : trajs)
-            if (ps[tr.prob]->m > 1) bases.kc[tr.prob] = std::max(bases.kc[tr.prob], (int)tr.segs[0].c.size());
-        std::vector<size_t> off(ps.size(), 0);
-        for (size_t p = 0; p < ps.size(); p++) {
-            const int kc = bases.kc[p];
-            const size_t b = sizeof(float) * N * (size_t)std::max(kc - 1, 0);
-            if (kc < 1 || kc > kCombineMaxK || basis_bytes + b > cap) {
-                bases.kc[p] = 0;
-                continue;
-            }
-            off[p] = basis_bytes;
-            basis_bytes += b;
+        bases.X.assign(ps.size(), nullptr);
+        bases.kx.assign(ps.size(), 0);
+        for (const Traj& tr : trajs) {
+            if (ps[tr.prob]->m < 2) continue;
+            if (o.shared_first_stage)
+                bases.kc[tr.prob] = std::max(bases.kc[tr.prob], (int)tr.segs.front().c.size());
+            if (o.final_readout) bases.kx[tr.prob] = std::max(bases.kx[tr.prob], (int)tr.segs.back().c.size());
         }
-        float* V = basis_bytes ? (float*)dev_get(c, "basis", basis_bytes) : nullptr;
-        if (basis_bytes && !V) return fail(c, MQW_ENOMEM, "device allocation failed for first-stage bases");
-        bool any = false;
-        for (size_t p = 0; p < ps.size(); p++)
-            if (bases.kc[p] > 0) {
-                bases.V[p] = (const float*)((const char*)V + off[p]);
-                any = true;
+        std::vector<size_t> offv(ps.size(), 0), offx(ps.size(), 0);
+        for (size_t p = 0; p < ps.size(); p++) {
+            const int kc = bases.kc[p], kx = bases.kx[p];
+            const size_t bv = sizeof(float) * N * (size_t)std::max(kc - 1, 0);
+            if (kc < 1 || kc > kCombineMaxK || basis_bytes + bv > cap) {
+                bases.kc[p] = 0;
+            } else {
+                offv[p] = basis_bytes;
+                basis_bytes += bv;
             }
-        if (any) {
+            const size_t bx = sizeof(float) * N * (size_t)std::max(kx, 0);
+            if (kx < 1 || kx > kReadMaxK || basis_bytes + bx > cap) {
+                bases.kx[p] = 0;
+            } else {
+                offx[p] = basis_bytes;
+                basis_bytes += bx;
+            }
+        }
+        char* base = basis_bytes ? (char*)dev_get(c, "basis", basis_bytes) : nullptr;
+        if (basis_bytes && !base) return fail(c, MQW_ENOMEM, "device allocation failed for the shared bases");
+        for (size_t p = 0; p < ps.size(); p++) {
+            if (bases.kc[p] > 0) {
+                bases.V[p] = (const float*)(base + offv[p]); // unused when kc = 1 (V_0 only)
+                use_bases = true;
+            }
+            if (bases.kx[p] > 0) {
+                bases.X[p] = (const float*)(base + offx[p]);
+                use_bases = true;
+            }
+        }
+        if (use_bases) {
             st = run_bases(c, n, k, ps, trajs, (const float*)E, bases, o.use_graph != 0);
             if (st) return st;
-            for (Traj& tr : trajs)
-                if (bases.kc[tr.prob] > 0) tr.first_done = true;
+            for (Traj& tr : trajs) {
+                tr.first_done = bases.kc[tr.prob] > 0;
+                tr.adjoint = bases.kx[tr.prob] > 0;
+            }
         }
     }
-    const bool use_bases = basis_bytes > 0;
 
     // groups sized to the memory-side cache, longest trajectories first
     const size_t nst = k.hi > 0 ? 8 : 6; // planar states per slot (+ the high-bit sums)
@@ -870,7 +1061,8 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     std::vector<size_t> cost(trajs.size(), 0);
     for (size_t t = 0; t < trajs.size(); t++) {
         order[t] = t;
-        for (size_t s = trajs[t].first_done ? 1 : 0; s < trajs[t].segs.size(); s++)
+        const size_t s1 = trajs[t].segs.size() - (trajs[t].adjoint ? 1 : 0);
+        for (size_t s = trajs[t].first_done ? 1 : 0; s < s1; s++)
             cost[t] += std::max<size_t>(trajs[t].segs[s].c.size() - 1, 1);
     }
     std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
@@ -901,11 +1093,12 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         std::vector<std::vector<size_t>> batch(groups.begin() + g0, groups.begin() + g1);
         Plan<real> plan;
         build_plan<real>(n, nparts, trajs, batch, ps, E, slots, probs_d, plan, k.hi, use_bases ? &bases : nullptr);
-        double ms = 0.0;
-        st = run_plan<real>(c, precision, n, k, plan, o.use_graph != 0, &ms);
+        double ms_sweep = 0.0, ms_aux = 0.0;
+        st = run_plan<real>(c, precision, n, k, plan, o.use_graph != 0, &ms_sweep, &ms_aux);
         if (st) return st;
-        c->stats.gpu_ms += ms;
-        c->stats.sweep_ms += ms;
+        c->stats.gpu_ms += ms_sweep + ms_aux;
+        c->stats.sweep_ms += ms_sweep;
+        c->stats.aux_ms += ms_aux;
         c->stats.aux_bytes += plan.aux_bytes;
         c->stats.aux_launches += plan.aux_launches;
         c->stats.sweeps += plan.sweeps;
@@ -1032,6 +1225,7 @@ void mqw_opts_default(mqw_opts* o)
     o->variant = -1;
     o->high_pass = -1;
     o->shared_first_stage = 1;
+    o->final_readout = 1;
     o->sample_probs = nullptr;
 }
 
@@ -1336,8 +1530,8 @@ mqw_status mqw_clenshaw(mqw_ctx* c, int n, const float* coeffs, int K, float* ps
     for (size_t q = 0; q < plan.jobs.size(); q++) plan.ops.push_back(Op{0, q, 1});
     plan.rjobs.push_back(fin);
     plan.ops.push_back(Op{1, 0, 1});
-    double ms = 0;
-    mqw_status st = run_plan<float>(c, MQW_FP32, n, k, plan, false, &ms);
+    double ms = 0, ms_aux = 0;
+    mqw_status st = run_plan<float>(c, MQW_FP32, n, k, plan, false, &ms, &ms_aux);
     if (st) return st;
     const float* outp = (const float*)plan.rjobs.back().out;
     HIPCHK(c, hipMemcpyAsync(psi, outp, sizeof(float) * 2 * N, hipMemcpyDeviceToHost, c->stream));

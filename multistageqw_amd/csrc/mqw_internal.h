@@ -117,6 +117,40 @@ struct CombineJob {
 int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem* items_dev, const float* coefs_dev,
                    int n, uint32_t nparts, uint32_t kc_max, void* stream);
 
+// Adjoint readout of the last stage.  The reference keeps only
+// |psi_m[eloc]|^2 / ||psi_m||^2 of the last stage's output psi_m = P psi
+// (P = sum_k a_k T_k(Ht), a_k = c_k i^(k&1)).  T_k(Ht) is real symmetric, so
+//     psi_m[eloc] = sum_k a_k <X_k, psi>,   X_k = T_k(Ht) e_eloc,
+// with one real basis X per instance (the same recurrence as the shared first
+// stage, started from the unit vector e_eloc), and ||psi_m|| = ||psi|| to
+// within the series' truncation (the tail sum of |c_k| beyond K, at most the
+// coefficient tolerance), so the probability is |sum_k a_k <X_k, psi>|^2 / ||psi||^2.
+constexpr int kReadTile = 64;   // samples per dot workgroup
+constexpr int kReadMaxK = 128;  // basis vectors per instance (X_0 included)
+struct ReadItem {
+    const float* psi;         // the last stage's input state (planar 2N, unnormalised)
+    const double* norm_part;  // its nparts partial sums of |psi|^2
+    double* prob;             // receives the probability
+    uint32_t coef_off;        // kx coefficients (zero padded) of the last stage
+    uint32_t pad;
+};
+struct ReadTile {
+    const float* X;           // X_0 .. X_{kx-1}, N floats each
+    uint32_t kx;              // basis size (<= kReadMaxK)
+    uint32_t ns;              // items of the tile (<= kReadTile)
+    uint32_t item_off;
+    uint32_t pad;
+};
+// e = unit vector k of length N.
+int launch_unit_vector(float* e, uint64_t N, uint64_t k, void* stream);
+// Partial amplitudes sum_k a_jk <X_k, psi_j> over nchunk amplitude chunks:
+//   dots[((tile * nchunk + chunk) * kReadTile + j) * 2 + {0: Re, 1: Im}]
+int launch_readout_dots(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev, const float* coefs_dev,
+                        int n, uint32_t nchunk, uint32_t kx_max, double* dots, void* stream);
+// Per item: sum the chunks and divide |amplitude|^2 by the norm.
+int launch_readout_finalize(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev, uint32_t nchunk,
+                            uint32_t nparts, const double* dots, void* stream);
+
 struct EnergyDev {
     double e0, emax;
     uint32_t eloc;

@@ -1464,6 +1464,143 @@ int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem
 }
 
 // ===========================================================================
+// Adjoint readout of the last stage (see ReadItem).  A workgroup takes one
+// chunk of N / nchunk amplitudes for a tile of up to kReadTile samples of one
+// instance.  Every thread keeps the KR basis values X_k[a] of its amplitude
+// in registers, forms phi_j[a] = sum_k a_jk X_k[a] with the sample's
+// coefficients from LDS (broadcast), and adds phi_j[a] psi_j[a]; the sum over
+// a wave is a cross-lane tree, and the waves' sums go to FP64 in LDS, in a
+// fixed order.  So the basis is read once per tile and each state once.
+// ===========================================================================
+__global__ void __launch_bounds__(256) unit_vector_kernel(float* __restrict__ e, uint64_t N, uint64_t k)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < N; i += (uint64_t)gridDim.x * 256)
+        e[i] = i == k ? 1.0f : 0.0f;
+}
+
+int launch_unit_vector(float* e, uint64_t N, uint64_t k, void* stream)
+{
+    const uint64_t blocks = (N + 255) / 256 < 4096 ? (N + 255) / 256 : 4096;
+    hipLaunchKernelGGL(unit_vector_kernel, dim3((uint32_t)blocks), dim3(256), 0, (hipStream_t)stream, e, N, k);
+    return check_launch("unit_vector");
+}
+
+template <int KR>
+__global__ void __launch_bounds__(256)
+readout_dot_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const ReadItem* __restrict__ items,
+                   const float* __restrict__ coefs, int n, uint32_t nchunk, double* __restrict__ dots)
+{
+    __shared__ float s_c[kReadTile * KR];
+    __shared__ double s_w[4][kReadTile][2];
+    const uint32_t tile = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
+    if (tile >= ntiles) return;
+    const ReadTile& T = tiles[tile];
+    const int kx = (int)T.kx, ns = (int)T.ns;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int i = t; i < ns * KR; i += 256) {
+        const int j = i / KR, k = i % KR;
+        s_c[i] = k < kx ? coefs[items[T.item_off + j].coef_off + k] : 0.0f;
+    }
+    for (int i = t; i < 4 * kReadTile * 2; i += 256) (&s_w[0][0][0])[i] = 0.0;
+    __syncthreads();
+    const uint64_t N = 1ull << n;
+    const uint64_t C = N / nchunk;
+    const uint64_t base = (uint64_t)chunk * C;
+    for (uint64_t a0 = 0; a0 < C; a0 += 256) {
+        const uint64_t a = base + a0 + (uint64_t)t;
+        float x[KR];
+#pragma unroll
+        for (int k = 0; k < KR; k++) x[k] = k < kx ? T.X[(uint64_t)k * N + a] : 0.0f;
+        for (int j = 0; j < ns; j++) {
+            const float* c = s_c + j * KR;
+            // phi_j = sum_k a_jk X_k: even k real, odd k imaginary
+            float pr = 0.0f, pi = 0.0f;
+#pragma unroll
+            for (int k = 0; k < KR; k += 2) {
+                pr += c[k] * x[k];
+                pi += c[k + 1] * x[k + 1];
+            }
+            const float* psi = items[T.item_off + j].psi;
+            const float yr = psi[a], yi = psi[N + a];
+            float re = pr * yr - pi * yi;
+            float im = pr * yi + pi * yr;
+#pragma unroll
+            for (int d = 32; d > 0; d >>= 1) {
+                re += __shfl_xor(re, d, 64);
+                im += __shfl_xor(im, d, 64);
+            }
+            if (lane == 0) {
+                s_w[w][j][0] += (double)re;
+                s_w[w][j][1] += (double)im;
+            }
+        }
+    }
+    __syncthreads();
+    if (t < 2 * ns) {
+        const int j = t >> 1, q = t & 1;
+        dots[((uint64_t)tile * nchunk + chunk) * kReadTile * 2 + 2 * j + q] =
+            ((s_w[0][j][q] + s_w[1][j][q]) + s_w[2][j][q]) + s_w[3][j][q];
+    }
+}
+
+int launch_readout_dots(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev, const float* coefs_dev,
+                        int n, uint32_t nchunk, uint32_t kx_max, double* dots, void* stream)
+{
+    if (ntiles == 0) return 0;
+    const uint64_t N = 1ull << n;
+    if (nchunk == 0 || N % nchunk != 0 || (N / nchunk) % 256 != 0) return -2;
+    if (kx_max < 1 || kx_max > (uint32_t)kReadMaxK) return -2;
+    const uint64_t grid = (uint64_t)ntiles * nchunk;
+    if (grid > 0x7fffffffull) return -3;
+    hipStream_t st = (hipStream_t)stream;
+    if (kx_max <= 32)
+        hipLaunchKernelGGL(readout_dot_kernel<32>, dim3((uint32_t)grid), dim3(256), 0, st, tiles_dev, ntiles, items_dev,
+                           coefs_dev, n, nchunk, dots);
+    else if (kx_max <= 64)
+        hipLaunchKernelGGL(readout_dot_kernel<64>, dim3((uint32_t)grid), dim3(256), 0, st, tiles_dev, ntiles, items_dev,
+                           coefs_dev, n, nchunk, dots);
+    else if (kx_max <= 96)
+        hipLaunchKernelGGL(readout_dot_kernel<96>, dim3((uint32_t)grid), dim3(256), 0, st, tiles_dev, ntiles, items_dev,
+                           coefs_dev, n, nchunk, dots);
+    else
+        hipLaunchKernelGGL(readout_dot_kernel<128>, dim3((uint32_t)grid), dim3(256), 0, st, tiles_dev, ntiles,
+                           items_dev, coefs_dev, n, nchunk, dots);
+    return check_launch("readout_dot");
+}
+
+// One thread per item: the amplitude is the chunk partials summed in chunk
+// order; the norm is the state's partial sums in order.
+__global__ void __launch_bounds__(64)
+readout_finalize_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const ReadItem* __restrict__ items,
+                        uint32_t nchunk, uint32_t nparts, const double* __restrict__ dots)
+{
+    const uint32_t tile = blockIdx.x, j = threadIdx.x;
+    if (tile >= ntiles) return;
+    const ReadTile& T = tiles[tile];
+    if (j >= T.ns) return;
+    const ReadItem& I = items[T.item_off + j];
+    double re = 0.0, im = 0.0;
+    for (uint32_t c = 0; c < nchunk; c++) {
+        const double* d = dots + ((uint64_t)tile * nchunk + c) * kReadTile * 2 + 2 * j;
+        re += d[0];
+        im += d[1];
+    }
+    double nrm = 0.0;
+    for (uint32_t q = 0; q < nparts; q++) nrm += I.norm_part[q];
+    *I.prob = (re * re + im * im) / nrm;
+}
+
+int launch_readout_finalize(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev, uint32_t nchunk,
+                            uint32_t nparts, const double* dots, void* stream)
+{
+    if (ntiles == 0) return 0;
+    static_assert(kReadTile <= 64, "one thread per item of a tile");
+    hipLaunchKernelGGL(readout_finalize_kernel, dim3(ntiles), dim3(64), 0, (hipStream_t)stream, tiles_dev, ntiles,
+                       items_dev, nchunk, nparts, dots);
+    return check_launch("readout_finalize");
+}
+
+// ===========================================================================
 // Stage reduction: norm, next-stage scale, probability readout
 // ===========================================================================
 template <typename real>

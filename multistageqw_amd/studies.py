@@ -32,13 +32,13 @@ PEAK = 8000.0
 def _apply_line(ctx, n, wall, **extra):
     st = ctx.last_stats()
     # complex H-applies of the term sweeps over their own device time; the
-    # shared first-stage basis (real-only recursion) is reported as basis_ms
+    # shared bases, combines and readouts are reported as aux_ms
     t = st["sweep_ms"] if st["sweep_ms"] > 0 else st["gpu_ms"]
     applies = st["term_units"] / (t * 1e-3)
     bpa = 36.0 if extra.get("precision", "fp32") == "fp32" else 72.0
     gbs = applies * bpa * (1 << n) / 1e9
     return dict(n=n, applies_per_s=applies, algorithmic_GBps=gbs, frac=gbs / PEAK, device_ms=st["gpu_ms"],
-                basis_ms=st["aux_ms"], wall_s=wall, applies=st["term_units"], **extra)
+                aux_ms=st["aux_ms"], wall_s=wall, applies=st["term_units"], **extra)
 
 
 def precision(ctx, n, m, samples, seed):

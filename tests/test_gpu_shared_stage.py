@@ -36,7 +36,7 @@ def test_shared_first_stage_vs_oracle_and_clenshaw(ctx, oracle, n, m, count, sam
     _, tt_off, sp_off = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0,
                                   shared_first_stage=False, **kw)
     st_off = ctx.last_stats()
-    assert st_off["aux_launches"] == 0 and st_off["psi_const_amp_terms"] > 0
+    assert st_off["psi_const_amp_terms"] > 0
     po, tto, spo = oracle.solve(n, m, recs, samples=samples, threads=8, with_samples=True)
     assert np.array_equal(tt, tto) and np.array_equal(tt_off, tto)
     assert np.max(np.abs(sp - sp_off)) < PATH_TOL
@@ -78,9 +78,9 @@ def test_shared_first_stage_with_high_bit_pass(ctx):
     n, m, samples = 25, 3, 2
     _say("shared first stage n=25 vs per-sample Clenshaw")
     rec = M.generate_sk(n, 1, 4301)
-    _, _, a = ctx.solve(n, m, rec, samples=samples, with_samples=True)
+    _, _, a = ctx.solve(n, m, rec, samples=samples, with_samples=True, final_readout=False)
     assert ctx.last_stats()["aux_launches"] > 0
-    _, _, b = ctx.solve(n, m, rec, samples=samples, with_samples=True, shared_first_stage=False)
+    _, _, b = ctx.solve(n, m, rec, samples=samples, with_samples=True, shared_first_stage=False, final_readout=False)
     assert ctx.last_stats()["aux_launches"] == 0
     assert np.max(np.abs(a - b)) < PATH_TOL
 
@@ -93,9 +93,10 @@ def test_long_first_stage_falls_back(ctx):
     rec = M.generate_sk(n, 1, 4401)[0]
     g = np.array([0.8, 0.3], np.float32)
     t = np.array([4.0, 0.5], np.float32)    # s = onenorm * t > 4 * 0.8 * 14: far above 64 terms
-    _, _, a = ctx.solve_batch([(n, m, rec, g, t)], samples=samples, with_samples=True, small_n_max=0)
+    _, _, a = ctx.solve_batch([(n, m, rec, g, t)], samples=samples, with_samples=True, small_n_max=0,
+                              final_readout=False)
     st = ctx.last_stats()
     assert st["aux_launches"] == 0 and st["psi_const_amp_terms"] > 0
     _, _, b = ctx.solve_batch([(n, m, rec, g, t)], samples=samples, with_samples=True, small_n_max=0,
-                              shared_first_stage=False)
+                              shared_first_stage=False, final_readout=False)
     assert np.array_equal(a, b)
