@@ -1412,10 +1412,15 @@ combine_kernel(const CombineJob* __restrict__ jobs, uint32_t njobs, const Combin
     const uint64_t base = (uint64_t)part * C;
     for (uint64_t a0 = 0; a0 < C; a0 += 256) {
         const uint64_t a = base + a0 + (uint64_t)t;
+        const MQW_GLOBAL float* V = gview(J.V);
         float v[KR];
         v[0] = J.psi0;
 #pragma unroll
-        for (int k = 1; k < KR; k++) v[k] = k < kc ? J.V[(uint64_t)(k - 1) * N + a] : 0.0f;
+        for (int k = 1; k < KR; k++) v[k] = k < kc ? V[(uint64_t)(k - 1) * N + a] : 0.0f;
+        // keep the column in registers for the whole sample loop (the loads
+        // are otherwise re-issued per sample)
+#pragma unroll
+        for (int k = 0; k < KR; k++) asm volatile("" : "+v"(v[k]));
         for (int j = 0; j < ns; j++) {
             const float* c = s_c + j * KR;
             float re = 0.0f, im = 0.0f;
@@ -1424,7 +1429,7 @@ combine_kernel(const CombineJob* __restrict__ jobs, uint32_t njobs, const Combin
                 re += c[k] * v[k];
                 im += c[k + 1] * v[k + 1];
             }
-            float* o = items[J.item_off + j].out;
+            MQW_GLOBAL float* o = gview(items[J.item_off + j].out);
             o[a] = re;
             o[N + a] = im;
             double s = (double)re * (double)re + (double)im * (double)im;
@@ -1508,9 +1513,13 @@ readout_dot_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const Re
     const uint64_t base = (uint64_t)chunk * C;
     for (uint64_t a0 = 0; a0 < C; a0 += 256) {
         const uint64_t a = base + a0 + (uint64_t)t;
+        const MQW_GLOBAL float* X = gview(T.X);
         float x[KR];
 #pragma unroll
-        for (int k = 0; k < KR; k++) x[k] = k < kx ? T.X[(uint64_t)k * N + a] : 0.0f;
+        for (int k = 0; k < KR; k++) x[k] = k < kx ? X[(uint64_t)k * N + a] : 0.0f;
+        // keep the column in registers for the whole sample loop
+#pragma unroll
+        for (int k = 0; k < KR; k++) asm volatile("" : "+v"(x[k]));
         for (int j = 0; j < ns; j++) {
             const float* c = s_c + j * KR;
             // phi_j = sum_k a_jk X_k: even k real, odd k imaginary
@@ -1520,7 +1529,7 @@ readout_dot_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const Re
                 pr += c[k] * x[k];
                 pi += c[k + 1] * x[k + 1];
             }
-            const float* psi = items[T.item_off + j].psi;
+            const MQW_GLOBAL float* psi = gview(items[T.item_off + j].psi);
             const float yr = psi[a], yi = psi[N + a];
             float re = pr * yr - pi * yi;
             float im = pr * yi + pi * yr;
