@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 5
+#define MQW_ABI_VERSION 6
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -111,6 +111,10 @@ typedef struct {
                                 probability moves by at most 2 tol relative (MultiQW.cpp:441-446
                                 renormalises the propagated state); 0 = the last stage runs as
                                 Clenshaw terms and its output is renormalised, as the reference */
+    int shared_second_stage; /* 1 (default) = for m >= 3, where it costs less than the samples' own terms,
+                                the second stage too comes from a shared basis: W_lk = T_l(H2/onenorm2) V_k
+                                per instance (kc1 x kc2 real vectors) and one matrix-core GEMM row of
+                                coefficient products per sample; 0 = per-sample Clenshaw terms */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are

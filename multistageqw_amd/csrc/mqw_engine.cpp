@@ -199,6 +199,7 @@ struct Traj {
     std::vector<Segment> segs;
     const void* psi_in = nullptr; // input of the first stage; nullptr = |+>
     bool first_done = false;      // the first stage comes from the shared basis (a combine op)
+    bool second_done = false;     // the first two stages come from the shared bases (a combine2 op)
     bool adjoint = false;         // the last stage is read out by the adjoint dots (no terms)
 };
 
@@ -295,10 +296,10 @@ mqw_status group_energies(mqw_ctx* c, int n, std::vector<Problem*>& ps, int mode
 struct Op {
     int kind; // 0 = term launch, 1 = reduce launch, 2 = high-bit pass over the jobs of the next term launch,
               // 3 = combine launch (shared first stage), 4 = real-half term launch (basis recurrences),
-              // 5 = adjoint readout dots, 6 = adjoint readout finalize
+              // 5 = adjoint readout dots, 6 = adjoint readout finalize, 7 = combine2 launch (second stage)
     size_t off;
     uint32_t count;
-    uint32_t aux = 0;  // kinds 3, 5: largest basis size of its jobs / tiles
+    uint32_t aux = 0;  // kinds 3, 5: largest basis size of its jobs / tiles; kind 7: largest tile
     uint32_t aux2 = 0; // kinds 5, 6: amplitude chunks of the dots
 };
 
@@ -311,6 +312,8 @@ struct Plan {
     std::vector<float> ccoef;
     std::vector<ReadTile> rtiles;
     std::vector<ReadItem> ritems;
+    std::vector<Combine2Tile> c2tiles;
+    std::vector<Combine2Item> c2items;
     size_t dots_doubles = 0;  // largest readout-dot scratch of one op
     double aux_bytes = 0.0;   // algorithmic bytes of the combine and readout launches
     uint64_t aux_launches = 0;
@@ -345,14 +348,15 @@ void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, cons
     int p = -1, a = 0, b = 1, par = 0;
     int64_t prev_slot = -1;
     size_t s0 = 0;
-    if (tr.first_done) {
-        // a combine op left the first stage's output in st[0] and its partial norms in part[0]
+    if (tr.first_done || tr.second_done) {
+        // a combine op left the first (combine2: second) stage's output in st[0]
+        // and its partial norms in part[0]
         p = 0;
         a = 1;
         b = 2;
         par = 1;
-        prev_slot = tr.segs[0].prob_slot;
-        s0 = 1;
+        s0 = tr.second_done ? 2 : 1;
+        prev_slot = tr.segs[s0 - 1].prob_slot;
     }
     auto rc = [](double v) { return (real)v; };
     // an adjoint trajectory stops before its last stage
@@ -463,7 +467,31 @@ struct Bases {
     std::vector<int> kc;
     std::vector<const float*> X;
     std::vector<int> kx;
+    // shared second stage: W (kc x kc2 vectors, l >= 1 stored), its row table and |+> as a vector
+    std::vector<const float*> W;
+    std::vector<int> kc2;
+    std::vector<const uint32_t*> qtab;
+    std::vector<uint32_t> q0, q1;
+    const float* plus = nullptr;
 };
+
+// Rows (l, k) of the second-stage GEMM in two classes, each padded to an even
+// count: real ((l&1) == (k&1), minus when both are odd), then imaginary.
+void second_stage_rows(int kc1, int kc2, std::vector<uint32_t>& rows, uint32_t& q0, uint32_t& q1)
+{
+    rows.clear();
+    for (int cls = 0; cls < 2; cls++) {
+        const size_t b = rows.size();
+        for (int l = 0; l < kc2; l++)
+            for (int k = 0; k < kc1; k++) {
+                if (((l & 1) != (k & 1)) != (cls == 1)) continue;
+                const uint32_t sg = (cls == 0 && (l & 1)) ? 2u : 1u;
+                rows.push_back((uint32_t)l | ((uint32_t)k << 8) | (sg << 16));
+            }
+        if ((rows.size() - b) & 1) rows.push_back(0u); // l = k = 0 with no weight
+        (cls ? q1 : q0) = (uint32_t)(rows.size() - b);
+    }
+}
 
 // Readout-dot tiles for the adjoint members of one group: per problem, tiles
 // of up to kReadTile samples; the dots are split over amplitude chunks so a
@@ -568,6 +596,63 @@ void combine_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_t
     }
 }
 
+// Combine2 tiles for the second_done members of one group: per problem,
+// tiles of up to kC2Tile samples, each writing its slot's st[0] and part[0].
+template <typename real>
+void combine2_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_t>& grp, const Bases& B,
+                  const std::vector<Slot<real>>& slots, Plan<real>& P)
+{
+    if constexpr (std::is_same<real, float>::value) {
+        const size_t N = (size_t)1 << n;
+        std::map<int, std::vector<size_t>> byprob;
+        for (size_t i = 0; i < grp.size(); i++)
+            if (trajs[grp[i]].second_done) byprob[trajs[grp[i]].prob].push_back(i);
+        if (byprob.empty()) return;
+        Op op{7, P.c2tiles.size(), 0, 0};
+        for (const auto& kv : byprob) {
+            const int pr = kv.first, kc1 = B.kc[pr], kc2 = B.kc2[pr];
+            const std::vector<size_t>& mem = kv.second;
+            for (size_t i0 = 0; i0 < mem.size(); i0 += kC2Tile) {
+                const size_t ns = std::min<size_t>(kC2Tile, mem.size() - i0);
+                Combine2Tile ct{};
+                ct.plus = B.plus;
+                ct.V = B.V[pr];
+                ct.W = B.W[pr];
+                ct.qtab = B.qtab[pr];
+                ct.q0 = B.q0[pr];
+                ct.q1 = B.q1[pr];
+                ct.kc1 = (uint32_t)kc1;
+                ct.kc2 = (uint32_t)kc2;
+                ct.item_off = (uint32_t)P.c2items.size();
+                ct.ns = (uint32_t)ns;
+                for (size_t q = 0; q < ns; q++) {
+                    const size_t i = mem[i0 + q];
+                    const Traj& tr = trajs[grp[i]];
+                    Combine2Item it{};
+                    it.out = slots[i].st[0];
+                    it.norm_part = slots[i].part[0];
+                    it.c1_off = (uint32_t)P.ccoef.size();
+                    for (int k = 0; k < kc1; k++)
+                        P.ccoef.push_back(k < (int)tr.segs[0].c.size() ? (float)tr.segs[0].c[k] : 0.0f);
+                    it.c2_off = (uint32_t)P.ccoef.size();
+                    for (int l = 0; l < kc2; l++)
+                        P.ccoef.push_back(l < (int)tr.segs[1].c.size() ? (float)tr.segs[1].c[l] : 0.0f);
+                    P.c2items.push_back(it);
+                }
+                P.c2tiles.push_back(ct);
+                op.count++;
+                op.aux = std::max<uint32_t>(op.aux, (uint32_t)ns);
+                // reads every basis row once, writes both halves of every sample
+                P.aux_bytes += (double)N * (4.0 * (double)(B.q0[pr] + B.q1[pr]) + 8.0 * (double)ns);
+            }
+        }
+        P.ops.push_back(op);
+        P.aux_launches++;
+    } else {
+        (void)n, (void)trajs, (void)grp, (void)B, (void)slots, (void)P;
+    }
+}
+
 // Groups of trajectories (indices into trajs), each run to completion before the next.
 template <typename real>
 void build_plan(int n, uint32_t parts, const std::vector<Traj>& trajs, const std::vector<std::vector<size_t>>& groups,
@@ -577,6 +662,7 @@ void build_plan(int n, uint32_t parts, const std::vector<Traj>& trajs, const std
     const size_t N = (size_t)1 << n;
     for (const auto& grp : groups) {
         if (B) combine_ops<real>(n, trajs, grp, *B, slots, P);
+        if (B && !B->W.empty()) combine2_ops<real>(n, trajs, grp, *B, slots, P);
         std::vector<std::vector<TermJob<real>>> seq(grp.size());
         std::vector<ReduceJob> fins;
         std::vector<AdjointIn> adj(grp.size());
@@ -646,6 +732,17 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
         HIPCHK(c, hipMemcpyAsync(ci, P.citems.data(), sizeof(CombineItem) * P.citems.size(), hipMemcpyHostToDevice,
                                  c->stream));
     }
+    Combine2Tile* c2t = nullptr;
+    Combine2Item* c2i = nullptr;
+    if (!P.c2tiles.empty()) {
+        c2t = (Combine2Tile*)dev_get(c, "plan_c2tiles", sizeof(Combine2Tile) * P.c2tiles.size());
+        c2i = (Combine2Item*)dev_get(c, "plan_c2items", sizeof(Combine2Item) * P.c2items.size());
+        if (!c2t || !c2i) return fail(c, MQW_ENOMEM, "device allocation failed for second-stage tiles");
+        HIPCHK(c, hipMemcpyAsync(c2t, P.c2tiles.data(), sizeof(Combine2Tile) * P.c2tiles.size(),
+                                 hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c2i, P.c2items.data(), sizeof(Combine2Item) * P.c2items.size(),
+                                 hipMemcpyHostToDevice, c->stream));
+    }
     if (!P.rtiles.empty()) {
         rtd = (ReadTile*)dev_get(c, "plan_rtiles", sizeof(ReadTile) * P.rtiles.size());
         rid = (ReadItem*)dev_get(c, "plan_ritems", sizeof(ReadItem) * P.ritems.size());
@@ -676,6 +773,8 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
                 rc = launch_readout_dots(rtd + op.off, op.count, rid, cf, n, op.aux2, op.aux, dots, c->stream);
             else if (op.kind == 6)
                 rc = launch_readout_finalize(rtd + op.off, op.count, rid, op.aux2, k.parts, dots, c->stream);
+            else if (op.kind == 7)
+                rc = launch_combine2(c2t + op.off, op.count, c2i, cf, n, k.parts, op.aux, c->stream);
             else if (op.kind == 2)
                 rc = launch_high_pass(k.hi, k.hwb, jd + op.off, op.count, n, c->stream);
             else if (k.nmask > 0)
@@ -834,7 +933,7 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
     // X_0 = e_eloc, ahead of the recurrence on the same stream
     for (size_t p = 0; p < ps.size(); p++)
         if (B.kx[p] > 0) {
-            if (launch_unit_vector(const_cast<float*>(B.X[p]), N, ps[p]->info.eloc, c->stream))
+            if (launch_fill(const_cast<float*>(B.X[p]), N, 0.0f, ps[p]->info.eloc, 1.0f, c->stream))
                 return fail(c, MQW_EDEVICE, "unit vector: %s", last_kernel_error());
             bytes += 4.0 * N;
         }
@@ -902,13 +1001,83 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
         P.ops.push_back(op);
     }
     c->stats.aux_bytes += bytes;
-    if (P.jobs.empty()) return MQW_OK;
-    double ms_sweep = 0.0, ms_aux = 0.0; // every op here is a basis launch (aux)
-    mqw_status st = run_plan<float>(c, MQW_FP32, n, k, P, use_graph, &ms_sweep, &ms_aux);
+    auto run = [&](Plan<float>& Q) -> mqw_status {
+        if (Q.jobs.empty()) return MQW_OK;
+        double ms_sweep = 0.0, ms_aux = 0.0; // every op here is a basis launch (aux)
+        mqw_status st = run_plan<float>(c, MQW_FP32, n, k, Q, use_graph, &ms_sweep, &ms_aux);
+        if (st) return st;
+        c->stats.gpu_ms += ms_sweep + ms_aux;
+        c->stats.aux_ms += ms_sweep + ms_aux;
+        c->stats.aux_launches += Q.ops.size();
+        return MQW_OK;
+    };
+    mqw_status st = run(P);
     if (st) return st;
-    c->stats.gpu_ms += ms_sweep + ms_aux;
-    c->stats.aux_ms += ms_sweep + ms_aux;
-    c->stats.aux_launches += P.ops.size();
+
+    // second stage, from the finished V:
+    //   W_{l+1,k} = 2 Ht_2 W_lk - W_{l-1,k},  W_1k = Ht_2 V_k,  W_0k = V_k (V_0 = |+>)
+    int wsteps = 0;
+    for (size_t p = 0; p < B.kc2.size(); p++) wsteps = std::max(wsteps, B.kc2[p] - 1);
+    if (wsteps > 0) {
+        if (launch_fill(const_cast<float*>(B.plus), N, psi0, N, 0.0f, c->stream))
+            return fail(c, MQW_EDEVICE, "fill: %s", last_kernel_error());
+        std::vector<const Segment*> seg1(ps.size(), nullptr);
+        for (const Traj& tr : trajs)
+            if (!seg1[tr.prob] && tr.segs.size() > 1) seg1[tr.prob] = &tr.segs[1];
+        Plan<float> Pw;
+        for (int l = 0; l < wsteps; l++) {
+            Op op{4, Pw.jobs.size(), 0};
+            for (size_t p = 0; p < ps.size(); p++) {
+                const int kc2 = B.kc2[p];
+                if (l >= kc2 - 1) continue;
+                const int kc1 = B.kc[p];
+                float* W = const_cast<float*>(B.W[p]);
+                const float* V = B.V[p];
+                auto row = [&](int ll, int kk) { return W + ((size_t)kk * (kc2 - 1) + (size_t)(ll - 1)) * N; };
+                for (int kk = 0; kk < kc1; kk++) {
+                    TermJob<float> j = job(p, *seg1[p], l);
+                    j.out = row(l + 1, kk);
+                    uint32_t flags = 0;
+                    double b = 8.0; // E read, output write
+                    if (l == 0) {
+                        if (kk == 0) {
+                            flags |= SRC1_PSI | F_PSI_CONST;
+                        } else {
+                            flags |= SRC1_ARRAY;
+                            j.b1 = V + (size_t)(kk - 1) * N;
+                            b += 4.0;
+                        }
+                        flags |= SRC2_ZERO;
+                    } else {
+                        flags |= SRC1_ARRAY;
+                        j.b1 = row(l, kk);
+                        j.w2r = 1.0f;
+                        b += 4.0;
+                        if (l == 1) {
+                            if (kk == 0) {
+                                flags |= SRC2_PSI | F_PSI_CONST;
+                            } else {
+                                flags |= SRC2_ARRAY;
+                                j.b2 = V + (size_t)(kk - 1) * N;
+                                b += 4.0;
+                            }
+                        } else {
+                            flags |= SRC2_ARRAY;
+                            j.b2 = row(l - 1, kk);
+                            b += 4.0;
+                        }
+                    }
+                    j.flags = flags;
+                    Pw.jobs.push_back(j);
+                    op.count++;
+                    c->stats.aux_bytes += b * N;
+                }
+            }
+            Pw.ops.push_back(op);
+        }
+        st = run(Pw);
+        if (st) return st;
+    }
     return MQW_OK;
 }
 
@@ -990,85 +1159,72 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     }
 
     const uint32_t nparts = k.parts;
-    // shared bases, one per instance with m > 1 (samples restart from |+>):
-    // the first stage's V (combined per sample) and the last stage's adjoint X
-    Bases bases;
-    size_t basis_bytes = 0;
-    bool use_bases = false;
+    // Shared bases, per instance with m > 1 (its samples restart from |+>):
+    // the first stage's V (kc vectors, combined per sample), the second
+    // stage's W (kc x kc2, from m = 3, when it costs less than the samples'
+    // own terms) and the last stage's adjoint X (kx).  0 = not shared.
     const bool basis_path =
         std::is_same<real, float>::value && k.half && !mixer && n >= k.tb && (N / nparts) % 256 == 0;
-    if (basis_path && (o.shared_first_stage || o.final_readout)) {
-        const size_t cap = o.max_device_bytes ? o.max_device_bytes / 4 : ((size_t)16 << 30);
-        bases.V.assign(ps.size(), nullptr);
-        bases.kc.assign(ps.size(), 0);
-        bases.X.assign(ps.size(), nullptr);
-        bases.kx.assign(ps.size(), 0);
+    std::vector<int> kc1(ps.size(), 0), kc2(ps.size(), 0), kx(ps.size(), 0);
+    if (basis_path) {
+        std::vector<double> own2(ps.size(), 0.0); // bytes per amplitude of the samples' own second-stage terms
         for (const Traj& tr : trajs) {
-            if (ps[tr.prob]->m < 2) continue;
-            if (o.shared_first_stage)
-                bases.kc[tr.prob] = std::max(bases.kc[tr.prob], (int)tr.segs.front().c.size());
-            if (o.final_readout) bases.kx[tr.prob] = std::max(bases.kx[tr.prob], (int)tr.segs.back().c.size());
+            const int pm = ps[tr.prob]->m;
+            if (pm < 2) continue;
+            if (o.shared_first_stage) kc1[tr.prob] = std::max(kc1[tr.prob], (int)tr.segs.front().c.size());
+            if (o.final_readout) kx[tr.prob] = std::max(kx[tr.prob], (int)tr.segs.back().c.size());
+            if (pm >= 3 && o.shared_second_stage) {
+                kc2[tr.prob] = std::max(kc2[tr.prob], (int)tr.segs[1].c.size());
+                own2[tr.prob] += 36.0 * (double)std::max<size_t>(tr.segs[1].c.size() - 1, 1);
+            }
         }
-        std::vector<size_t> offv(ps.size(), 0), offx(ps.size(), 0);
         for (size_t p = 0; p < ps.size(); p++) {
-            const int kc = bases.kc[p], kx = bases.kx[p];
-            const size_t bv = sizeof(float) * N * (size_t)std::max(kc - 1, 0);
-            if (kc < 1 || kc > kCombineMaxK || basis_bytes + bv > cap) {
-                bases.kc[p] = 0;
-            } else {
-                offv[p] = basis_bytes;
-                basis_bytes += bv;
-            }
-            const size_t bx = sizeof(float) * N * (size_t)std::max(kx, 0);
-            if (kx < 1 || kx > kReadMaxK || basis_bytes + bx > cap) {
-                bases.kx[p] = 0;
-            } else {
-                offx[p] = basis_bytes;
-                basis_bytes += bx;
-            }
-        }
-        char* base = basis_bytes ? (char*)dev_get(c, "basis", basis_bytes) : nullptr;
-        if (basis_bytes && !base) return fail(c, MQW_ENOMEM, "device allocation failed for the shared bases");
-        for (size_t p = 0; p < ps.size(); p++) {
-            if (bases.kc[p] > 0) {
-                bases.V[p] = (const float*)(base + offv[p]); // unused when kc = 1 (V_0 only)
-                use_bases = true;
-            }
-            if (bases.kx[p] > 0) {
-                bases.X[p] = (const float*)(base + offx[p]);
-                use_bases = true;
-            }
-        }
-        if (use_bases) {
-            st = run_bases(c, n, k, ps, trajs, (const float*)E, bases, o.use_graph != 0);
-            if (st) return st;
-            for (Traj& tr : trajs) {
-                tr.first_done = bases.kc[tr.prob] > 0;
-                tr.adjoint = bases.kx[tr.prob] > 0;
-            }
+            if (kc1[p] > kCombineMaxK) kc1[p] = 0;
+            if (kx[p] > kReadMaxK) kx[p] = 0;
+            // W: recurrences (16 B per vector and amplitude) plus one GEMM read (4 B), against the
+            // samples' own terms, with a margin for the matrix work
+            const double shared2 = 20.0 * (double)kc1[p] * (double)kc2[p];
+            if (kc1[p] < 1 || kc1[p] > kC2MaxK || kc2[p] < 2 || kc2[p] > kC2MaxK || shared2 > 0.8 * own2[p])
+                kc2[p] = 0;
         }
     }
+    const size_t vcap = o.max_device_bytes ? o.max_device_bytes / 4 : ((size_t)16 << 30);
+    const size_t wcap = o.max_device_bytes ? o.max_device_bytes / 4 : ((size_t)32 << 30);
+    auto wbytes = [&](size_t p) -> size_t {
+        return kc2[p] ? sizeof(float) * N * (size_t)kc1[p] * (size_t)(kc2[p] - 1) : 0;
+    };
+    for (size_t p = 0; p < ps.size(); p++)
+        if (wbytes(p) > wcap) kc2[p] = 0;
+    // problem batches whose second-stage bases fit together
+    std::vector<std::pair<size_t, size_t>> pbatches;
+    {
+        size_t b0 = 0, wsum = 0;
+        for (size_t p = 0; p < ps.size(); p++) {
+            if (p > b0 && wsum + wbytes(p) > wcap) {
+                pbatches.emplace_back(b0, p);
+                b0 = p;
+                wsum = 0;
+            }
+            wsum += wbytes(p);
+        }
+        pbatches.emplace_back(b0, ps.size());
+    }
+    std::vector<std::vector<size_t>> ptraj(ps.size());
+    for (size_t t = 0; t < trajs.size(); t++) ptraj[trajs[t].prob].push_back(t);
+    size_t max_batch_trajs = 0;
+    for (const auto& pb : pbatches) {
+        size_t cnt = 0;
+        for (size_t p = pb.first; p < pb.second; p++) cnt += ptraj[p].size();
+        max_batch_trajs = std::max(max_batch_trajs, cnt);
+    }
 
-    // groups sized to the memory-side cache, longest trajectories first
+    // trajectory slots, sized to the memory-side cache budget
     const size_t nst = k.hi > 0 ? 8 : 6; // planar states per slot (+ the high-bit sums)
     const size_t per_traj = sizeof(real) * nst * N + sizeof(double) * 2 * nparts;
     const size_t budget = o.group_bytes ? o.group_bytes : ((size_t)32 << 30);
     size_t G = std::max<size_t>(1, budget / per_traj);
-    if (o.max_device_bytes)
-        G = std::min(G, std::max<size_t>(1, (o.max_device_bytes - std::min(o.max_device_bytes, basis_bytes)) / per_traj));
-    G = std::min(G, trajs.size());
-    std::vector<size_t> order(trajs.size());
-    std::vector<size_t> cost(trajs.size(), 0);
-    for (size_t t = 0; t < trajs.size(); t++) {
-        order[t] = t;
-        const size_t s1 = trajs[t].segs.size() - (trajs[t].adjoint ? 1 : 0);
-        for (size_t s = trajs[t].first_done ? 1 : 0; s < s1; s++)
-            cost[t] += std::max<size_t>(trajs[t].segs[s].c.size() - 1, 1);
-    }
-    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
-    std::vector<std::vector<size_t>> groups;
-    for (size_t i = 0; i < order.size(); i += G)
-        groups.emplace_back(order.begin() + i, order.begin() + std::min(order.size(), i + G));
+    if (o.max_device_bytes) G = std::min(G, std::max<size_t>(1, o.max_device_bytes / 2 / per_traj));
+    G = std::min(G, std::max<size_t>(1, max_batch_trajs));
     real* state = (real*)dev_get(c, "state", sizeof(real) * nst * N * G);
     double* parts = (double*)dev_get(c, "norm_part", sizeof(double) * 2 * nparts * G);
     if (!state || !parts) return fail(c, MQW_ENOMEM, "device allocation failed for %zu trajectory slots at n=%d", G, n);
@@ -1078,34 +1234,135 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         if (k.hi > 0) slots[i].hs = state + (3 * G + i) * 2 * N;
         for (int q = 0; q < 2; q++) slots[i].part[q] = parts + (i * 2 + (size_t)q) * nparts;
     }
-    // bound the host job table: run the groups in batches
-    const size_t jobs_cap = (size_t)1 << 22;
-    size_t g0 = 0;
-    while (g0 < groups.size()) {
-        size_t g1 = g0, njobs = 0;
-        while (g1 < groups.size()) {
-            size_t gj = 0;
-            for (size_t t : groups[g1]) gj += cost[t];
-            if (g1 > g0 && njobs + gj > jobs_cap) break;
-            njobs += gj;
-            g1++;
+
+    for (const auto& pb : pbatches) {
+        // the batch's bases
+        Bases bases;
+        bool use_bases = false;
+        if (basis_path) {
+            bases.V.assign(ps.size(), nullptr);
+            bases.kc.assign(ps.size(), 0);
+            bases.X.assign(ps.size(), nullptr);
+            bases.kx.assign(ps.size(), 0);
+            bases.W.assign(ps.size(), nullptr);
+            bases.kc2.assign(ps.size(), 0);
+            bases.qtab.assign(ps.size(), nullptr);
+            bases.q0.assign(ps.size(), 0);
+            bases.q1.assign(ps.size(), 0);
+            size_t vbytes = 0, total = 0;
+            std::vector<size_t> offv(ps.size(), 0), offx(ps.size(), 0), offw(ps.size(), 0);
+            std::vector<std::vector<uint32_t>> rows(ps.size());
+            size_t nrows = 0;
+            bool any_w = false;
+            for (size_t p = pb.first; p < pb.second; p++) {
+                const size_t bv = sizeof(float) * N * (size_t)std::max(kc1[p] - 1, 0);
+                const size_t bx = sizeof(float) * N * (size_t)kx[p];
+                if (kc1[p] > 0 && vbytes + bv <= vcap) {
+                    bases.kc[p] = kc1[p];
+                    offv[p] = total;
+                    total += bv;
+                    vbytes += bv;
+                }
+                if (kx[p] > 0 && vbytes + bx <= vcap) {
+                    bases.kx[p] = kx[p];
+                    offx[p] = total;
+                    total += bx;
+                    vbytes += bx;
+                }
+                if (kc2[p] > 0 && bases.kc[p] > 0) {
+                    bases.kc2[p] = kc2[p];
+                    offw[p] = total;
+                    total += wbytes(p);
+                    second_stage_rows(kc1[p], kc2[p], rows[p], bases.q0[p], bases.q1[p]);
+                    nrows += rows[p].size();
+                    any_w = true;
+                }
+            }
+            const size_t plus_off = total;
+            if (any_w) total += sizeof(float) * N;
+            char* base = total ? (char*)dev_get(c, "basis", total) : nullptr;
+            if (total && !base) return fail(c, MQW_ENOMEM, "device allocation failed for the shared bases");
+            uint32_t* qdev = nullptr;
+            if (any_w) {
+                bases.plus = (const float*)(base + plus_off);
+                qdev = (uint32_t*)dev_get(c, "basis_rows", sizeof(uint32_t) * nrows);
+                if (!qdev) return fail(c, MQW_ENOMEM, "device allocation failed for second-stage rows");
+                std::vector<uint32_t> all;
+                all.reserve(nrows);
+                for (size_t p = pb.first; p < pb.second; p++) {
+                    if (!bases.kc2[p]) continue;
+                    bases.qtab[p] = qdev + all.size();
+                    all.insert(all.end(), rows[p].begin(), rows[p].end());
+                }
+                HIPCHK(c, hipMemcpyAsync(qdev, all.data(), sizeof(uint32_t) * all.size(), hipMemcpyHostToDevice,
+                                         c->stream));
+            }
+            for (size_t p = pb.first; p < pb.second; p++) {
+                if (bases.kc[p] > 0) bases.V[p] = (const float*)(base + offv[p]); // unused when kc = 1 (V_0 only)
+                if (bases.kx[p] > 0) bases.X[p] = (const float*)(base + offx[p]);
+                if (bases.kc2[p] > 0) bases.W[p] = (const float*)(base + offw[p]);
+                use_bases |= bases.kc[p] > 0 || bases.kx[p] > 0;
+            }
+            if (use_bases) {
+                st = run_bases(c, n, k, ps, trajs, (const float*)E, bases, o.use_graph != 0);
+                if (st) return st;
+            }
+            if (!any_w) {
+                bases.W.clear();
+                bases.kc2.clear();
+            }
         }
-        std::vector<std::vector<size_t>> batch(groups.begin() + g0, groups.begin() + g1);
-        Plan<real> plan;
-        build_plan<real>(n, nparts, trajs, batch, ps, E, slots, probs_d, plan, k.hi, use_bases ? &bases : nullptr);
-        double ms_sweep = 0.0, ms_aux = 0.0;
-        st = run_plan<real>(c, precision, n, k, plan, o.use_graph != 0, &ms_sweep, &ms_aux);
-        if (st) return st;
-        c->stats.gpu_ms += ms_sweep + ms_aux;
-        c->stats.sweep_ms += ms_sweep;
-        c->stats.aux_ms += ms_aux;
-        c->stats.aux_bytes += plan.aux_bytes;
-        c->stats.aux_launches += plan.aux_launches;
-        c->stats.sweeps += plan.sweeps;
-        c->stats.term_units += plan.terms;
-        c->stats.amp_terms += plan.amp_terms;
-        c->stats.psi_const_amp_terms += plan.psi_const_amp_terms;
-        g0 = g1;
+        // the batch's trajectories, longest first
+        std::vector<size_t> order;
+        for (size_t p = pb.first; p < pb.second; p++)
+            for (size_t t : ptraj[p]) {
+                Traj& tr = trajs[t];
+                tr.second_done = use_bases && !bases.kc2.empty() && bases.kc2[p] > 0;
+                tr.first_done = use_bases && !tr.second_done && bases.kc[p] > 0;
+                tr.adjoint = use_bases && bases.kx[p] > 0;
+                order.push_back(t);
+            }
+        std::vector<size_t> cost(trajs.size(), 0);
+        for (size_t t : order) {
+            const Traj& tr = trajs[t];
+            const size_t s1 = tr.segs.size() - (tr.adjoint ? 1 : 0);
+            for (size_t s = tr.second_done ? 2 : tr.first_done ? 1 : 0; s < s1; s++)
+                cost[t] += std::max<size_t>(tr.segs[s].c.size() - 1, 1);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+        std::vector<std::vector<size_t>> groups;
+        for (size_t i = 0; i < order.size(); i += G)
+            groups.emplace_back(order.begin() + i, order.begin() + std::min(order.size(), i + G));
+        // bound the host job table: run the groups in batches
+        const size_t jobs_cap = (size_t)1 << 22;
+        size_t g0 = 0;
+        while (g0 < groups.size()) {
+            size_t g1 = g0, njobs = 0;
+            while (g1 < groups.size()) {
+                size_t gj = 0;
+                for (size_t t : groups[g1]) gj += cost[t];
+                if (g1 > g0 && njobs + gj > jobs_cap) break;
+                njobs += gj;
+                g1++;
+            }
+            std::vector<std::vector<size_t>> batch(groups.begin() + g0, groups.begin() + g1);
+            Plan<real> plan;
+            build_plan<real>(n, nparts, trajs, batch, ps, E, slots, probs_d, plan, k.hi,
+                             use_bases ? &bases : nullptr);
+            double ms_sweep = 0.0, ms_aux = 0.0;
+            st = run_plan<real>(c, precision, n, k, plan, o.use_graph != 0, &ms_sweep, &ms_aux);
+            if (st) return st;
+            c->stats.gpu_ms += ms_sweep + ms_aux;
+            c->stats.sweep_ms += ms_sweep;
+            c->stats.aux_ms += ms_aux;
+            c->stats.aux_bytes += plan.aux_bytes;
+            c->stats.aux_launches += plan.aux_launches;
+            c->stats.sweeps += plan.sweeps;
+            c->stats.term_units += plan.terms;
+            c->stats.amp_terms += plan.amp_terms;
+            c->stats.psi_const_amp_terms += plan.psi_const_amp_terms;
+            g0 = g1;
+        }
     }
     HIPCHK(c, hipMemcpyAsync(probs.data(), probs_d, sizeof(double) * probs.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1226,6 +1483,7 @@ void mqw_opts_default(mqw_opts* o)
     o->high_pass = -1;
     o->shared_first_stage = 1;
     o->final_readout = 1;
+    o->shared_second_stage = 1;
     o->sample_probs = nullptr;
 }
 

@@ -117,6 +117,34 @@ struct CombineJob {
 int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem* items_dev, const float* coefs_dev,
                    int n, uint32_t nparts, uint32_t kc_max, void* stream);
 
+// Shared second stage.  The second stage's output of sample j is
+//     psi2_j = sum_{l,k} d_jl c_jk i^(l&1) i^(k&1) W_lk,  W_lk = T_l(Ht_2) V_k,
+// with c_j, d_j the sample's first- and second-stage coefficients and V_k
+// the shared first-stage basis: one basis W of kc1 x kc2 real vectors per
+// instance (real-half recurrences from each V_k), and per sample a GEMM row.
+// The rows (l, k) split into a real class ((l&1) == (k&1); sign -1 when both
+// are odd) and an imaginary class.  The scale of psi2 is irrelevant: every
+// later stage and the readout normalise their input.
+constexpr int kC2MaxK = 64;   // kc1 and kc2 limits
+constexpr int kC2Tile = 128;  // samples per workgroup (4 MFMA row blocks of 32)
+struct Combine2Item {
+    float* out;           // planar output (2N floats)
+    double* norm_part;    // nparts partial sums of |out|^2, one per workgroup
+    uint32_t c1_off;      // kc1 first-stage coefficients (zero padded)
+    uint32_t c2_off;      // kc2 second-stage coefficients (zero padded)
+};
+struct Combine2Tile {
+    const float* plus;    // |+> as N floats: W_00
+    const float* V;       // V_1 .. V_{kc1-1}: W_0k
+    const float* W;       // W_lk for l >= 1 at W + (k (kc2 - 1) + l - 1) N
+    const uint32_t* qtab; // q0 real rows then q1 imaginary rows: l | k << 8 | s << 16, s = 1 plus, 2 minus, 0 pad
+    uint32_t q0, q1;      // even row counts
+    uint32_t kc1, kc2;
+    uint32_t item_off, ns;
+};
+int launch_combine2(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combine2Item* items_dev,
+                    const float* coefs_dev, int n, uint32_t nparts, uint32_t ns_max, void* stream);
+
 // Adjoint readout of the last stage.  The reference keeps only
 // |psi_m[eloc]|^2 / ||psi_m||^2 of the last stage's output psi_m = P psi
 // (P = sum_k a_k T_k(Ht), a_k = c_k i^(k&1)).  T_k(Ht) is real symmetric, so
@@ -141,8 +169,8 @@ struct ReadTile {
     uint32_t item_off;
     uint32_t pad;
 };
-// e = unit vector k of length N.
-int launch_unit_vector(float* e, uint64_t N, uint64_t k, void* stream);
+// x[i] = v for i != k, x[k] = vk (k >= N: every element v).
+int launch_fill(float* x, uint64_t N, float v, uint64_t k, float vk, void* stream);
 // Partial amplitudes sum_k a_jk <X_k, psi_j> over nchunk amplitude chunks:
 //   dots[((tile * nchunk + chunk) * kReadTile + j) * 2 + {0: Re, 1: Im}]
 int launch_readout_dots(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev, const float* coefs_dev,

@@ -1469,6 +1469,183 @@ int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem
 }
 
 // ===========================================================================
+// Shared second stage (see Combine2Tile): out_j = E_j W on the matrix cores.
+// A workgroup of 8 waves takes one chunk of N / nparts amplitudes for up to
+// 32 MB samples; per 256 amplitudes, wave w owns 32 of them and MB row blocks
+// of 32 samples, both classes: 2 MB accumulator tiles of
+// v_mfma_f32_32x32x2f32 (A = coefficients, 32 samples x 2 rows; B = basis,
+// 2 rows x 32 amplitudes).  A lane's A entry d_jl c_jk s is formed from the
+// coefficient rows in LDS; its B entry is one load of row (l, k) of the basis.
+// Loads run UNR row pairs ahead of the matrix ops.  The squares of each
+// output go through a 32-lane tree into FP64 per wave, summed in wave order.
+// ===========================================================================
+typedef float mqw_f32x16 __attribute__((ext_vector_type(16)));
+
+template <int MB>
+__global__ void __launch_bounds__(512)
+combine2_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const Combine2Item* __restrict__ items,
+                const float* __restrict__ coefs, int n, uint32_t nparts)
+{
+    constexpr int NS = MB * 32;
+    constexpr int CP = kC2MaxK + 1;   // padded coefficient rows: 32 samples hit 32 banks
+    constexpr int UNR = 8;            // row pairs loaded ahead
+    __shared__ float s_c1[NS * CP];
+    __shared__ float s_c2[NS * CP];
+    __shared__ uint32_t s_q[kC2MaxK * kC2MaxK + 2];
+    __shared__ double s_n[8][NS];
+    const uint32_t tile = blockIdx.x / nparts, part = blockIdx.x % nparts;
+    if (tile >= ntiles) return;
+    const Combine2Tile& T = tiles[tile];
+    const int ns = (int)T.ns, kc1 = (int)T.kc1, kc2 = (int)T.kc2;
+    const int q0 = (int)T.q0, nq = (int)(T.q0 + T.q1);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int col = lane & 31, half = lane >> 5;
+    for (int i = t; i < NS * CP; i += 512) {
+        const int j = i / CP, k = i % CP;
+        float a = 0.0f, b = 0.0f;
+        if (j < ns) {
+            const Combine2Item& I = items[T.item_off + j];
+            if (k < kc1) a = coefs[I.c1_off + k];
+            if (k < kc2) b = coefs[I.c2_off + k];
+        }
+        s_c1[i] = a;
+        s_c2[i] = b;
+    }
+    for (int i = t; i < nq; i += 512) s_q[i] = T.qtab[i];
+    for (int i = t; i < 8 * NS; i += 512) (&s_n[0][0])[i] = 0.0;
+    __syncthreads();
+    const uint64_t N = 1ull << n;
+    const uint64_t C = N / nparts;
+    const uint64_t base = (uint64_t)part * C;
+    const MQW_GLOBAL float* plus = gview(T.plus);
+    const MQW_GLOBAL float* V = gview(T.V);
+    const MQW_GLOBAL float* W = gview(T.W);
+    // basis row q of this lane's half, at amplitude a
+    auto load_b = [&](int q, uint64_t a, int& l, int& k, float& sg) -> float {
+        const uint32_t e = s_q[q];
+        l = (int)(e & 255u);
+        k = (int)((e >> 8) & 255u);
+        const uint32_t s = e >> 16;
+        sg = s == 1u ? 1.0f : (s == 2u ? -1.0f : 0.0f);
+        const MQW_GLOBAL float* row =
+            l == 0 ? (k == 0 ? plus : V + (uint64_t)(k - 1) * N) : W + ((uint64_t)k * (kc2 - 1) + (l - 1)) * N;
+        return row[a];
+    };
+    for (uint64_t a0 = 0; a0 < C; a0 += 256) {
+        const uint64_t a = base + a0 + (uint64_t)(w * 32 + col);
+        mqw_f32x16 acc[2][MB];
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[c][mb][r] = 0.0f;
+#pragma unroll
+        for (int cls = 0; cls < 2; cls++) {
+            const int qb = cls ? q0 : 0, qe = cls ? nq : q0;
+            float bv[UNR], sv[UNR];
+            int lv[UNR], kv[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; u++) {
+                const int q = qb + 2 * u;
+                if (q < qe) {
+                    bv[u] = load_b(q + half, a, lv[u], kv[u], sv[u]);
+                } else {
+                    bv[u] = 0.0f;
+                    sv[u] = 0.0f;
+                    lv[u] = kv[u] = 0;
+                }
+            }
+            for (int q = qb; q < qe; q += 2 * UNR) {
+                float bn[UNR], sn[UNR];
+                int ln[UNR], kn[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; u++) {
+                    const int qq = q + 2 * UNR + 2 * u;
+                    if (qq < qe) {
+                        bn[u] = load_b(qq + half, a, ln[u], kn[u], sn[u]);
+                    } else {
+                        bn[u] = 0.0f;
+                        sn[u] = 0.0f;
+                        ln[u] = kn[u] = 0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UNR; u++)
+#pragma unroll
+                    for (int mb = 0; mb < MB; mb++) {
+                        const int j = mb * 32 + col;
+                        const float av = sv[u] * s_c2[j * CP + lv[u]] * s_c1[j * CP + kv[u]];
+                        acc[cls][mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv[u], acc[cls][mb], 0, 0, 0);
+                    }
+#pragma unroll
+                for (int u = 0; u < UNR; u++) {
+                    bv[u] = bn[u];
+                    sv[u] = sn[u];
+                    lv[u] = ln[u];
+                    kv[u] = kn[u];
+                }
+            }
+        }
+        // lane (col, half) holds rows 8 (r / 4) + 4 half + r % 4 of column col
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int j = mb * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+                const float re = acc[0][mb][r], im = acc[1][mb][r];
+                if (j < ns) {
+                    MQW_GLOBAL float* o = gview(items[T.item_off + j].out);
+                    o[a] = re;
+                    o[N + a] = im;
+                }
+                float sq = re * re + im * im;
+#pragma unroll
+                for (int d = 1; d < 32; d <<= 1) sq += __shfl_xor(sq, d, 64);
+                if (col == 0) s_n[w][j] += (double)sq;
+            }
+    }
+    __syncthreads();
+    if (t < ns) {
+        double sum = 0.0;
+        for (int q = 0; q < 8; q++) sum += s_n[q][t];
+        items[T.item_off + t].norm_part[part] = sum;
+    }
+}
+
+int launch_combine2(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combine2Item* items_dev,
+                    const float* coefs_dev, int n, uint32_t nparts, uint32_t ns_max, void* stream)
+{
+    if (ntiles == 0) return 0;
+    const uint64_t N = 1ull << n;
+    if (nparts == 0 || N % nparts != 0 || (N / nparts) % 256 != 0) return -2;
+    if (ns_max < 1 || ns_max > (uint32_t)kC2Tile) return -2;
+    const uint64_t grid = (uint64_t)ntiles * nparts;
+    if (grid > 0x7fffffffull) return -3;
+    hipStream_t st = (hipStream_t)stream;
+    const int mb = (int)((ns_max + 31) / 32);
+    switch (mb) {
+    case 1:
+        hipLaunchKernelGGL(combine2_kernel<1>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
+                           coefs_dev, n, nparts);
+        break;
+    case 2:
+        hipLaunchKernelGGL(combine2_kernel<2>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
+                           coefs_dev, n, nparts);
+        break;
+    case 3:
+        hipLaunchKernelGGL(combine2_kernel<3>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
+                           coefs_dev, n, nparts);
+        break;
+    default:
+        hipLaunchKernelGGL(combine2_kernel<4>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
+                           coefs_dev, n, nparts);
+        break;
+    }
+    return check_launch("combine2");
+}
+
+// ===========================================================================
 // Adjoint readout of the last stage (see ReadItem).  A workgroup takes one
 // chunk of N / nchunk amplitudes for a tile of up to kReadTile samples of one
 // instance.  Every thread keeps the KR basis values X_k[a] of its amplitude
@@ -1477,17 +1654,17 @@ int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem
 // a wave is a cross-lane tree, and the waves' sums go to FP64 in LDS, in a
 // fixed order.  So the basis is read once per tile and each state once.
 // ===========================================================================
-__global__ void __launch_bounds__(256) unit_vector_kernel(float* __restrict__ e, uint64_t N, uint64_t k)
+__global__ void __launch_bounds__(256) fill_kernel(float* __restrict__ x, uint64_t N, float v, uint64_t k, float vk)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < N; i += (uint64_t)gridDim.x * 256)
-        e[i] = i == k ? 1.0f : 0.0f;
+        x[i] = i == k ? vk : v;
 }
 
-int launch_unit_vector(float* e, uint64_t N, uint64_t k, void* stream)
+int launch_fill(float* x, uint64_t N, float v, uint64_t k, float vk, void* stream)
 {
     const uint64_t blocks = (N + 255) / 256 < 4096 ? (N + 255) / 256 : 4096;
-    hipLaunchKernelGGL(unit_vector_kernel, dim3((uint32_t)blocks), dim3(256), 0, (hipStream_t)stream, e, N, k);
-    return check_launch("unit_vector");
+    hipLaunchKernelGGL(fill_kernel, dim3((uint32_t)blocks), dim3(256), 0, (hipStream_t)stream, x, N, v, k, vk);
+    return check_launch("fill");
 }
 
 template <int KR>
