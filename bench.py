@@ -10,9 +10,11 @@ the input records already in host memory.  `value` is instances/s over the
 timed steps.
 
 Also reported: the n = 24 apply rate (H|psi> applies/s and algorithmic GB/s of
-one n = 24, m = 3 instance), the roofline of the dominant kernel (the fused
-Clenshaw term kernel) and a CPU baseline: the FP32 restatement of MultiQW.cpp
-(oracle/) on the host cores for a bounded sample of the same workload.
+one n = 24, m = 3 instance), the roofline of the kernel class with the most
+device time (term sweeps, shared-basis recurrences or the second-stage GEMM;
+all three are listed) and a CPU baseline: the FP32 restatement of
+MultiQW.cpp (oracle/) on the host cores for a bounded sample of the same
+workload.
 """
 from __future__ import annotations
 
@@ -28,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 
 
 def main() -> int:
@@ -64,45 +67,66 @@ def main() -> int:
 
     for k in range(args.warmup):
         step(k)
-    agg = dict(amp_terms=0, gpu_ms=0.0, sweep_ms=0.0, sweeps=0, term_units=0, alg=0.0, req=0.0, aux_ms=0.0,
-               aux_bytes=0.0, aux_launches=0)
+    keys = ("amp_terms", "gpu_ms", "sweep_ms", "sweeps", "term_units", "algorithmic_bytes", "required_bytes",
+            "aux_ms", "aux_bytes", "aux_launches", "basis_ms", "basis_bytes", "basis_launches", "gemm_ms",
+            "gemm_flops", "gemm_useful_flops", "gemm_launches", "gemm_bytes")
+    agg = {key: 0.0 for key in keys}
     t0 = time.perf_counter()
     probs = []
     for k in range(args.steps):
         p, _ = step(args.warmup + k)
         st = ctx.last_stats()
-        agg["amp_terms"] += st["amp_terms"]
-        agg["gpu_ms"] += st["gpu_ms"]
-        agg["sweeps"] += st["sweeps"]
-        agg["term_units"] += st["term_units"]
-        agg["alg"] += st["algorithmic_bytes"]
-        agg["req"] += st["required_bytes"]
-        for key in ("sweep_ms", "aux_ms", "aux_bytes", "aux_launches"):
+        for key in keys:
             agg[key] += st[key]
         probs.append(p)
     wall = time.perf_counter() - t0
     inst = args.steps * args.per_step
     value = inst / wall if wall > 0 else 0.0
-    # Roofline of the fused term kernel: algorithmic bytes (36 B per amplitude
-    # per complex H-apply) over the event-timed device time of the term sweeps
-    # (term launches and the small per-group reduce launches).
-    achieved = agg["alg"] / (agg["sweep_ms"] * 1e-3) / 1e9 if agg["sweep_ms"] > 0 else 0.0
-    # The same without the psi reads of terms whose psi is the constant |+>.
-    achieved_req = agg["req"] / (agg["sweep_ms"] * 1e-3) / 1e9 if agg["sweep_ms"] > 0 else 0.0
-    per_launch_bytes = agg["alg"] / max(1, agg["sweeps"])
-    per_launch_ms = agg["sweep_ms"] / max(1, agg["sweeps"])
-    # Every propagation kernel together: the shared bases, combines and adjoint
-    # readouts (aux) with the term sweeps, over all event-timed device time.
-    achieved_all = (agg["alg"] + agg["aux_bytes"]) / (agg["gpu_ms"] * 1e-3) / 1e9 if agg["gpu_ms"] > 0 else 0.0
-    # HBM traffic per launch: the calibrated FETCH_SIZE + WRITE_SIZE ratio to
-    # algorithmic bytes measured on this workload (profiles/, rocprofv3 --pmc).
+
+    def rate(x, ms, scale=1e9):
+        return x / (ms * 1e-3) / scale if ms > 0 else 0.0
+
+    # Each kernel class, event-timed on the library's stream:
+    #  - term sweeps: algorithmic bytes 36 B per amplitude per complex H-apply;
+    #  - basis recurrences (the same term kernel on real halves): 16 B per vector and amplitude;
+    #  - second-stage GEMM (matrix cores, FP32): executed flops.
+    classes = {
+        "term_sweep": dict(kernel="half_term_kernel (mode 0-2) / term_kernel", bound="hbm", unit="GB/s",
+                           peak=HBM_PEAK_GBS, ms=agg["sweep_ms"], launches=int(agg["sweeps"]),
+                           work=agg["algorithmic_bytes"], achieved=rate(agg["algorithmic_bytes"], agg["sweep_ms"]),
+                           profile="half_term_kernel_n20_m3_traffic.md"),
+        "basis": dict(kernel="half_term_kernel (mode 3, real halves)", bound="hbm", unit="GB/s", peak=HBM_PEAK_GBS,
+                      ms=agg["basis_ms"], launches=int(agg["basis_launches"]), work=agg["basis_bytes"],
+                      achieved=rate(agg["basis_bytes"], agg["basis_ms"]), profile="basis_recurrence_n20_m3_traffic.md"),
+        "gemm": dict(kernel="combine2_kernel (v_mfma_f32_32x32x2f32)", bound="mfma", unit="TFLOP/s",
+                     peak=MFMA_FP32_PEAK_TFLOPS, ms=agg["gemm_ms"], launches=int(agg["gemm_launches"]),
+                     work=agg["gemm_flops"], achieved=rate(agg["gemm_flops"], agg["gemm_ms"], 1e12),
+                     achieved_useful=rate(agg["gemm_useful_flops"], agg["gemm_ms"], 1e12),
+                     hbm_gbs=rate(agg["gemm_bytes"], agg["gemm_ms"]), profile="combine2_kernel_n20_m3_traffic.md"),
+    }
+    other_ms = agg["gpu_ms"] - agg["sweep_ms"] - agg["basis_ms"] - agg["gemm_ms"]
+    dom_name = max(classes, key=lambda c: classes[c]["ms"])
+    dom = classes[dom_name]
+    per_launch_work = dom["work"] / max(1, dom["launches"])
+    # HBM bytes per launch of the dominant kernel: its calibrated FETCH_SIZE +
+    # WRITE_SIZE per algorithmic byte (or per flop), measured with rocprofv3
+    # --pmc on this workload (profiles/).
     traffic, traffic_src = None, None
-    prof = os.path.join(ROOT, "profiles", "half_term_kernel_n20_m3_traffic.md")
+    prof = os.path.join(ROOT, "profiles", dom["profile"])
     if os.path.exists(prof) and n == 20 and m == 3:
         for line in open(prof):
-            if line.startswith("traffic_per_algorithmic_byte:"):
-                traffic = float(line.split(":")[1]) * per_launch_bytes
-                traffic_src = "profiles/half_term_kernel_n20_m3_traffic.md"
+            if line.startswith("traffic_per_unit:"):
+                traffic = float(line.split(":")[1]) * per_launch_work
+                traffic_src = "profiles/" + dom["profile"]
+    roofline = {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
+                "frac": dom["achieved"] / dom["peak"], "traffic": traffic, "traffic_source": traffic_src,
+                "kernel": dom["kernel"], "class": dom_name, "work_per_launch": per_launch_work,
+                "avg_launch_ms": dom["ms"] / max(1, dom["launches"]), "launches": dom["launches"],
+                "device_ms": agg["gpu_ms"], "other_ms": other_ms,
+                "classes": {c: {key: v for key, v in d.items() if key not in ("profile",)}
+                            for c, d in classes.items()}}
+    for c in roofline["classes"].values():
+        c["frac"] = c["achieved"] / c["peak"] if c["peak"] else 0.0
 
     apply = None
     if args.apply_n > 0:
@@ -149,15 +173,7 @@ def main() -> int:
         "config": {"workload": f"SK n={n} m={m}, {args.instances} instances seed 2, 100 samples, "
                                f"{args.per_step} instances per step", "n": n, "m": m,
                    "instances_per_step": args.per_step, "samples": 100},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "achieved_required": achieved_req, "frac_required": achieved_req / HBM_PEAK_GBS,
-                     "kernel": "fused Clenshaw sweep (half_term_kernel / term_kernel)",
-                     "alg_bytes_per_launch": per_launch_bytes, "avg_launch_ms": per_launch_ms,
-                     "launches": agg["sweeps"], "achieved_all_kernels": achieved_all,
-                     "frac_all_kernels": achieved_all / HBM_PEAK_GBS, "device_ms": agg["gpu_ms"],
-                     "sweep_ms": agg["sweep_ms"], "aux_ms": agg["aux_ms"], "aux_bytes": agg["aux_bytes"],
-                     "aux_launches": agg["aux_launches"]},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "apply_rate": apply,
         "mean_prob": float(np.mean(np.concatenate(probs))) if probs else None,

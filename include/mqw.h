@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 6
+#define MQW_ABI_VERSION 7
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -248,6 +248,14 @@ typedef struct {
     double aux_ms;           /* device time of the shared bases, combine and readout launches (in gpu_ms) */
     double aux_bytes;        /* algorithmic bytes of the basis recursions, combine and readout launches */
     uint64_t aux_launches;   /* basis, combine and readout launches (not in sweeps) */
+    double basis_ms;         /* of aux_ms: the shared bases' recurrences (real-half term kernel) */
+    double basis_bytes;      /* their algorithmic bytes: 16 B per vector and amplitude (less for |+>, e_Eloc) */
+    double gemm_ms;          /* of aux_ms: the second-stage GEMM launches (matrix cores, FP32) */
+    double gemm_flops;       /* their executed flops (2 per multiply-add, row blocks of 32 samples) */
+    double gemm_useful_flops;/* of which on real samples */
+    uint64_t basis_launches; /* launches of the recurrences */
+    uint64_t gemm_launches;  /* launches of the GEMM */
+    double gemm_bytes;       /* the GEMM's algorithmic bytes: every basis row read once per tile, outputs written */
 } mqw_stats;
 mqw_status mqw_last_stats(const mqw_ctx* ctx, mqw_stats* out);
 

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmqw.so")
 CLI_PATH = os.path.join(HERE, "mqw")
 
-ABI_VERSION = 6  # MQW_ABI_VERSION of include/mqw.h this binding matches
+ABI_VERSION = 7  # MQW_ABI_VERSION of include/mqw.h this binding matches
 MAX_STAGES = 64
 DEFAULT_SAMPLES = 100
 DEFAULT_TOL = 1e-6
@@ -71,7 +71,9 @@ class Stats(C.Structure):
                 ("algorithmic_bytes", C.c_double), ("gpu_ms", C.c_double), ("sweep_ms", C.c_double),
                 ("lds_trajectories", C.c_uint64), ("psi_const_amp_terms", C.c_uint64),
                 ("required_bytes", C.c_double), ("aux_ms", C.c_double), ("aux_bytes", C.c_double),
-                ("aux_launches", C.c_uint64)]
+                ("aux_launches", C.c_uint64), ("basis_ms", C.c_double), ("basis_bytes", C.c_double),
+                ("gemm_ms", C.c_double), ("gemm_flops", C.c_double), ("gemm_useful_flops", C.c_double),
+                ("basis_launches", C.c_uint64), ("gemm_launches", C.c_uint64), ("gemm_bytes", C.c_double)]
 
 
 _lib = None

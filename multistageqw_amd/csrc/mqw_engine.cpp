@@ -316,6 +316,9 @@ struct Plan {
     std::vector<Combine2Item> c2items;
     size_t dots_doubles = 0;  // largest readout-dot scratch of one op
     double aux_bytes = 0.0;   // algorithmic bytes of the combine and readout launches
+    double gemm_flops = 0.0;        // matrix-core flops the combine2 launches execute (padded rows)
+    double gemm_useful_flops = 0.0; // of which on real samples
+    double gemm_bytes = 0.0;        // their algorithmic bytes (also in aux_bytes)
     uint64_t aux_launches = 0;
     std::vector<Op> ops;
     uint64_t amp_terms = 0;
@@ -644,6 +647,10 @@ void combine2_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_
                 op.aux = std::max<uint32_t>(op.aux, (uint32_t)ns);
                 // reads every basis row once, writes both halves of every sample
                 P.aux_bytes += (double)N * (4.0 * (double)(B.q0[pr] + B.q1[pr]) + 8.0 * (double)ns);
+                const double rows = (double)(B.q0[pr] + B.q1[pr]);
+                P.gemm_flops += 2.0 * rows * 32.0 * (double)((ns + 31) / 32) * (double)N;
+                P.gemm_useful_flops += 2.0 * rows * (double)ns * (double)N;
+                P.gemm_bytes += (double)N * (4.0 * rows + 8.0 * (double)ns);
             }
         }
         P.ops.push_back(op);
@@ -706,7 +713,7 @@ template <typename real>
 // launches) form one hipGraph (or direct launches) between two events, so
 // the sweep's device time is measured apart from the rest.
 mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<real>& P, bool use_graph,
-                    double* ms_sweep, double* ms_aux)
+                    double* ms_sweep, double* ms_aux, double* ms_basis = nullptr, double* ms_gemm = nullptr)
 {
     TermJob<real>* jd = (TermJob<real>*)dev_get(c, "plan_jobs", sizeof(TermJob<real>) * std::max<size_t>(1, P.jobs.size()));
     ReduceJob* rd = (ReduceJob*)dev_get(c, "plan_rjobs", sizeof(ReduceJob) * std::max<size_t>(1, P.rjobs.size()));
@@ -787,15 +794,17 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
         }
         return 0;
     };
-    auto aux_kind = [](int kind) { return kind >= 3; };
+    // timing classes: 0 sweep (terms, reduce, high-bit pass), 1 basis recurrences,
+    // 2 second-stage GEMM, 3 other auxiliary launches (combine, readout)
+    auto op_class = [](int kind) { return kind <= 2 ? 0 : kind == 4 ? 1 : kind == 7 ? 2 : 3; };
     struct Run {
         size_t b, e;
-        bool aux;
+        int cls;
     };
     std::vector<Run> runs;
     for (size_t q = 0; q < P.ops.size(); q++) {
-        const bool a = aux_kind(P.ops[q].kind);
-        if (runs.empty() || runs.back().aux != a) runs.push_back(Run{q, q, a});
+        const int cl = op_class(P.ops[q].kind);
+        if (runs.empty() || runs.back().cls != cl) runs.push_back(Run{q, q, cl});
         runs.back().e = q + 1;
     }
     std::deque<EventPair> evs;
@@ -834,7 +843,9 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
     for (size_t q = 0; q < runs.size(); q++) {
         float t = 0.0f;
         hipEventElapsedTime(&t, evs[q].a, evs[q].b);
-        *(runs[q].aux ? ms_aux : ms_sweep) += t;
+        *(runs[q].cls ? ms_aux : ms_sweep) += t;
+        if (runs[q].cls == 1 && ms_basis) *ms_basis += t;
+        if (runs[q].cls == 2 && ms_gemm) *ms_gemm += t;
     }
     return st;
 }
@@ -1001,13 +1012,16 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
         P.ops.push_back(op);
     }
     c->stats.aux_bytes += bytes;
+    c->stats.basis_bytes += bytes;
     auto run = [&](Plan<float>& Q) -> mqw_status {
         if (Q.jobs.empty()) return MQW_OK;
-        double ms_sweep = 0.0, ms_aux = 0.0; // every op here is a basis launch (aux)
-        mqw_status st = run_plan<float>(c, MQW_FP32, n, k, Q, use_graph, &ms_sweep, &ms_aux);
+        double ms_sweep = 0.0, ms_aux = 0.0, ms_basis = 0.0; // every op here is a basis launch
+        mqw_status st = run_plan<float>(c, MQW_FP32, n, k, Q, use_graph, &ms_sweep, &ms_aux, &ms_basis);
         if (st) return st;
         c->stats.gpu_ms += ms_sweep + ms_aux;
         c->stats.aux_ms += ms_sweep + ms_aux;
+        c->stats.basis_ms += ms_basis;
+        c->stats.basis_launches += Q.ops.size();
         c->stats.aux_launches += Q.ops.size();
         return MQW_OK;
     };
@@ -1071,6 +1085,7 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
                     Pw.jobs.push_back(j);
                     op.count++;
                     c->stats.aux_bytes += b * N;
+                    c->stats.basis_bytes += b * N;
                 }
             }
             Pw.ops.push_back(op);
@@ -1349,12 +1364,17 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             Plan<real> plan;
             build_plan<real>(n, nparts, trajs, batch, ps, E, slots, probs_d, plan, k.hi,
                              use_bases ? &bases : nullptr);
-            double ms_sweep = 0.0, ms_aux = 0.0;
-            st = run_plan<real>(c, precision, n, k, plan, o.use_graph != 0, &ms_sweep, &ms_aux);
+            double ms_sweep = 0.0, ms_aux = 0.0, ms_basis = 0.0, ms_gemm = 0.0;
+            st = run_plan<real>(c, precision, n, k, plan, o.use_graph != 0, &ms_sweep, &ms_aux, &ms_basis, &ms_gemm);
             if (st) return st;
             c->stats.gpu_ms += ms_sweep + ms_aux;
             c->stats.sweep_ms += ms_sweep;
             c->stats.aux_ms += ms_aux;
+            c->stats.gemm_ms += ms_gemm;
+            c->stats.gemm_flops += plan.gemm_flops;
+            c->stats.gemm_useful_flops += plan.gemm_useful_flops;
+            c->stats.gemm_bytes += plan.gemm_bytes;
+            for (const Op& op : plan.ops) c->stats.gemm_launches += op.kind == 7;
             c->stats.aux_bytes += plan.aux_bytes;
             c->stats.aux_launches += plan.aux_launches;
             c->stats.sweeps += plan.sweeps;

@@ -126,7 +126,7 @@ int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem
 // are odd) and an imaginary class.  The scale of psi2 is irrelevant: every
 // later stage and the readout normalise their input.
 constexpr int kC2MaxK = 64;   // kc1 and kc2 limits
-constexpr int kC2Tile = 128;  // samples per workgroup (4 MFMA row blocks of 32)
+constexpr int kC2Tile = 64;   // samples per workgroup (2 MFMA row blocks of 32)
 struct Combine2Item {
     float* out;           // planar output (2N floats)
     double* norm_part;    // nparts partial sums of |out|^2, one per workgroup

@@ -1474,24 +1474,28 @@ int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem
 // 32 MB samples; per 256 amplitudes, wave w owns 32 of them and MB row blocks
 // of 32 samples, both classes: 2 MB accumulator tiles of
 // v_mfma_f32_32x32x2f32 (A = coefficients, 32 samples x 2 rows; B = basis,
-// 2 rows x 32 amplitudes).  A lane's A entry d_jl c_jk s is formed from the
-// coefficient rows in LDS; its B entry is one load of row (l, k) of the basis.
-// Loads run UNR row pairs ahead of the matrix ops.  The squares of each
-// output go through a 32-lane tree into FP64 per wave, summed in wave order.
+// 2 rows x 32 amplitudes).  A = d_jl c_jk s is built by the workgroup, 32 rows
+// at a time, into a double-buffered LDS chunk (one barrier per chunk); each
+// lane's B entry is one load of row (l, k) of the basis, UNR row pairs ahead
+// of the matrix ops.  The squares of each output go through a 32-lane tree
+// into FP64 per wave, summed in wave order.
 // ===========================================================================
 typedef float mqw_f32x16 __attribute__((ext_vector_type(16)));
 
 template <int MB>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512, 4)
 combine2_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const Combine2Item* __restrict__ items,
                 const float* __restrict__ coefs, int n, uint32_t nparts)
 {
     constexpr int NS = MB * 32;
-    constexpr int CP = kC2MaxK + 1;   // padded coefficient rows: 32 samples hit 32 banks
-    constexpr int UNR = 8;            // row pairs loaded ahead
+    constexpr int CP = kC2MaxK + 1;   // padded coefficient rows: consecutive samples hit consecutive banks
+    constexpr int UNR = 8;            // row pairs per block, loaded one block ahead
+    constexpr int QC = 32;            // A rows per LDS chunk (a multiple of 2 UNR)
+    constexpr int BPC = QC / (2 * UNR);
     __shared__ float s_c1[NS * CP];
     __shared__ float s_c2[NS * CP];
     __shared__ uint32_t s_q[kC2MaxK * kC2MaxK + 2];
+    __shared__ float s_a[2][QC][NS];
     __shared__ double s_n[8][NS];
     const uint32_t tile = blockIdx.x / nparts, part = blockIdx.x % nparts;
     if (tile >= ntiles) return;
@@ -1520,13 +1524,25 @@ combine2_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const C
     const MQW_GLOBAL float* plus = gview(T.plus);
     const MQW_GLOBAL float* V = gview(T.V);
     const MQW_GLOBAL float* W = gview(T.W);
+    // A rows [r0, r1) into buffer b: s_a[b][r - r0][j] = s d_jl c_jk
+    auto fill_a = [&](int b, int r0, int r1) {
+        for (int i = t; i < QC * NS; i += 512) {
+            const int r = i / NS, j = i % NS;
+            float v = 0.0f;
+            if (r0 + r < r1) {
+                const uint32_t e = s_q[r0 + r];
+                const int l = (int)(e & 255u), k = (int)((e >> 8) & 255u);
+                const uint32_t sg = e >> 16;
+                const float f = s_c2[j * CP + l] * s_c1[j * CP + k];
+                v = sg == 1u ? f : (sg == 2u ? -f : 0.0f);
+            }
+            s_a[b][r][j] = v;
+        }
+    };
     // basis row q of this lane's half, at amplitude a
-    auto load_b = [&](int q, uint64_t a, int& l, int& k, float& sg) -> float {
+    auto load_b = [&](int q, uint64_t a) -> float {
         const uint32_t e = s_q[q];
-        l = (int)(e & 255u);
-        k = (int)((e >> 8) & 255u);
-        const uint32_t s = e >> 16;
-        sg = s == 1u ? 1.0f : (s == 2u ? -1.0f : 0.0f);
+        const int l = (int)(e & 255u), k = (int)((e >> 8) & 255u);
         const MQW_GLOBAL float* row =
             l == 0 ? (k == 0 ? plus : V + (uint64_t)(k - 1) * N) : W + ((uint64_t)k * (kc2 - 1) + (l - 1)) * N;
         return row[a];
@@ -1543,48 +1559,45 @@ combine2_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const C
 #pragma unroll
         for (int cls = 0; cls < 2; cls++) {
             const int qb = cls ? q0 : 0, qe = cls ? nq : q0;
-            float bv[UNR], sv[UNR];
-            int lv[UNR], kv[UNR];
+            const int nblk = (qe - qb + 2 * UNR - 1) / (2 * UNR);
+            if (nblk == 0) continue;
+            const int nch = (qe - qb + QC - 1) / QC;
+            float bv[UNR];
 #pragma unroll
             for (int u = 0; u < UNR; u++) {
                 const int q = qb + 2 * u;
-                if (q < qe) {
-                    bv[u] = load_b(q + half, a, lv[u], kv[u], sv[u]);
-                } else {
-                    bv[u] = 0.0f;
-                    sv[u] = 0.0f;
-                    lv[u] = kv[u] = 0;
-                }
+                bv[u] = q < qe ? load_b(q + half, a) : 0.0f;
             }
-            for (int q = qb; q < qe; q += 2 * UNR) {
-                float bn[UNR], sn[UNR];
-                int ln[UNR], kn[UNR];
+            fill_a(0, qb, qe < qb + QC ? qe : qb + QC);
+            __syncthreads();
+            for (int blk = 0; blk < nblk; blk++) {
+                const int ch = blk / BPC;
+                if (blk % BPC == 0 && ch + 1 < nch) {
+                    const int r0 = qb + (ch + 1) * QC;
+                    fill_a((ch + 1) & 1, r0, qe < r0 + QC ? qe : r0 + QC);
+                }
+                const int rs = qb + blk * 2 * UNR;
+                float bn[UNR];
 #pragma unroll
                 for (int u = 0; u < UNR; u++) {
-                    const int qq = q + 2 * UNR + 2 * u;
-                    if (qq < qe) {
-                        bn[u] = load_b(qq + half, a, ln[u], kn[u], sn[u]);
-                    } else {
-                        bn[u] = 0.0f;
-                        sn[u] = 0.0f;
-                        ln[u] = kn[u] = 0;
-                    }
+                    const int q = rs + 2 * UNR + 2 * u;
+                    bn[u] = q < qe ? load_b(q + half, a) : 0.0f;
                 }
+                const float(*A)[NS] = s_a[ch & 1];
+                const int rc = rs - (qb + ch * QC); // first row of the block inside the chunk
 #pragma unroll
-                for (int u = 0; u < UNR; u++)
+                for (int u = 0; u < UNR; u++) {
+                    if (rs + 2 * u >= qe) break; // uniform
 #pragma unroll
                     for (int mb = 0; mb < MB; mb++) {
-                        const int j = mb * 32 + col;
-                        const float av = sv[u] * s_c2[j * CP + lv[u]] * s_c1[j * CP + kv[u]];
+                        const float av = A[rc + 2 * u + half][mb * 32 + col];
                         acc[cls][mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv[u], acc[cls][mb], 0, 0, 0);
                     }
-#pragma unroll
-                for (int u = 0; u < UNR; u++) {
-                    bv[u] = bn[u];
-                    sv[u] = sn[u];
-                    lv[u] = ln[u];
-                    kv[u] = kn[u];
                 }
+#pragma unroll
+                for (int u = 0; u < UNR; u++) bv[u] = bn[u];
+                // end of a chunk: the next chunk is written, this one's reads are done
+                if (blk % BPC == BPC - 1 || blk == nblk - 1) __syncthreads();
             }
         }
         // lane (col, half) holds rows 8 (r / 4) + 4 half + r % 4 of column col
@@ -1623,25 +1636,12 @@ int launch_combine2(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combin
     const uint64_t grid = (uint64_t)ntiles * nparts;
     if (grid > 0x7fffffffull) return -3;
     hipStream_t st = (hipStream_t)stream;
-    const int mb = (int)((ns_max + 31) / 32);
-    switch (mb) {
-    case 1:
+    if (ns_max <= 32)
         hipLaunchKernelGGL(combine2_kernel<1>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
                            coefs_dev, n, nparts);
-        break;
-    case 2:
+    else
         hipLaunchKernelGGL(combine2_kernel<2>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
                            coefs_dev, n, nparts);
-        break;
-    case 3:
-        hipLaunchKernelGGL(combine2_kernel<3>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
-                           coefs_dev, n, nparts);
-        break;
-    default:
-        hipLaunchKernelGGL(combine2_kernel<4>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
-                           coefs_dev, n, nparts);
-        break;
-    }
     return check_launch("combine2");
 }
 
