@@ -300,19 +300,25 @@ struct EventPair {
         hipError_t e = hipEventCreate(&a);
         return e != hipSuccess ? e : hipEventCreate(&b);
     }
+    EventPair() = default;
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
     ~EventPair()
     {
-        if (a) hipEventDestroy(a);
-        if (b) hipEventDestroy(b);
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
     }
 };
 struct GraphGuard {
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
+    GraphGuard() = default;
+    GraphGuard(const GraphGuard&) = delete;
+    GraphGuard& operator=(const GraphGuard&) = delete;
     ~GraphGuard()
     {
-        if (ge) hipGraphExecDestroy(ge);
-        if (g) hipGraphDestroy(g);
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
     }
 };
 
