@@ -96,8 +96,9 @@ def uneven(ctx, count, samples, seed, chunk=256, fp64_every=0, tune_cache=None):
             tuned[n] = ctx.tune(n, tune_cache)
         print(json.dumps(dict(study="uneven", tuned=tuned)), flush=True)
     probs = np.zeros(count, np.float32)
-    dev_ms = alg = 0.0
-    lds = 0
+    keys = ("gpu_ms", "sweep_ms", "algorithmic_bytes", "basis_ms", "basis_bytes", "gemm_ms", "gemm_flops",
+            "lds_trajectories")
+    tot = {k: 0.0 for k in keys}
     wall = 0.0
     for c0 in range(0, count, chunk):
         part = inst[c0:c0 + chunk]
@@ -105,14 +106,20 @@ def uneven(ctx, count, samples, seed, chunk=256, fp64_every=0, tune_cache=None):
         p, _ = ctx.solve_batch(part, samples=samples)
         wall += time.time() - t0
         st = ctx.last_stats()
-        dev_ms += st["gpu_ms"]
-        alg += st["algorithmic_bytes"]
-        lds += st["lds_trajectories"]
+        for k in keys:
+            tot[k] += st[k]
         probs[c0:c0 + len(part)] = p
         print(json.dumps(dict(study="uneven", done=c0 + len(part), wall_s=round(wall, 2))), flush=True)
+
+    def rate(x, ms, scale):
+        return x / (ms * 1e-3) / scale if ms > 0 else 0.0
+
     out = dict(study="uneven", count=count, samples=samples, instances_per_s=count / wall, wall_s=wall,
-               device_ms=dev_ms, algorithmic_GBps=alg / (dev_ms * 1e-3) / 1e9, lds_trajectories=lds,
-               mean_p=float(np.mean(probs)))
+               device_ms=tot["gpu_ms"], term_sweep_ms=tot["sweep_ms"],
+               term_sweep_GBps=rate(tot["algorithmic_bytes"], tot["sweep_ms"], 1e9), basis_ms=tot["basis_ms"],
+               basis_GBps=rate(tot["basis_bytes"], tot["basis_ms"], 1e9), gemm_ms=tot["gemm_ms"],
+               gemm_TFLOPs=rate(tot["gemm_flops"], tot["gemm_ms"], 1e12),
+               lds_trajectories=int(tot["lds_trajectories"]), mean_p=float(np.mean(probs)))
     if fp64_every:
         idx = list(range(0, count, fp64_every))
         t0 = time.time()
