@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 7
+#define MQW_ABI_VERSION 8
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -115,6 +115,10 @@ typedef struct {
                                 the second stage too comes from a shared basis: W_lk = T_l(H2/onenorm2) V_k
                                 per instance (kc1 x kc2 real vectors) and one matrix-core GEMM row of
                                 coefficient products per sample; 0 = per-sample Clenshaw terms */
+    int overlap_bases;       /* 1 (default) = when a solve holds several batches of shared bases, the
+                                next batch's basis recurrences run on a second stream while this
+                                batch's GEMM, readout and sweeps run (two basis buffers); 0 = one batch
+                                after the other on one stream.  Results are identical either way. */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are
@@ -256,6 +260,10 @@ typedef struct {
     uint64_t basis_launches; /* launches of the recurrences */
     uint64_t gemm_launches;  /* launches of the GEMM */
     double gemm_bytes;       /* the GEMM's algorithmic bytes: every basis row read once per tile, outputs written */
+    uint64_t basis_batches;  /* problem batches of shared bases (each fills the basis buffer once) */
+    uint64_t overlap_batches; /* of those, batches whose recurrences ran on the second stream beside the main
+                                 stream's work (overlap_bases); their event times overlap, so gpu_ms then
+                                 exceeds the device's busy time */
 } mqw_stats;
 mqw_status mqw_last_stats(const mqw_ctx* ctx, mqw_stats* out);
 

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmqw.so")
 CLI_PATH = os.path.join(HERE, "mqw")
 
-ABI_VERSION = 7  # MQW_ABI_VERSION of include/mqw.h this binding matches
+ABI_VERSION = 8  # MQW_ABI_VERSION of include/mqw.h this binding matches
 MAX_STAGES = 64
 DEFAULT_SAMPLES = 100
 DEFAULT_TOL = 1e-6
@@ -58,7 +58,8 @@ class Opts(C.Structure):
                 ("use_graph", C.c_int), ("max_device_bytes", C.c_size_t), ("group_bytes", C.c_size_t), ("variant", C.c_int),
                 ("sample_probs", C.POINTER(C.c_float)), ("high_pass", C.c_int),
                 ("mixer_masks", C.POINTER(C.c_uint32)), ("mixer_count", C.c_int), ("fixed_times", C.c_int),
-                ("shared_first_stage", C.c_int), ("final_readout", C.c_int), ("shared_second_stage", C.c_int)]
+                ("shared_first_stage", C.c_int), ("final_readout", C.c_int), ("shared_second_stage", C.c_int),
+                ("overlap_bases", C.c_int)]
 
 
 class Instance(C.Structure):
@@ -73,7 +74,8 @@ class Stats(C.Structure):
                 ("required_bytes", C.c_double), ("aux_ms", C.c_double), ("aux_bytes", C.c_double),
                 ("aux_launches", C.c_uint64), ("basis_ms", C.c_double), ("basis_bytes", C.c_double),
                 ("gemm_ms", C.c_double), ("gemm_flops", C.c_double), ("gemm_useful_flops", C.c_double),
-                ("basis_launches", C.c_uint64), ("gemm_launches", C.c_uint64), ("gemm_bytes", C.c_double)]
+                ("basis_launches", C.c_uint64), ("gemm_launches", C.c_uint64), ("gemm_bytes", C.c_double),
+                ("basis_batches", C.c_uint64), ("overlap_batches", C.c_uint64)]
 
 
 _lib = None

@@ -109,7 +109,8 @@ def write_output(n: int, m: int, probs, directory: str = ".") -> str:
 def make_opts(samples=L.DEFAULT_SAMPLES, tol=L.DEFAULT_TOL, precision=L.FP32, energy_mode=L.ENERGY_COMPAT,
               tile_bits=0, chunk_bits=-1, small_n_max=-1, use_graph=True, max_device_bytes=0,
               group_bytes=0, variant=-1, high_pass=-1, mixer_masks=None, fixed_times=False,
-              shared_first_stage=True, final_readout=True, shared_second_stage=True) -> L.Opts:
+              shared_first_stage=True, final_readout=True, shared_second_stage=True,
+              overlap_bases=True) -> L.Opts:
     o = L.Opts()
     L.load().mqw_opts_default(C.byref(o))
     o.samples = samples
@@ -128,6 +129,7 @@ def make_opts(samples=L.DEFAULT_SAMPLES, tol=L.DEFAULT_TOL, precision=L.FP32, en
     o.shared_first_stage = int(bool(shared_first_stage))
     o.final_readout = int(bool(final_readout))
     o.shared_second_stage = int(bool(shared_second_stage))
+    o.overlap_bases = int(bool(overlap_bases))
     if mixer_masks is not None:
         masks = np.ascontiguousarray(mixer_masks, np.uint32)
         o._masks = masks  # keeps the array alive as long as the options

@@ -43,13 +43,14 @@ mqw_status mqw::fail(mqw_ctx* c, mqw_status s, const char* fmt, ...)
     return s;
 }
 
-// Grow-only named device buffer.  Growing synchronises the stream first.
+// Grow-only named device buffer.  Growing synchronises both streams first.
 void* mqw::dev_get(mqw_ctx* c, const std::string& name, size_t bytes)
 {
     if (bytes == 0) bytes = 256;
     DevBuf& b = c->bufs[name];
     if (b.cap >= bytes) return b.ptr;
     hipStreamSynchronize(c->stream);
+    if (c->stream2) hipStreamSynchronize(c->stream2);
     if (b.ptr) hipFree(b.ptr);
     b.ptr = nullptr;
     b.cap = 0;
@@ -66,6 +67,7 @@ void* mqw::dev_get(mqw_ctx* c, const std::string& name, size_t bytes)
 void mqw::release_bufs(mqw_ctx* c)
 {
     hipStreamSynchronize(c->stream);
+    if (c->stream2) hipStreamSynchronize(c->stream2);
     for (auto& kv : c->bufs)
         if (kv.second.ptr) hipFree(kv.second.ptr);
     c->bufs.clear();
@@ -340,6 +342,27 @@ struct AdjointIn {
     const void* psi = nullptr;
     const double* norm_part = nullptr;
 };
+
+// Event pairs and graphs of launched op runs, with each run's timing class
+// (0 sweep, 1 basis recurrences, 2 GEMM, 3 other auxiliary launches); kept
+// until the work has finished and its times are read.
+struct Timed {
+    std::deque<EventPair> evs;
+    std::deque<GraphGuard> graphs;
+    std::vector<int> cls;
+};
+
+// Elapsed times of finished runs, by class.
+void collect(Timed& tm, double* ms_sweep, double* ms_aux, double* ms_basis, double* ms_gemm)
+{
+    for (size_t q = 0; q < tm.cls.size() && q < tm.evs.size(); q++) {
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, tm.evs[q].a, tm.evs[q].b) != hipSuccess) continue;
+        *(tm.cls[q] ? ms_aux : ms_sweep) += t;
+        if (tm.cls[q] == 1 && ms_basis) *ms_basis += t;
+        if (tm.cls[q] == 2 && ms_gemm) *ms_gemm += t;
+    }
+}
 
 template <typename real>
 void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, const real* E, uint32_t eloc,
@@ -713,10 +736,15 @@ template <typename real>
 // launches) form one hipGraph (or direct launches) between two events, so
 // the sweep's device time is measured apart from the rest.
 mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<real>& P, bool use_graph,
-                    double* ms_sweep, double* ms_aux, double* ms_basis = nullptr, double* ms_gemm = nullptr)
+                    double* ms_sweep, double* ms_aux, double* ms_basis = nullptr, double* ms_gemm = nullptr,
+                    Timed* pending = nullptr, hipStream_t stream = nullptr, const std::string& tag = "plan")
 {
-    TermJob<real>* jd = (TermJob<real>*)dev_get(c, "plan_jobs", sizeof(TermJob<real>) * std::max<size_t>(1, P.jobs.size()));
-    ReduceJob* rd = (ReduceJob*)dev_get(c, "plan_rjobs", sizeof(ReduceJob) * std::max<size_t>(1, P.rjobs.size()));
+    // pending: launch only; the caller synchronises and collects the timings.
+    // stream / tag: the stream and the name prefix of the device job buffers
+    const hipStream_t stm = stream ? stream : c->stream;
+    TermJob<real>* jd =
+        (TermJob<real>*)dev_get(c, tag + "_jobs", sizeof(TermJob<real>) * std::max<size_t>(1, P.jobs.size()));
+    ReduceJob* rd = (ReduceJob*)dev_get(c, tag + "_rjobs", sizeof(ReduceJob) * std::max<size_t>(1, P.rjobs.size()));
     if (!jd || !rd) return fail(c, MQW_ENOMEM, "device allocation failed for %zu sweep jobs", P.jobs.size());
     CombineJob* cd = nullptr;
     CombineItem* ci = nullptr;
@@ -725,71 +753,71 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
     ReadItem* rid = nullptr;
     double* dots = nullptr;
     if (!P.ccoef.empty()) {
-        cf = (float*)dev_get(c, "plan_ccoef", sizeof(float) * P.ccoef.size());
+        cf = (float*)dev_get(c, tag + "_ccoef", sizeof(float) * P.ccoef.size());
         if (!cf) return fail(c, MQW_ENOMEM, "device allocation failed for coefficient rows");
         HIPCHK(c, hipMemcpyAsync(cf, P.ccoef.data(), sizeof(float) * P.ccoef.size(), hipMemcpyHostToDevice,
-                                 c->stream));
+                                 stm));
     }
     if (!P.cjobs.empty()) {
-        cd = (CombineJob*)dev_get(c, "plan_cjobs", sizeof(CombineJob) * P.cjobs.size());
-        ci = (CombineItem*)dev_get(c, "plan_citems", sizeof(CombineItem) * P.citems.size());
+        cd = (CombineJob*)dev_get(c, tag + "_cjobs", sizeof(CombineJob) * P.cjobs.size());
+        ci = (CombineItem*)dev_get(c, tag + "_citems", sizeof(CombineItem) * P.citems.size());
         if (!cd || !ci) return fail(c, MQW_ENOMEM, "device allocation failed for combine jobs");
         HIPCHK(c, hipMemcpyAsync(cd, P.cjobs.data(), sizeof(CombineJob) * P.cjobs.size(), hipMemcpyHostToDevice,
-                                 c->stream));
+                                 stm));
         HIPCHK(c, hipMemcpyAsync(ci, P.citems.data(), sizeof(CombineItem) * P.citems.size(), hipMemcpyHostToDevice,
-                                 c->stream));
+                                 stm));
     }
     Combine2Tile* c2t = nullptr;
     Combine2Item* c2i = nullptr;
     if (!P.c2tiles.empty()) {
-        c2t = (Combine2Tile*)dev_get(c, "plan_c2tiles", sizeof(Combine2Tile) * P.c2tiles.size());
-        c2i = (Combine2Item*)dev_get(c, "plan_c2items", sizeof(Combine2Item) * P.c2items.size());
+        c2t = (Combine2Tile*)dev_get(c, tag + "_c2tiles", sizeof(Combine2Tile) * P.c2tiles.size());
+        c2i = (Combine2Item*)dev_get(c, tag + "_c2items", sizeof(Combine2Item) * P.c2items.size());
         if (!c2t || !c2i) return fail(c, MQW_ENOMEM, "device allocation failed for second-stage tiles");
         HIPCHK(c, hipMemcpyAsync(c2t, P.c2tiles.data(), sizeof(Combine2Tile) * P.c2tiles.size(),
-                                 hipMemcpyHostToDevice, c->stream));
+                                 hipMemcpyHostToDevice, stm));
         HIPCHK(c, hipMemcpyAsync(c2i, P.c2items.data(), sizeof(Combine2Item) * P.c2items.size(),
-                                 hipMemcpyHostToDevice, c->stream));
+                                 hipMemcpyHostToDevice, stm));
     }
     if (!P.rtiles.empty()) {
-        rtd = (ReadTile*)dev_get(c, "plan_rtiles", sizeof(ReadTile) * P.rtiles.size());
-        rid = (ReadItem*)dev_get(c, "plan_ritems", sizeof(ReadItem) * P.ritems.size());
-        dots = (double*)dev_get(c, "plan_dots", sizeof(double) * P.dots_doubles);
+        rtd = (ReadTile*)dev_get(c, tag + "_rtiles", sizeof(ReadTile) * P.rtiles.size());
+        rid = (ReadItem*)dev_get(c, tag + "_ritems", sizeof(ReadItem) * P.ritems.size());
+        dots = (double*)dev_get(c, tag + "_dots", sizeof(double) * P.dots_doubles);
         if (!rtd || !rid || !dots) return fail(c, MQW_ENOMEM, "device allocation failed for readout tiles");
         HIPCHK(c, hipMemcpyAsync(rtd, P.rtiles.data(), sizeof(ReadTile) * P.rtiles.size(), hipMemcpyHostToDevice,
-                                 c->stream));
+                                 stm));
         HIPCHK(c, hipMemcpyAsync(rid, P.ritems.data(), sizeof(ReadItem) * P.ritems.size(), hipMemcpyHostToDevice,
-                                 c->stream));
+                                 stm));
     }
     if (!P.jobs.empty())
         HIPCHK(c, hipMemcpyAsync(jd, P.jobs.data(), sizeof(TermJob<real>) * P.jobs.size(), hipMemcpyHostToDevice,
-                                 c->stream));
+                                 stm));
     if (!P.rjobs.empty())
         HIPCHK(c, hipMemcpyAsync(rd, P.rjobs.data(), sizeof(ReduceJob) * P.rjobs.size(), hipMemcpyHostToDevice,
-                                 c->stream));
+                                 stm));
     auto issue = [&](size_t b, size_t e) -> int {
         for (size_t q = b; q < e; q++) {
             const Op& op = P.ops[q];
             int rc;
             if (op.kind == 1)
-                rc = launch_reduce(precision, rd + op.off, op.count, c->stream);
+                rc = launch_reduce(precision, rd + op.off, op.count, stm);
             else if (op.kind == 3)
-                rc = launch_combine(cd + op.off, op.count, ci, cf, n, k.parts, op.aux, c->stream);
+                rc = launch_combine(cd + op.off, op.count, ci, cf, n, k.parts, op.aux, stm);
             else if (op.kind == 4)
-                rc = launch_half_real(k.tb, k.variant, jd + op.off, op.count, n, k.lc, c->stream);
+                rc = launch_half_real(k.tb, k.variant, jd + op.off, op.count, n, k.lc, stm);
             else if (op.kind == 5)
-                rc = launch_readout_dots(rtd + op.off, op.count, rid, cf, n, op.aux2, op.aux, dots, c->stream);
+                rc = launch_readout_dots(rtd + op.off, op.count, rid, cf, n, op.aux2, op.aux, dots, stm);
             else if (op.kind == 6)
-                rc = launch_readout_finalize(rtd + op.off, op.count, rid, op.aux2, k.parts, dots, c->stream);
+                rc = launch_readout_finalize(rtd + op.off, op.count, rid, op.aux2, k.parts, dots, stm);
             else if (op.kind == 7)
-                rc = launch_combine2(c2t + op.off, op.count, c2i, cf, n, k.parts, op.aux, c->stream);
+                rc = launch_combine2(c2t + op.off, op.count, c2i, cf, n, k.parts, op.aux, stm);
             else if (op.kind == 2)
-                rc = launch_high_pass(k.hi, k.hwb, jd + op.off, op.count, n, c->stream);
+                rc = launch_high_pass(k.hi, k.hwb, jd + op.off, op.count, n, stm);
             else if (k.nmask > 0)
-                rc = launch_mask_term(precision, jd + op.off, op.count, n, k.masks, k.nmask, c->stream);
+                rc = launch_mask_term(precision, jd + op.off, op.count, n, k.masks, k.nmask, stm);
             else if (k.half)
-                rc = launch_half_term(k.tb, k.variant, jd + op.off, op.count, n, k.lc, k.hi, c->stream);
+                rc = launch_half_term(k.tb, k.variant, jd + op.off, op.count, n, k.lc, k.hi, stm);
             else
-                rc = launch_term(precision, k.tb, k.variant, jd + op.off, op.count, n, k.lc, c->stream);
+                rc = launch_term(precision, k.tb, k.variant, jd + op.off, op.count, n, k.lc, stm);
             if (rc) return rc;
         }
         return 0;
@@ -807,26 +835,29 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
         if (runs.empty() || runs.back().cls != cl) runs.push_back(Run{q, q, cl});
         runs.back().e = q + 1;
     }
-    std::deque<EventPair> evs;
-    std::deque<GraphGuard> graphs;
+    Timed local;
+    Timed& tm = pending ? *pending : local;
+    std::deque<EventPair>& evs = tm.evs;
+    std::deque<GraphGuard>& graphs = tm.graphs;
     mqw_status st = MQW_OK;
     for (const Run& r : runs) {
+        tm.cls.push_back(r.cls);
         evs.emplace_back();
         EventPair& ev = evs.back();
         HIPCHK(c, ev.create());
-        HIPCHK(c, hipEventRecord(ev.a, c->stream));
+        HIPCHK(c, hipEventRecord(ev.a, stm));
         if (use_graph) {
             graphs.emplace_back();
             GraphGuard& gr = graphs.back();
-            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            HIPCHK(c, hipStreamBeginCapture(stm, hipStreamCaptureModeThreadLocal));
             const int rc = issue(r.b, r.e);
-            hipError_t ce = hipStreamEndCapture(c->stream, &gr.g);
+            hipError_t ce = hipStreamEndCapture(stm, &gr.g);
             if (rc || ce != hipSuccess)
                 return fail(c, MQW_EDEVICE, "graph capture failed: %s / %s", last_kernel_error(),
                             hipGetErrorString(ce));
             hipError_t ie = hipGraphInstantiate(&gr.ge, gr.g, nullptr, nullptr, 0);
             if (ie != hipSuccess) return fail(c, MQW_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ie));
-            hipError_t le = hipGraphLaunch(gr.ge, c->stream);
+            hipError_t le = hipGraphLaunch(gr.ge, stm);
             if (le != hipSuccess) {
                 st = fail(c, MQW_EDEVICE, "hipGraphLaunch: %s", hipGetErrorString(le));
                 break;
@@ -835,18 +866,13 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
             st = fail(c, MQW_EDEVICE, "launch: %s", last_kernel_error());
             break;
         }
-        HIPCHK(c, hipEventRecord(ev.b, c->stream));
+        HIPCHK(c, hipEventRecord(ev.b, stm));
     }
-    hipError_t se = hipStreamSynchronize(c->stream);
+    if (pending) return st; // the caller synchronises and collects
+    hipError_t se = hipStreamSynchronize(stm);
     if (se != hipSuccess) return fail(c, MQW_EDEVICE, "stream sync: %s", hipGetErrorString(se));
     if (st) return st;
-    for (size_t q = 0; q < runs.size(); q++) {
-        float t = 0.0f;
-        hipEventElapsedTime(&t, evs[q].a, evs[q].b);
-        *(runs[q].cls ? ms_aux : ms_sweep) += t;
-        if (runs[q].cls == 1 && ms_basis) *ms_basis += t;
-        if (runs[q].cls == 2 && ms_gemm) *ms_gemm += t;
-    }
+    collect(tm, ms_sweep, ms_aux, ms_basis, ms_gemm);
     return st;
 }
 
@@ -929,9 +955,14 @@ mqw_status run_lds(mqw_ctx* c, int n, const std::vector<Traj>& trajs, const std:
 // with Ht_i = (H_P - gamma_i H_G) / onenorm_i, the stage's gamma and bound as
 // its Clenshaw terms use.
 // ---------------------------------------------------------------------------
+// With pending, the launches go to `stream` and are not waited for: the
+// plans (whose host tables the copies read) move into keep, the events into
+// pending, and the caller synchronises and collects the times.
 mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Problem*>& ps,
-                     const std::vector<Traj>& trajs, const float* E, const Bases& B, bool use_graph)
+                     const std::vector<Traj>& trajs, const float* E, const Bases& B, bool use_graph,
+                     Timed* pending = nullptr, hipStream_t stream = nullptr, std::deque<Plan<float>>* keep = nullptr)
 {
+    const hipStream_t stm = stream ? stream : c->stream;
     const size_t N = (size_t)1 << n;
     const float psi0 = (float)(1 / sqrt((double)N));
     std::vector<const Segment*> seg0(ps.size(), nullptr), segl(ps.size(), nullptr);
@@ -944,7 +975,7 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
     // X_0 = e_eloc, ahead of the recurrence on the same stream
     for (size_t p = 0; p < ps.size(); p++)
         if (B.kx[p] > 0) {
-            if (launch_fill(const_cast<float*>(B.X[p]), N, 0.0f, ps[p]->info.eloc, 1.0f, c->stream))
+            if (launch_fill(const_cast<float*>(B.X[p]), N, 0.0f, ps[p]->info.eloc, 1.0f, stm))
                 return fail(c, MQW_EDEVICE, "unit vector: %s", last_kernel_error());
             bytes += 4.0 * N;
         }
@@ -1015,14 +1046,21 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
     c->stats.basis_bytes += bytes;
     auto run = [&](Plan<float>& Q) -> mqw_status {
         if (Q.jobs.empty()) return MQW_OK;
+        c->stats.basis_launches += Q.ops.size();
+        c->stats.aux_launches += Q.ops.size();
+        if (pending) {
+            mqw_status st = run_plan<float>(c, MQW_FP32, n, k, Q, use_graph, nullptr, nullptr, nullptr, nullptr,
+                                            pending, stm, "bases");
+            keep->push_back(std::move(Q)); // the copies read its host tables
+            return st;
+        }
         double ms_sweep = 0.0, ms_aux = 0.0, ms_basis = 0.0; // every op here is a basis launch
-        mqw_status st = run_plan<float>(c, MQW_FP32, n, k, Q, use_graph, &ms_sweep, &ms_aux, &ms_basis);
+        mqw_status st = run_plan<float>(c, MQW_FP32, n, k, Q, use_graph, &ms_sweep, &ms_aux, &ms_basis, nullptr,
+                                        nullptr, stm, "bases");
         if (st) return st;
         c->stats.gpu_ms += ms_sweep + ms_aux;
         c->stats.aux_ms += ms_sweep + ms_aux;
         c->stats.basis_ms += ms_basis;
-        c->stats.basis_launches += Q.ops.size();
-        c->stats.aux_launches += Q.ops.size();
         return MQW_OK;
     };
     mqw_status st = run(P);
@@ -1033,7 +1071,7 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
     int wsteps = 0;
     for (size_t p = 0; p < B.kc2.size(); p++) wsteps = std::max(wsteps, B.kc2[p] - 1);
     if (wsteps > 0) {
-        if (launch_fill(const_cast<float*>(B.plus), N, psi0, N, 0.0f, c->stream))
+        if (launch_fill(const_cast<float*>(B.plus), N, psi0, N, 0.0f, stm))
             return fail(c, MQW_EDEVICE, "fill: %s", last_kernel_error());
         std::vector<const Segment*> seg1(ps.size(), nullptr);
         for (const Traj& tr : trajs)
@@ -1250,11 +1288,23 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         for (int q = 0; q < 2; q++) slots[i].part[q] = parts + (i * 2 + (size_t)q) * nparts;
     }
 
-    for (const auto& pb : pbatches) {
-        // the batch's bases
-        Bases bases;
-        bool use_bases = false;
-        if (basis_path) {
+    // Each problem batch's bases: offsets in a basis buffer and the row tables
+    // of its second-stage GEMM; the pointers are bound when the batch is given
+    // a buffer.
+    struct BatchBases {
+        Bases b;
+        std::vector<size_t> offv, offx, offw, qoff;
+        std::vector<uint32_t> rows;
+        size_t total = 0, plus_off = 0;
+        bool any_w = false, use = false;
+    };
+    std::vector<BatchBases> bbs(pbatches.size());
+    size_t max_total = 0, max_rows = 0;
+    if (basis_path) {
+        for (size_t q = 0; q < pbatches.size(); q++) {
+            const auto& pb = pbatches[q];
+            BatchBases& bb = bbs[q];
+            Bases& bases = bb.b;
             bases.V.assign(ps.size(), nullptr);
             bases.kc.assign(ps.size(), 0);
             bases.X.assign(ps.size(), nullptr);
@@ -1264,69 +1314,138 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             bases.qtab.assign(ps.size(), nullptr);
             bases.q0.assign(ps.size(), 0);
             bases.q1.assign(ps.size(), 0);
-            size_t vbytes = 0, total = 0;
-            std::vector<size_t> offv(ps.size(), 0), offx(ps.size(), 0), offw(ps.size(), 0);
-            std::vector<std::vector<uint32_t>> rows(ps.size());
-            size_t nrows = 0;
-            bool any_w = false;
+            bb.offv.assign(ps.size(), 0);
+            bb.offx.assign(ps.size(), 0);
+            bb.offw.assign(ps.size(), 0);
+            bb.qoff.assign(ps.size(), 0);
+            size_t vbytes = 0;
+            std::vector<uint32_t> rows;
             for (size_t p = pb.first; p < pb.second; p++) {
                 const size_t bv = sizeof(float) * N * (size_t)std::max(kc1[p] - 1, 0);
                 const size_t bx = sizeof(float) * N * (size_t)kx[p];
                 if (kc1[p] > 0 && vbytes + bv <= vcap) {
                     bases.kc[p] = kc1[p];
-                    offv[p] = total;
-                    total += bv;
+                    bb.offv[p] = bb.total;
+                    bb.total += bv;
                     vbytes += bv;
                 }
                 if (kx[p] > 0 && vbytes + bx <= vcap) {
                     bases.kx[p] = kx[p];
-                    offx[p] = total;
-                    total += bx;
+                    bb.offx[p] = bb.total;
+                    bb.total += bx;
                     vbytes += bx;
                 }
                 if (kc2[p] > 0 && bases.kc[p] > 0) {
                     bases.kc2[p] = kc2[p];
-                    offw[p] = total;
-                    total += wbytes(p);
-                    second_stage_rows(kc1[p], kc2[p], rows[p], bases.q0[p], bases.q1[p]);
-                    nrows += rows[p].size();
-                    any_w = true;
+                    bb.offw[p] = bb.total;
+                    bb.total += wbytes(p);
+                    second_stage_rows(kc1[p], kc2[p], rows, bases.q0[p], bases.q1[p]);
+                    bb.qoff[p] = bb.rows.size();
+                    bb.rows.insert(bb.rows.end(), rows.begin(), rows.end());
+                    bb.any_w = true;
                 }
+                bb.use |= bases.kc[p] > 0 || bases.kx[p] > 0;
             }
-            const size_t plus_off = total;
-            if (any_w) total += sizeof(float) * N;
-            char* base = total ? (char*)dev_get(c, "basis", total) : nullptr;
-            if (total && !base) return fail(c, MQW_ENOMEM, "device allocation failed for the shared bases");
-            uint32_t* qdev = nullptr;
-            if (any_w) {
-                bases.plus = (const float*)(base + plus_off);
-                qdev = (uint32_t*)dev_get(c, "basis_rows", sizeof(uint32_t) * nrows);
-                if (!qdev) return fail(c, MQW_ENOMEM, "device allocation failed for second-stage rows");
-                std::vector<uint32_t> all;
-                all.reserve(nrows);
-                for (size_t p = pb.first; p < pb.second; p++) {
-                    if (!bases.kc2[p]) continue;
-                    bases.qtab[p] = qdev + all.size();
-                    all.insert(all.end(), rows[p].begin(), rows[p].end());
-                }
-                HIPCHK(c, hipMemcpyAsync(qdev, all.data(), sizeof(uint32_t) * all.size(), hipMemcpyHostToDevice,
-                                         c->stream));
-            }
-            for (size_t p = pb.first; p < pb.second; p++) {
-                if (bases.kc[p] > 0) bases.V[p] = (const float*)(base + offv[p]); // unused when kc = 1 (V_0 only)
-                if (bases.kx[p] > 0) bases.X[p] = (const float*)(base + offx[p]);
-                if (bases.kc2[p] > 0) bases.W[p] = (const float*)(base + offw[p]);
-                use_bases |= bases.kc[p] > 0 || bases.kx[p] > 0;
-            }
-            if (use_bases) {
-                st = run_bases(c, n, k, ps, trajs, (const float*)E, bases, o.use_graph != 0);
-                if (st) return st;
-            }
-            if (!any_w) {
-                bases.W.clear();
-                bases.kc2.clear();
+            bb.plus_off = bb.total;
+            if (bb.any_w) bb.total += sizeof(float) * N;
+            max_total = std::max(max_total, bb.total);
+            max_rows = std::max(max_rows, bb.rows.size());
+        }
+    }
+    // Several batches: two basis buffers, the next batch's recurrences on the
+    // second stream while this batch's plans run on the main one.
+    const bool overlap = o.overlap_bases && c->stream2 && pbatches.size() > 1 && max_total > 0 &&
+                         (!o.max_device_bytes || 2 * max_total <= o.max_device_bytes / 2);
+    const int nbuf = overlap ? 2 : 1;
+    char* bbase[2] = {nullptr, nullptr};
+    uint32_t* bq[2] = {nullptr, nullptr};
+    for (int b = 0; b < nbuf && max_total > 0; b++) {
+        bbase[b] = (char*)dev_get(c, "basis" + std::to_string(b), max_total);
+        if (!bbase[b]) return fail(c, MQW_ENOMEM, "device allocation failed for the shared bases");
+        if (max_rows) {
+            bq[b] = (uint32_t*)dev_get(c, "basis_rows" + std::to_string(b), sizeof(uint32_t) * max_rows);
+            if (!bq[b]) return fail(c, MQW_ENOMEM, "device allocation failed for second-stage rows");
+        }
+    }
+    // the second stream's work in flight: its plans (whose host tables the
+    // copies read), graphs and events; drained before any of them is freed
+    std::deque<Plan<float>> keep[2];
+    Timed timed[2];
+    bool recorded[2] = {false, false};
+    struct Drain {
+        mqw_ctx* c;
+        bool on;
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        ~Drain()
+        {
+            if (on) (void)hipStreamSynchronize(c->stream2);
+            for (hipEvent_t e : ev)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } drain{c, overlap};
+    if (overlap)
+        for (int b = 0; b < 2; b++) HIPCHK(c, hipEventCreateWithFlags(&drain.ev[b], hipEventDisableTiming));
+    // binds batch q's bases to its buffer and runs their recurrences: at once
+    // on the main stream, or queued on the second stream when overlapping
+    auto bind = [&](size_t q) -> mqw_status {
+        BatchBases& bb = bbs[q];
+        Bases& bases = bb.b;
+        const int b = overlap ? (int)(q & 1) : 0;
+        const hipStream_t bs = overlap ? c->stream2 : c->stream;
+        char* base = bbase[b];
+        const auto& pb = pbatches[q];
+        if (bb.any_w) {
+            bases.plus = (const float*)(base + bb.plus_off);
+            HIPCHK(c, hipMemcpyAsync(bq[b], bb.rows.data(), sizeof(uint32_t) * bb.rows.size(), hipMemcpyHostToDevice,
+                                     bs));
+        }
+        for (size_t p = pb.first; p < pb.second; p++) {
+            if (bases.kc[p] > 0) bases.V[p] = (const float*)(base + bb.offv[p]); // unused when kc = 1 (V_0 only)
+            if (bases.kx[p] > 0) bases.X[p] = (const float*)(base + bb.offx[p]);
+            if (bases.kc2[p] > 0) {
+                bases.W[p] = (const float*)(base + bb.offw[p]);
+                bases.qtab[p] = bq[b] + bb.qoff[p];
             }
         }
+        mqw_status rs = MQW_OK;
+        if (bb.use) {
+            if (overlap) {
+                rs = run_bases(c, n, k, ps, trajs, (const float*)E, bases, o.use_graph != 0, &timed[b], c->stream2,
+                               &keep[b]);
+                if (rs) return rs;
+                HIPCHK(c, hipEventRecord(drain.ev[b], c->stream2));
+                recorded[b] = true;
+            } else {
+                rs = run_bases(c, n, k, ps, trajs, (const float*)E, bases, o.use_graph != 0);
+                if (rs) return rs;
+            }
+        }
+        if (!bb.any_w) {
+            bases.W.clear();
+            bases.kc2.clear();
+        }
+        return MQW_OK;
+    };
+    if (overlap) {
+        st = bind(0);
+        if (st) return st;
+    }
+    for (size_t q = 0; q < pbatches.size(); q++) {
+        const auto& pb = pbatches[q];
+        if (overlap) {
+            if (q + 1 < pbatches.size()) {
+                st = bind(q + 1);
+                if (st) return st;
+            }
+            if (recorded[q & 1]) HIPCHK(c, hipStreamWaitEvent(c->stream, drain.ev[q & 1], 0));
+        } else if (basis_path) {
+            st = bind(q);
+            if (st) return st;
+        }
+        Bases& bases = bbs[q].b;
+        const bool use_bases = bbs[q].use;
+        c->stats.basis_batches += use_bases;
+        c->stats.overlap_batches += recorded[q & 1];
         // the batch's trajectories, longest first
         std::vector<size_t> order;
         for (size_t p = pb.first; p < pb.second; p++)
@@ -1382,6 +1501,22 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             c->stats.amp_terms += plan.amp_terms;
             c->stats.psi_const_amp_terms += plan.psi_const_amp_terms;
             g0 = g1;
+        }
+        if (recorded[q & 1]) {
+            // the batch's recurrences ran on the second stream (their event
+            // times overlap the main stream's)
+            Timed& tm = timed[q & 1];
+            HIPCHK(c, hipEventSynchronize(drain.ev[q & 1]));
+            double ms_sweep = 0.0, ms_aux = 0.0, ms_basis = 0.0;
+            collect(tm, &ms_sweep, &ms_aux, &ms_basis, nullptr);
+            c->stats.gpu_ms += ms_sweep + ms_aux;
+            c->stats.aux_ms += ms_sweep + ms_aux;
+            c->stats.basis_ms += ms_basis;
+            tm.evs.clear();
+            tm.graphs.clear();
+            tm.cls.clear();
+            keep[q & 1].clear();
+            recorded[q & 1] = false;
         }
     }
     HIPCHK(c, hipMemcpyAsync(probs.data(), probs_d, sizeof(double) * probs.size(), hipMemcpyDeviceToHost, c->stream));
@@ -1504,6 +1639,7 @@ void mqw_opts_default(mqw_opts* o)
     o->shared_first_stage = 1;
     o->final_readout = 1;
     o->shared_second_stage = 1;
+    o->overlap_bases = 1;
     o->sample_probs = nullptr;
 }
 
@@ -1527,6 +1663,10 @@ mqw_status mqw_create(int device, mqw_ctx** out)
         c->tuned_hp[i] = -1;
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return MQW_EDEVICE;
+    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        return MQW_EDEVICE;
+    }
     *out = c.release();
     return MQW_OK;
 }
@@ -1535,8 +1675,10 @@ void mqw_destroy(mqw_ctx* c)
 {
     if (!c) return;
     hipSetDevice(c->device);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     release_bufs(c);
     if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream2) hipStreamDestroy(c->stream2);
     delete c;
 }
 

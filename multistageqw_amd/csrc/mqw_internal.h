@@ -281,6 +281,7 @@ struct SweepCfg {
 struct mqw_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr; // the shared bases' recurrences, overlapping the main stream's work
     std::string err;
     std::map<std::string, mqw::DevBuf> bufs;
     mqw_stats stats{};

@@ -1,0 +1,65 @@
+"""Overlapped batches of shared bases: when a solve's shared bases take several
+batches, the next batch's basis recurrences run on a second stream while the
+current batch's GEMM, readout and sweeps run on the main one.  The kernels and
+their inputs are the same as one batch after the other, so the results must be
+bit-identical with overlap_bases off; also checked against the oracle."""
+import numpy as np
+import pytest
+
+import multistageqw_amd as M
+
+pytestmark = pytest.mark.gpu
+
+PROB_TOL = 1e-5
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.mark.parametrize("n,m,count,samples,kw", [
+    (16, 3, 3, 64, dict(max_device_bytes=2 << 30)),                     # three batches (W 336-394 MiB each)
+    (16, 3, 2, 64, dict(max_device_bytes=2 << 30, use_graph=False)),    # two batches, no graphs
+    (16, 3, 3, 70, dict(max_device_bytes=2 << 30, group_bytes=24 << 20)),  # several groups per batch
+    # middle stages as terms; the seed of the second-stage test, whose two instances both share
+    # their second stage (W 203 MiB each)
+    (16, 5, 2, 48, dict(max_device_bytes=3 << 29, seed=6165)),
+])
+def test_overlapped_bases_identical(ctx, oracle, n, m, count, samples, kw):
+    _say("overlapped bases n", n, "m", m, "count", count, "samples", samples, kw)
+    kw = dict(kw)
+    recs = M.generate_sk(n, count, kw.pop("seed", 7000 + 10 * n + m))
+    p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, **kw)
+    st = ctx.last_stats()
+    assert st["basis_batches"] == count and st["overlap_batches"] == count
+    p_off, tt_off, sp_off = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0,
+                                      overlap_bases=False, **kw)
+    st_off = ctx.last_stats()
+    assert st_off["basis_batches"] == count and st_off["overlap_batches"] == 0
+    assert st["gemm_launches"] == st_off["gemm_launches"] > 0
+    assert np.array_equal(sp, sp_off) and np.array_equal(p, p_off) and np.array_equal(tt, tt_off)
+    po, tto, spo = oracle.solve(n, m, recs, samples=samples, threads=8, with_samples=True)
+    assert np.array_equal(tt, tto)
+    assert np.max(np.abs(sp - spo)) < PROB_TOL
+    assert np.max(np.abs(p - po)) < PROB_TOL
+
+
+def test_overlap_repeated_solves(ctx):
+    """Back-to-back overlapped solves on one context reuse both basis buffers."""
+    n, m, samples = 16, 3, 64
+    recs = M.generate_sk(n, 3, 7101)
+    _say("overlapped bases: three solves in a row")
+    first = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, max_device_bytes=2 << 30)[2]
+    for _ in range(2):
+        again = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, max_device_bytes=2 << 30)[2]
+        assert ctx.last_stats()["overlap_batches"] == 3
+        assert np.array_equal(first, again)
+
+
+def test_single_batch_does_not_overlap(ctx):
+    """One batch of bases has nothing to overlap with: it runs on the main stream."""
+    _say("overlapped bases: one batch")
+    recs = M.generate_sk(14, 2, 7201)
+    ctx.solve(14, 3, recs, samples=64, small_n_max=0)
+    st = ctx.last_stats()
+    assert st["basis_batches"] == 1 and st["overlap_batches"] == 0
