@@ -15,6 +15,13 @@ device time (term sweeps, shared-basis recurrences or the second-stage GEMM;
 all three are listed) and a CPU baseline: the FP32 restatement of
 MultiQW.cpp (oracle/) on the host cores for a bounded sample of the same
 workload.
+
+The batches of shared bases overlap (the next batch's recurrences run on a
+second stream beside this batch's GEMM), so the kernels' event times in the
+timed steps include time they share the GPU.  "roofline" is measured over the
+timed steps as they ran; "roofline_isolated" repeats the measurement on one
+extra step after the timed region with the batches one after the other, where
+each kernel has the GPU to itself.
 """
 from __future__ import annotations
 
@@ -31,6 +38,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
+MFMA_FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 matrix
 
 
 def main() -> int:
@@ -46,6 +54,9 @@ def main() -> int:
     ap.add_argument("--apply-samples", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores); -1 = skip")
     ap.add_argument("--cpu-samples", type=int, default=8)
+    ap.add_argument("--gemm", default="split_fp16", choices=("split_fp16", "fp32"),
+                    help="arithmetic of the second-stage GEMM")
+    ap.add_argument("--overlap", type=int, default=1, help="overlap batches of shared bases (0/1)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -60,10 +71,10 @@ def main() -> int:
     recs = M.generate_sk(n, args.instances, 2)
     ctx = M.Context(0)
 
-    def step(k):
+    def step(k, overlap=bool(args.overlap)):
         lo = (k * args.per_step) % args.instances
         idx = [(lo + i) % args.instances for i in range(args.per_step)]
-        return ctx.solve(n, m, recs[idx])
+        return ctx.solve(n, m, recs[idx], gemm=args.gemm, overlap_bases=overlap)
 
     for k in range(args.warmup):
         step(k)
@@ -86,47 +97,70 @@ def main() -> int:
     def rate(x, ms, scale=1e9):
         return x / (ms * 1e-3) / scale if ms > 0 else 0.0
 
-    # Each kernel class, event-timed on the library's stream:
-    #  - term sweeps: algorithmic bytes 36 B per amplitude per complex H-apply;
-    #  - basis recurrences (the same term kernel on real halves): 16 B per vector and amplitude;
-    #  - second-stage GEMM (matrix cores, FP32): executed flops.
-    classes = {
-        "term_sweep": dict(kernel="half_term_kernel (mode 0-2) / term_kernel", bound="hbm", unit="GB/s",
-                           peak=HBM_PEAK_GBS, ms=agg["sweep_ms"], launches=int(agg["sweeps"]),
-                           work=agg["algorithmic_bytes"], achieved=rate(agg["algorithmic_bytes"], agg["sweep_ms"]),
-                           profile="half_term_kernel_n20_m3_traffic.md"),
-        "basis": dict(kernel="half_term_kernel (mode 3, real halves)", bound="hbm", unit="GB/s", peak=HBM_PEAK_GBS,
-                      ms=agg["basis_ms"], launches=int(agg["basis_launches"]), work=agg["basis_bytes"],
-                      achieved=rate(agg["basis_bytes"], agg["basis_ms"]), profile="basis_recurrence_n20_m3_traffic.md"),
-        "gemm": dict(kernel="combine2_kernel (v_mfma_f32_32x32x2f32)", bound="mfma", unit="TFLOP/s",
-                     peak=MFMA_FP32_PEAK_TFLOPS, ms=agg["gemm_ms"], launches=int(agg["gemm_launches"]),
-                     work=agg["gemm_flops"], achieved=rate(agg["gemm_flops"], agg["gemm_ms"], 1e12),
-                     achieved_useful=rate(agg["gemm_useful_flops"], agg["gemm_ms"], 1e12),
-                     hbm_gbs=rate(agg["gemm_bytes"], agg["gemm_ms"]), profile="combine2_kernel_n20_m3_traffic.md"),
-    }
-    other_ms = agg["gpu_ms"] - agg["sweep_ms"] - agg["basis_ms"] - agg["gemm_ms"]
-    dom_name = max(classes, key=lambda c: classes[c]["ms"])
-    dom = classes[dom_name]
-    per_launch_work = dom["work"] / max(1, dom["launches"])
-    # HBM bytes per launch of the dominant kernel: its calibrated FETCH_SIZE +
-    # WRITE_SIZE per algorithmic byte (or per flop), measured with rocprofv3
-    # --pmc on this workload (profiles/).
-    traffic, traffic_src = None, None
-    prof = os.path.join(ROOT, "profiles", dom["profile"])
-    if os.path.exists(prof) and n == 20 and m == 3:
-        for line in open(prof):
-            if line.startswith("traffic_per_unit:"):
-                traffic = float(line.split(":")[1]) * per_launch_work
-                traffic_src = "profiles/" + dom["profile"]
-    roofline = {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
-                "frac": dom["achieved"] / dom["peak"], "traffic": traffic, "traffic_source": traffic_src,
-                "kernel": dom["kernel"], "class": dom_name, "work_per_launch": per_launch_work,
-                "avg_launch_ms": dom["ms"] / max(1, dom["launches"]), "launches": dom["launches"],
-                "device_ms": agg["gpu_ms"], "other_ms": other_ms,
-                "classes": {c: {key: v for key, v in d.items() if key not in ("profile",)}
-                            for c, d in classes.items()}}
-    for c in roofline["classes"].values():
-        c["frac"] = c["achieved"] / c["peak"] if c["peak"] else 0.0
+    def roofline_of(agg):
+        # Each kernel class, event-timed on the stream it runs on:
+        #  - term sweeps: algorithmic bytes 36 B per amplitude per complex H-apply;
+        #  - basis recurrences (the same term kernel on real halves): 16 B per vector and amplitude;
+        #  - second-stage GEMM: basis rows read once per tile and outputs written (HBM), and the
+        #    matrix-core flops it executes (three FP16 products per FP32 product when split).
+        split = args.gemm == "split_fp16"
+        gemm_peak = MFMA_FP16_PEAK_TFLOPS if split else MFMA_FP32_PEAK_TFLOPS
+        gemm_gbs = rate(agg["gemm_bytes"], agg["gemm_ms"])
+        gemm_tf = rate(agg["gemm_flops"], agg["gemm_ms"], 1e12)
+        gemm_hbm_bound = split  # the split GEMM executes 192 FP16 flops per basis byte: HBM-bound
+        classes = {
+            "term_sweep": dict(kernel="half_term_kernel (mode 0-2) / term_kernel", bound="hbm", unit="GB/s",
+                               peak=HBM_PEAK_GBS, ms=agg["sweep_ms"], launches=int(agg["sweeps"]),
+                               work=agg["algorithmic_bytes"],
+                               achieved=rate(agg["algorithmic_bytes"], agg["sweep_ms"]),
+                               profile="half_term_kernel_n20_m3_traffic.md"),
+            "basis": dict(kernel="half_term_kernel (mode 3, real halves)", bound="hbm", unit="GB/s",
+                          peak=HBM_PEAK_GBS, ms=agg["basis_ms"], launches=int(agg["basis_launches"]),
+                          work=agg["basis_bytes"], achieved=rate(agg["basis_bytes"], agg["basis_ms"]),
+                          profile="basis_recurrence_n20_m3_traffic.md"),
+            "gemm": dict(kernel=("combine2h_kernel (v_mfma_f32_32x32x16_f16, split FP16 operands)" if split
+                                 else "combine2_kernel (v_mfma_f32_32x32x2f32)"),
+                         bound="hbm" if gemm_hbm_bound else "mfma", unit="GB/s" if gemm_hbm_bound else "TFLOP/s",
+                         peak=HBM_PEAK_GBS if gemm_hbm_bound else gemm_peak, ms=agg["gemm_ms"],
+                         launches=int(agg["gemm_launches"]),
+                         work=agg["gemm_bytes"] if gemm_hbm_bound else agg["gemm_flops"],
+                         achieved=gemm_gbs if gemm_hbm_bound else gemm_tf,
+                         mfma_tflops=gemm_tf, mfma_peak=gemm_peak, mfma_frac=gemm_tf / gemm_peak,
+                         useful_fp32_tflops=rate(agg["gemm_useful_flops"], agg["gemm_ms"], 1e12), hbm_gbs=gemm_gbs,
+                         profile=("combine2h_kernel_n20_m3_traffic.md" if split
+                                  else "combine2_kernel_n20_m3_traffic.md")),
+        }
+        other_ms = agg["gpu_ms"] - agg["sweep_ms"] - agg["basis_ms"] - agg["gemm_ms"]
+        dom_name = max(classes, key=lambda c: classes[c]["ms"])
+        dom = classes[dom_name]
+        per_launch_work = dom["work"] / max(1, dom["launches"])
+        # HBM bytes per launch of the dominant kernel: its calibrated FETCH_SIZE +
+        # WRITE_SIZE per algorithmic byte (or per flop), measured with rocprofv3
+        # --pmc on this workload (profiles/).
+        traffic, traffic_src = None, None
+        prof = os.path.join(ROOT, "profiles", dom["profile"])
+        if os.path.exists(prof) and n == 20 and m == 3:
+            for line in open(prof):
+                if line.startswith("traffic_per_unit:"):
+                    traffic = float(line.split(":")[1]) * per_launch_work
+                    traffic_src = "profiles/" + dom["profile"]
+        roofline = {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
+                    "frac": dom["achieved"] / dom["peak"], "traffic": traffic, "traffic_source": traffic_src,
+                    "kernel": dom["kernel"], "class": dom_name, "work_per_launch": per_launch_work,
+                    "avg_launch_ms": dom["ms"] / max(1, dom["launches"]), "launches": dom["launches"],
+                    "device_ms": agg["gpu_ms"], "other_ms": other_ms,
+                    "classes": {c: {key: v for key, v in d.items() if key not in ("profile",)}
+                                for c, d in classes.items()}}
+        for c in roofline["classes"].values():
+            c["frac"] = c["achieved"] / c["peak"] if c["peak"] else 0.0
+        return roofline
+
+    roofline = roofline_of(agg)
+    # one extra step, not timed, with the batches of bases one after the other
+    step(args.warmup + args.steps, overlap=False)
+    st = ctx.last_stats()
+    roofline_iso = roofline_of({key: float(st[key]) for key in keys})
+    roofline_iso["step"] = "one step after the timed region, batches of bases not overlapped"
 
     apply = None
     if args.apply_n > 0:
@@ -172,8 +206,10 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"SK n={n} m={m}, {args.instances} instances seed 2, 100 samples, "
                                f"{args.per_step} instances per step", "n": n, "m": m,
-                   "instances_per_step": args.per_step, "samples": 100},
+                   "instances_per_step": args.per_step, "samples": 100, "second_stage_gemm": args.gemm,
+                   "overlap_bases": bool(args.overlap)},
         "roofline": roofline,
+        "roofline_isolated": roofline_iso,
         "cpu_baseline": cpu,
         "apply_rate": apply,
         "mean_prob": float(np.mean(np.concatenate(probs))) if probs else None,

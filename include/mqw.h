@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 8
+#define MQW_ABI_VERSION 9
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -47,6 +47,7 @@ typedef enum {
 
 typedef enum { MQW_ENERGY_COMPAT = 0, MQW_ENERGY_EXACT = 1 } mqw_energy_mode;
 typedef enum { MQW_FP32 = 0, MQW_FP64 = 1 } mqw_precision;
+typedef enum { MQW_GEMM_SPLIT_FP16 = 0, MQW_GEMM_FP32 = 1 } mqw_gemm;
 
 typedef struct mqw_ctx mqw_ctx;
 
@@ -119,6 +120,10 @@ typedef struct {
                                 next batch's basis recurrences run on a second stream while this
                                 batch's GEMM, readout and sweeps run (two basis buffers); 0 = one batch
                                 after the other on one stream.  Results are identical either way. */
+    int second_stage_gemm;   /* arithmetic of the shared second stage's GEMM: MQW_GEMM_SPLIT_FP16 (default)
+                                = FP16 matrix ops on operands split into high and low FP16 parts
+                                (three products, FP32 accumulation: 22-bit operands, FP32-level
+                                error); MQW_GEMM_FP32 = FP32 matrix ops */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are

@@ -302,7 +302,8 @@ struct Op {
     size_t off;
     uint32_t count;
     uint32_t aux = 0;  // kinds 3, 5: largest basis size of its jobs / tiles; kind 7: largest tile
-    uint32_t aux2 = 0; // kinds 5, 6: amplitude chunks of the dots
+    uint32_t aux2 = 0; // kinds 5, 6: amplitude chunks of the dots; kind 7: 1 = split-FP16 kernel
+    uint32_t aux3 = 0; // kind 5: 1 = dots on the matrix cores (split FP16)
 };
 
 template <typename real>
@@ -320,6 +321,7 @@ struct Plan {
     double aux_bytes = 0.0;   // algorithmic bytes of the combine and readout launches
     double gemm_flops = 0.0;        // matrix-core flops the combine2 launches execute (padded rows)
     double gemm_useful_flops = 0.0; // of which on real samples
+    size_t c2a_halves = 0;          // FP16 coefficient matrices of the split-FP16 combine2 launches
     double gemm_bytes = 0.0;        // their algorithmic bytes (also in aux_bytes)
     uint64_t aux_launches = 0;
     std::vector<Op> ops;
@@ -499,10 +501,12 @@ struct Bases {
     std::vector<const uint32_t*> qtab;
     std::vector<uint32_t> q0, q1;
     const float* plus = nullptr;
+    bool split = true; // the GEMM on FP16 matrix ops with split operands (else FP32 matrix ops)
 };
 
-// Rows (l, k) of the second-stage GEMM in two classes, each padded to an even
-// count: real ((l&1) == (k&1), minus when both are odd), then imaginary.
+// Rows (l, k) of the second-stage GEMM in two classes, each padded to a
+// multiple of kC2RowPad: real ((l&1) == (k&1), minus when both are odd), then
+// imaginary.
 void second_stage_rows(int kc1, int kc2, std::vector<uint32_t>& rows, uint32_t& q0, uint32_t& q1)
 {
     rows.clear();
@@ -514,7 +518,7 @@ void second_stage_rows(int kc1, int kc2, std::vector<uint32_t>& rows, uint32_t& 
                 const uint32_t sg = (cls == 0 && (l & 1)) ? 2u : 1u;
                 rows.push_back((uint32_t)l | ((uint32_t)k << 8) | (sg << 16));
             }
-        if ((rows.size() - b) & 1) rows.push_back(0u); // l = k = 0 with no weight
+        while ((rows.size() - b) % kC2RowPad) rows.push_back(0u); // l = k = 0 with no weight
         (cls ? q1 : q0) = (uint32_t)(rows.size() - b);
     }
 }
@@ -543,6 +547,7 @@ void readout_ops(int n, uint32_t parts, const std::vector<Traj>& trajs, const st
             rt.kx = (uint32_t)kx;
             rt.ns = (uint32_t)ns;
             rt.item_off = (uint32_t)P.ritems.size();
+            float amax = 0.0f; // largest coefficient magnitude of the tile
             for (size_t q = 0; q < ns; q++) {
                 const size_t i = mem[i0 + q];
                 const Traj& tr = trajs[grp[i]];
@@ -552,9 +557,16 @@ void readout_ops(int n, uint32_t parts, const std::vector<Traj>& trajs, const st
                 it.norm_part = adj[i].norm_part;
                 it.prob = probs + tr.segs.back().prob_slot;
                 it.coef_off = (uint32_t)P.ccoef.size();
-                for (int k = 0; k < kx; k++) P.ccoef.push_back(k < (int)cc.size() ? (float)cc[k] : 0.0f);
+                for (int k = 0; k < kx; k++) {
+                    P.ccoef.push_back(k < (int)cc.size() ? (float)cc[k] : 0.0f);
+                    amax = std::max(amax, fabsf(P.ccoef.back()));
+                }
                 P.ritems.push_back(it);
             }
+            // FP16 coefficients: a power of two taking the largest to [2^13, 2^14]
+            const int e = amax > 0.0f ? std::max(-30, std::min(60, (int)floor(log2(16384.0 / amax)))) : 0;
+            rt.ascale = (float)ldexp(1.0, e);
+            rt.oscale = (float)ldexp(1.0, -e - 12); // and the basis's 2^12
             P.rtiles.push_back(rt);
             kx_max = std::max<uint32_t>(kx_max, (uint32_t)kx);
             // reads the kx basis vectors and the ns states once
@@ -566,7 +578,7 @@ void readout_ops(int n, uint32_t parts, const std::vector<Traj>& trajs, const st
     uint32_t nchunk = 1;
     while ((uint64_t)nchunk * ntiles < 2048 && (N / (nchunk * 2)) >= 4096) nchunk *= 2;
     P.dots_doubles = std::max<size_t>(P.dots_doubles, (size_t)ntiles * nchunk * kReadTile * 2);
-    Op d{5, t0, ntiles, kx_max, nchunk};
+    Op d{5, t0, ntiles, kx_max, nchunk, B.split ? 1u : 0u};
     Op f{6, t0, ntiles, kx_max, nchunk};
     P.ops.push_back(d);
     P.ops.push_back(f);
@@ -634,12 +646,13 @@ void combine2_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_
         for (size_t i = 0; i < grp.size(); i++)
             if (trajs[grp[i]].second_done) byprob[trajs[grp[i]].prob].push_back(i);
         if (byprob.empty()) return;
-        Op op{7, P.c2tiles.size(), 0, 0};
+        Op op{7, P.c2tiles.size(), 0, 0, B.split ? 1u : 0u};
+        const size_t tsz = B.split ? kC2TileH : kC2Tile;
         for (const auto& kv : byprob) {
             const int pr = kv.first, kc1 = B.kc[pr], kc2 = B.kc2[pr];
             const std::vector<size_t>& mem = kv.second;
-            for (size_t i0 = 0; i0 < mem.size(); i0 += kC2Tile) {
-                const size_t ns = std::min<size_t>(kC2Tile, mem.size() - i0);
+            for (size_t i0 = 0; i0 < mem.size(); i0 += tsz) {
+                const size_t ns = std::min<size_t>(tsz, mem.size() - i0);
                 Combine2Tile ct{};
                 ct.plus = B.plus;
                 ct.V = B.V[pr];
@@ -651,6 +664,7 @@ void combine2_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_
                 ct.kc2 = (uint32_t)kc2;
                 ct.item_off = (uint32_t)P.c2items.size();
                 ct.ns = (uint32_t)ns;
+                float m1 = 0.0f, m2 = 0.0f; // largest coefficient magnitudes of the tile
                 for (size_t q = 0; q < ns; q++) {
                     const size_t i = mem[i0 + q];
                     const Traj& tr = trajs[grp[i]];
@@ -658,23 +672,45 @@ void combine2_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_
                     it.out = slots[i].st[0];
                     it.norm_part = slots[i].part[0];
                     it.c1_off = (uint32_t)P.ccoef.size();
-                    for (int k = 0; k < kc1; k++)
+                    for (int k = 0; k < kc1; k++) {
                         P.ccoef.push_back(k < (int)tr.segs[0].c.size() ? (float)tr.segs[0].c[k] : 0.0f);
+                        m1 = std::max(m1, fabsf(P.ccoef.back()));
+                    }
                     it.c2_off = (uint32_t)P.ccoef.size();
-                    for (int l = 0; l < kc2; l++)
+                    for (int l = 0; l < kc2; l++) {
                         P.ccoef.push_back(l < (int)tr.segs[1].c.size() ? (float)tr.segs[1].c[l] : 0.0f);
+                        m2 = std::max(m2, fabsf(P.ccoef.back()));
+                    }
                     P.c2items.push_back(it);
                 }
+                // FP16 coefficient matrix: a power of two taking the largest
+                // product |c1| |c2| to [2^13, 2^14] (far below FP16's 65504)
+                const double amax = (double)m1 * (double)m2;
+                const int e = amax > 0.0 ? std::max(-30, std::min(60, (int)floor(log2(16384.0 / amax)))) : 0;
+                ct.ascale = (float)ldexp(1.0, e);
+                ct.oscale = (float)ldexp(1.0, -e - 12); // and the basis's 2^12
                 P.c2tiles.push_back(ct);
                 op.count++;
                 op.aux = std::max<uint32_t>(op.aux, (uint32_t)ns);
-                // reads every basis row once, writes both halves of every sample
-                P.aux_bytes += (double)N * (4.0 * (double)(B.q0[pr] + B.q1[pr]) + 8.0 * (double)ns);
-                const double rows = (double)(B.q0[pr] + B.q1[pr]);
-                P.gemm_flops += 2.0 * rows * 32.0 * (double)((ns + 31) / 32) * (double)N;
-                P.gemm_useful_flops += 2.0 * rows * (double)ns * (double)N;
-                P.gemm_bytes += (double)N * (4.0 * rows + 8.0 * (double)ns);
             }
+        }
+        // per tile: rows x padded samples, both FP16 parts
+        const uint32_t nsp = combine2h_nsp(op.aux);
+        for (size_t q = op.off; q < op.off + op.count; q++) {
+            Combine2Tile& ct = P.c2tiles[q];
+            const double rows = (double)(ct.q0 + ct.q1);
+            const uint32_t ns = ct.ns;
+            if (B.split) {
+                ct.a_off = (uint32_t)P.c2a_halves;
+                P.c2a_halves += 2 * (size_t)(ct.q0 + ct.q1) * nsp;
+                P.gemm_flops += 3.0 * 2.0 * rows * (double)nsp * (double)N;
+            } else {
+                P.gemm_flops += 2.0 * rows * 32.0 * (double)((ns + 31) / 32) * (double)N;
+            }
+            // reads every basis row once, writes both halves of every sample
+            P.aux_bytes += (double)N * (4.0 * rows + 8.0 * (double)ns);
+            P.gemm_useful_flops += 2.0 * rows * (double)ns * (double)N;
+            P.gemm_bytes += (double)N * (4.0 * rows + 8.0 * (double)ns);
         }
         P.ops.push_back(op);
         P.aux_launches++;
@@ -769,6 +805,11 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
     }
     Combine2Tile* c2t = nullptr;
     Combine2Item* c2i = nullptr;
+    uint16_t* c2a = nullptr;
+    if (P.c2a_halves) {
+        c2a = (uint16_t*)dev_get(c, tag + "_c2a", sizeof(uint16_t) * P.c2a_halves);
+        if (!c2a) return fail(c, MQW_ENOMEM, "device allocation failed for FP16 coefficient matrices");
+    }
     if (!P.c2tiles.empty()) {
         c2t = (Combine2Tile*)dev_get(c, tag + "_c2tiles", sizeof(Combine2Tile) * P.c2tiles.size());
         c2i = (Combine2Item*)dev_get(c, tag + "_c2items", sizeof(Combine2Item) * P.c2items.size());
@@ -804,10 +845,14 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
                 rc = launch_combine(cd + op.off, op.count, ci, cf, n, k.parts, op.aux, stm);
             else if (op.kind == 4)
                 rc = launch_half_real(k.tb, k.variant, jd + op.off, op.count, n, k.lc, stm);
+            else if (op.kind == 5 && op.aux3)
+                rc = launch_readout_dots_mfma(rtd + op.off, op.count, rid, cf, n, op.aux2, dots, stm);
             else if (op.kind == 5)
                 rc = launch_readout_dots(rtd + op.off, op.count, rid, cf, n, op.aux2, op.aux, dots, stm);
             else if (op.kind == 6)
                 rc = launch_readout_finalize(rtd + op.off, op.count, rid, op.aux2, k.parts, dots, stm);
+            else if (op.kind == 7 && op.aux2)
+                rc = launch_combine2h(c2t + op.off, op.count, c2i, cf, c2a, n, k.parts, op.aux, stm);
             else if (op.kind == 7)
                 rc = launch_combine2(c2t + op.off, op.count, c2i, cf, n, k.parts, op.aux, stm);
             else if (op.kind == 2)
@@ -1305,6 +1350,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             const auto& pb = pbatches[q];
             BatchBases& bb = bbs[q];
             Bases& bases = bb.b;
+            bases.split = o.second_stage_gemm == MQW_GEMM_SPLIT_FP16;
             bases.V.assign(ps.size(), nullptr);
             bases.kc.assign(ps.size(), 0);
             bases.X.assign(ps.size(), nullptr);
@@ -1533,6 +1579,8 @@ mqw_status solve_problems(mqw_ctx* c, std::vector<Problem>& probs_in, const mqw_
     if (o.precision != MQW_FP32 && o.precision != MQW_FP64) return fail(c, MQW_EINVAL, "bad precision");
     if (o.energy_mode != MQW_ENERGY_COMPAT && o.energy_mode != MQW_ENERGY_EXACT)
         return fail(c, MQW_EINVAL, "bad energy mode");
+    if (o.second_stage_gemm != MQW_GEMM_SPLIT_FP16 && o.second_stage_gemm != MQW_GEMM_FP32)
+        return fail(c, MQW_EINVAL, "bad second_stage_gemm %d", o.second_stage_gemm);
     if (o.mixer_count < 0 || o.mixer_count > 64 || (o.mixer_count > 0 && !o.mixer_masks))
         return fail(c, MQW_EINVAL, "mixer_count must be 0..64 with masks given");
     for (int q = 0; q < o.mixer_count; q++)
@@ -1640,6 +1688,7 @@ void mqw_opts_default(mqw_opts* o)
     o->final_readout = 1;
     o->shared_second_stage = 1;
     o->overlap_bases = 1;
+    o->second_stage_gemm = MQW_GEMM_SPLIT_FP16;
     o->sample_probs = nullptr;
 }
 

@@ -126,7 +126,10 @@ int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem
 // are odd) and an imaginary class.  The scale of psi2 is irrelevant: every
 // later stage and the readout normalise their input.
 constexpr int kC2MaxK = 64;   // kc1 and kc2 limits
-constexpr int kC2Tile = 64;   // samples per workgroup (2 MFMA row blocks of 32)
+constexpr int kC2Tile = 64;   // samples per workgroup of the FP32 kernel (2 MFMA row blocks of 32)
+constexpr int kC2TileH = 128; // samples per workgroup of the split-FP16 kernel (4 row blocks of 32)
+constexpr int kC2RowPad = 16; // each class's rows padded to a multiple of this (one FP16 MFMA k block)
+constexpr int kC2MaxRows = kC2MaxK * kC2MaxK + 2 * kC2RowPad;
 struct Combine2Item {
     float* out;           // planar output (2N floats)
     double* norm_part;    // nparts partial sums of |out|^2, one per workgroup
@@ -138,12 +141,26 @@ struct Combine2Tile {
     const float* V;       // V_1 .. V_{kc1-1}: W_0k
     const float* W;       // W_lk for l >= 1 at W + (k (kc2 - 1) + l - 1) N
     const uint32_t* qtab; // q0 real rows then q1 imaginary rows: l | k << 8 | s << 16, s = 1 plus, 2 minus, 0 pad
-    uint32_t q0, q1;      // even row counts
+    uint32_t q0, q1;      // row counts, multiples of kC2RowPad
     uint32_t kc1, kc2;
     uint32_t item_off, ns;
+    // split-FP16 kernel: the tile's coefficient matrix (written by the prep
+    // launch) at A + a_off halves, scaled by ascale (a power of two putting
+    // its largest entry near 2^14); outputs are scaled back by oscale
+    uint32_t a_off;
+    float ascale, oscale;
+    uint32_t pad;
 };
 int launch_combine2(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combine2Item* items_dev,
                     const float* coefs_dev, int n, uint32_t nparts, uint32_t ns_max, void* stream);
+// Same product with FP32 accuracy from FP16 matrix ops: each operand split
+// into high and low FP16 parts, three products per 16-row block, FP32
+// accumulation.  A_dev: scratch of FP16 values (bit patterns), combine2h_nsp(ns_max)
+// * 2 * (q0 + q1) per tile, placed at each tile's a_off.
+int launch_combine2h(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combine2Item* items_dev,
+                     const float* coefs_dev, uint16_t* A_dev, int n, uint32_t nparts, uint32_t ns_max, void* stream);
+// padded samples per tile of the split kernel for a launch whose largest tile has ns_max samples
+inline uint32_t combine2h_nsp(uint32_t ns_max) { return ns_max <= 32 ? 32u : ns_max <= 64 ? 64u : 128u; }
 
 // Adjoint readout of the last stage.  The reference keeps only
 // |psi_m[eloc]|^2 / ||psi_m||^2 of the last stage's output psi_m = P psi
@@ -168,6 +185,7 @@ struct ReadTile {
     uint32_t ns;              // items of the tile (<= kReadTile)
     uint32_t item_off;
     uint32_t pad;
+    float ascale, oscale;     // matrix-core dots: coefficient scale (a power of two) and its inverse x 2^-12
 };
 // x[i] = v for i != k, x[k] = vk (k >= N: every element v).
 int launch_fill(float* x, uint64_t N, float v, uint64_t k, float vk, void* stream);
@@ -175,6 +193,10 @@ int launch_fill(float* x, uint64_t N, float v, uint64_t k, float vk, void* strea
 //   dots[((tile * nchunk + chunk) * kReadTile + j) * 2 + {0: Re, 1: Im}]
 int launch_readout_dots(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev, const float* coefs_dev,
                         int n, uint32_t nchunk, uint32_t kx_max, double* dots, void* stream);
+// The same dots on the FP16 matrix cores with split operands (FP32 accuracy):
+// phi = X a as a product over the basis index, then sum_a phi psi per lane.
+int launch_readout_dots_mfma(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev,
+                             const float* coefs_dev, int n, uint32_t nchunk, double* dots, void* stream);
 // Per item: sum the chunks and divide |amplitude|^2 by the norm.
 int launch_readout_finalize(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev, uint32_t nchunk,
                             uint32_t nparts, const double* dots, void* stream);

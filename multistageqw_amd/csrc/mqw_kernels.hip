@@ -1646,6 +1646,202 @@ int launch_combine2(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combin
 }
 
 // ===========================================================================
+// Shared second stage on the FP16 matrix cores with FP32 accuracy.  Each
+// operand x is split as hi = fp16(x), lo = fp16(x - hi) (22 significant bits
+// together; the basis scaled by 2^12 and the coefficients by the tile's power
+// of two, so both parts stay in FP16's normal range for every entry that
+// matters) and out = Ahi Bhi + Ahi Blo + Alo Bhi, accumulated in FP32 by
+// v_mfma_f32_32x32x16_f16; the dropped Alo Blo is below 2^-22 of each product.
+//
+// combine2_prep_kernel writes each tile's A = s d_jl c_jk (scaled) once, as
+// 16-row blocks [block][hi, lo][NSP samples][16 rows] of FP16, the layout the
+// main kernel copies into LDS.  combine2h_kernel: a workgroup of 8 waves takes
+// one chunk of N / nparts amplitudes for up to NSP = 32 MB samples; per 256
+// amplitudes, wave w owns 32 of them and MB accumulator tiles of 32 samples,
+// one class (real rows, then imaginary rows) at a time.  Lane (col, half)
+// loads rows 8 half + i (i < 8) of each 16-row block at its amplitude, two
+// blocks ahead of the matrix ops; A arrives through a double-buffered LDS
+// chunk of two blocks (one barrier per chunk).
+// ===========================================================================
+typedef _Float16 mqw_h8 __attribute__((ext_vector_type(8)));
+
+__global__ void __launch_bounds__(256)
+combine2_prep_kernel(const Combine2Tile* __restrict__ tiles, const Combine2Item* __restrict__ items,
+                     const float* __restrict__ coefs, _Float16* __restrict__ A, uint32_t nsp)
+{
+    const Combine2Tile& T = tiles[blockIdx.y];
+    const uint32_t nq = T.q0 + T.q1;
+    const uint32_t total = nq * nsp;
+    _Float16* At = A + T.a_off;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < total; i += gridDim.x * 256u) {
+        const uint32_t r = i / nsp, j = i % nsp;
+        const uint32_t e = T.qtab[r];
+        const uint32_t l = e & 255u, k = (e >> 8) & 255u, sg = e >> 16;
+        float v = 0.0f;
+        if (j < T.ns && sg != 0u) {
+            const Combine2Item& I = items[T.item_off + j];
+            const float f = coefs[I.c2_off + l] * coefs[I.c1_off + k];
+            v = (sg == 1u ? f : -f) * T.ascale;
+        }
+        const _Float16 hi = (_Float16)v;
+        const _Float16 lo = (_Float16)(v - (float)hi);
+        const uint32_t b = r >> 4, rr = r & 15u;
+        At[((size_t)(2u * b) * nsp + j) * 16u + rr] = hi;
+        At[((size_t)(2u * b + 1u) * nsp + j) * 16u + rr] = lo;
+    }
+}
+
+template <int MB>
+__global__ void __launch_bounds__(512, 2)
+combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const Combine2Item* __restrict__ items,
+                 const _Float16* __restrict__ A, int n, uint32_t nparts)
+{
+    constexpr int NSP = MB * 32;
+    constexpr int BLK = 2 * NSP * 16;  // halves of one 16-row block (hi and lo)
+    constexpr float kWScale = 4096.0f; // 2^12: basis entries are at most 1 in magnitude
+    __shared__ __attribute__((aligned(16))) _Float16 s_a[2][2 * BLK];
+    __shared__ uint64_t s_rp[kC2MaxRows];
+    __shared__ float* s_out[NSP];
+    __shared__ double s_n[8][NSP];
+    const uint32_t tile = blockIdx.x / nparts, part = blockIdx.x % nparts;
+    if (tile >= ntiles) return;
+    const Combine2Tile& T = tiles[tile];
+    const int ns = (int)T.ns, kc2 = (int)T.kc2;
+    const int q0 = (int)T.q0, nq = (int)(T.q0 + T.q1);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int col = lane & 31, half = lane >> 5;
+    const uint64_t N = 1ull << n;
+    for (int i = t; i < nq; i += 512) {
+        const uint32_t e = T.qtab[i];
+        const int l = (int)(e & 255u), k = (int)((e >> 8) & 255u);
+        const float* row = l == 0 ? (k == 0 ? T.plus : T.V + (uint64_t)(k - 1) * N)
+                                  : T.W + ((uint64_t)k * (kc2 - 1) + (l - 1)) * N;
+        s_rp[i] = (uint64_t)row;
+    }
+    for (int i = t; i < NSP; i += 512) s_out[i] = i < ns ? items[T.item_off + i].out : nullptr;
+    for (int i = t; i < 8 * NSP; i += 512) (&s_n[0][0])[i] = 0.0;
+    __syncthreads();
+    const _Float16* At = A + T.a_off;
+    const uint64_t C = N / nparts;
+    const uint64_t base = (uint64_t)part * C;
+    const float osc = T.oscale;
+    // blocks [b0, b0 + nblk) of A (nblk <= 2) into LDS buffer buf
+    auto copy_a = [&](int buf, int b0, int nblk) {
+        const int4* src = reinterpret_cast<const int4*>(At + (size_t)b0 * BLK);
+        int4* dst = reinterpret_cast<int4*>(s_a[buf]);
+        const int cnt = nblk * BLK / 8;
+        for (int i = t; i < cnt; i += 512) dst[i] = src[i];
+    };
+    for (uint64_t a0 = 0; a0 < C; a0 += 256) {
+        const uint64_t a = base + a0 + (uint64_t)(w * 32 + col);
+        const uint64_t aoff = a * sizeof(float);
+        // this lane's rows 8 half + i of the 16-row block starting at row q
+        auto load_b = [&](float (&bv)[8], int q) {
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                bv[i] = *reinterpret_cast<const MQW_GLOBAL float*>(s_rp[q + 8 * half + i] + aoff);
+        };
+#pragma unroll 1
+        for (int cls = 0; cls < 2; cls++) {
+            const int qb = cls ? q0 : 0, qe = cls ? nq : q0;
+            const int nb = (qe - qb) / kC2RowPad;
+            mqw_f32x16 acc[MB];
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[mb][r] = 0.0f;
+            if (nb > 0) {
+                const int bb = qb / kC2RowPad;
+                float bv[2][8];
+                load_b(bv[0], qb);
+                if (nb > 1) load_b(bv[1], qb + 16);
+                copy_a(0, bb, nb < 2 ? nb : 2);
+                __syncthreads();
+                // two blocks (one LDS chunk) per iteration
+                for (int blk = 0; blk < nb; blk += 2) {
+                    const int ch = blk >> 1;
+                    if (blk + 2 < nb) copy_a((ch + 1) & 1, bb + blk + 2, nb - blk - 2 < 2 ? nb - blk - 2 : 2);
+                    const _Float16* Ach = s_a[ch & 1];
+#pragma unroll
+                    for (int p = 0; p < 2; p++) {
+                        if (blk + p >= nb) break; // uniform
+                        mqw_h8 bh, bl;
+#pragma unroll
+                        for (int i = 0; i < 8; i++) {
+                            const float xs = bv[p][i] * kWScale;
+                            const _Float16 h = (_Float16)xs;
+                            bh[i] = h;
+                            bl[i] = (_Float16)(xs - (float)h);
+                        }
+                        if (blk + p + 2 < nb) load_b(bv[p], qb + (blk + p + 2) * 16);
+                        const _Float16* Ab = Ach + p * BLK;
+#pragma unroll
+                        for (int mb = 0; mb < MB; mb++) {
+                            const int j = mb * 32 + col;
+                            const mqw_h8 ah = *reinterpret_cast<const mqw_h8*>(Ab + j * 16 + 8 * half);
+                            const mqw_h8 al = *reinterpret_cast<const mqw_h8*>(Ab + (NSP + j) * 16 + 8 * half);
+                            acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[mb], 0, 0, 0);
+                            acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[mb], 0, 0, 0);
+                            acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[mb], 0, 0, 0);
+                        }
+                    }
+                    // this chunk's reads are done, the next chunk is written
+                    __syncthreads();
+                }
+            }
+            // lane (col, half) holds rows 8 (r / 4) + 4 half + r % 4 of column col
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int j = mb * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+                    const float v = acc[mb][r] * osc;
+                    if (j < ns) {
+                        MQW_GLOBAL float* o = gview(s_out[j]);
+                        o[(uint64_t)cls * N + a] = v;
+                    }
+                    float sq = v * v;
+#pragma unroll
+                    for (int d = 1; d < 32; d <<= 1) sq += __shfl_xor(sq, d, 64);
+                    if (col == 0) s_n[w][j] += (double)sq;
+                }
+        }
+    }
+    __syncthreads();
+    if (t < ns) {
+        double sum = 0.0;
+        for (int q = 0; q < 8; q++) sum += s_n[q][t];
+        items[T.item_off + t].norm_part[part] = sum;
+    }
+}
+
+int launch_combine2h(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combine2Item* items_dev,
+                     const float* coefs_dev, uint16_t* A_dev, int n, uint32_t nparts, uint32_t ns_max, void* stream)
+{
+    if (ntiles == 0) return 0;
+    const uint64_t N = 1ull << n;
+    if (nparts == 0 || N % nparts != 0 || (N / nparts) % 256 != 0) return -2;
+    if (ns_max < 1 || ns_max > (uint32_t)kC2TileH || !A_dev) return -2;
+    const uint64_t grid = (uint64_t)ntiles * nparts;
+    if (grid > 0x7fffffffull || ntiles > 65535u) return -3;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t nsp = combine2h_nsp(ns_max);
+    _Float16* A = reinterpret_cast<_Float16*>(A_dev);
+    hipLaunchKernelGGL(combine2_prep_kernel, dim3(64, ntiles), dim3(256), 0, st, tiles_dev, items_dev, coefs_dev, A,
+                       nsp);
+    if (nsp == 32)
+        hipLaunchKernelGGL(combine2h_kernel<1>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
+                           (const _Float16*)A, n, nparts);
+    else if (nsp == 64)
+        hipLaunchKernelGGL(combine2h_kernel<2>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
+                           (const _Float16*)A, n, nparts);
+    else
+        hipLaunchKernelGGL(combine2h_kernel<4>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
+                           (const _Float16*)A, n, nparts);
+    return check_launch("combine2h");
+}
+
+// ===========================================================================
 // Adjoint readout of the last stage (see ReadItem).  A workgroup takes one
 // chunk of N / nchunk amplitudes for a tile of up to kReadTile samples of one
 // instance.  Every thread keeps the KR basis values X_k[a] of its amplitude
@@ -1752,6 +1948,151 @@ int launch_readout_dots(const ReadTile* tiles_dev, uint32_t ntiles, const ReadIt
         hipLaunchKernelGGL(readout_dot_kernel<128>, dim3((uint32_t)grid), dim3(256), 0, st, tiles_dev, ntiles,
                            items_dev, coefs_dev, n, nchunk, dots);
     return check_launch("readout_dot");
+}
+
+// The same dots on the FP16 matrix cores (operands split as in combine2h):
+// phi_j[a] = sum_k a_jk X_k[a] is D[a][j] = sum_k X_k[a] a_jk with the basis
+// index as the inner dimension, even k (real part of a_jk X_k) and odd k
+// (imaginary part) as two classes of rows k = 2 q + class.  Per 256
+// amplitudes, wave w takes 32 of them: lane (col, half) loads rows 8 half + i
+// of each 16-row block at amplitude col (A), the tile's coefficients come
+// from LDS (B), and afterwards the lane holds phi of sample col at 16
+// amplitudes, multiplies by psi there and keeps its own partial dot: one
+// FP32 sum per 256 amplitudes, then FP64 across passes, the two halves of the
+// wave and the 8 waves in order.
+template <int MB>
+__global__ void __launch_bounds__(512, 2)
+readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const ReadItem* __restrict__ items,
+                    const float* __restrict__ coefs, int n, uint32_t nchunk, double* __restrict__ dots)
+{
+    constexpr int NSP = MB * 32;
+    constexpr int KB = kReadMaxK / 32; // 16-row blocks per class
+    constexpr int BLK = 2 * NSP * 16;  // halves of one block (hi and lo)
+    constexpr float kXScale = 4096.0f; // 2^12: basis entries are at most 1 in magnitude
+    __shared__ __attribute__((aligned(16))) _Float16 s_b[2 * KB * BLK];
+    __shared__ const float* s_psi[NSP];
+    __shared__ double s_w[8][NSP][2];
+    const uint32_t tile = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
+    if (tile >= ntiles) return;
+    const ReadTile& T = tiles[tile];
+    const int kx = (int)T.kx, ns = (int)T.ns;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int col = lane & 31, half = lane >> 5;
+    for (int i = t; i < 2 * KB * NSP * 16; i += 512) {
+        const int rr = i & 15, j = (i >> 4) % NSP, cb = (i >> 4) / NSP; // cb = class * KB + block
+        const int k = 2 * (16 * (cb % KB) + rr) + cb / KB;
+        const float v = (j < ns && k < kx) ? coefs[items[T.item_off + j].coef_off + k] * T.ascale : 0.0f;
+        const _Float16 hi = (_Float16)v;
+        s_b[cb * BLK + j * 16 + rr] = hi;
+        s_b[cb * BLK + (NSP + j) * 16 + rr] = (_Float16)(v - (float)hi);
+    }
+    for (int i = t; i < NSP; i += 512) s_psi[i] = i < ns ? items[T.item_off + i].psi : nullptr;
+    for (int i = t; i < 8 * NSP * 2; i += 512) (&s_w[0][0][0])[i] = 0.0;
+    __syncthreads();
+    const uint64_t N = 1ull << n;
+    const uint64_t C = N / nchunk;
+    const uint64_t base = (uint64_t)chunk * C;
+    const MQW_GLOBAL float* X = gview(T.X);
+    const float osc = T.oscale;
+    const int nbc[2] = {((kx + 1) / 2 + 15) / 16, (kx / 2 + 15) / 16};
+    double dre[MB], dim[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++) dre[mb] = dim[mb] = 0.0;
+    for (uint64_t a0 = 0; a0 < C; a0 += 256) {
+        const uint64_t a = base + a0 + (uint64_t)(w * 32 + col);
+        mqw_f32x16 acc[2][MB];
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[c][mb][r] = 0.0f;
+#pragma unroll
+        for (int cls = 0; cls < 2; cls++) {
+            for (int b = 0; b < nbc[cls]; b++) {
+                float xv[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const int k = 2 * (16 * b + 8 * half + i) + cls;
+                    xv[i] = k < kx ? X[(uint64_t)k * N + a] : 0.0f;
+                }
+                mqw_h8 xh, xl;
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const float xs = xv[i] * kXScale;
+                    const _Float16 h = (_Float16)xs;
+                    xh[i] = h;
+                    xl[i] = (_Float16)(xs - (float)h);
+                }
+                const _Float16* Bb = s_b + (cls * KB + b) * BLK;
+#pragma unroll
+                for (int mb = 0; mb < MB; mb++) {
+                    const int j = mb * 32 + col;
+                    const mqw_h8 bh = *reinterpret_cast<const mqw_h8*>(Bb + j * 16 + 8 * half);
+                    const mqw_h8 bl = *reinterpret_cast<const mqw_h8*>(Bb + (NSP + j) * 16 + 8 * half);
+                    acc[cls][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bh, acc[cls][mb], 0, 0, 0);
+                    acc[cls][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bl, acc[cls][mb], 0, 0, 0);
+                    acc[cls][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xl, bh, acc[cls][mb], 0, 0, 0);
+                }
+            }
+        }
+        // lane (col, half) holds rows 8 (r / 4) + 4 half + r % 4 of column col:
+        // phi of sample mb * 32 + col at amplitudes a0 + 32 w + 8 g + 4 half + e
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) {
+            const int j = mb * 32 + col;
+            if (j >= ns) continue;
+            const MQW_GLOBAL float* psi = gview(s_psi[j]);
+            float sr = 0.0f, si = 0.0f;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const uint64_t am = base + a0 + (uint64_t)(w * 32 + 8 * g + 4 * half);
+                typedef float f4 __attribute__((ext_vector_type(4)));
+                const f4 yr = *reinterpret_cast<const MQW_GLOBAL f4*>(psi + am);
+                const f4 yi = *reinterpret_cast<const MQW_GLOBAL f4*>(psi + N + am);
+                const float yre[4] = {yr[0], yr[1], yr[2], yr[3]};
+                const float yim[4] = {yi[0], yi[1], yi[2], yi[3]};
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const float pr = acc[0][mb][4 * g + e] * osc, pi = acc[1][mb][4 * g + e] * osc;
+                    sr += pr * yre[e] - pi * yim[e];
+                    si += pr * yim[e] + pi * yre[e];
+                }
+            }
+            dre[mb] += (double)sr;
+            dim[mb] += (double)si;
+        }
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++) {
+        const double r2 = __shfl_xor(dre[mb], 32, 64), i2 = __shfl_xor(dim[mb], 32, 64);
+        const int j = mb * 32 + col;
+        if (half == 0 && j < ns) {
+            s_w[w][j][0] = dre[mb] + r2;
+            s_w[w][j][1] = dim[mb] + i2;
+        }
+    }
+    __syncthreads();
+    if (t < 2 * ns) {
+        const int j = t >> 1, q = t & 1;
+        double sum = 0.0;
+        for (int v = 0; v < 8; v++) sum += s_w[v][j][q];
+        dots[((uint64_t)tile * nchunk + chunk) * kReadTile * 2 + 2 * j + q] = sum;
+    }
+}
+
+int launch_readout_dots_mfma(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev,
+                             const float* coefs_dev, int n, uint32_t nchunk, double* dots, void* stream)
+{
+    if (ntiles == 0) return 0;
+    static_assert(kReadTile == 64 && kReadMaxK % 32 == 0, "tiles of two 32-sample blocks, whole row blocks");
+    const uint64_t N = 1ull << n;
+    if (nchunk == 0 || N % nchunk != 0 || (N / nchunk) % 256 != 0) return -2;
+    const uint64_t grid = (uint64_t)ntiles * nchunk;
+    if (grid > 0x7fffffffull) return -3;
+    hipLaunchKernelGGL(readout_mfma_kernel<2>, dim3((uint32_t)grid), dim3(512), 0, (hipStream_t)stream, tiles_dev,
+                       ntiles, items_dev, coefs_dev, n, nchunk, dots);
+    return check_launch("readout_dot_mfma");
 }
 
 // One thread per item: the amplitude is the chunk partials summed in chunk

@@ -121,3 +121,25 @@ def test_final_readout_in_uneven_batch(ctx):
         _, _, b = ctx.solve_batch(inst, samples=7, with_samples=True, small_n_max=0, fixed_times=fixed,
                                   final_readout=False)
         assert np.max(np.abs(a - b)) < READOUT_TOL
+
+
+@pytest.mark.parametrize("n,m,count,samples,kw", [
+    (14, 2, 2, 100, {}),                               # tiles of 64 and 36 samples
+    (16, 3, 2, 20, dict(shared_second_stage=False)),   # one partial tile per instance
+    (15, 4, 1, 64, dict(tile_bits=13, variant=1)),     # one full tile
+])
+def test_matrix_core_readout_matches_vector_readout(ctx, oracle, n, m, count, samples, kw):
+    """Readout dots on the FP16 matrix cores (split operands, the default) against the FP32
+    vector-unit dots (gemm="fp32"), the propagated last stage and the oracle."""
+    _say("matrix-core readout n", n, "m", m, "samples", samples, kw)
+    recs = M.generate_sk(n, count, 5300 + 10 * n + m)
+    p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, **kw)
+    assert ctx.last_stats()["aux_launches"] > 0
+    p32, tt32, sp32 = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, gemm="fp32", **kw)
+    _, _, spp = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, final_readout=False, **kw)
+    po, tto, spo = oracle.solve(n, m, recs, samples=samples, threads=8, with_samples=True)
+    assert np.array_equal(tt, tto) and np.array_equal(tt32, tto)
+    assert np.max(np.abs(sp - sp32)) < 1e-6
+    assert np.max(np.abs(sp - spp)) < READOUT_TOL
+    assert np.max(np.abs(sp - spo)) < PROB_TOL
+    assert np.max(np.abs(p - po)) < PROB_TOL

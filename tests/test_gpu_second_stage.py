@@ -91,3 +91,34 @@ def test_second_stage_in_uneven_batch(ctx):
     _, _, a = ctx.solve_batch(inst, samples=48, with_samples=True, small_n_max=0)
     _, _, b = ctx.solve_batch(inst, samples=48, with_samples=True, small_n_max=0, shared_second_stage=False)
     assert np.max(np.abs(a - b)) < PATH_TOL
+
+
+@pytest.mark.parametrize("n,m,count,samples,kw", [
+    (14, 3, 2, 150, {}),                                  # tiles of 128 and 22 samples
+    (16, 3, 2, 100, {}),                                  # the headline's 100 samples: one tile
+    (15, 4, 2, 40, dict(tile_bits=13, variant=1)),        # one 64-sample tile, the third stage as terms
+    (14, 3, 3, 32, dict(final_readout=False)),            # one 32-sample tile, the last stage as terms
+])
+def test_split_fp16_gemm_matches_fp32_gemm(ctx, oracle, n, m, count, samples, kw):
+    """The split-FP16 GEMM (FP16 matrix ops on high/low operand parts, FP32 accumulation) against
+    the FP32 matrix-op GEMM, the per-sample Clenshaw path and the oracle."""
+    _say("split fp16 vs fp32 gemm n", n, "m", m, "samples", samples, kw)
+    recs = M.generate_sk(n, count, 6500 + 10 * n + m)
+    p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, **kw)
+    st = ctx.last_stats()
+    assert st["gemm_launches"] > 0
+    p32, _, sp32 = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, gemm="fp32", **kw)
+    assert ctx.last_stats()["gemm_launches"] == st["gemm_launches"]
+    _, _, spc = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0,
+                          shared_second_stage=False, **kw)
+    po, tto, spo = oracle.solve(n, m, recs, samples=samples, threads=8, with_samples=True)
+    assert np.array_equal(tt, tto)
+    assert np.max(np.abs(sp - sp32)) < PATH_TOL
+    assert np.max(np.abs(sp - spc)) < PATH_TOL
+    assert np.max(np.abs(sp - spo)) < PROB_TOL
+    assert np.max(np.abs(p - po)) < PROB_TOL
+
+
+def test_gemm_option_rejects_unknown():
+    with pytest.raises(ValueError):
+        M.api.make_opts(gemm="bf16")
