@@ -1994,10 +1994,25 @@ readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const R
     const uint64_t base = (uint64_t)chunk * C;
     const MQW_GLOBAL float* X = gview(T.X);
     const float osc = T.oscale;
-    const int nbc[2] = {((kx + 1) / 2 + 15) / 16, (kx / 2 + 15) / 16};
+    // 16-row blocks: class 0 (even k) then class 1 (odd k), the total padded to
+    // an even count (a padding block has no rows: zero loads, zero products)
+    const int nb0 = ((kx + 1) / 2 + 15) / 16;
+    const int nbt = (nb0 + (kx / 2 + 15) / 16 + 1) & ~1;
+    // block g of this lane's rows at amplitude a
+    auto load_x = [&](float (&xv)[8], int g, uint64_t a) {
+        const int cls = g >= nb0 ? 1 : 0, b = g - (cls ? nb0 : 0);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int k = 2 * (16 * b + 8 * half + i) + cls;
+            xv[i] = k < kx ? X[(uint64_t)k * N + a] : 0.0f;
+        }
+    };
     double dre[MB], dim[MB];
 #pragma unroll
     for (int mb = 0; mb < MB; mb++) dre[mb] = dim[mb] = 0.0;
+    // one block ahead: the next block of the pass, or the first of the next pass
+    float xv[2][8];
+    load_x(xv[0], 0, base + (uint64_t)(w * 32 + col));
     for (uint64_t a0 = 0; a0 < C; a0 += 256) {
         const uint64_t a = base + a0 + (uint64_t)(w * 32 + col);
         mqw_f32x16 acc[2][MB];
@@ -2007,32 +2022,39 @@ readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const R
             for (int mb = 0; mb < MB; mb++)
 #pragma unroll
                 for (int r = 0; r < 16; r++) acc[c][mb][r] = 0.0f;
+        for (int g = 0; g < nbt; g += 2) {
 #pragma unroll
-        for (int cls = 0; cls < 2; cls++) {
-            for (int b = 0; b < nbc[cls]; b++) {
-                float xv[8];
-#pragma unroll
-                for (int i = 0; i < 8; i++) {
-                    const int k = 2 * (16 * b + 8 * half + i) + cls;
-                    xv[i] = k < kx ? X[(uint64_t)k * N + a] : 0.0f;
-                }
+            for (int p = 0; p < 2; p++) {
+                const int gg = g + p;
+                if (gg + 1 < nbt)
+                    load_x(xv[p ^ 1], gg + 1, a);
+                else if (a0 + 256 < C)
+                    load_x(xv[p ^ 1], 0, a + 256);
                 mqw_h8 xh, xl;
 #pragma unroll
                 for (int i = 0; i < 8; i++) {
-                    const float xs = xv[i] * kXScale;
+                    const float xs = xv[p][i] * kXScale;
                     const _Float16 h = (_Float16)xs;
                     xh[i] = h;
                     xl[i] = (_Float16)(xs - (float)h);
                 }
+                const int cls = gg >= nb0 ? 1 : 0, b = gg - (cls ? nb0 : 0);
+                if (b >= KB) continue; // a padding block past the last class's blocks
                 const _Float16* Bb = s_b + (cls * KB + b) * BLK;
 #pragma unroll
                 for (int mb = 0; mb < MB; mb++) {
                     const int j = mb * 32 + col;
                     const mqw_h8 bh = *reinterpret_cast<const mqw_h8*>(Bb + j * 16 + 8 * half);
                     const mqw_h8 bl = *reinterpret_cast<const mqw_h8*>(Bb + (NSP + j) * 16 + 8 * half);
-                    acc[cls][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bh, acc[cls][mb], 0, 0, 0);
-                    acc[cls][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bl, acc[cls][mb], 0, 0, 0);
-                    acc[cls][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xl, bh, acc[cls][mb], 0, 0, 0);
+                    if (cls == 0) {
+                        acc[0][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bh, acc[0][mb], 0, 0, 0);
+                        acc[0][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bl, acc[0][mb], 0, 0, 0);
+                        acc[0][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xl, bh, acc[0][mb], 0, 0, 0);
+                    } else {
+                        acc[1][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bh, acc[1][mb], 0, 0, 0);
+                        acc[1][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bl, acc[1][mb], 0, 0, 0);
+                        acc[1][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xl, bh, acc[1][mb], 0, 0, 0);
+                    }
                 }
             }
         }
