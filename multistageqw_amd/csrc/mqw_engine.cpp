@@ -1024,8 +1024,31 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
                 return fail(c, MQW_EDEVICE, "unit vector: %s", last_kernel_error());
             bytes += 4.0 * N;
         }
+    // One plan of recurrence steps.  Step t holds V_{t+1} and X_{t+1} of every
+    // instance and, as a wavefront, the second stage's W_{l+1,k} with
+    // l + k = t: W_1k needs V_k (made at step k - 1), W_{l+1,k} needs W_lk and
+    // W_{l-1,k} (steps t - 1, t - 2), so every launch is wide from the first
+    // steps on instead of a narrow V/X phase followed by the W launches.
+    //   V_{k+1} = 2 Ht_1 V_k - V_{k-1},  X_{k+1} = 2 Ht_m X_k - X_{k-1}
+    //   W_{l+1,k} = 2 Ht_2 W_lk - W_{l-1,k},  W_1k = Ht_2 V_k,  W_0k = V_k (V_0 = |+>)
+    // kc2 holds zeros for the instances (or the whole batch) without a shared second stage
+    bool has_w = false;
+    for (size_t p = 0; p < B.kc2.size(); p++) has_w |= B.kc2[p] > 0;
+    if (has_w && !B.plus) return fail(c, MQW_EINVAL, "second-stage bases without the |+> row");
+    std::vector<const Segment*> seg1(ps.size(), nullptr);
+    if (has_w)
+        for (const Traj& tr : trajs)
+            if (!seg1[tr.prob] && tr.segs.size() > 1) seg1[tr.prob] = &tr.segs[1];
     int steps = 0;
-    for (size_t p = 0; p < ps.size(); p++) steps = std::max({steps, B.kc[p] - 1, B.kx[p] - 1});
+    for (size_t p = 0; p < ps.size(); p++) {
+        steps = std::max({steps, B.kc[p] - 1, B.kx[p] - 1});
+        if (has_w && B.kc2[p] > 0) steps = std::max(steps, B.kc[p] - 1 + B.kc2[p] - 1);
+    }
+    if (has_w) {
+        // |+> as a vector: the GEMM's row (l, k) = (0, 0)
+        if (launch_fill(const_cast<float*>(B.plus), N, psi0, N, 0.0f, stm))
+            return fail(c, MQW_EDEVICE, "fill: %s", last_kernel_error());
+    }
     Plan<float> P;
     auto job = [&](size_t p, const Segment& sg, int q) {
         TermJob<float> j{};
@@ -1084,98 +1107,74 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
                 P.jobs.push_back(j);
                 op.count++;
             }
+            const int kc2 = has_w ? B.kc2[p] : 0;
+            if (kc2 <= 0) continue;
+            const int kc1 = B.kc[p];
+            float* W = const_cast<float*>(B.W[p]);
+            const float* V = B.V[p];
+            auto row = [&](int ll, int kk) { return W + ((size_t)kk * (kc2 - 1) + (size_t)(ll - 1)) * N; };
+            // W_{l+1,kk} with l = q - kk
+            for (int kk = 0; kk < kc1 && kk <= q; kk++) {
+                const int l = q - kk;
+                if (l >= kc2 - 1) continue;
+                TermJob<float> j = job(p, *seg1[p], l);
+                j.out = row(l + 1, kk);
+                uint32_t flags = 0;
+                double b = 8.0; // E read, output write
+                if (l == 0) {
+                    if (kk == 0) {
+                        flags |= SRC1_PSI | F_PSI_CONST;
+                    } else {
+                        flags |= SRC1_ARRAY;
+                        j.b1 = V + (size_t)(kk - 1) * N;
+                        b += 4.0;
+                    }
+                    flags |= SRC2_ZERO;
+                } else {
+                    flags |= SRC1_ARRAY;
+                    j.b1 = row(l, kk);
+                    j.w2r = 1.0f;
+                    b += 4.0;
+                    if (l == 1) {
+                        if (kk == 0) {
+                            flags |= SRC2_PSI | F_PSI_CONST;
+                        } else {
+                            flags |= SRC2_ARRAY;
+                            j.b2 = V + (size_t)(kk - 1) * N;
+                            b += 4.0;
+                        }
+                    } else {
+                        flags |= SRC2_ARRAY;
+                        j.b2 = row(l - 1, kk);
+                        b += 4.0;
+                    }
+                }
+                j.flags = flags;
+                P.jobs.push_back(j);
+                op.count++;
+                bytes += b * N;
+            }
         }
         P.ops.push_back(op);
     }
     c->stats.aux_bytes += bytes;
     c->stats.basis_bytes += bytes;
-    auto run = [&](Plan<float>& Q) -> mqw_status {
-        if (Q.jobs.empty()) return MQW_OK;
-        c->stats.basis_launches += Q.ops.size();
-        c->stats.aux_launches += Q.ops.size();
-        if (pending) {
-            mqw_status st = run_plan<float>(c, MQW_FP32, n, k, Q, use_graph, nullptr, nullptr, nullptr, nullptr,
-                                            pending, stm, "bases");
-            keep->push_back(std::move(Q)); // the copies read its host tables
-            return st;
-        }
-        double ms_sweep = 0.0, ms_aux = 0.0, ms_basis = 0.0; // every op here is a basis launch
-        mqw_status st = run_plan<float>(c, MQW_FP32, n, k, Q, use_graph, &ms_sweep, &ms_aux, &ms_basis, nullptr,
-                                        nullptr, stm, "bases");
-        if (st) return st;
-        c->stats.gpu_ms += ms_sweep + ms_aux;
-        c->stats.aux_ms += ms_sweep + ms_aux;
-        c->stats.basis_ms += ms_basis;
-        return MQW_OK;
-    };
-    mqw_status st = run(P);
-    if (st) return st;
-
-    // second stage, from the finished V:
-    //   W_{l+1,k} = 2 Ht_2 W_lk - W_{l-1,k},  W_1k = Ht_2 V_k,  W_0k = V_k (V_0 = |+>)
-    int wsteps = 0;
-    for (size_t p = 0; p < B.kc2.size(); p++) wsteps = std::max(wsteps, B.kc2[p] - 1);
-    if (wsteps > 0) {
-        if (launch_fill(const_cast<float*>(B.plus), N, psi0, N, 0.0f, stm))
-            return fail(c, MQW_EDEVICE, "fill: %s", last_kernel_error());
-        std::vector<const Segment*> seg1(ps.size(), nullptr);
-        for (const Traj& tr : trajs)
-            if (!seg1[tr.prob] && tr.segs.size() > 1) seg1[tr.prob] = &tr.segs[1];
-        Plan<float> Pw;
-        for (int l = 0; l < wsteps; l++) {
-            Op op{4, Pw.jobs.size(), 0};
-            for (size_t p = 0; p < ps.size(); p++) {
-                const int kc2 = B.kc2[p];
-                if (l >= kc2 - 1) continue;
-                const int kc1 = B.kc[p];
-                float* W = const_cast<float*>(B.W[p]);
-                const float* V = B.V[p];
-                auto row = [&](int ll, int kk) { return W + ((size_t)kk * (kc2 - 1) + (size_t)(ll - 1)) * N; };
-                for (int kk = 0; kk < kc1; kk++) {
-                    TermJob<float> j = job(p, *seg1[p], l);
-                    j.out = row(l + 1, kk);
-                    uint32_t flags = 0;
-                    double b = 8.0; // E read, output write
-                    if (l == 0) {
-                        if (kk == 0) {
-                            flags |= SRC1_PSI | F_PSI_CONST;
-                        } else {
-                            flags |= SRC1_ARRAY;
-                            j.b1 = V + (size_t)(kk - 1) * N;
-                            b += 4.0;
-                        }
-                        flags |= SRC2_ZERO;
-                    } else {
-                        flags |= SRC1_ARRAY;
-                        j.b1 = row(l, kk);
-                        j.w2r = 1.0f;
-                        b += 4.0;
-                        if (l == 1) {
-                            if (kk == 0) {
-                                flags |= SRC2_PSI | F_PSI_CONST;
-                            } else {
-                                flags |= SRC2_ARRAY;
-                                j.b2 = V + (size_t)(kk - 1) * N;
-                                b += 4.0;
-                            }
-                        } else {
-                            flags |= SRC2_ARRAY;
-                            j.b2 = row(l - 1, kk);
-                            b += 4.0;
-                        }
-                    }
-                    j.flags = flags;
-                    Pw.jobs.push_back(j);
-                    op.count++;
-                    c->stats.aux_bytes += b * N;
-                    c->stats.basis_bytes += b * N;
-                }
-            }
-            Pw.ops.push_back(op);
-        }
-        st = run(Pw);
-        if (st) return st;
+    if (P.jobs.empty()) return MQW_OK;
+    c->stats.basis_launches += P.ops.size();
+    c->stats.aux_launches += P.ops.size();
+    if (pending) {
+        mqw_status st = run_plan<float>(c, MQW_FP32, n, k, P, use_graph, nullptr, nullptr, nullptr, nullptr, pending,
+                                        stm, "bases");
+        keep->push_back(std::move(P)); // the copies read its host tables
+        return st;
     }
+    double ms_sweep = 0.0, ms_aux = 0.0, ms_basis = 0.0; // every op here is a basis launch
+    mqw_status st = run_plan<float>(c, MQW_FP32, n, k, P, use_graph, &ms_sweep, &ms_aux, &ms_basis, nullptr, nullptr,
+                                    stm, "bases");
+    if (st) return st;
+    c->stats.gpu_ms += ms_sweep + ms_aux;
+    c->stats.aux_ms += ms_sweep + ms_aux;
+    c->stats.basis_ms += ms_basis;
     return MQW_OK;
 }
 

@@ -1858,6 +1858,7 @@ __global__ void __launch_bounds__(256) fill_kernel(float* __restrict__ x, uint64
 
 int launch_fill(float* x, uint64_t N, float v, uint64_t k, float vk, void* stream)
 {
+    if (!x) return -2;
     const uint64_t blocks = (N + 255) / 256 < 4096 ? (N + 255) / 256 : 4096;
     hipLaunchKernelGGL(fill_kernel, dim3((uint32_t)blocks), dim3(256), 0, (hipStream_t)stream, x, N, v, k, vk);
     return check_launch("fill");
