@@ -1692,14 +1692,14 @@ combine2_prep_kernel(const Combine2Tile* __restrict__ tiles, const Combine2Item*
 }
 
 template <int MB>
-__global__ void __launch_bounds__(512, 2)
+__global__ void __launch_bounds__(512, 4)
 combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const Combine2Item* __restrict__ items,
                  const _Float16* __restrict__ A, int n, uint32_t nparts)
 {
     constexpr int NSP = MB * 32;
     constexpr int BLK = 2 * NSP * 16;  // halves of one 16-row block (hi and lo)
     constexpr float kWScale = 4096.0f; // 2^12: basis entries are at most 1 in magnitude
-    __shared__ __attribute__((aligned(16))) _Float16 s_a[2][2 * BLK];
+    __shared__ __attribute__((aligned(16))) _Float16 s_a[4 * BLK];
     __shared__ uint64_t s_rp[kC2MaxRows];
     __shared__ float* s_out[NSP];
     __shared__ double s_n[8][NSP];
@@ -1725,10 +1725,10 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
     const uint64_t C = N / nparts;
     const uint64_t base = (uint64_t)part * C;
     const float osc = T.oscale;
-    // blocks [b0, b0 + nblk) of A (nblk <= 2) into LDS buffer buf
-    auto copy_a = [&](int buf, int b0, int nblk) {
+    // blocks [b0, b0 + nblk) of A (nblk <= 4) into LDS
+    auto copy_a = [&](int b0, int nblk) {
         const int4* src = reinterpret_cast<const int4*>(At + (size_t)b0 * BLK);
-        int4* dst = reinterpret_cast<int4*>(s_a[buf]);
+        int4* dst = reinterpret_cast<int4*>(s_a);
         const int cnt = nblk * BLK / 8;
         for (int i = t; i < cnt; i += 512) dst[i] = src[i];
     };
@@ -1755,26 +1755,25 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
                 float bv[2][8];
                 load_b(bv[0], qb);
                 if (nb > 1) load_b(bv[1], qb + 16);
-                copy_a(0, bb, nb < 2 ? nb : 2);
-                __syncthreads();
-                // two blocks (one LDS chunk) per iteration
-                for (int blk = 0; blk < nb; blk += 2) {
-                    const int ch = blk >> 1;
-                    if (blk + 2 < nb) copy_a((ch + 1) & 1, bb + blk + 2, nb - blk - 2 < 2 ? nb - blk - 2 : 2);
-                    const _Float16* Ach = s_a[ch & 1];
+                // four blocks (one LDS chunk) per iteration: the previous
+                // chunk's reads end at a barrier, the copy at another
+                for (int blk = 0; blk < nb; blk += 4) {
+                    __syncthreads();
+                    copy_a(bb + blk, nb - blk < 4 ? nb - blk : 4);
+                    __syncthreads();
 #pragma unroll
-                    for (int p = 0; p < 2; p++) {
+                    for (int p = 0; p < 4; p++) {
                         if (blk + p >= nb) break; // uniform
                         mqw_h8 bh, bl;
 #pragma unroll
                         for (int i = 0; i < 8; i++) {
-                            const float xs = bv[p][i] * kWScale;
+                            const float xs = bv[p & 1][i] * kWScale;
                             const _Float16 h = (_Float16)xs;
                             bh[i] = h;
                             bl[i] = (_Float16)(xs - (float)h);
                         }
-                        if (blk + p + 2 < nb) load_b(bv[p], qb + (blk + p + 2) * 16);
-                        const _Float16* Ab = Ach + p * BLK;
+                        if (blk + p + 2 < nb) load_b(bv[p & 1], qb + (blk + p + 2) * 16);
+                        const _Float16* Ab = s_a + p * BLK;
 #pragma unroll
                         for (int mb = 0; mb < MB; mb++) {
                             const int j = mb * 32 + col;
@@ -1785,8 +1784,6 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
                             acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[mb], 0, 0, 0);
                         }
                     }
-                    // this chunk's reads are done, the next chunk is written
-                    __syncthreads();
                 }
             }
             // lane (col, half) holds rows 8 (r / 4) + 4 half + r % 4 of column col
