@@ -866,6 +866,9 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
             if (MODE == 2) {
 #pragma unroll
                 for (int q = 0; q < R; q++) e[q] = e2[q];
+            } else if (MODE == 3) {
+                // the recurrences of one instance share its diagonal: cached loads
+                vload<float, R>(J.E + off, e, PS);
             } else {
                 vload_nt<float, R>(J.E + off, e, PS);
             }
@@ -1970,8 +1973,12 @@ readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const R
     __shared__ __attribute__((aligned(16))) _Float16 s_b[2 * KB * BLK];
     __shared__ const float* s_psi[NSP];
     __shared__ double s_w[8][NSP][2];
-    const uint32_t tile = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
-    if (tile >= ntiles) return;
+    // XCD-aware order: workgroup b runs on XCD b % 8 and each XCD takes a run
+    // of consecutive units, chunk-major, so the tiles of one amplitude chunk
+    // (two tiles of an instance share its basis) meet in one L2
+    const uint32_t u = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const uint32_t chunk = u / ntiles, tile = u % ntiles;
+    if (chunk >= nchunk) return;
     const ReadTile& T = tiles[tile];
     const int kx = (int)T.kx, ns = (int)T.ns;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -2108,7 +2115,7 @@ int launch_readout_dots_mfma(const ReadTile* tiles_dev, uint32_t ntiles, const R
     static_assert(kReadTile == 64 && kReadMaxK % 32 == 0, "tiles of two 32-sample blocks, whole row blocks");
     const uint64_t N = 1ull << n;
     if (nchunk == 0 || N % nchunk != 0 || (N / nchunk) % 256 != 0) return -2;
-    const uint64_t grid = (uint64_t)ntiles * nchunk;
+    const uint64_t grid = ((uint64_t)ntiles * nchunk + 7) / 8 * 8; // a multiple of the 8 XCDs
     if (grid > 0x7fffffffull) return -3;
     hipLaunchKernelGGL(readout_mfma_kernel<2>, dim3((uint32_t)grid), dim3(512), 0, (hipStream_t)stream, tiles_dev,
                        ntiles, items_dev, coefs_dev, n, nchunk, dots);
