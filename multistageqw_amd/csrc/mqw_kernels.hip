@@ -2122,35 +2122,45 @@ int launch_readout_dots_mfma(const ReadTile* tiles_dev, uint32_t ntiles, const R
     return check_launch("readout_dot_mfma");
 }
 
-// One thread per item: the amplitude is the chunk partials summed in chunk
-// order; the norm is the state's partial sums in order.
-__global__ void __launch_bounds__(64)
+// One wave per item: lane l sums the chunk partials l, l + 64, ... and the
+// norm partials the same way, then a fixed-order tree over the lanes (the same
+// order on every run).
+__global__ void __launch_bounds__(256)
 readout_finalize_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const ReadItem* __restrict__ items,
                         uint32_t nchunk, uint32_t nparts, const double* __restrict__ dots)
 {
-    const uint32_t tile = blockIdx.x, j = threadIdx.x;
+    const uint32_t tile = blockIdx.x / (kReadTile / 4);
+    const uint32_t j = (blockIdx.x % (kReadTile / 4)) * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
     if (tile >= ntiles) return;
     const ReadTile& T = tiles[tile];
-    if (j >= T.ns) return;
+    if (j >= T.ns) return; // uniform per wave
     const ReadItem& I = items[T.item_off + j];
-    double re = 0.0, im = 0.0;
-    for (uint32_t c = 0; c < nchunk; c++) {
+    double re = 0.0, im = 0.0, nrm = 0.0;
+    for (uint32_t c = lane; c < nchunk; c += 64) {
         const double* d = dots + ((uint64_t)tile * nchunk + c) * kReadTile * 2 + 2 * j;
         re += d[0];
         im += d[1];
     }
-    double nrm = 0.0;
-    for (uint32_t q = 0; q < nparts; q++) nrm += I.norm_part[q];
-    *I.prob = (re * re + im * im) / nrm;
+    for (uint32_t q = lane; q < nparts; q += 64) nrm += I.norm_part[q];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        re += __shfl_xor(re, d, 64);
+        im += __shfl_xor(im, d, 64);
+        nrm += __shfl_xor(nrm, d, 64);
+    }
+    if (lane == 0) *I.prob = (re * re + im * im) / nrm;
 }
 
 int launch_readout_finalize(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev, uint32_t nchunk,
                             uint32_t nparts, const double* dots, void* stream)
 {
     if (ntiles == 0) return 0;
-    static_assert(kReadTile <= 64, "one thread per item of a tile");
-    hipLaunchKernelGGL(readout_finalize_kernel, dim3(ntiles), dim3(64), 0, (hipStream_t)stream, tiles_dev, ntiles,
-                       items_dev, nchunk, nparts, dots);
+    static_assert(kReadTile % 4 == 0, "four items per workgroup");
+    const uint64_t grid = (uint64_t)ntiles * (kReadTile / 4);
+    if (grid > 0x7fffffffull) return -3;
+    hipLaunchKernelGGL(readout_finalize_kernel, dim3((uint32_t)grid), dim3(256), 0, (hipStream_t)stream, tiles_dev,
+                       ntiles, items_dev, nchunk, nparts, dots);
     return check_launch("readout_finalize");
 }
 
