@@ -19,9 +19,10 @@ workload.
 The batches of shared bases overlap (the next batch's recurrences run on a
 second stream beside this batch's GEMM), so the kernels' event times in the
 timed steps include time they share the GPU.  "roofline" is measured over the
-timed steps as they ran; "roofline_isolated" repeats the measurement on one
-extra step after the timed region with the batches one after the other, where
-each kernel has the GPU to itself.
+timed steps as they ran, with "roofline.all_kernels" the algorithmic bytes of
+every kernel over the timed wall clock; "roofline_isolated" repeats the
+measurement on one extra step after the timed region with the batches one
+after the other, where each kernel has the GPU to itself.
 """
 from __future__ import annotations
 
@@ -156,6 +157,12 @@ def main() -> int:
         return roofline
 
     roofline = roofline_of(agg)
+    # every kernel's algorithmic bytes (term sweeps, recurrences, GEMM, readout,
+    # combines) over the timed wall clock: the device as a whole, both streams
+    all_bytes = agg["algorithmic_bytes"] + agg["aux_bytes"]
+    roofline["all_kernels"] = {"algorithmic_bytes": all_bytes, "wall_s": wall,
+                               "gbs": all_bytes / wall / 1e9 if wall > 0 else 0.0,
+                               "frac": all_bytes / wall / 1e9 / HBM_PEAK_GBS if wall > 0 else 0.0}
     # one extra step, not timed, with the batches of bases one after the other
     step(args.warmup + args.steps, overlap=False)
     st = ctx.last_stats()
