@@ -73,9 +73,11 @@ void mqw::release_bufs(mqw_ctx* c)
     c->bufs.clear();
 }
 
-// Smallest n at which the high-bit pass is on by default, and its row width
-// (128 B rows: measured best of 64, 128 and 256 B at n = 23..28).
-static constexpr int kHighPassMinN = 25;
+// Smallest n at which the high-bit pass is on by default (at n = 24 the
+// register variant over 5 bits runs 4.8 % more applies/s than partner loads
+// alone), and the row width of the LDS variant (128 B rows: measured best of
+// 64, 128 and 256 B at n = 23..28).
+static constexpr int kHighPassMinN = 24;
 static constexpr int kHighPassWb = 1;
 
 static int default_tile_bits(int precision, int n)

@@ -43,7 +43,7 @@ def _j0_record(n, seed):
 @pytest.mark.parametrize("n,m,samples", [(24, 2, 2), (26, 2, 1), (28, 2, 1)])
 def test_fullsize_j0_closed_form(ctx, oracle, n, m, samples, high_pass):
     """n = 24 and n = 28 (the large single-instance size): GPU walk vs the product formula,
-    with the default sweep (the high-bit pass from n = 25), with partner loads only, and with the
+    with the default sweep (the high-bit pass from n = 24), with partner loads only, and with the
     register variant of the pass."""
     _say("J=0 closed form n", n, "m", m, "samples", samples, "high_pass", high_pass)
     rec = _j0_record(n, 40 + n)
