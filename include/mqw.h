@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 9
+#define MQW_ABI_VERSION 10
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -48,6 +48,7 @@ typedef enum {
 typedef enum { MQW_ENERGY_COMPAT = 0, MQW_ENERGY_EXACT = 1 } mqw_energy_mode;
 typedef enum { MQW_FP32 = 0, MQW_FP64 = 1 } mqw_precision;
 typedef enum { MQW_GEMM_SPLIT_FP16 = 0, MQW_GEMM_FP32 = 1 } mqw_gemm;
+typedef enum { MQW_MIXER_XSTRING = 0, MQW_MIXER_XY = 1 } mqw_mixer;
 
 typedef struct mqw_ctx mqw_ctx;
 
@@ -124,6 +125,11 @@ typedef struct {
                                 = FP16 matrix ops on operands split into high and low FP16 parts
                                 (three products, FP32 accumulation: 22-bit operands, FP32-level
                                 error); MQW_GEMM_FP32 = FP32 matrix ops */
+    int mixer_kind;          /* with mixer_masks: MQW_MIXER_XSTRING (default) = sum_k prod_{i in m_k} X_i;
+                                MQW_MIXER_XY = sum_k (X_a X_b + Y_a Y_b) over pairs m_k = 2^a + 2^b
+                                (exactly two bits each; the reference README.md:41 "XY" to-do):
+                                (H_M psi)[x] = sum_k 2 psi[x ^ m_k] where bits a, b of x differ.  It
+                                keeps Hamming weight; spectral bound E_abs + gamma * 2 * count. */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are

@@ -110,7 +110,7 @@ def make_opts(samples=L.DEFAULT_SAMPLES, tol=L.DEFAULT_TOL, precision=L.FP32, en
               tile_bits=0, chunk_bits=-1, small_n_max=-1, use_graph=True, max_device_bytes=0,
               group_bytes=0, variant=-1, high_pass=-1, mixer_masks=None, fixed_times=False,
               shared_first_stage=True, final_readout=True, shared_second_stage=True,
-              overlap_bases=True, gemm="split_fp16") -> L.Opts:
+              overlap_bases=True, gemm="split_fp16", mixer="xstring") -> L.Opts:
     o = L.Opts()
     L.load().mqw_opts_default(C.byref(o))
     o.samples = samples
@@ -134,6 +134,10 @@ def make_opts(samples=L.DEFAULT_SAMPLES, tol=L.DEFAULT_TOL, precision=L.FP32, en
     if gemm not in gemms:
         raise ValueError(f"gemm must be one of {sorted(gemms)}, got {gemm!r}")
     o.second_stage_gemm = gemms[gemm]
+    mixers = {"xstring": 0, "xy": 1}
+    if mixer not in mixers:
+        raise ValueError(f"mixer must be one of {sorted(mixers)}, got {mixer!r}")
+    o.mixer_kind = mixers[mixer]
     if mixer_masks is not None:
         masks = np.ascontiguousarray(mixer_masks, np.uint32)
         o._masks = masks  # keeps the array alive as long as the options

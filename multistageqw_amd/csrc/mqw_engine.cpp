@@ -860,7 +860,7 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
             else if (op.kind == 2)
                 rc = launch_high_pass(k.hi, k.hwb, jd + op.off, op.count, n, stm);
             else if (k.nmask > 0)
-                rc = launch_mask_term(precision, jd + op.off, op.count, n, k.masks, k.nmask, stm);
+                rc = launch_mask_term(precision, jd + op.off, op.count, n, k.masks, k.nmask, k.xy, stm);
             else if (k.half)
                 rc = launch_half_term(k.tb, k.variant, jd + op.off, op.count, n, k.lc, k.hi, stm);
             else
@@ -1205,9 +1205,12 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         k.parts = 1u << (n - k.tb);
         k.masks = md;
         k.nmask = o.mixer_count;
+        k.xy = o.mixer_kind == MQW_MIXER_XY;
     }
-    // spectral bound of the mixer: n for sum_i X_i, the mask count for X-strings
-    const float mix_norm = mixer ? (float)o.mixer_count : (float)(unsigned)n;
+    // spectral bound of the mixer: n for sum_i X_i, the mask count for
+    // X-strings, 2 per pair for XY (each X_a X_b + Y_a Y_b has norm 2)
+    const float mix_norm = mixer ? (float)o.mixer_count * (o.mixer_kind == MQW_MIXER_XY ? 2.0f : 1.0f)
+                                 : (float)(unsigned)n;
     real* E = (real*)dev_get(c, "E_group", sizeof(real) * N * ps.size());
     if (!E) return fail(c, MQW_ENOMEM, "device allocation failed for %zu diagonals at n=%d", ps.size(), n);
     mqw_status st = group_energies(c, n, ps, o.energy_mode, precision, E);
@@ -1586,6 +1589,12 @@ mqw_status solve_problems(mqw_ctx* c, std::vector<Problem>& probs_in, const mqw_
         return fail(c, MQW_EINVAL, "mixer_count must be 0..64 with masks given");
     for (int q = 0; q < o.mixer_count; q++)
         if (o.mixer_masks[q] == 0u) return fail(c, MQW_EINVAL, "mixer mask %d is empty", q);
+    if (o.mixer_kind != MQW_MIXER_XSTRING && o.mixer_kind != MQW_MIXER_XY)
+        return fail(c, MQW_EINVAL, "bad mixer_kind %d", o.mixer_kind);
+    if (o.mixer_kind == MQW_MIXER_XY)
+        for (int q = 0; q < o.mixer_count; q++)
+            if (__builtin_popcount(o.mixer_masks[q]) != 2)
+                return fail(c, MQW_EINVAL, "XY mixer mask %d (0x%x) must have exactly two bits", q, o.mixer_masks[q]);
     std::map<int, std::vector<Problem*>> groups;
     size_t slots = 0;
     for (Problem& p : probs_in) {
@@ -1689,6 +1698,7 @@ void mqw_opts_default(mqw_opts* o)
     o->final_readout = 1;
     o->shared_second_stage = 1;
     o->overlap_bases = 1;
+    o->mixer_kind = MQW_MIXER_XSTRING;
     o->second_stage_gemm = MQW_GEMM_SPLIT_FP16;
     o->sample_probs = nullptr;
 }

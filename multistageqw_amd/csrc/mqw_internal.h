@@ -244,9 +244,11 @@ int high_pass_supported(int n, int hi, int wb);
 int launch_high_pass(int hi, int wb, const void* jobs_dev, uint32_t njobs, int n, void* stream);
 // X-string mixer sweep (FP32/FP64): sum over nmask flip masks instead of the
 // hypercube; 2^mask_tile_bits(n) amplitudes (one partial norm) per workgroup.
+// xy: every mask is a pair 2^a + 2^b and contributes X_a X_b + Y_a Y_b, i.e.
+// 2 psi[x ^ m] where bits a and b of x differ, else nothing.
 int mask_tile_bits(int n);
 int launch_mask_term(int precision, const void* jobs_dev, uint32_t njobs, int n, const uint32_t* masks_dev, int nmask,
-                     void* stream);
+                     bool xy, void* stream);
 int launch_reduce(int precision, const ReduceJob* jobs_dev, uint32_t njobs, void* stream);
 int launch_half_step(const float* b1, float* b2, const float* hp, const float* psi, int n, float alpha,
                      float beta, float coef, void* stream);
@@ -294,8 +296,9 @@ struct SweepCfg {
     uint32_t parts = 1; // partial norm sums per output
     int hi = 0;        // top bits summed by the high-bit pass (0 = none)
     int hwb = 0;       // its pencil rows are 64 * 2^hwb bytes
-    const uint32_t* masks = nullptr; // X-string mixer on the device (nmask > 0), else the hypercube
+    const uint32_t* masks = nullptr; // X-string or XY mixer on the device (nmask > 0), else the hypercube
     int nmask = 0;
+    bool xy = false;                 // the masks are XY pairs
 };
 
 } // namespace mqw

@@ -1028,7 +1028,17 @@ __device__ __forceinline__ double block_sum(double v, double* red, int nt)
     return r;
 }
 
-template <typename real>
+// XY: the masks are pairs; the flip of pair m reaches amplitude x with weight
+// 2 where exactly one of its two bits is set in x (X_a X_b + Y_a Y_b moves
+// |01> <-> |10> with amplitude 2 and annihilates |00>, |11>).
+template <typename real, bool XY>
+__device__ __forceinline__ real mask_weight(uint64_t x, uint64_t m)
+{
+    if (!XY) return real(1);
+    return __popcll(x & m) == 1 ? real(2) : real(0);
+}
+
+template <typename real, bool XY>
 __global__ void __launch_bounds__(256)
 mask_term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, const uint32_t* __restrict__ masks,
                  int nmask)
@@ -1079,11 +1089,20 @@ mask_term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, 
     if (src1 == SRC1_ZERO) {
 #pragma unroll
         for (int q = 0; q < 4; q++) ur[q] = ui[q] = real(0);
-    } else if (src1 == SRC1_PSI && psi_const) {
+    } else if (src1 == SRC1_PSI && psi_const && !XY) {
         // |+> is an eigenvector of every X-string, eigenvalue 1 each
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             ur[q] = J.alpha * e[q] * J.psi0 - J.beta * (real)nmask * J.psi0;
+            ui[q] = real(0);
+        }
+    } else if (src1 == SRC1_PSI && psi_const) {
+        // |+> under XY pairs: psi0 times the sum of the weights at x
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            real w = real(0);
+            for (int k = 0; k < nmask; k++) w += mask_weight<real, true>(x0 + q, masks[k]);
+            ur[q] = J.alpha * e[q] * J.psi0 - J.beta * w * J.psi0;
             ui[q] = real(0);
         }
     } else {
@@ -1110,8 +1129,9 @@ mask_term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, 
             vload<real, 4>(src + N + base, ti);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                ar[q] += tr[q ^ lo];
-                ai[q] += ti[q ^ lo];
+                const real w = mask_weight<real, XY>(x0 + q, m);
+                ar[q] += w * tr[q ^ lo];
+                ai[q] += w * ti[q ^ lo];
             }
         }
 #pragma unroll
@@ -1157,7 +1177,7 @@ mask_term_kernel(const TermJob<real>* __restrict__ jobs, uint32_t njobs, int n, 
 int mask_tile_bits(int n) { return n < kMaskTileBits ? n : kMaskTileBits; }
 
 int launch_mask_term(int precision, const void* jobs_dev, uint32_t njobs, int n, const uint32_t* masks_dev, int nmask,
-                     void* stream)
+                     bool xy, void* stream)
 {
     if (n < 4 || nmask < 1) return -2;
     const int tmb = mask_tile_bits(n);
@@ -1166,11 +1186,17 @@ int launch_mask_term(int precision, const void* jobs_dev, uint32_t njobs, int n,
     if (grid > 0x7fffffffull) return -3;
     hipStream_t st = (hipStream_t)stream;
     const dim3 block(1u << (tmb - 2));
-    if (precision == MQW_FP32)
-        hipLaunchKernelGGL(mask_term_kernel<float>, dim3((uint32_t)grid), block, 0, st,
+    if (precision == MQW_FP32 && xy)
+        hipLaunchKernelGGL((mask_term_kernel<float, true>), dim3((uint32_t)grid), block, 0, st,
                            (const TermJob<float>*)jobs_dev, njobs, n, masks_dev, nmask);
+    else if (precision == MQW_FP32)
+        hipLaunchKernelGGL((mask_term_kernel<float, false>), dim3((uint32_t)grid), block, 0, st,
+                           (const TermJob<float>*)jobs_dev, njobs, n, masks_dev, nmask);
+    else if (xy)
+        hipLaunchKernelGGL((mask_term_kernel<double, true>), dim3((uint32_t)grid), block, 0, st,
+                           (const TermJob<double>*)jobs_dev, njobs, n, masks_dev, nmask);
     else
-        hipLaunchKernelGGL(mask_term_kernel<double>, dim3((uint32_t)grid), block, 0, st,
+        hipLaunchKernelGGL((mask_term_kernel<double, false>), dim3((uint32_t)grid), block, 0, st,
                            (const TermJob<double>*)jobs_dev, njobs, n, masks_dev, nmask);
     return check_launch("mask_term");
 }
