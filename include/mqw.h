@@ -156,6 +156,15 @@ void mqw_opts_default(mqw_opts* o);
 mqw_status mqw_unpack(int n, const double* rec, float* J, float* h);
 /* Heuristic gamma and times per stage (MultiQW.cpp:365-423). */
 mqw_status mqw_schedule_heuristic(int n, int m, const double* rec, mqw_schedule* out);
+/* The same heuristic with the ground-state depth a taken from the diagonal's
+ * own shape: the 1/N and 1/(eN) lower-tail quantiles corrected by
+ * Cornish-Fisher for its skewness and excess kurtosis (from mqw_diag_moments)
+ * instead of a Gaussian's.  No counterpart in the reference, whose skewness
+ * and kurtosis terms are commented out (MultiQW.cpp:379-380); zero moments
+ * give mqw_schedule_heuristic's schedule exactly.  Pass the result as an
+ * override (mqw_instance.gamma/short_t). */
+mqw_status mqw_schedule_heuristic_moments(int n, int m, const double* rec, double skewness, double excess_kurtosis,
+                                          mqw_schedule* out);
 /* Monte-Carlo times times[i*samples+j] (MultiQW.cpp:387,417-430): MT19937
  * re-seeded per problem, U(short_t, 2 short_t) stage-major; m == 1 sorts and
  * differences.  short_t has m entries. */

@@ -26,7 +26,7 @@ FP32, FP64 = 0, 1
 # Symbols declared in include/mqw.h (checked by the CPU test suite).
 EXPORTS = [
     "mqw_create", "mqw_destroy", "mqw_last_error", "mqw_status_string", "mqw_abi_version", "mqw_opts_default",
-    "mqw_unpack", "mqw_schedule_heuristic", "mqw_sample_times", "mqw_chebyshev_coeffs", "mqw_generate_sk",
+    "mqw_unpack", "mqw_schedule_heuristic", "mqw_schedule_heuristic_moments", "mqw_sample_times", "mqw_chebyshev_coeffs", "mqw_generate_sk",
     "mqw_energy", "mqw_clenshaw_step_dev", "mqw_clenshaw", "mqw_solve", "mqw_solve_batch", "mqw_tune",
     "mqw_last_stats", "mqw_count_instances", "mqw_read_instances", "mqw_append_output", "mqw_features",
     "mqw_mlp_create", "mqw_mlp_destroy", "mqw_mlp_get_params", "mqw_mlp_set_params", "mqw_mlp_forward",
@@ -104,6 +104,8 @@ def load():
         "mqw_opts_default": ([C.POINTER(Opts)], None),
         "mqw_unpack": ([C.c_int, dp, fp, fp], C.c_int),
         "mqw_schedule_heuristic": ([C.c_int, C.c_int, dp, C.POINTER(Schedule)], C.c_int),
+        "mqw_schedule_heuristic_moments": ([C.c_int, C.c_int, dp, C.c_double, C.c_double, C.POINTER(Schedule)],
+                                           C.c_int),
         "mqw_sample_times": ([C.c_int, C.c_int, fp, fp], C.c_int),
         "mqw_chebyshev_coeffs": ([C.c_double, C.c_double, dp, C.c_int, ip], C.c_int),
         "mqw_generate_sk": ([C.c_int, C.c_int, C.c_uint64, dp], C.c_int),

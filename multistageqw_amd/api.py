@@ -66,10 +66,16 @@ class Schedule:
     delta2: np.float32
 
 
-def schedule_heuristic(n: int, m: int, rec) -> Schedule:
+def schedule_heuristic(n: int, m: int, rec, skewness=None, excess_kurtosis=None) -> Schedule:
+    """MultiQW.cpp:365-423; with the diagonal's skewness and excess kurtosis
+    (Context.diag_moments), the Cornish-Fisher corrected ground-state depth."""
     rec = np.ascontiguousarray(rec, np.float64)
     s = L.Schedule()
-    L.check(L.load().mqw_schedule_heuristic(n, m, L.f64p(rec), C.byref(s)))
+    if skewness is None and excess_kurtosis is None:
+        L.check(L.load().mqw_schedule_heuristic(n, m, L.f64p(rec), C.byref(s)))
+    else:
+        L.check(L.load().mqw_schedule_heuristic_moments(n, m, L.f64p(rec), C.c_double(skewness or 0.0),
+                                                        C.c_double(excess_kurtosis or 0.0), C.byref(s)))
     return Schedule(np.array(s.gamma[:m], np.float32), np.array(s.short_t[:m], np.float32),
                     np.float32(s.total_t), np.float32(s.a), np.float32(s.hp2), np.float32(s.delta2))
 

@@ -1768,6 +1768,18 @@ mqw_status mqw_schedule_heuristic(int n, int m, const double* rec, mqw_schedule*
     return MQW_OK;
 }
 
+mqw_status mqw_schedule_heuristic_moments(int n, int m, const double* rec, double skewness, double excess_kurtosis,
+                                          mqw_schedule* out)
+{
+    if (n < MQW_MIN_N || n > MQW_MAX_N || m < 1 || m > MQW_MAX_STAGES || !rec || !out) return MQW_EINVAL;
+    if (!finite_rec(n, rec) || !std::isfinite(skewness) || !std::isfinite(excess_kurtosis)) return MQW_EINVAL;
+    float J[kMaxN * kMaxN], h[kMaxN];
+    unpack(n, rec, J, h);
+    const double shape[2] = {skewness, excess_kurtosis};
+    schedule_heuristic(n, m, J, h, out, shape);
+    return MQW_OK;
+}
+
 mqw_status mqw_sample_times(int m, int samples, const float* short_t, float* times)
 {
     if (m < 1 || m > MQW_MAX_STAGES || samples < 1 || !short_t || !times) return MQW_EINVAL;

@@ -124,13 +124,28 @@ double ppnd16(double p)
 
 // MultiQW.cpp:365-423.  my_normcdfinvf(x) evaluates -Phi^{-1}(x) (its erfcinv
 // returns the negated value, :77); here Phi^{-1} is AS241 in FP64, rounded.
-void schedule_heuristic(int n, int m, const float* J, const float* h, mqw_schedule* out)
+// Cornish-Fisher correction of a standard normal quantile z for a
+// distribution of skewness s and excess kurtosis k (the terms to second order):
+// x = z + (z^2 - 1) s/6 + (z^3 - 3z) k/24 - (2z^3 - 5z) s^2/36.
+static double cornish_fisher_shift(double z, double s, double k)
+{
+    return (z * z - 1) * s / 6 + (z * z * z - 3 * z) * k / 24 - (2 * z * z * z - 5 * z) * s * s / 36;
+}
+
+void schedule_heuristic(int n, int m, const float* J, const float* h, mqw_schedule* out, const double* shape)
 {
     const unsigned N = 1u << n, un = (unsigned)n, um = (unsigned)m;
-    const float b = (float)(-ppnd16((double)(1 / (float)N)));
+    float b = (float)(-ppnd16((double)(1 / (float)N)));
     const float e_m = 0.577215664901532860;
     const float e = 2.718281828459045;
-    const float c = (float)(-ppnd16((double)(1 / (e * N))));
+    float c = (float)(-ppnd16((double)(1 / (e * N))));
+    if (shape) {
+        // depths below the mean of the lower-tail quantiles 1/N and 1/(eN) of
+        // the energies' own distribution instead of a Gaussian's (b, c are
+        // -z at those probabilities); zero moments leave b and c unchanged
+        b = (float)((double)b - cornish_fisher_shift(-(double)b, shape[0], shape[1]));
+        c = (float)((double)c - cornish_fisher_shift(-(double)c, shape[0], shape[1]));
+    }
     const float a = fmaf(1 - e_m, b, e_m * c);
     float jj[kMaxN * kMaxN], hh[kMaxN];
     for (int i = 0; i < n * n; i++) jj[i] = J[i] * J[i];

@@ -29,7 +29,10 @@ constexpr int kMaxN = MQW_MAX_N;
 float eigen_sum_f32(const float* x, long size);
 void unpack(int n, const double* rec, float* J, float* h);
 double ppnd16(double p);
-void schedule_heuristic(int n, int m, const float* J, const float* h, mqw_schedule* out);
+// shape: optional {skewness, excess kurtosis} of the diagonal, for a
+// Cornish-Fisher corrected ground-state depth a (nullptr: the reference's).
+void schedule_heuristic(int n, int m, const float* J, const float* h, mqw_schedule* out,
+                        const double* shape = nullptr);
 void sample_times(int m, int samples, const float* short_t, float* times);
 // Coefficients c[0..K) of cos(sx)+sin(sx); returns K or -1 when cap is too small.
 int cheb_coeffs(double s, double tol, double* c, int cap);

@@ -12,6 +12,10 @@
   python -m multistageqw_amd.studies uneven --count 4096 --seed 3 --fp64-every 16 --tune-cache tune.txt
       the uneven workload (n uniform in 12..22, m uniform in 1..5): instances/s,
       bytes-weighted GB/s, per-instance error against FP64 runs of a subset.
+  python -m multistageqw_amd.studies moments --n 20 --m 3 --count 256 --seed 6 --samples 100
+      the heuristic schedule against the same heuristic with a Cornish-Fisher
+      ground-state depth from the diagonal's skewness and kurtosis: mean
+      ground-state probability of each, and how often the corrected one wins.
 
 Each prints JSON lines.
 """
@@ -24,7 +28,7 @@ import time
 
 import numpy as np
 
-from . import FP32, FP64, Context, generate_sk
+from . import FP32, FP64, Context, generate_sk, schedule_heuristic
 
 PEAK = 8000.0
 
@@ -130,9 +134,42 @@ def uneven(ctx, count, samples, seed, chunk=256, fp64_every=0, tune_cache=None):
     return out
 
 
+def moments(ctx, n, m, count, samples, seed, chunk=64):
+    """Mean ground-state probability under the reference heuristic and under the
+    heuristic with the Cornish-Fisher depth (each instance's own diagonal
+    moments, computed on the GPU), same Monte-Carlo draws."""
+    recs = generate_sk(n, count, seed)
+    base, corr, skews, kurts, depth = [], [], [], [], []
+    t0 = time.perf_counter()
+    for c0 in range(0, count, chunk):
+        inst_b, inst_c = [], []
+        for rec in recs[c0:c0 + chunk]:
+            mo = ctx.diag_moments(n, rec)
+            sb = schedule_heuristic(n, m, rec)
+            sc = schedule_heuristic(n, m, rec, skewness=mo["skewness"], excess_kurtosis=mo["excess_kurtosis"])
+            inst_b.append((n, m, rec, sb.gamma, sb.short_t))
+            inst_c.append((n, m, rec, sc.gamma, sc.short_t))
+            skews.append(mo["skewness"])
+            kurts.append(mo["excess_kurtosis"])
+            depth.append((float(sb.a), float(sc.a)))
+        pb, _ = ctx.solve_batch(inst_b, samples=samples)
+        pc, _ = ctx.solve_batch(inst_c, samples=samples)
+        base.extend(pb.tolist())
+        corr.extend(pc.tolist())
+    base, corr = np.array(base), np.array(corr)
+    d = np.array(depth)
+    return dict(study="moments", n=n, m=m, count=count, samples=samples, seed=seed,
+                mean_p_heuristic=float(base.mean()), mean_p_corrected=float(corr.mean()),
+                ratio=float(corr.mean() / base.mean()), corrected_wins=float(np.mean(corr > base)),
+                skewness_mean=float(np.mean(skews)), skewness_sd=float(np.std(skews)),
+                excess_kurtosis_mean=float(np.mean(kurts)), a_heuristic=float(d[:, 0].mean()),
+                a_corrected_mean=float(d[:, 1].mean()), a_corrected_sd=float(d[:, 1].std()),
+                wall_s=time.perf_counter() - t0)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["precision", "large", "sweep", "uneven", "batch"])
+    ap.add_argument("study", choices=["precision", "large", "sweep", "uneven", "batch", "moments"])
     ap.add_argument("--n", type=int, default=24)
     ap.add_argument("--ns", type=int, nargs="*", default=[14, 16, 18, 20, 22, 24, 26])
     ap.add_argument("--m", type=int, default=3)
@@ -156,6 +193,8 @@ def main(argv=None):
             print(json.dumps(large(ctx, n, a.m, a.samples, a.seed, high_pass=a.high_pass)), flush=True)
     elif a.study == "batch":
         print(json.dumps(batch(ctx, a.n, a.m, a.count, a.samples, a.seed)), flush=True)
+    elif a.study == "moments":
+        print(json.dumps(moments(ctx, a.n, a.m, a.count, a.samples, a.seed)), flush=True)
     else:
         print(json.dumps(uneven(ctx, a.count, a.samples, a.seed, a.chunk, a.fp64_every, a.tune_cache)), flush=True)
     ctx.close()

@@ -46,3 +46,21 @@ def test_moments_j0_closed_form_at_n20(ctx):
     assert abs(got["skewness"]) < 1e-6
     assert abs(got["excess_kurtosis"] - (-2 * np.sum(h ** 4) / np.sum(h ** 2) ** 2)) < 1e-6
     assert abs(got["variance"] / np.sum(h ** 2) - 1) < 1e-6
+
+
+def test_moment_corrected_schedule_end_to_end(ctx, oracle):
+    """Diagonal moments on the GPU -> Cornish-Fisher schedule -> solve with it as an override,
+    against the dense FP64 propagator with the same schedule and Monte-Carlo times."""
+    from oracle import dense as D
+    n, m, samples = 10, 3, 6
+    rec = M.generate_sk(n, 1, 995)[0]
+    mo = ctx.diag_moments(n, rec)
+    sc = M.schedule_heuristic(n, m, rec, skewness=mo["skewness"], excess_kurtosis=mo["excess_kurtosis"])
+    sb = M.schedule_heuristic(n, m, rec)
+    assert sc.a != sb.a
+    _, _, sp = ctx.solve_batch([(n, m, rec, sc.gamma, sc.short_t)], samples=samples, with_samples=True)
+    J, h = oracle.unpack(n, rec)
+    _, e, info = oracle.energy_compat(n, J, h)
+    times = oracle.sample_times(m, samples, sc.short_t)
+    ref = D.walk_probability(e - (info["emax"] + info["e0"]) / 2, sc.gamma.astype(np.float64), times, info["eloc"])
+    assert np.max(np.abs(sp[0] - ref)) < 1e-5
