@@ -32,6 +32,28 @@ def test_headline_size_vs_oracle(ctx, oracle):
     assert np.max(np.abs(p - po)) < PROB_TOL
 
 
+def test_headline_config_shared_path_vs_propagated(ctx, oracle):
+    """The headline configuration itself: n = 20, m = 3, 100 samples (the second stage from the
+    shared basis and the split-FP16 GEMM in 128-sample tiles, the adjoint readout), against the
+    same samples propagated stage by stage with per-sample Clenshaw terms (the path the 3-sample
+    oracle test pins) and against the FP32 matrix-op GEMM and vector readout."""
+    _say("n=20 m=3 x2, 100 samples: shared path vs propagated")
+    recs = M.generate_sk(20, 2, 2)
+    p, tt, sp = ctx.solve(20, 3, recs, samples=100, with_samples=True)
+    st = ctx.last_stats()
+    assert st["gemm_launches"] > 0 and st["sweeps"] == 0
+    p32, _, sp32 = ctx.solve(20, 3, recs, samples=100, with_samples=True, gemm="fp32")
+    pt, ttt, spt = ctx.solve(20, 3, recs, samples=100, with_samples=True, shared_first_stage=False,
+                             shared_second_stage=False, final_readout=False)
+    assert ctx.last_stats()["gemm_launches"] == 0
+    sched = [M.schedule_heuristic(20, 3, r) for r in recs]
+    assert np.array_equal(tt, np.array([s.total_t for s in sched], np.float32)) and np.array_equal(tt, ttt)
+    assert np.max(np.abs(sp - sp32)) < 2e-6
+    # FP32 rounding of two evaluation orders (2e-6) plus the readout's norm (<= 2 tol relative)
+    assert np.max(np.abs(sp - spt)) < 5e-6
+    assert np.max(np.abs(p - pt)) < 5e-6
+
+
 def _j0_record(n, seed):
     rng = np.random.default_rng(seed)
     rec = np.zeros(n * (n + 1) // 2)
