@@ -1689,8 +1689,10 @@ int launch_combine2(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combin
 // amplitudes, wave w owns 32 of them and MB accumulator tiles of 32 samples,
 // one class (real rows, then imaginary rows) at a time.  Lane (col, half)
 // loads rows 8 half + i (i < 8) of each 16-row block at its amplitude, two
-// blocks ahead of the matrix ops; A arrives through a double-buffered LDS
-// chunk of two blocks (one barrier per chunk).
+// blocks ahead of the matrix ops; A arrives through an LDS chunk of four
+// blocks copied between two barriers.  Vector loads complete in order, so the
+// wait for that copy also waits for the basis loads issued before it: four
+// blocks per chunk drain the prefetch half as often as two.
 // ===========================================================================
 typedef _Float16 mqw_h8 __attribute__((ext_vector_type(8)));
 
