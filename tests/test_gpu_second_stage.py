@@ -98,6 +98,7 @@ def test_second_stage_in_uneven_batch(ctx):
     (16, 3, 2, 100, {}),                                  # the headline's 100 samples: one tile
     (15, 4, 2, 40, dict(tile_bits=13, variant=1)),        # one 64-sample tile, the third stage as terms
     (14, 3, 3, 32, dict(final_readout=False)),            # one 32-sample tile, the last stage as terms
+    (14, 3, 1, 129, {}),                                  # tiles of 128 and 1 sample
 ])
 def test_split_fp16_gemm_matches_fp32_gemm(ctx, oracle, n, m, count, samples, kw):
     """The split-FP16 GEMM (FP16 matrix ops on high/low operand parts, FP32 accumulation) against
@@ -122,3 +123,33 @@ def test_split_fp16_gemm_matches_fp32_gemm(ctx, oracle, n, m, count, samples, kw
 def test_gemm_option_rejects_unknown():
     with pytest.raises(ValueError):
         M.api.make_opts(gemm="bf16")
+
+
+def test_split_gemm_near_the_basis_limit(ctx, oracle):
+    """First and second stages of 55-64 Chebyshev terms (fixed times, so every sample has the same
+    s): the largest bases the shared second stage takes (kc1, kc2 <= 64, rows up to 64 x 64)."""
+    n, m, samples = 14, 3, 64
+    _say("split gemm near kc = 64")
+    rec = M.generate_sk(n, 1, 6601)[0]
+    J, h = oracle.unpack(n, rec)
+    _, _, info = oracle.energy_compat(n, J, h)
+    e_abs = (info["emax"] - info["e0"]) / 2
+    gam = np.array([0.8, 0.5, 0.2], np.float32)
+    onenorm = e_abs + np.abs(gam.astype(np.float64)) * n
+    # stage times giving s = onenorm t = 44 for the first two stages (about 58 terms), a short last stage
+    st = np.array([44.0 / onenorm[0], 44.0 / onenorm[1], 0.5], np.float32)
+    k1 = len(oracle.cheb_coeffs(float(np.float32(onenorm[0]) * st[0])))
+    k2 = len(oracle.cheb_coeffs(float(np.float32(onenorm[1]) * st[1])))
+    assert 50 <= k1 <= 64 and 50 <= k2 <= 64
+    inst = [(n, m, rec, gam, st)]
+    _, _, a = ctx.solve_batch(inst, samples=samples, with_samples=True, small_n_max=0, fixed_times=True)
+    assert ctx.last_stats()["gemm_launches"] > 0
+    _, _, b = ctx.solve_batch(inst, samples=samples, with_samples=True, small_n_max=0, fixed_times=True,
+                              gemm="fp32")
+    _, _, c = ctx.solve_batch(inst, samples=samples, with_samples=True, small_n_max=0, fixed_times=True,
+                              shared_second_stage=False)
+    assert ctx.last_stats()["gemm_launches"] == 0
+    assert np.max(np.abs(a - b)) < PATH_TOL
+    assert np.max(np.abs(a - c)) < PATH_TOL
+    # fixed times: every sample is the same walk
+    assert np.max(np.abs(a - a[0, 0])) < 1e-7
