@@ -143,3 +143,29 @@ def test_matrix_core_readout_matches_vector_readout(ctx, oracle, n, m, count, sa
     assert np.max(np.abs(sp - spp)) < READOUT_TOL
     assert np.max(np.abs(sp - spo)) < PROB_TOL
     assert np.max(np.abs(p - po)) < PROB_TOL
+
+
+def test_readout_near_the_basis_limit(ctx, oracle):
+    """A last stage of 100-128 Chebyshev terms (fixed times): the largest adjoint basis the readout
+    takes (kx <= 128, the blocks of both classes and a padding block), matrix-core and vector dots
+    against the propagated last stage."""
+    n, m, samples = 14, 2, 40
+    _say("readout near kx = 128")
+    rec = M.generate_sk(n, 1, 5701)[0]
+    J, h = oracle.unpack(n, rec)
+    _, _, info = oracle.energy_compat(n, J, h)
+    e_abs = (info["emax"] - info["e0"]) / 2
+    gam = np.array([0.7, 0.3], np.float32)
+    onenorm = e_abs + np.abs(gam.astype(np.float64)) * n
+    st = np.array([0.6, 95.0 / onenorm[1]], np.float32)
+    kx = len(oracle.cheb_coeffs(float(np.float32(onenorm[1]) * st[1])))
+    assert 100 <= kx <= 128
+    inst = [(n, m, rec, gam, st)]
+    _, _, a = ctx.solve_batch(inst, samples=samples, with_samples=True, small_n_max=0, fixed_times=True)
+    assert ctx.last_stats()["aux_launches"] > 0
+    _, _, b = ctx.solve_batch(inst, samples=samples, with_samples=True, small_n_max=0, fixed_times=True,
+                              gemm="fp32")
+    _, _, c = ctx.solve_batch(inst, samples=samples, with_samples=True, small_n_max=0, fixed_times=True,
+                              final_readout=False)
+    assert np.max(np.abs(a - b)) < 1e-6
+    assert np.max(np.abs(a - c)) < READOUT_TOL
