@@ -63,3 +63,26 @@ def test_single_batch_does_not_overlap(ctx):
     ctx.solve(14, 3, recs, samples=64, small_n_max=0)
     st = ctx.last_stats()
     assert st["basis_batches"] == 1 and st["overlap_batches"] == 0
+
+
+def test_overlap_in_uneven_batch(ctx):
+    """Several batches of bases in one n-group of an uneven batch, with m = 1 chains (no bases),
+    an instance whose second stage stays as terms (few samples would not pay) and override
+    schedules: overlapped and one-after-the-other results are bit-identical."""
+    _say("overlapped bases: uneven batch")
+    inst = [
+        (16, 3, M.generate_sk(16, 1, 7301)[0]),
+        (16, 1, M.generate_sk(16, 1, 7302)[0]),
+        (16, 3, M.generate_sk(16, 1, 7303)[0]),
+        (16, 4, M.generate_sk(16, 1, 7304)[0], np.array([0.9, 0.6, 0.4, 0.2], np.float32),
+         np.array([0.5, 0.7, 0.6, 0.4], np.float32)),
+        (16, 3, M.generate_sk(16, 1, 7305)[0]),
+        (14, 3, M.generate_sk(14, 1, 7306)[0]),
+    ]
+    kw = dict(samples=64, with_samples=True, small_n_max=0, max_device_bytes=2 << 30)
+    p, tt, sp = ctx.solve_batch(inst, **kw)
+    st = ctx.last_stats()
+    assert st["overlap_batches"] >= 2
+    p0, tt0, sp0 = ctx.solve_batch(inst, overlap_bases=False, **kw)
+    assert ctx.last_stats()["overlap_batches"] == 0
+    assert np.array_equal(sp, sp0) and np.array_equal(p, p0) and np.array_equal(tt, tt0)
