@@ -1689,10 +1689,9 @@ int launch_combine2(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combin
 // amplitudes, wave w owns 32 of them and MB accumulator tiles of 32 samples,
 // one class (real rows, then imaginary rows) at a time.  Lane (col, half)
 // loads rows 8 half + i (i < 8) of each 16-row block at its amplitude, two
-// blocks ahead of the matrix ops; A arrives through an LDS chunk of four
-// blocks copied between two barriers.  Vector loads complete in order, so the
-// wait for that copy also waits for the basis loads issued before it: four
-// blocks per chunk drain the prefetch half as often as two.
+// blocks ahead of the matrix ops.  A arrives in chunks of two blocks through
+// two LDS buffers, by direct global-to-LDS loads issued a chunk ahead; the
+// wait for a chunk's copy leaves the basis prefetch in flight (see the loop).
 // ===========================================================================
 typedef _Float16 mqw_h8 __attribute__((ext_vector_type(8)));
 
@@ -1722,6 +1721,30 @@ combine2_prep_kernel(const Combine2Tile* __restrict__ tiles, const Combine2Item*
     }
 }
 
+// s_waitcnt with vmcnt <= VM and lgkmcnt <= LGKM, expcnt not waited for
+// (gfx9 encoding: vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <int VM, int LGKM>
+__device__ __forceinline__ void wait_vm_lgkm()
+{
+    static_assert(VM >= 0 && VM < 64 && LGKM >= 0 && LGKM < 16, "waitcnt field out of range");
+    __builtin_amdgcn_s_waitcnt((VM & 15) | ((VM >> 4) << 14) | (7 << 4) | (LGKM << 8));
+}
+
+// 16 bytes per lane from global memory straight into LDS: lane l's bytes land
+// at LDS byte address lds + 16 l (lds is wave-uniform).  Written as inline
+// assembly so that the compiler's wait insertion, which cannot tell LDS
+// buffers apart, does not make every later LDS read wait for the copy; the
+// caller waits for it explicitly (vmcnt) before a barrier.  The compiler's
+// own vmcnt waits stay correct: loads it does not count only make them
+// stricter, since vector loads complete in issue order.
+__device__ __forceinline__ void lds_copy16(const void* src, uint32_t lds)
+{
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                 :
+                 : "s"(lds), "v"(src)
+                 : "memory", "m0");
+}
+
 template <int MB>
 __global__ void __launch_bounds__(512, 4)
 combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const Combine2Item* __restrict__ items,
@@ -1730,7 +1753,8 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
     constexpr int NSP = MB * 32;
     constexpr int BLK = 2 * NSP * 16;  // halves of one 16-row block (hi and lo)
     constexpr float kWScale = 4096.0f; // 2^12: basis entries are at most 1 in magnitude
-    __shared__ __attribute__((aligned(16))) _Float16 s_a[4 * BLK];
+    // two LDS buffers of two 16-row blocks of A
+    __shared__ __attribute__((aligned(16))) _Float16 s_a[2 * 2 * BLK];
     __shared__ uint64_t s_rp[kC2MaxRows];
     __shared__ float* s_out[NSP];
     __shared__ double s_n[8][NSP];
@@ -1756,12 +1780,18 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
     const uint64_t C = N / nparts;
     const uint64_t base = (uint64_t)part * C;
     const float osc = T.oscale;
-    // blocks [b0, b0 + nblk) of A (nblk <= 4) into LDS
-    auto copy_a = [&](int b0, int nblk) {
-        const int4* src = reinterpret_cast<const int4*>(At + (size_t)b0 * BLK);
-        int4* dst = reinterpret_cast<int4*>(s_a);
-        const int cnt = nblk * BLK / 8;
-        for (int i = t; i < cnt; i += 512) dst[i] = src[i];
+    const uint32_t s_a_lds = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) _Float16*)s_a;
+    // blocks [b0, b0 + nblk) of A (nblk <= 2) into LDS buffer bf, in slots of
+    // 1 KiB (64 lanes x 16 B, one instruction), slot s by wave s % 8
+    constexpr int SPB = BLK * 2 / 1024; // slots per block
+    static_assert(SPB * 1024 == BLK * 2, "a block is a whole number of slots");
+    auto copy_a = [&](int bf, int b0, int nblk) {
+        const int nsl = nblk * SPB;
+        for (int slot = w; slot < nsl; slot += 8) { // uniform per wave
+            const int4* src = reinterpret_cast<const int4*>(At + (size_t)b0 * BLK) + slot * 64 + lane;
+            const uint32_t dst = s_a_lds + (uint32_t)(bf * 2 * BLK * 2 + slot * 1024);
+            lds_copy16(src, __builtin_amdgcn_readfirstlane(dst));
+        }
     };
     for (uint64_t a0 = 0; a0 < C; a0 += 256) {
         const uint64_t a = base + a0 + (uint64_t)(w * 32 + col);
@@ -1783,36 +1813,63 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
                 for (int r = 0; r < 16; r++) acc[mb][r] = 0.0f;
             if (nb > 0) {
                 const int bb = qb / kC2RowPad;
+                const int nch = (nb + 1) / 2;
+                // Two blocks (one LDS chunk) per iteration, double-buffered:
+                // chunk c + 1's copy is issued right after the barrier that
+                // opens chunk c, and exactly 16 basis loads (two blocks, the
+                // prefetch) follow every copy, so waiting until at most 16
+                // loads are in flight completes the copy without draining the
+                // prefetch.  Past the last block the prefetch reloads it.
+                // Every wave is done with both buffers (its LDS reads waited
+                // for), and no output store of the previous class is in
+                // flight, before the first copy.
+                __builtin_amdgcn_sched_barrier(0);
+                wait_vm_lgkm<0, 0>();
+                __builtin_amdgcn_s_barrier();
+                asm volatile("" ::: "memory");
+                __builtin_amdgcn_sched_barrier(0);
+                copy_a(0, bb, nb < 2 ? nb : 2);
+                __builtin_amdgcn_sched_barrier(0);
                 float bv[2][8];
                 load_b(bv[0], qb);
-                if (nb > 1) load_b(bv[1], qb + 16);
-                // four blocks (one LDS chunk) per iteration: the previous
-                // chunk's reads end at a barrier, the copy at another
-                for (int blk = 0; blk < nb; blk += 4) {
-                    __syncthreads();
-                    copy_a(bb + blk, nb - blk < 4 ? nb - blk : 4);
-                    __syncthreads();
+                load_b(bv[1], qb + (nb > 1 ? 16 : 0));
+#pragma unroll 1
+                for (int c = 0; c < nch; c++) {
+                    // chunk c is in buffer c & 1 once this wave's copy is done
+                    // (at most the 16 later basis loads in flight) and every
+                    // wave has passed the barrier; this wave's LDS reads of
+                    // chunk c - 1 are complete before buffer (c + 1) & 1 is
+                    // overwritten
+                    __builtin_amdgcn_sched_barrier(0);
+                    wait_vm_lgkm<16, 0>();
+                    __builtin_amdgcn_s_barrier();
+                    asm volatile("" ::: "memory");
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (c + 1 < nch) copy_a((c + 1) & 1, bb + 2 * c + 2, nb - 2 * c - 2 < 2 ? nb - 2 * c - 2 : 2);
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int p = 0; p < 4; p++) {
-                        if (blk + p >= nb) break; // uniform
+                    for (int p = 0; p < 2; p++) {
+                        const int b = 2 * c + p;
                         mqw_h8 bh, bl;
 #pragma unroll
                         for (int i = 0; i < 8; i++) {
-                            const float xs = bv[p & 1][i] * kWScale;
+                            const float xs = bv[p][i] * kWScale;
                             const _Float16 h = (_Float16)xs;
                             bh[i] = h;
                             bl[i] = (_Float16)(xs - (float)h);
                         }
-                        if (blk + p + 2 < nb) load_b(bv[p & 1], qb + (blk + p + 2) * 16);
-                        const _Float16* Ab = s_a + p * BLK;
+                        load_b(bv[p], qb + (b + 2 < nb ? b + 2 : nb - 1) * 16);
+                        if (b < nb) { // uniform: only an odd last chunk's second block is absent
+                            const _Float16* Ab = s_a + ((c & 1) * 2 + p) * BLK;
 #pragma unroll
-                        for (int mb = 0; mb < MB; mb++) {
-                            const int j = mb * 32 + col;
-                            const mqw_h8 ah = *reinterpret_cast<const mqw_h8*>(Ab + j * 16 + 8 * half);
-                            const mqw_h8 al = *reinterpret_cast<const mqw_h8*>(Ab + (NSP + j) * 16 + 8 * half);
-                            acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[mb], 0, 0, 0);
-                            acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[mb], 0, 0, 0);
-                            acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[mb], 0, 0, 0);
+                            for (int mb = 0; mb < MB; mb++) {
+                                const int j = mb * 32 + col;
+                                const mqw_h8 ah = *reinterpret_cast<const mqw_h8*>(Ab + j * 16 + 8 * half);
+                                const mqw_h8 al = *reinterpret_cast<const mqw_h8*>(Ab + (NSP + j) * 16 + 8 * half);
+                                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[mb], 0, 0, 0);
+                                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[mb], 0, 0, 0);
+                                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[mb], 0, 0, 0);
+                            }
                         }
                     }
                 }
