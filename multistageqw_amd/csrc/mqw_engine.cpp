@@ -1286,7 +1286,9 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             // W: recurrences (16 B per vector and amplitude) plus one GEMM read (4 B), against the
             // samples' own terms, with a margin for the matrix work
             const double shared2 = 20.0 * (double)kc1[p] * (double)kc2[p];
-            if (kc1[p] < 1 || kc1[p] > kC2MaxK || kc2[p] < 2 || kc2[p] > kC2MaxK || shared2 > 0.8 * own2[p])
+            // (the split-FP16 kernel takes amplitude chunks of 512: always so with half tiles)
+            if (kc1[p] < 1 || kc1[p] > kC2MaxK || kc2[p] < 2 || kc2[p] > kC2MaxK || shared2 > 0.8 * own2[p] ||
+                (o.second_stage_gemm == MQW_GEMM_SPLIT_FP16 && (N / nparts) % 512 != 0))
                 kc2[p] = 0;
         }
     }

@@ -1684,14 +1684,17 @@ int launch_combine2(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combin
 //
 // combine2_prep_kernel writes each tile's A = s d_jl c_jk (scaled) once, as
 // 16-row blocks [block][hi, lo][NSP samples][16 rows] of FP16, the layout the
-// main kernel copies into LDS.  combine2h_kernel: a workgroup of 8 waves takes
-// one chunk of N / nparts amplitudes for up to NSP = 32 MB samples; per 256
-// amplitudes, wave w owns 32 of them and MB accumulator tiles of 32 samples,
-// one class (real rows, then imaginary rows) at a time.  Lane (col, half)
-// loads rows 8 half + i (i < 8) of each 16-row block at its amplitude, two
-// blocks ahead of the matrix ops.  A arrives in chunks of two blocks through
-// two LDS buffers, by direct global-to-LDS loads issued a chunk ahead; the
-// wait for a chunk's copy leaves the basis prefetch in flight (see the loop).
+// main kernel copies into LDS.  combine2h_kernel: a workgroup of NW = 16 waves
+// takes one chunk of N / nparts amplitudes for up to NSP = 32 MB samples; per
+// 512 amplitudes, wave w owns 32 of them and MB accumulator tiles of 32
+// samples, one class (real rows, then imaginary rows) at a time.  Everything
+// reaches LDS by direct global-to-LDS loads (16 B per lane): each wave copies
+// its own 32 amplitudes of every 16-row basis block (128 B per row) into its
+// own three-slot ring, three blocks ahead, and the waves copy each block of A
+// into a three-buffer ring shared by the workgroup, two blocks ahead.  Lane
+// (col, half) then reads rows 8 half + i (i < 8) of the block at its
+// amplitude.  The copies' order makes a count of loads in flight enough to
+// know a block has arrived (see the loop), so no wait drains the pipeline.
 // ===========================================================================
 typedef _Float16 mqw_h8 __attribute__((ext_vector_type(8)));
 
@@ -1721,43 +1724,63 @@ combine2_prep_kernel(const Combine2Tile* __restrict__ tiles, const Combine2Item*
     }
 }
 
-// s_waitcnt with vmcnt <= VM and lgkmcnt <= LGKM, expcnt not waited for
-// (gfx9 encoding: vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
-template <int VM, int LGKM>
-__device__ __forceinline__ void wait_vm_lgkm()
+// s_waitcnt as inline assembly with a memory clobber, so the compiler moves no
+// memory access across it: at most VM vector memory operations of this wave
+// left in flight, and no LDS (or scalar memory) operation.
+template <int VM>
+__device__ __forceinline__ void wait_vm_lgkm0()
 {
-    static_assert(VM >= 0 && VM < 64 && LGKM >= 0 && LGKM < 16, "waitcnt field out of range");
-    __builtin_amdgcn_s_waitcnt((VM & 15) | ((VM >> 4) << 14) | (7 << 4) | (LGKM << 8));
+    static_assert(VM >= 0 && VM < 16, "vmcnt out of range");
+    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)" : : "n"(VM) : "memory");
+}
+// the same with a run-time count (wave-uniform), one of 0, 2, 3, 4, 5; any
+// other value waits for everything
+__device__ __forceinline__ void wait_vm_lgkm0(int vm)
+{
+    switch (vm) {
+    case 5: wait_vm_lgkm0<5>(); break;
+    case 4: wait_vm_lgkm0<4>(); break;
+    case 3: wait_vm_lgkm0<3>(); break;
+    case 2: wait_vm_lgkm0<2>(); break;
+    default: wait_vm_lgkm0<0>(); break;
+    }
 }
 
 // 16 bytes per lane from global memory straight into LDS: lane l's bytes land
-// at LDS byte address lds + 16 l (lds is wave-uniform).  Written as inline
-// assembly so that the compiler's wait insertion, which cannot tell LDS
-// buffers apart, does not make every later LDS read wait for the copy; the
-// caller waits for it explicitly (vmcnt) before a barrier.  The compiler's
-// own vmcnt waits stay correct: loads it does not count only make them
-// stricter, since vector loads complete in issue order.
+// at LDS byte address lds + 16 l (lds is wave-uniform).  Inline assembly, so
+// the compiler neither counts these loads nor makes its LDS reads wait for
+// them (its wait insertion cannot tell LDS buffers apart): the caller waits
+// explicitly.  This wave's earlier LDS reads complete first, since the copy
+// may overwrite what they read.  The compiler's own vmcnt waits stay correct:
+// loads it does not count only make them stricter (vector loads complete in
+// issue order).
 __device__ __forceinline__ void lds_copy16(const void* src, uint32_t lds)
 {
-    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
                  :
                  : "s"(lds), "v"(src)
                  : "memory", "m0");
 }
 
-template <int MB>
-__global__ void __launch_bounds__(512, 4)
+template <int MB, int NW>
+__global__ void __launch_bounds__(NW * 64, 4)
 combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const Combine2Item* __restrict__ items,
                  const _Float16* __restrict__ A, int n, uint32_t nparts)
 {
+    constexpr int NT = NW * 64;
     constexpr int NSP = MB * 32;
     constexpr int BLK = 2 * NSP * 16;  // halves of one 16-row block (hi and lo)
     constexpr float kWScale = 4096.0f; // 2^12: basis entries are at most 1 in magnitude
-    // two LDS buffers of two 16-row blocks of A
-    __shared__ __attribute__((aligned(16))) _Float16 s_a[2 * 2 * BLK];
-    __shared__ uint64_t s_rp[kC2MaxRows];
+    constexpr int SPB = BLK * 2 / 1024; // 1 KiB copy slots per block of A
+    static_assert(SPB * 1024 == BLK * 2 && SPB <= NW, "a block is a whole number of slots, at most one per wave");
+    // A: three buffers of one 16-row block, shared by the waves, two blocks
+    // ahead.  Basis: per wave, three slots of one block (16 rows x the wave's
+    // 32 amplitudes), three blocks ahead.
+    __shared__ __attribute__((aligned(16))) _Float16 s_a[3 * BLK];
+    __shared__ __attribute__((aligned(16))) float s_b[NW][3][16 * 32];
+    __shared__ uint16_t s_rc[kC2MaxRows];
     __shared__ float* s_out[NSP];
-    __shared__ double s_n[8][NSP];
+    __shared__ double s_n[NW][NSP];
     const uint32_t tile = blockIdx.x / nparts, part = blockIdx.x % nparts;
     if (tile >= ntiles) return;
     const Combine2Tile& T = tiles[tile];
@@ -1766,42 +1789,54 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int col = lane & 31, half = lane >> 5;
     const uint64_t N = 1ull << n;
-    for (int i = t; i < nq; i += 512) {
+    // row codes: 0 for |+>, k for V_k (k < 64), 64 + k (kc2 - 1) + l - 1 for
+    // W_lk (below 4096 with kc1, kc2 <= 64)
+    for (int i = t; i < nq; i += NT) {
         const uint32_t e = T.qtab[i];
         const int l = (int)(e & 255u), k = (int)((e >> 8) & 255u);
-        const float* row = l == 0 ? (k == 0 ? T.plus : T.V + (uint64_t)(k - 1) * N)
-                                  : T.W + ((uint64_t)k * (kc2 - 1) + (l - 1)) * N;
-        s_rp[i] = (uint64_t)row;
+        s_rc[i] = (uint16_t)(l == 0 ? k : 64 + k * (kc2 - 1) + (l - 1));
     }
-    for (int i = t; i < NSP; i += 512) s_out[i] = i < ns ? items[T.item_off + i].out : nullptr;
-    for (int i = t; i < 8 * NSP; i += 512) (&s_n[0][0])[i] = 0.0;
+    for (int i = t; i < NSP; i += NT) s_out[i] = i < ns ? items[T.item_off + i].out : nullptr;
+    for (int i = t; i < NW * NSP; i += NT) (&s_n[0][0])[i] = 0.0;
     __syncthreads();
     const _Float16* At = A + T.a_off;
     const uint64_t C = N / nparts;
     const uint64_t base = (uint64_t)part * C;
     const float osc = T.oscale;
     const uint32_t s_a_lds = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) _Float16*)s_a;
-    // blocks [b0, b0 + nblk) of A (nblk <= 2) into LDS buffer bf, in slots of
-    // 1 KiB (64 lanes x 16 B, one instruction), slot s by wave s % 8
-    constexpr int SPB = BLK * 2 / 1024; // slots per block
-    static_assert(SPB * 1024 == BLK * 2, "a block is a whole number of slots");
-    auto copy_a = [&](int bf, int b0, int nblk) {
-        const int nsl = nblk * SPB;
-        for (int slot = w; slot < nsl; slot += 8) { // uniform per wave
-            const int4* src = reinterpret_cast<const int4*>(At + (size_t)b0 * BLK) + slot * 64 + lane;
-            const uint32_t dst = s_a_lds + (uint32_t)(bf * 2 * BLK * 2 + slot * 1024);
-            lds_copy16(src, __builtin_amdgcn_readfirstlane(dst));
+    const uint32_t s_b_lds = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) float*)&s_b[w][0][0];
+    // the tile's row bases in registers (the copies' memory clobbers would
+    // otherwise reload them)
+    const float* const rW = T.W;
+    const float* const rV = T.V;
+    const float* const rP = T.plus;
+    auto row_ptr = [&](uint32_t code) -> const float* {
+        const bool isw = code >= 64u;
+        const float* const rb = isw ? rW : (code ? rV : rP);
+        const uint32_t idx = isw ? code - 64u : (code ? code - 1u : 0u);
+        return rb + (uint64_t)idx * N;
+    };
+    // block b of A into buffer bf: slot s (1 KiB) by wave s, so this wave
+    // issues ca (0 or 1) instructions per block
+    const int ca = w < SPB ? 1 : 0;
+    auto copy_a = [&](int bf, int b) {
+        if (ca) { // uniform per wave
+            const int4* src = reinterpret_cast<const int4*>(At + (size_t)b * BLK) + w * 64 + lane;
+            lds_copy16(src, __builtin_amdgcn_readfirstlane(s_a_lds + (uint32_t)(bf * BLK * 2 + w * 1024)));
         }
     };
-    for (uint64_t a0 = 0; a0 < C; a0 += 256) {
-        const uint64_t a = base + a0 + (uint64_t)(w * 32 + col);
-        const uint64_t aoff = a * sizeof(float);
-        // this lane's rows 8 half + i of the 16-row block starting at row q
-        auto load_b = [&](float (&bv)[8], int q) {
+    // rows [q, q + 16) of this wave's 32 amplitudes from aw into its slot sl:
+    // two instructions of 8 rows, 8 lanes (128 B) per row
+    auto copy_b = [&](int sl, int q, uint64_t aw) {
 #pragma unroll
-            for (int i = 0; i < 8; i++)
-                bv[i] = *reinterpret_cast<const MQW_GLOBAL float*>(s_rp[q + 8 * half + i] + aoff);
-        };
+        for (int h = 0; h < 2; h++) {
+            const float* src = row_ptr(s_rc[q + 8 * h + (lane >> 3)]) + aw + (uint64_t)((lane & 7) * 4);
+            lds_copy16(src, __builtin_amdgcn_readfirstlane(s_b_lds + (uint32_t)(sl * 2048 + h * 1024)));
+        }
+    };
+    for (uint64_t a0 = 0; a0 < C; a0 += NW * 32) {
+        const uint64_t aw = base + a0 + (uint64_t)(w * 32);
+        const uint64_t a = aw + (uint64_t)col;
 #pragma unroll 1
         for (int cls = 0; cls < 2; cls++) {
             const int qb = cls ? q0 : 0, qe = cls ? nq : q0;
@@ -1813,64 +1848,59 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
                 for (int r = 0; r < 16; r++) acc[mb][r] = 0.0f;
             if (nb > 0) {
                 const int bb = qb / kC2RowPad;
-                const int nch = (nb + 1) / 2;
-                // Two blocks (one LDS chunk) per iteration, double-buffered:
-                // chunk c + 1's copy is issued right after the barrier that
-                // opens chunk c, and exactly 16 basis loads (two blocks, the
-                // prefetch) follow every copy, so waiting until at most 16
-                // loads are in flight completes the copy without draining the
-                // prefetch.  Past the last block the prefetch reloads it.
-                // Every wave is done with both buffers (its LDS reads waited
-                // for), and no output store of the previous class is in
-                // flight, before the first copy.
+                // This wave's copies, in issue order: A(0), A(1), B(0), B(1),
+                // B(2), then per block b: A(b + 2), B(b + 3), each only if
+                // that block exists (A: ca instructions, B: two).  So A(b) is
+                // issued after B(b) for b >= 2, and at block b the copies
+                // issued after both are B(b + 1), A(b + 1), B(b + 2) (only
+                // B(1), B(2) at block 0): that many left in flight means A(b)
+                // and B(b) are in LDS (vector loads complete in order).
+                // First, every store of the previous class is complete and
+                // every wave is done with the buffers.
                 __builtin_amdgcn_sched_barrier(0);
-                wait_vm_lgkm<0, 0>();
+                wait_vm_lgkm0<0>();
                 __builtin_amdgcn_s_barrier();
                 asm volatile("" ::: "memory");
                 __builtin_amdgcn_sched_barrier(0);
-                copy_a(0, bb, nb < 2 ? nb : 2);
-                __builtin_amdgcn_sched_barrier(0);
-                float bv[2][8];
-                load_b(bv[0], qb);
-                load_b(bv[1], qb + (nb > 1 ? 16 : 0));
+                copy_a(0, bb);
+                if (nb > 1) copy_a(1, bb + 1);
+                copy_b(0, qb, aw);
+                if (nb > 1) copy_b(1, qb + 16, aw);
+                if (nb > 2) copy_b(2, qb + 32, aw);
 #pragma unroll 1
-                for (int c = 0; c < nch; c++) {
-                    // chunk c is in buffer c & 1 once this wave's copy is done
-                    // (at most the 16 later basis loads in flight) and every
-                    // wave has passed the barrier; this wave's LDS reads of
-                    // chunk c - 1 are complete before buffer (c + 1) & 1 is
-                    // overwritten
+                for (int b = 0; b < nb; b++) {
+                    // after the barrier A(b) of every wave is in LDS, and
+                    // every wave's reads of A(b - 1) are complete, so its
+                    // buffer takes A(b + 2)
+                    const int left = b == 0 ? 2 * (nb > 1) + 2 * (nb > 2)
+                                            : (2 + ca) * (b + 1 < nb) + 2 * (b + 2 < nb);
                     __builtin_amdgcn_sched_barrier(0);
-                    wait_vm_lgkm<16, 0>();
+                    wait_vm_lgkm0(left);
                     __builtin_amdgcn_s_barrier();
                     asm volatile("" ::: "memory");
                     __builtin_amdgcn_sched_barrier(0);
-                    if (c + 1 < nch) copy_a((c + 1) & 1, bb + 2 * c + 2, nb - 2 * c - 2 < 2 ? nb - 2 * c - 2 : 2);
-                    __builtin_amdgcn_sched_barrier(0);
+                    const int sl = b % 3;
+                    if (b + 2 < nb) copy_a((b + 2) % 3, bb + b + 2);
+                    const float* sb = &s_b[w][sl][0];
+                    mqw_h8 bh, bl;
 #pragma unroll
-                    for (int p = 0; p < 2; p++) {
-                        const int b = 2 * c + p;
-                        mqw_h8 bh, bl;
+                    for (int i = 0; i < 8; i++) {
+                        const float xs = sb[(8 * half + i) * 32 + col] * kWScale;
+                        const _Float16 h = (_Float16)xs;
+                        bh[i] = h;
+                        bl[i] = (_Float16)(xs - (float)h);
+                    }
+                    // the slot's reads above complete before the copy
+                    if (b + 3 < nb) copy_b(sl, qb + (b + 3) * 16, aw);
+                    const _Float16* Ab = s_a + sl * BLK;
 #pragma unroll
-                        for (int i = 0; i < 8; i++) {
-                            const float xs = bv[p][i] * kWScale;
-                            const _Float16 h = (_Float16)xs;
-                            bh[i] = h;
-                            bl[i] = (_Float16)(xs - (float)h);
-                        }
-                        load_b(bv[p], qb + (b + 2 < nb ? b + 2 : nb - 1) * 16);
-                        if (b < nb) { // uniform: only an odd last chunk's second block is absent
-                            const _Float16* Ab = s_a + ((c & 1) * 2 + p) * BLK;
-#pragma unroll
-                            for (int mb = 0; mb < MB; mb++) {
-                                const int j = mb * 32 + col;
-                                const mqw_h8 ah = *reinterpret_cast<const mqw_h8*>(Ab + j * 16 + 8 * half);
-                                const mqw_h8 al = *reinterpret_cast<const mqw_h8*>(Ab + (NSP + j) * 16 + 8 * half);
-                                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[mb], 0, 0, 0);
-                                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[mb], 0, 0, 0);
-                                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[mb], 0, 0, 0);
-                            }
-                        }
+                    for (int mb = 0; mb < MB; mb++) {
+                        const int j = mb * 32 + col;
+                        const mqw_h8 ah = *reinterpret_cast<const mqw_h8*>(Ab + j * 16 + 8 * half);
+                        const mqw_h8 al = *reinterpret_cast<const mqw_h8*>(Ab + (NSP + j) * 16 + 8 * half);
+                        acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[mb], 0, 0, 0);
+                        acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[mb], 0, 0, 0);
+                        acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[mb], 0, 0, 0);
                     }
                 }
             }
@@ -1895,7 +1925,7 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
     __syncthreads();
     if (t < ns) {
         double sum = 0.0;
-        for (int q = 0; q < 8; q++) sum += s_n[q][t];
+        for (int q = 0; q < NW; q++) sum += s_n[q][t];
         items[T.item_off + t].norm_part[part] = sum;
     }
 }
@@ -1905,7 +1935,7 @@ int launch_combine2h(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combi
 {
     if (ntiles == 0) return 0;
     const uint64_t N = 1ull << n;
-    if (nparts == 0 || N % nparts != 0 || (N / nparts) % 256 != 0) return -2;
+    if (nparts == 0 || N % nparts != 0 || (N / nparts) % 512 != 0) return -2;
     if (ns_max < 1 || ns_max > (uint32_t)kC2TileH || !A_dev) return -2;
     const uint64_t grid = (uint64_t)ntiles * nparts;
     if (grid > 0x7fffffffull || ntiles > 65535u) return -3;
@@ -1914,15 +1944,15 @@ int launch_combine2h(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combi
     _Float16* A = reinterpret_cast<_Float16*>(A_dev);
     hipLaunchKernelGGL(combine2_prep_kernel, dim3(64, ntiles), dim3(256), 0, st, tiles_dev, items_dev, coefs_dev, A,
                        nsp);
+    // 16 waves, 512 amplitudes per pass (one workgroup per CU: 145 KB of LDS)
+    const dim3 g((uint32_t)grid), b(1024);
+    const _Float16* Ac = A;
     if (nsp == 32)
-        hipLaunchKernelGGL(combine2h_kernel<1>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
-                           (const _Float16*)A, n, nparts);
+        hipLaunchKernelGGL((combine2h_kernel<1, 16>), g, b, 0, st, tiles_dev, ntiles, items_dev, Ac, n, nparts);
     else if (nsp == 64)
-        hipLaunchKernelGGL(combine2h_kernel<2>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
-                           (const _Float16*)A, n, nparts);
+        hipLaunchKernelGGL((combine2h_kernel<2, 16>), g, b, 0, st, tiles_dev, ntiles, items_dev, Ac, n, nparts);
     else
-        hipLaunchKernelGGL(combine2h_kernel<4>, dim3((uint32_t)grid), dim3(512), 0, st, tiles_dev, ntiles, items_dev,
-                           (const _Float16*)A, n, nparts);
+        hipLaunchKernelGGL((combine2h_kernel<4, 16>), g, b, 0, st, tiles_dev, ntiles, items_dev, Ac, n, nparts);
     return check_launch("combine2h");
 }
 
