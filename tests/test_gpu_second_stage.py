@@ -153,3 +153,34 @@ def test_split_gemm_near_the_basis_limit(ctx, oracle):
     assert np.max(np.abs(a - c)) < PATH_TOL
     # fixed times: every sample is the same walk
     assert np.max(np.abs(a - a[0, 0])) < 1e-7
+
+
+def test_split_gemm_smallest_bases(ctx, oracle):
+    """Second-stage bases of 5 x 5, 6 x 6 and 8 x 10 terms in one batch: one, two and three 16-row
+    blocks per class, the shortest runs of the split GEMM's copy pipeline (its first copies and
+    waits differ below four blocks)."""
+    n, m, samples = 14, 3, 64
+    _say("split gemm, 1-3 blocks per class")
+    gam = np.array([0.8, 0.5, 0.2], np.float32)
+    inst, blocks = [], []
+    for i, (s1, s2) in enumerate([(0.2, 0.2), (0.5, 0.5), (1.0, 2.0)]):
+        rec = M.generate_sk(n, 1, 6700 + i)[0]
+        J, h = oracle.unpack(n, rec)
+        _, _, info = oracle.energy_compat(n, J, h)
+        onenorm = (info["emax"] - info["e0"]) / 2 + np.abs(gam.astype(np.float64)) * n
+        st = np.array([s1 / onenorm[0], s2 / onenorm[1], 0.5], np.float32)
+        k1 = len(oracle.cheb_coeffs(float(np.float32(onenorm[0]) * st[0])))
+        k2 = len(oracle.cheb_coeffs(float(np.float32(onenorm[1]) * st[1])))
+        real_rows = sum(1 for l in range(k2) for k in range(k1) if (l & 1) == (k & 1))
+        blocks.append(((real_rows + 15) // 16, (k1 * k2 - real_rows + 15) // 16))
+        inst.append((n, m, rec, gam, st))
+    assert blocks == [(1, 1), (2, 2), (3, 3)], blocks
+    _, _, a = ctx.solve_batch(inst, samples=samples, with_samples=True, small_n_max=0, fixed_times=True)
+    assert ctx.last_stats()["gemm_launches"] > 0
+    _, _, b = ctx.solve_batch(inst, samples=samples, with_samples=True, small_n_max=0, fixed_times=True,
+                              gemm="fp32")
+    _, _, c = ctx.solve_batch(inst, samples=samples, with_samples=True, small_n_max=0, fixed_times=True,
+                              shared_second_stage=False)
+    assert ctx.last_stats()["gemm_launches"] == 0
+    assert np.max(np.abs(a - b)) < PATH_TOL
+    assert np.max(np.abs(a - c)) < PATH_TOL
