@@ -1683,7 +1683,8 @@ int launch_combine2(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combin
 // v_mfma_f32_32x32x16_f16; the dropped Alo Blo is below 2^-22 of each product.
 //
 // combine2_prep_kernel writes each tile's A = s d_jl c_jk (scaled) once, as
-// 16-row blocks [block][hi, lo][NSP samples][16 rows] of FP16, the layout the
+// 16-row blocks [block][hi, lo][rows 0-7, rows 8-15][NSP samples][8 rows] of
+// FP16 (so the lanes of a wave read consecutive 16-byte pieces), the layout the
 // main kernel copies into LDS.  combine2h_kernel: a workgroup of NW = 16 waves
 // takes one chunk of N / nparts amplitudes for up to NSP = 32 MB samples; per
 // 512 amplitudes, wave w owns 32 of them and MB accumulator tiles of 32
@@ -1719,8 +1720,9 @@ combine2_prep_kernel(const Combine2Tile* __restrict__ tiles, const Combine2Item*
         const _Float16 hi = (_Float16)v;
         const _Float16 lo = (_Float16)(v - (float)hi);
         const uint32_t b = r >> 4, rr = r & 15u;
-        At[((size_t)(2u * b) * nsp + j) * 16u + rr] = hi;
-        At[((size_t)(2u * b + 1u) * nsp + j) * 16u + rr] = lo;
+        const size_t pos = ((size_t)(rr >> 3) * nsp + j) * 8u + (rr & 7u); // within one part of the block
+        At[(size_t)(2u * b) * nsp * 16u + pos] = hi;
+        At[(size_t)(2u * b + 1u) * nsp * 16u + pos] = lo;
     }
 }
 
@@ -1896,8 +1898,8 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
 #pragma unroll
                     for (int mb = 0; mb < MB; mb++) {
                         const int j = mb * 32 + col;
-                        const mqw_h8 ah = *reinterpret_cast<const mqw_h8*>(Ab + j * 16 + 8 * half);
-                        const mqw_h8 al = *reinterpret_cast<const mqw_h8*>(Ab + (NSP + j) * 16 + 8 * half);
+                        const mqw_h8 ah = *reinterpret_cast<const mqw_h8*>(Ab + (half * NSP + j) * 8);
+                        const mqw_h8 al = *reinterpret_cast<const mqw_h8*>(Ab + NSP * 16 + (half * NSP + j) * 8);
                         acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[mb], 0, 0, 0);
                         acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[mb], 0, 0, 0);
                         acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[mb], 0, 0, 0);
