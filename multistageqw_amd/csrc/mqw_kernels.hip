@@ -2105,8 +2105,11 @@ readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const R
         const int k = 2 * (16 * (cb % KB) + rr) + cb / KB;
         const float v = (j < ns && k < kx) ? coefs[items[T.item_off + j].coef_off + k] * T.ascale : 0.0f;
         const _Float16 hi = (_Float16)v;
-        s_b[cb * BLK + j * 16 + rr] = hi;
-        s_b[cb * BLK + (NSP + j) * 16 + rr] = (_Float16)(v - (float)hi);
+        // [block][hi, lo][rows 0-7, rows 8-15][NSP samples][8 rows]: a wave's
+        // lanes read consecutive 16-byte pieces (no LDS bank conflicts)
+        const int pos = ((rr >> 3) * NSP + j) * 8 + (rr & 7);
+        s_b[cb * BLK + pos] = hi;
+        s_b[cb * BLK + NSP * 16 + pos] = (_Float16)(v - (float)hi);
     }
     for (int i = t; i < NSP; i += 512) s_psi[i] = i < ns ? items[T.item_off + i].psi : nullptr;
     for (int i = t; i < 8 * NSP * 2; i += 512) (&s_w[0][0][0])[i] = 0.0;
@@ -2166,8 +2169,8 @@ readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const R
 #pragma unroll
                 for (int mb = 0; mb < MB; mb++) {
                     const int j = mb * 32 + col;
-                    const mqw_h8 bh = *reinterpret_cast<const mqw_h8*>(Bb + j * 16 + 8 * half);
-                    const mqw_h8 bl = *reinterpret_cast<const mqw_h8*>(Bb + (NSP + j) * 16 + 8 * half);
+                    const mqw_h8 bh = *reinterpret_cast<const mqw_h8*>(Bb + (half * NSP + j) * 8);
+                    const mqw_h8 bl = *reinterpret_cast<const mqw_h8*>(Bb + NSP * 16 + (half * NSP + j) * 8);
                     if (cls == 0) {
                         acc[0][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bh, acc[0][mb], 0, 0, 0);
                         acc[0][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bl, acc[0][mb], 0, 0, 0);
