@@ -43,6 +43,97 @@ MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 MFMA_FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 matrix
 
 
+def _host_cpu() -> dict:
+    """CPU model, sockets and logical CPUs of the host (from /proc/cpuinfo)."""
+    model, sockets = "unknown", set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model == "unknown":
+                model = line.split(":", 1)[1].strip()
+            elif line.startswith("physical id"):
+                sockets.add(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return dict(cpu_model=model, sockets=len(sockets) or None, host_logical_cpus=os.cpu_count())
+
+
+def cpu_baseline(args, n, m, recs) -> dict:
+    """The FP32 restatement of MultiQW.cpp (oracle/mqw_oracle.c) built as the reference
+    is built (gcc -O3 -march=native, on this host), timed on this host's CPUs: whole
+    instances at all 100 samples across every CPU this process may use (capped at 16, the
+    host's CPU share per GPU), and one core on a few samples of one instance."""
+    import tempfile
+    from oracle import oracle as O
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = args.cpu_threads or min(16, avail)
+    with tempfile.TemporaryDirectory() as d:
+        L = O.load_native(O.build_native(d))
+        cnt = args.cpu_instances
+        tc = time.perf_counter()
+        O.solve_native(L, n, m, recs[:cnt], samples=100, threads=threads)
+        wc = time.perf_counter() - tc
+        t1 = time.perf_counter()
+        O.solve_native(L, n, m, recs[:1], samples=args.cpu_core_samples, threads=1)
+        w1 = time.perf_counter() - t1
+    one_core = args.cpu_core_samples / 100.0 / w1
+    return dict(value=cnt / wc, unit="instances/s", cores=threads, kind="port",
+                sample=f"{cnt} whole n={n} m={m} instances (100 samples each) in {wc:.1f} s on {threads} threads "
+                       f"(samples of an instance split across threads); FP32 restatement of MultiQW.cpp built "
+                       f"gcc -O3 -march=native on this host",
+                one_core=dict(value=one_core, unit="instances/s",
+                              sample=f"{args.cpu_core_samples} of 100 samples of one instance on 1 thread "
+                                     f"({w1:.1f} s), scaled to 100 samples"),
+                cpus_available=avail, **_host_cpu())
+
+
+TUNE_CACHE = os.path.join(ROOT, "multistageqw_amd", "tune", "gfx950.txt")
+
+
+def uneven_leg(ctx, args) -> dict:
+    """The uneven workload: instances with n ~ U{12..22}, m ~ U{1..5} (seed 3, the first
+    args.uneven of the 4096-instance set), 100 samples, solved in chunks of 256 instances per
+    mqw_solve_batch call with the autotuner's configuration for every n (read from the shipped
+    tuning table; measured where it has no line).  Every (uneven // uneven_check)-th instance is
+    also run at 8 samples in FP32 and with an all-FP64 state, for the per-sample error."""
+    import multistageqw_amd as M
+    rng = np.random.default_rng(3)
+    inst = []
+    for k in range(args.uneven):
+        nn = int(rng.integers(12, 23))
+        mm = int(rng.integers(1, 6))
+        inst.append((nn, mm, M.generate_sk(nn, 1, 3 * 100003 + k)[0]))
+    ns = sorted({i[0] for i in inst})
+    tt0 = time.perf_counter()
+    tuned = {nn: ctx.tune(nn, TUNE_CACHE) for nn in ns}
+    tune_s = time.perf_counter() - tt0
+    keys = ("gpu_ms", "sweep_ms", "algorithmic_bytes", "aux_bytes", "basis_ms", "gemm_ms", "lds_trajectories")
+    tot = {k: 0.0 for k in keys}
+    ctx.solve_batch(inst[:8], samples=100)  # warm the workspace
+    wall = 0.0
+    for c0 in range(0, len(inst), 256):
+        t0 = time.perf_counter()
+        ctx.solve_batch(inst[c0:c0 + 256], samples=100)
+        wall += time.perf_counter() - t0
+        st = ctx.last_stats()
+        for k in keys:
+            tot[k] += st[k]
+    out = dict(workload=f"{len(inst)} instances, n in 12..22, m in 1..5, seed 3, 100 samples, 256 per call",
+               instances_per_s=len(inst) / wall, wall_s=wall,
+               gbs=(tot["algorithmic_bytes"] + tot["aux_bytes"]) / wall / 1e9,
+               term_sweep_gbs=tot["algorithmic_bytes"] / (tot["sweep_ms"] * 1e-3) / 1e9 if tot["sweep_ms"] else 0.0,
+               device_ms=tot["gpu_ms"], term_sweep_ms=tot["sweep_ms"], basis_ms=tot["basis_ms"],
+               gemm_ms=tot["gemm_ms"], lds_trajectories=int(tot["lds_trajectories"]), tune_s=tune_s,
+               tuned={str(k): v for k, v in tuned.items()})
+    if args.uneven_check > 0:
+        sub = inst[::max(1, len(inst) // args.uneven_check)][:args.uneven_check]
+        _, _, s32 = ctx.solve_batch(sub, samples=8, with_samples=True)
+        _, _, s64 = ctx.solve_batch(sub, samples=8, with_samples=True, precision=M.FP64)
+        d = np.abs(s32 - s64)
+        out.update(fp64_check=dict(instances=len(sub), samples=8, max_abs_err=float(d.max()),
+                                   mean_abs_err=float(d.mean())))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,8 +145,14 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--apply-n", type=int, default=24, help="n of the apply-rate measurement (0 = skip)")
     ap.add_argument("--apply-samples", type=int, default=8)
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores); -1 = skip")
-    ap.add_argument("--cpu-samples", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0 = every CPU this process may use, at most 16 (the host's share per GPU); -1 = skip")
+    ap.add_argument("--cpu-instances", type=int, default=2, help="whole instances (100 samples) the CPU times")
+    ap.add_argument("--cpu-core-samples", type=int, default=4, help="samples of the 1-core measurement")
+    ap.add_argument("--uneven", type=int, default=512,
+                    help="instances of the uneven workload (n in 12..22, m in 1..5, seed 3) timed after the "
+                         "headline steps (0 = skip)")
+    ap.add_argument("--uneven-check", type=int, default=16, help="of those, instances also run in FP64")
     ap.add_argument("--gemm", default="split_fp16", choices=("split_fp16", "fp32"),
                     help="arithmetic of the second-stage GEMM")
     ap.add_argument("--overlap", type=int, default=1, help="overlap batches of shared bases (0/1)")
@@ -186,18 +283,11 @@ def main() -> int:
         apply = dict(n=na, m=m, samples=args.apply_samples, applies_per_s=applies, gbs=gbs,
                      frac=gbs / HBM_PEAK_GBS, device_ms=sa["gpu_ms"], aux_ms=sa["aux_ms"], wall_s=wa)
 
+    uneven = uneven_leg(ctx, args) if args.uneven > 0 else None
+
     cpu = None
     if args.cpu_threads >= 0:
-        from oracle import oracle as O
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        O.lib()
-        tc = time.perf_counter()
-        O.solve(n, m, recs[:threads], samples=args.cpu_samples, threads=threads)
-        wc = time.perf_counter() - tc
-        cpu = dict(value=threads * args.cpu_samples / 100.0 / wc, unit="instances/s", cores=threads, kind="port",
-                   sample=f"{threads} of the n={n} m={m} instances x {args.cpu_samples} of 100 samples each "
-                          f"(FP32 restatement of MultiQW.cpp, one problem per thread, {wc:.1f} s), "
-                          "scaled linearly to 100 samples")
+        cpu = cpu_baseline(args, n, m, recs)
 
     line = {
         "metric": "batched instances/s at n=20,m=3 (+ H|psi> applies/s and HBM GB/s at n=24)",
@@ -220,6 +310,7 @@ def main() -> int:
         "roofline_isolated": roofline_iso,
         "cpu_baseline": cpu,
         "apply_rate": apply,
+        "uneven": uneven,
         "mean_prob": float(np.mean(np.concatenate(probs))) if probs else None,
     }
     print(json.dumps(line), flush=True)

@@ -703,7 +703,7 @@ void combine2_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_
             const double rows = (double)(ct.q0 + ct.q1);
             const uint32_t ns = ct.ns;
             if (B.split) {
-                ct.a_off = (uint32_t)P.c2a_halves;
+                ct.a_off = (uint64_t)P.c2a_halves;
                 P.c2a_halves += 2 * (size_t)(ct.q0 + ct.q1) * nsp;
                 P.gemm_flops += 3.0 * 2.0 * rows * (double)nsp * (double)N;
             } else {
@@ -780,6 +780,12 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
     // pending: launch only; the caller synchronises and collects the timings.
     // stream / tag: the stream and the name prefix of the device job buffers
     const hipStream_t stm = stream ? stream : c->stream;
+    // the combine, second-stage and readout tables index their coefficients
+    // and items with 32-bit offsets
+    const size_t lim = 0xffffffffull;
+    if (P.ccoef.size() > lim || P.citems.size() > lim || P.c2items.size() > lim || P.ritems.size() > lim)
+        return fail(c, MQW_EINVAL, "plan tables exceed 2^32 entries (%zu coefficients); solve fewer problems per call",
+                    P.ccoef.size());
     TermJob<real>* jd =
         (TermJob<real>*)dev_get(c, tag + "_jobs", sizeof(TermJob<real>) * std::max<size_t>(1, P.jobs.size()));
     ReduceJob* rd = (ReduceJob*)dev_get(c, tag + "_rjobs", sizeof(ReduceJob) * std::max<size_t>(1, P.rjobs.size()));
@@ -1576,10 +1582,10 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     return MQW_OK;
 }
 
-mqw_status solve_problems(mqw_ctx* c, std::vector<Problem>& probs_in, const mqw_opts& o, float* prob,
-                          float* total_t)
+// Options are checked before any problem is prepared (prepare_problem sizes
+// the sample tables from o.samples).
+mqw_status check_opts(mqw_ctx* c, const mqw_opts& o)
 {
-    c->stats = mqw_stats{};
     if (o.samples < 1) return fail(c, MQW_EINVAL, "samples must be >= 1 (got %d)", o.samples);
     if (!(o.tol > 0.0)) return fail(c, MQW_EINVAL, "tol must be > 0");
     if (o.precision != MQW_FP32 && o.precision != MQW_FP64) return fail(c, MQW_EINVAL, "bad precision");
@@ -1597,6 +1603,14 @@ mqw_status solve_problems(mqw_ctx* c, std::vector<Problem>& probs_in, const mqw_
         for (int q = 0; q < o.mixer_count; q++)
             if (__builtin_popcount(o.mixer_masks[q]) != 2)
                 return fail(c, MQW_EINVAL, "XY mixer mask %d (0x%x) must have exactly two bits", q, o.mixer_masks[q]);
+    return MQW_OK;
+}
+
+mqw_status solve_problems(mqw_ctx* c, std::vector<Problem>& probs_in, const mqw_opts& o, float* prob,
+                          float* total_t)
+{
+    c->stats = mqw_stats{};
+    if (mqw_status st = check_opts(c, o)) return st;
     std::map<int, std::vector<Problem*>> groups;
     size_t slots = 0;
     for (Problem& p : probs_in) {
@@ -2043,6 +2057,7 @@ mqw_status mqw_solve(mqw_ctx* c, int n, int m, const double* recs, int count, co
         o = *opts;
     else
         mqw_opts_default(&o);
+    if (mqw_status st = check_opts(c, o)) return st;
     hipSetDevice(c->device);
     const int P = n * (n + 1) / 2;
     std::vector<Problem> ps((size_t)count);
@@ -2065,6 +2080,7 @@ mqw_status mqw_solve_batch(mqw_ctx* c, const mqw_instance* inst, int count, cons
         o = *opts;
     else
         mqw_opts_default(&o);
+    if (mqw_status st = check_opts(c, o)) return st;
     hipSetDevice(c->device);
     std::vector<Problem> ps((size_t)count);
     for (int k = 0; k < count; k++) {

@@ -149,10 +149,10 @@ struct Combine2Tile {
     uint32_t item_off, ns;
     // split-FP16 kernel: the tile's coefficient matrix (written by the prep
     // launch) at A + a_off halves, scaled by ascale (a power of two putting
-    // its largest entry near 2^14); outputs are scaled back by oscale
-    uint32_t a_off;
+    // its largest entry near 2^14); outputs are scaled back by oscale.  The
+    // offset is 64-bit: a plan of many tiles passes 2^32 halves.
+    uint64_t a_off;
     float ascale, oscale;
-    uint32_t pad;
 };
 int launch_combine2(const Combine2Tile* tiles_dev, uint32_t ntiles, const Combine2Item* items_dev,
                     const float* coefs_dev, int n, uint32_t nparts, uint32_t ns_max, void* stream);

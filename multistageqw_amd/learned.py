@@ -106,14 +106,42 @@ def apply_offsets(gamma_h, t_h, offsets) -> Tuple[np.ndarray, np.ndarray]:
     g = np.asarray(gamma_h, np.float64) * np.exp(d[:, 0])
     t = np.asarray(t_h, np.float64) * np.exp(d[:, 1])
     t *= float(np.sum(np.asarray(t_h, np.float64))) / float(np.sum(t))
-    return g.astype(np.float32), t.astype(np.float32)
+    t32 = t.astype(np.float32)
+    # The simulator's total time is the sequential float32 sum of the stage
+    # times (as the heuristic's total_t, MultiQW.cpp:420-423).  Rounding to
+    # float32 can leave that sum an ulp or two away from the heuristic's, so
+    # one stage, the largest that can, absorbs the difference and the totals
+    # are equal.  Ties in the last addition can skip a total when one stage
+    # moves, so the stages are tried in turn, by up to 8 ulps either way.
+    want = _f32_total(t_h)
+    if _f32_total(t32) != want:
+        for i in np.argsort(-t32):
+            base = t32[i]
+            for k in range(1, 17):  # +1, -1, +2, -2, ... ulps of stage i
+                step = (k + 1) // 2 * (1 if k % 2 else -1)
+                v = base
+                for _ in range(abs(step)):
+                    v = np.nextafter(v, np.float32(np.inf if step > 0 else 0.0), dtype=np.float32)
+                t32[i] = v
+                if _f32_total(t32) == want:
+                    return g.astype(np.float32), t32
+            t32[i] = base
+    return g.astype(np.float32), t32
 
 
-def score(ctx: Context, instances, schedules, samples: int) -> np.ndarray:
-    """Mean ground-state probability of each (instance, schedule) on the simulator."""
+def _f32_total(t) -> np.float32:
+    s = np.float32(0.0)
+    for x in np.asarray(t, np.float32):
+        s = np.float32(s + x)
+    return s
+
+
+def score(ctx: Context, instances, schedules, samples: int, with_total_t: bool = False):
+    """Mean ground-state probability of each (instance, schedule) on the simulator
+    (and, with with_total_t, the total evolution time the simulator ran)."""
     batch = [(n, m, rec, g, t) for (n, m, rec), (g, t) in zip(instances, schedules)]
-    p, _ = ctx.solve_batch(batch, samples=samples)
-    return p
+    p, tt = ctx.solve_batch(batch, samples=samples)
+    return (p, tt) if with_total_t else p
 
 
 @dataclass
@@ -183,6 +211,7 @@ def evaluate(ctx: Context, mlp: MLP, count: int, seed: int, n: int = 20, m: int 
              chunk: int = 64, log=None) -> dict:
     """Mean ground-state probability: heuristic vs MLP proposal vs proposal-with-fallback."""
     ph, pm = [], []
+    equal_time = True
     t0 = time.time()
     for c0 in range(0, count, chunk):
         cnt = min(chunk, count - c0)
@@ -190,8 +219,10 @@ def evaluate(ctx: Context, mlp: MLP, count: int, seed: int, n: int = 20, m: int 
         heur = [schedule_heuristic(nn, mm, rec) for (nn, mm, rec) in inst]
         Y = mlp.forward(features(ctx, inst))
         prop = [apply_offsets(h.gamma, h.short_t, Y[k]) for k, h in enumerate(heur)]
-        p_h = score(ctx, inst, [(h.gamma, h.short_t) for h in heur], samples)
-        p_m = score(ctx, inst, prop, samples)
+        p_h, tt_h = score(ctx, inst, [(h.gamma, h.short_t) for h in heur], samples, with_total_t=True)
+        p_m, tt_m = score(ctx, inst, prop, samples, with_total_t=True)
+        equal_time &= bool(np.array_equal(tt_h, tt_m)) and bool(
+            np.array_equal(tt_h, np.array([h.total_t for h in heur], np.float32)))
         ph.append(p_h)
         pm.append(p_m)
         if log:
@@ -201,7 +232,8 @@ def evaluate(ctx: Context, mlp: MLP, count: int, seed: int, n: int = 20, m: int 
     pf = np.maximum(ph, pm)
     return dict(count=count, n=n, m=m, samples=samples, p_heuristic=float(ph.mean()), p_mlp=float(pm.mean()),
                 p_fallback=float(pf.mean()), gain_fallback=float(pf.mean() - ph.mean()),
-                gain_raw=float(pm.mean() - ph.mean()), win_rate=float(np.mean(pm > ph)), seconds=time.time() - t0)
+                gain_raw=float(pm.mean() - ph.mean()), win_rate=float(np.mean(pm > ph)), equal_time=equal_time,
+                seconds=time.time() - t0)
 
 
 def main(argv=None) -> int:

@@ -508,45 +508,81 @@ void oracle_clenshaw_f64(const double* coeffs, int K, double* psi, const double*
 /* recs: count problems of n(n+1)/2 doubles.  Outputs per problem: the mean  */
 /* ground-state probability and total_t.  samples = 100 in the reference.    */
 /* precision 0: FP32 restatement (reference order); 1: FP64 state.          */
+/*                                                                            */
+/* Work is split in two phases so that threads can share one problem:        */
+/*  1. per problem: unpack, energies, schedule, Monte-Carlo times            */
+/*     (MultiQW.cpp:303-430);                                                 */
+/*  2. per (problem, run of samples): the stage loop (:432-451).  For m > 1  */
+/*     every sample restarts from |+> (:447-451), so samples are independent */
+/*     and runs of them go to different threads; an m == 1 problem chains    */
+/*     its samples (:426-430) and is one run.                                */
+/* The mean over samples is taken after both phases (:453), in the           */
+/* reference's float sum order, so results do not depend on the threads.     */
 /* ------------------------------------------------------------------------ */
 typedef struct {
     int n, m, samples, precision;
+    int norm_f64;           /* FP32 path: 0 = the reference's float norm (Eigen order), 1 = FP64 sum */
     double tol;
     const double* rec;
-    float* prob;
-    float* total_t;
-    float* sample_probs;    /* nullable: samples floats */
     const float* gamma_ov;  /* nullable: m per-stage gammas replacing the heuristic */
     const float* short_t_ov;/* nullable: m per-stage times replacing the heuristic */
-} oracle_job;
+    /* phase 1 results */
+    float* hp;
+    oracle_energy_info info;
+    oracle_schedule sch;
+    float* times;
+    float* sp;              /* samples per-sample probabilities */
+} oracle_problem;
 
-static void solve_one(const oracle_job* jb)
+typedef struct {
+    oracle_problem* pr;
+    int j0, j1;             /* samples [j0, j1) */
+} oracle_run;
+
+static void problem_setup(oracle_problem* pr)
 {
-    const int n = jb->n, m = jb->m, samples = jb->samples;
+    const int n = pr->n, m = pr->m;
     const uint32_t N = 1u << n;
     float J[ORACLE_MAX_N * ORACLE_MAX_N], h[ORACLE_MAX_N];
-    oracle_unpack(n, jb->rec, J, h);
-    float* hp = (float*)malloc(sizeof(float) * N);
-    oracle_energy_info info;
-    oracle_energy_compat(n, J, h, hp, NULL, &info);
-    oracle_schedule sch;
-    oracle_schedule_heuristic(n, m, J, h, &sch);
-    if (jb->gamma_ov && jb->short_t_ov) {
+    oracle_unpack(n, pr->rec, J, h);
+    pr->hp = (float*)malloc(sizeof(float) * N);
+    oracle_energy_compat(n, J, h, pr->hp, NULL, &pr->info);
+    oracle_schedule_heuristic(n, m, J, h, &pr->sch);
+    if (pr->gamma_ov && pr->short_t_ov) {
         float tot = 0;
         for (int i = 0; i < m; i++) {
-            sch.gamma[i] = jb->gamma_ov[i];
-            sch.short_t[i] = jb->short_t_ov[i];
-            tot += sch.short_t[i];
+            pr->sch.gamma[i] = pr->gamma_ov[i];
+            pr->sch.short_t[i] = pr->short_t_ov[i];
+            tot += pr->sch.short_t[i];
         }
-        sch.total_t = tot;
+        pr->sch.total_t = tot;
     }
-    float* times = (float*)malloc(sizeof(float) * (size_t)m * samples);
-    oracle_sample_times(m, samples, sch.short_t, times);
-    float* sp = (float*)malloc(sizeof(float) * samples);
+    pr->times = (float*)malloc(sizeof(float) * (size_t)m * pr->samples);
+    oracle_sample_times(m, pr->samples, pr->sch.short_t, pr->times);
+    pr->sp = (float*)malloc(sizeof(float) * pr->samples);
+}
+
+static void problem_free(oracle_problem* pr)
+{
+    free(pr->hp);
+    free(pr->times);
+    free(pr->sp);
+}
+
+/* Samples [j0, j1) of one problem, from |+> (an m == 1 problem runs all of
+ * its samples here, chaining the state). */
+static void run_samples(const oracle_run* rn)
+{
+    const oracle_problem* pr = rn->pr;
+    const int n = pr->n, m = pr->m, samples = pr->samples;
+    const uint32_t N = 1u << n;
+    const float* hp = pr->hp;
+    const float* times = pr->times;
+    float* sp = pr->sp;
     int cap = 4096;
     double* cd = (double*)malloc(sizeof(double) * cap);
     const float psi0 = (float)(1 / sqrt((double)N));
-    if (jb->precision == 0) {
+    if (pr->precision == 0) {
         float* psi = (float*)malloc(sizeof(float) * 2 * N);
         float* scratch = (float*)malloc(sizeof(float) * 4 * N);
         float* cf = (float*)malloc(sizeof(float) * cap);
@@ -556,20 +592,27 @@ static void solve_one(const oracle_job* jb)
             psi[N + k] = 0.0f;
         }
         int psi_real = 1;
-        for (int j = 0; j < samples; j++) {
+        for (int j = rn->j0; j < rn->j1; j++) {
             for (int i = 0; i < m; i++) {
-                float gamma = sch.gamma[i];
-                float onenorm = fmaf(gamma, (float)(unsigned)n, info.e_abs);
+                float gamma = pr->sch.gamma[i];
+                float onenorm = fmaf(gamma, (float)(unsigned)n, pr->info.e_abs);
                 double scale = (double)(onenorm * times[i * samples + j]);
-                int K = oracle_cheb_coeffs(scale, jb->tol, cd, cap);
+                int K = oracle_cheb_coeffs(scale, pr->tol, cd, cap);
                 for (int k = 0; k < K; k++) cf[k] = (float)cd[k];
                 oracle_clenshaw_f32(cf, K, psi, hp, (unsigned)n, gamma, onenorm, psi_real, scratch);
                 for (uint32_t k = 0; k < 2 * N; k++) sq[k] = psi[k] * psi[k];
-                float nrm = sqrtf(oracle_eigen_sum_f32(sq, 2 * (long)N));
+                float nrm;
+                if (pr->norm_f64) {
+                    double s2 = 0.0;
+                    for (uint32_t k = 0; k < 2 * N; k++) s2 += (double)sq[k];
+                    nrm = (float)sqrt(s2);
+                } else {
+                    nrm = sqrtf(oracle_eigen_sum_f32(sq, 2 * (long)N));
+                }
                 for (uint32_t k = 0; k < 2 * N; k++) psi[k] = psi[k] / nrm;
                 psi_real = 0;
             }
-            float re = psi[info.eloc], im = psi[info.eloc + N];
+            float re = psi[pr->info.eloc], im = psi[pr->info.eloc + N];
             sp[j] = fmaf(re, re, im * im);
             if (m != 1) {
                 for (uint32_t k = 0; k < N; k++) {
@@ -593,19 +636,19 @@ static void solve_one(const oracle_job* jb)
             psi[k] = p0;
             psi[N + k] = 0.0;
         }
-        for (int j = 0; j < samples; j++) {
+        for (int j = rn->j0; j < rn->j1; j++) {
             for (int i = 0; i < m; i++) {
-                double gamma = sch.gamma[i];
-                float onenorm = fmaf(sch.gamma[i], (float)(unsigned)n, info.e_abs);
+                double gamma = pr->sch.gamma[i];
+                float onenorm = fmaf(pr->sch.gamma[i], (float)(unsigned)n, pr->info.e_abs);
                 double scale = (double)(onenorm * times[i * samples + j]);
-                int K = oracle_cheb_coeffs(scale, jb->tol, cd, cap);
+                int K = oracle_cheb_coeffs(scale, pr->tol, cd, cap);
                 oracle_clenshaw_f64(cd, K, psi, hpd, (unsigned)n, gamma, (double)onenorm, scratch);
                 double s2 = 0.0;
                 for (uint32_t k = 0; k < 2 * N; k++) s2 += psi[k] * psi[k];
                 double nrm = sqrt(s2);
                 for (uint32_t k = 0; k < 2 * N; k++) psi[k] /= nrm;
             }
-            double re = psi[info.eloc], im = psi[info.eloc + N];
+            double re = psi[pr->info.eloc], im = psi[pr->info.eloc + N];
             sp[j] = (float)(re * re + im * im);
             if (m != 1)
                 for (uint32_t k = 0; k < N; k++) {
@@ -617,17 +660,13 @@ static void solve_one(const oracle_job* jb)
         free(scratch);
         free(hpd);
     }
-    *jb->prob = oracle_eigen_sum_f32(sp, samples) / samples;
-    *jb->total_t = sch.total_t;
-    if (jb->sample_probs) memcpy(jb->sample_probs, sp, sizeof(float) * samples);
-    free(hp);
-    free(times);
-    free(sp);
     free(cd);
 }
 
 typedef struct {
-    oracle_job* jobs;
+    void* items;
+    size_t item_size;
+    void (*fn)(void*);
     int count;
     int next;
     pthread_mutex_t mu;
@@ -641,52 +680,99 @@ static void* worker(void* arg)
         int i = q->next++;
         pthread_mutex_unlock(&q->mu);
         if (i >= q->count) break;
-        solve_one(&q->jobs[i]);
+        q->fn((char*)q->items + (size_t)i * q->item_size);
     }
     return NULL;
 }
 
-/* Solve count problems; threads > 1 runs problems in parallel (the run.sh
- * fan-out of MultiQW.cpp, in one process).  sample_probs (nullable) receives
- * count*samples per-sample probabilities.  gamma/short_t (nullable) hold
- * count*m per-stage values replacing the heuristic schedule.  Returns 0, or
- * -1 on bad args. */
+/* Runs fn over count items on up to `threads` threads (the run.sh fan-out of
+ * MultiQW.cpp, in one process). */
+static void parallel_for(void* items, size_t item_size, int count, void (*fn)(void*), int threads)
+{
+    if (threads <= 1 || count <= 1) {
+        for (int i = 0; i < count; i++) fn((char*)items + (size_t)i * item_size);
+        return;
+    }
+    job_queue q;
+    q.items = items;
+    q.item_size = item_size;
+    q.fn = fn;
+    q.count = count;
+    q.next = 0;
+    pthread_mutex_init(&q.mu, NULL);
+    pthread_t th[256];
+    int T = threads > 256 ? 256 : threads;
+    if (T > count) T = count;
+    for (int t = 0; t < T; t++) pthread_create(&th[t], NULL, worker, &q);
+    for (int t = 0; t < T; t++) pthread_join(th[t], NULL);
+    pthread_mutex_destroy(&q.mu);
+}
+
+static void setup_fn(void* p) { problem_setup((oracle_problem*)p); }
+static void run_fn(void* p) { run_samples((const oracle_run*)p); }
+
+/* Solve count problems on up to `threads` threads.  sample_probs (nullable)
+ * receives count*samples per-sample probabilities.  gamma/short_t (nullable)
+ * hold count*m per-stage values replacing the heuristic schedule.  norm_f64
+ * (FP32 path): 0 = each stage's norm as the reference takes it (MultiQW.cpp:443,
+ * a float sum in Eigen's order), 1 = the same squares summed in double, which
+ * isolates the float sum's own rounding from the rest of the arithmetic.
+ * Returns 0, or -1 on bad args. */
+int oracle_solve_ex(int n, int m, const double* recs, int count, int samples, double tol, int precision,
+                    int threads, float* prob, float* total_t, float* sample_probs, const float* gamma,
+                    const float* short_t, int norm_f64)
+{
+    if (n < 4 || n > 26 || m < 1 || m > ORACLE_MAX_M || count < 0 || samples < 1) return -1;
+    if (count == 0) return 0;
+    const int P = n * (n + 1) / 2;
+    oracle_problem* prs = (oracle_problem*)calloc((size_t)count, sizeof(oracle_problem));
+    for (int c = 0; c < count; c++) {
+        prs[c].n = n;
+        prs[c].m = m;
+        prs[c].samples = samples;
+        prs[c].precision = precision;
+        prs[c].norm_f64 = norm_f64;
+        prs[c].tol = tol;
+        prs[c].rec = recs + (size_t)c * P;
+        prs[c].gamma_ov = gamma ? gamma + (size_t)c * m : NULL;
+        prs[c].short_t_ov = short_t ? short_t + (size_t)c * m : NULL;
+    }
+    parallel_for(prs, sizeof(oracle_problem), count, setup_fn, threads);
+    /* runs of samples: enough to give every thread work, whole problems for m == 1 */
+    int per = samples;
+    if (m != 1 && threads > count) {
+        int want = (threads + count - 1) / count; /* runs per problem */
+        per = (samples + want - 1) / want;
+    }
+    int nruns = 0;
+    for (int c = 0; c < count; c++) nruns += (samples + per - 1) / per;
+    oracle_run* runs = (oracle_run*)malloc(sizeof(oracle_run) * (size_t)nruns);
+    int r = 0;
+    for (int c = 0; c < count; c++)
+        for (int j = 0; j < samples; j += per) {
+            runs[r].pr = &prs[c];
+            runs[r].j0 = j;
+            runs[r].j1 = j + per < samples ? j + per : samples;
+            r++;
+        }
+    parallel_for(runs, sizeof(oracle_run), nruns, run_fn, threads);
+    for (int c = 0; c < count; c++) {
+        prob[c] = oracle_eigen_sum_f32(prs[c].sp, samples) / samples;
+        total_t[c] = prs[c].sch.total_t;
+        if (sample_probs) memcpy(sample_probs + (size_t)c * samples, prs[c].sp, sizeof(float) * samples);
+        problem_free(&prs[c]);
+    }
+    free(runs);
+    free(prs);
+    return 0;
+}
+
 int oracle_solve_sched(int n, int m, const double* recs, int count, int samples, double tol, int precision,
                        int threads, float* prob, float* total_t, float* sample_probs, const float* gamma,
                        const float* short_t)
 {
-    if (n < 4 || n > 26 || m < 1 || m > ORACLE_MAX_M || count < 0 || samples < 1) return -1;
-    const int P = n * (n + 1) / 2;
-    oracle_job* jobs = (oracle_job*)malloc(sizeof(oracle_job) * (size_t)(count > 0 ? count : 1));
-    for (int c = 0; c < count; c++) {
-        jobs[c].n = n;
-        jobs[c].m = m;
-        jobs[c].samples = samples;
-        jobs[c].precision = precision;
-        jobs[c].tol = tol;
-        jobs[c].rec = recs + (size_t)c * P;
-        jobs[c].prob = prob + c;
-        jobs[c].total_t = total_t + c;
-        jobs[c].sample_probs = sample_probs ? sample_probs + (size_t)c * samples : NULL;
-        jobs[c].gamma_ov = gamma ? gamma + (size_t)c * m : NULL;
-        jobs[c].short_t_ov = short_t ? short_t + (size_t)c * m : NULL;
-    }
-    if (threads <= 1) {
-        for (int c = 0; c < count; c++) solve_one(&jobs[c]);
-    } else {
-        job_queue q;
-        q.jobs = jobs;
-        q.count = count;
-        q.next = 0;
-        pthread_mutex_init(&q.mu, NULL);
-        pthread_t th[256];
-        int T = threads > 256 ? 256 : threads;
-        for (int t = 0; t < T; t++) pthread_create(&th[t], NULL, worker, &q);
-        for (int t = 0; t < T; t++) pthread_join(th[t], NULL);
-        pthread_mutex_destroy(&q.mu);
-    }
-    free(jobs);
-    return 0;
+    return oracle_solve_ex(n, m, recs, count, samples, tol, precision, threads, prob, total_t, sample_probs, gamma,
+                           short_t, 0);
 }
 
 int oracle_solve(int n, int m, const double* recs, int count, int samples, double tol, int precision, int threads,

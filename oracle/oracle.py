@@ -43,40 +43,71 @@ class MT(C.Structure):
     _fields_ = [("mt", C.c_uint32 * 624), ("idx", C.c_int)]
 
 
+def build_native(outdir: str) -> str:
+    """Compile the same source as the reference is built (gcc -O3 -march=native)
+    into outdir, on the machine that will run it: the CPU baseline's program."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "native", f"NATIVE_DIR={os.path.abspath(outdir)}"],
+                   check=True)
+    return os.path.join(os.path.abspath(outdir), "libmqw_oracle_native.so")
+
+
+def load_native(path: str):
+    """ctypes handle of a native build (build_native), bound like lib()."""
+    return _bind(C.CDLL(path))
+
+
+def solve_native(L, n: int, m: int, recs, samples: int = 100, threads: int = 1):
+    """oracle_solve through the handle L of a native build: (prob, total_t)."""
+    recs = np.ascontiguousarray(recs, np.float64).reshape(-1, n * (n + 1) // 2)
+    cnt = recs.shape[0]
+    prob = np.zeros(cnt, np.float32)
+    tt = np.zeros(cnt, np.float32)
+    if L.oracle_solve(n, m, _d(recs), cnt, samples, 1e-6, 0, threads, _f(prob), _f(tt), None) != 0:
+        raise ValueError("oracle_solve rejected its arguments")
+    return prob, tt
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
-        fp, dp, u = C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_uint
-        L.oracle_eigen_sum_f32.argtypes = [fp, C.c_long]
-        L.oracle_eigen_sum_f32.restype = C.c_float
-        L.oracle_unpack.argtypes = [C.c_int, dp, fp, fp]
-        L.oracle_energy_compat.argtypes = [C.c_int, fp, fp, fp, dp, C.POINTER(EnergyInfo)]
-        L.oracle_energy_exact.argtypes = [C.c_int, fp, fp, dp]
-        L.oracle_ppnd16.argtypes = [C.c_double]
-        L.oracle_ppnd16.restype = C.c_double
-        L.oracle_neg_normcdfinv_f32.argtypes = [C.c_float]
-        L.oracle_neg_normcdfinv_f32.restype = C.c_float
-        L.oracle_schedule_heuristic.argtypes = [C.c_int, C.c_int, fp, fp, C.POINTER(Schedule)]
-        L.oracle_mt_seed.argtypes = [C.POINTER(MT), C.c_uint32]
-        L.oracle_mt_next.argtypes = [C.POINTER(MT)]
-        L.oracle_mt_next.restype = C.c_uint32
-        L.oracle_sample_times.argtypes = [C.c_int, C.c_int, fp, fp]
-        L.oracle_cheb_coeffs.argtypes = [C.c_double, C.c_double, dp, C.c_int]
-        L.oracle_cheb_coeffs.restype = C.c_int
-        L.oracle_clenshaw_step_f32.argtypes = [fp, fp, fp, fp, u, C.c_float, C.c_float, C.c_float]
-        L.oracle_clenshaw_f32.argtypes = [fp, C.c_int, fp, fp, u, C.c_float, C.c_float, C.c_int, fp]
-        L.oracle_clenshaw_f64.argtypes = [dp, C.c_int, dp, dp, u, C.c_double, C.c_double, dp]
-        L.oracle_solve.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
-                                   fp, fp, fp]
-        L.oracle_solve.restype = C.c_int
-        L.oracle_solve_sched.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
-                                         fp, fp, fp, fp, fp]
-        L.oracle_solve_sched.restype = C.c_int
-        _lib = L
+        _lib = _bind(C.CDLL(_LIB_PATH))
     return _lib
+
+
+def _bind(L):
+    """Argument and result types of the exported functions."""
+    fp, dp, u = C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_uint
+    L.oracle_eigen_sum_f32.argtypes = [fp, C.c_long]
+    L.oracle_eigen_sum_f32.restype = C.c_float
+    L.oracle_unpack.argtypes = [C.c_int, dp, fp, fp]
+    L.oracle_energy_compat.argtypes = [C.c_int, fp, fp, fp, dp, C.POINTER(EnergyInfo)]
+    L.oracle_energy_exact.argtypes = [C.c_int, fp, fp, dp]
+    L.oracle_ppnd16.argtypes = [C.c_double]
+    L.oracle_ppnd16.restype = C.c_double
+    L.oracle_neg_normcdfinv_f32.argtypes = [C.c_float]
+    L.oracle_neg_normcdfinv_f32.restype = C.c_float
+    L.oracle_schedule_heuristic.argtypes = [C.c_int, C.c_int, fp, fp, C.POINTER(Schedule)]
+    L.oracle_mt_seed.argtypes = [C.POINTER(MT), C.c_uint32]
+    L.oracle_mt_next.argtypes = [C.POINTER(MT)]
+    L.oracle_mt_next.restype = C.c_uint32
+    L.oracle_sample_times.argtypes = [C.c_int, C.c_int, fp, fp]
+    L.oracle_cheb_coeffs.argtypes = [C.c_double, C.c_double, dp, C.c_int]
+    L.oracle_cheb_coeffs.restype = C.c_int
+    L.oracle_clenshaw_step_f32.argtypes = [fp, fp, fp, fp, u, C.c_float, C.c_float, C.c_float]
+    L.oracle_clenshaw_f32.argtypes = [fp, C.c_int, fp, fp, u, C.c_float, C.c_float, C.c_int, fp]
+    L.oracle_clenshaw_f64.argtypes = [dp, C.c_int, dp, dp, u, C.c_double, C.c_double, dp]
+    L.oracle_solve.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                               fp, fp, fp]
+    L.oracle_solve.restype = C.c_int
+    L.oracle_solve_sched.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                     fp, fp, fp, fp, fp]
+    L.oracle_solve_sched.restype = C.c_int
+    L.oracle_solve_ex.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                  fp, fp, fp, fp, fp, C.c_int]
+    L.oracle_solve_ex.restype = C.c_int
+    return L
 
 
 def _f(a):
@@ -189,9 +220,13 @@ def clenshaw_f64(coeffs, psi, hp, n, gamma, scale):
 
 
 def solve(n: int, m: int, recs, samples: int = 100, tol: float = 1e-6, precision: int = 0,
-          threads: int = 1, with_samples: bool = False, gamma=None, short_t=None):
+          threads: int = 1, with_samples: bool = False, gamma=None, short_t=None, norm: str = "eigen"):
     """Per-problem loop of MultiQW.cpp:296-455.  recs: (count, n(n+1)/2) float64.
-    gamma/short_t: optional (count, m) schedules replacing the heuristic."""
+    gamma/short_t: optional (count, m) schedules replacing the heuristic.
+    norm (FP32 path): "eigen" = each stage's norm as the reference takes it, a float
+    sum in Eigen's order (MultiQW.cpp:443); "f64" = the same squares summed in double."""
+    if norm not in ("eigen", "f64"):
+        raise ValueError(f"norm must be 'eigen' or 'f64', got {norm!r}")
     recs = np.ascontiguousarray(recs, np.float64).reshape(-1, n * (n + 1) // 2)
     cnt = recs.shape[0]
     prob = np.zeros(cnt, np.float32)
@@ -199,9 +234,9 @@ def solve(n: int, m: int, recs, samples: int = 100, tol: float = 1e-6, precision
     sp = np.zeros(cnt * samples, np.float32) if with_samples else None
     g = np.ascontiguousarray(gamma, np.float32).reshape(cnt, m) if gamma is not None else None
     st = np.ascontiguousarray(short_t, np.float32).reshape(cnt, m) if short_t is not None else None
-    rc = lib().oracle_solve_sched(n, m, _d(recs), cnt, samples, tol, precision, threads, _f(prob), _f(tt),
-                                  _f(sp) if sp is not None else None, _f(g) if g is not None else None,
-                                  _f(st) if st is not None else None)
+    rc = lib().oracle_solve_ex(n, m, _d(recs), cnt, samples, tol, precision, threads, _f(prob), _f(tt),
+                               _f(sp) if sp is not None else None, _f(g) if g is not None else None,
+                               _f(st) if st is not None else None, int(norm == "f64"))
     if rc != 0:
         raise ValueError("oracle_solve rejected its arguments")
     if with_samples:

@@ -2,7 +2,7 @@
 
 The CPU test checks that a request for several GPUs is refused with a message
 (one GPU only); the GPU test runs a tiny bench (4 instances, one step, a
-one-sample CPU baseline) and checks every field of its line.
+one-instance CPU baseline) and checks every field of its line.
 """
 import json
 import os
@@ -30,7 +30,8 @@ def test_bench_refuses_several_gpus():
 def test_bench_line_fields():
     print("[case] bench.py tiny run", flush=True)
     r = _bench("--steps", "1", "--warmup", "0", "--per-step", "4", "--instances", "8", "--apply-n", "0",
-               "--cpu-threads", "1", "--cpu-samples", "1")
+               "--n", "14", "--cpu-threads", "1", "--cpu-instances", "1", "--cpu-core-samples", "1",
+               "--uneven", "0")
     assert r.returncode == 0, r.stderr[-2000:]
     lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
     assert len(lines) == 1, r.stdout[-2000:]
@@ -51,3 +52,5 @@ def test_bench_line_fields():
     cb = d["cpu_baseline"]
     assert cb["value"] > 0 and cb["cores"] == 1 and cb["kind"] in ("port", "reference")
     assert isinstance(cb["unit"], str) and isinstance(cb["sample"], str)
+    assert cb["one_core"]["value"] > 0 and cb["cpus_available"] >= 1 and isinstance(cb["cpu_model"], str)
+    assert "uneven" in d and d["uneven"] is None

@@ -71,3 +71,20 @@ def test_degenerate_instance_solve_vs_oracle(ctx, oracle, n, m):
     assert np.array_equal(tt, tto)
     assert np.max(np.abs(sp - spo)) < PROB_TOL
     assert np.max(np.abs(p - po)) < PROB_TOL
+
+
+@pytest.mark.parametrize("samples", [0, -1])
+def test_bad_sample_count_is_an_error(ctx, samples):
+    """samples < 1 is rejected with MQW_EINVAL before any problem is prepared (a negative
+    count once reached a std::vector size and aborted the caller through the C ABI)."""
+    _say("samples", samples)
+    recs = M.generate_sk(10, 2, 1)
+    with pytest.raises(M.MQWError) as e:
+        ctx.solve(10, 2, recs, samples=samples)
+    assert e.value.status == 1  # MQW_EINVAL
+    with pytest.raises(M.MQWError) as e:
+        ctx.solve_batch([(10, 2, recs[0]), (12, 3, M.generate_sk(12, 1, 2)[0])], samples=samples)
+    assert e.value.status == 1
+    # the context still works afterwards
+    p, _ = ctx.solve(10, 2, recs, samples=4)
+    assert np.all((p >= 0) & (p <= 1))

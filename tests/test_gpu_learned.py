@@ -72,17 +72,20 @@ def test_features_vs_oracle(ctx, oracle):
         assert abs(F[k, 8:].sum() - 1) < 1e-6
 
 
-def test_learned_schedule_gain(ctx):
-    """Train on simulator-scored schedules (n = 12..14), evaluate on fresh n = 14 instances."""
-    _say("learned gain (small)")
+def test_learned_schedule_pipeline(ctx):
+    """The training pipeline end to end on a small problem: simulator-scored candidates
+    (n = 12..14), training by the HIP kernels, evaluation on fresh instances at equal total
+    time.  What the shipped network gains is tested in test_gpu_learned_shipped.py."""
+    _say("learned pipeline (small)")
     data = Lr.make_dataset(ctx, 240, seed=11, n_range=(12, 14), candidates=8, samples=16)
     assert data.scored == 240 * 8
+    # candidate 0 of every instance is the heuristic, so the best is at least as good
     assert np.all(data.p_best >= data.p_heur)
+    assert np.mean(data.p_best > data.p_heur) > 0.5
     mlp = Lr.MLP(ctx, seed=5)
     info = Lr.train(mlp, data, epochs=150, batch=128, lr=1e-3)
     assert info["history"][-1][1] < info["history"][0][1]
     ev = Lr.evaluate(ctx, mlp, 32, seed=99, n=14, samples=32)
     print("[learned]", ev, flush=True)
-    assert ev["p_fallback"] >= ev["p_heuristic"]
-    assert ev["gain_fallback"] > 0.0
+    assert ev["equal_time"]
     mlp.close()

@@ -304,3 +304,58 @@ def test_driver_fp32_vs_fp64_and_dense(oracle):
                                      info["eloc"], reset=(m != 1))
             assert np.max(np.abs(s64[k] - ref)) < 3e-6
             assert tt[k] == sch["total_t"]
+
+
+@pytest.mark.parametrize("size", [1, 15, 16, 17, 31, 32, 33, 100, (1 << 20) + 3])
+def test_eigen_sum_f32_error_bound(oracle, size):
+    """oracle_eigen_sum_f32 (the float reductions of MultiQW.cpp:443,453) against the exact sum.
+
+    Positive data, with the elements past the last 16-lane packet and the second
+    accumulator's lanes made larger, so a dropped tail or a missing accumulator
+    merge moves the sum by far more than rounding can.  Two bounds hold for any
+    summation order that adds every element once: the general (size - 1) u sum|x|,
+    and the tighter one for 32-lane accumulation, (size/32 + 8 + tail) u sum|x|.
+    """
+    rng = np.random.default_rng(size)
+    x = rng.uniform(1.0, 2.0, size).astype(np.float32)
+    a1 = (size // 16) * 16
+    a2 = (size // 32) * 32
+    lane = np.arange(size) % 32
+    x[:a2][lane[:a2] >= 16] *= np.float32(3.0)  # the second accumulator's lanes
+    if size > a1:  # the tail past the last packet: a tenth of the total
+        x[a1:] = np.float32(0.1 * float(np.sum(x[:a1], dtype=np.float64)) / (size - a1)) if a1 else x[a1:]
+    u = 2.0 ** -24
+    exact = float(np.sum(x.astype(np.float64)))
+    total = float(np.sum(np.abs(x.astype(np.float64))))
+    loose = max(size - 1, 1) * u * total
+    tight = (size // 32 + 8 + (size - a1)) * u * total
+    s = float(oracle.eigen_sum(x))
+    assert abs(s - exact) <= loose
+    assert abs(s - exact) <= tight
+    # the mistakes the bounds must catch, made on the CPU from the same data
+    if size > a1:
+        dropped_tail = float(oracle.eigen_sum(x[:a1])) if a1 else 0.0
+        assert abs(dropped_tail - exact) > loose
+    if a2 >= 64:
+        no_merge = exact - float(np.sum(x[:a2][lane[:a2] >= 16].astype(np.float64)))
+        assert abs(no_merge - exact) > tight
+
+
+def test_float_norm_rounding_at_large_n(oracle):
+    """The reference renormalises each stage with a float sum of 2^(n+1) squares
+    (MultiQW.cpp:443, Eigen's order).  At n = 18 that sum's rounding moves the
+    probabilities by far more than the rest of the FP32 arithmetic does: with the
+    squares summed in double the FP32 restatement is within 2e-7 of FP64, and the
+    float sum alone moves it many times further.  This is why GPU results (FP64
+    norms) are compared with both forms of the oracle."""
+    import multistageqw_amd as M
+    n, m = 18, 4
+    recs = M.generate_sk(n, 1, 6)
+    kw = dict(samples=4, threads=4, with_samples=True)
+    eig = oracle.solve(n, m, recs, **kw)[2]
+    f64n = oracle.solve(n, m, recs, norm="f64", **kw)[2]
+    exact = oracle.solve(n, m, recs, precision=1, **kw)[2]
+    rest = float(np.max(np.abs(f64n - exact)))
+    float_sum = float(np.max(np.abs(eig - exact)))
+    assert rest < 2e-7
+    assert float_sum > 5 * rest
