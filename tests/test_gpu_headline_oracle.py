@@ -12,7 +12,7 @@ renormalises after each stage as MultiQW.cpp:432-454 does.
 The reference takes each stage's norm as a float sum (Eigen's order, 32 float
 accumulators over 2^21 squares at n = 20, MultiQW.cpp:443), which by itself
 moves its probabilities by about 1e-6 here; the GPU sums norms in FP64.  So the
-GPU is held to 1e-5 against the oracle as the reference computes, and to 2e-6
+GPU is held to 1e-5 against the oracle as the reference computes, and to 5e-7
 against the same oracle with its norms summed in double.
 """
 import os
@@ -58,7 +58,7 @@ def test_headline_path_vs_oracle(ctx, headline_oracle, gemm):
           f"mean p gpu={p.mean():.8f} oracle={po.mean():.8f}; vs FP64-norm oracle max|dp|={dd.max():.3e}; "
           f"the reference's float-norm shift max={np.abs(spo - spd).max():.3e}", flush=True)
     assert dp.max() < PROB_TOL
-    assert dd.max() < 2e-6
+    assert dd.max() < 5e-7
     assert np.max(np.abs(p - po)) < PROB_TOL
     # the probabilities are not degenerate: the check compares real signal
     assert np.all(spo > 1e-4)
