@@ -11,7 +11,7 @@ renormalises after each stage as MultiQW.cpp:432-454 does.
 
 The reference takes each stage's norm as a float sum (Eigen's order, 32 float
 accumulators over 2^21 squares at n = 20, MultiQW.cpp:443), which by itself
-moves its probabilities by about 1e-6 here; the GPU sums norms in FP64.  So the
+moves its probabilities by up to 2.6e-6 here; the GPU sums norms in FP64.  So the
 GPU is held to 1e-5 against the oracle as the reference computes, and to 5e-7
 against the same oracle with its norms summed in double.
 """
