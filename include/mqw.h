@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 10
+#define MQW_ABI_VERSION 11
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -210,6 +210,18 @@ mqw_status mqw_clenshaw_step_dev(mqw_ctx* ctx, const float* b1_dev, float* b2_de
  * (unnormalised) result; hp: 2^n floats. */
 mqw_status mqw_clenshaw(mqw_ctx* ctx, int n, const float* coeffs, int K, float* psi, const float* hp,
                         float gamma, float scale, int psi_real);
+
+/* mqw_clenshaw without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
+ * results as mqw_clenshaw, but every device buffer (the diagonal, the states,
+ * partial norms, the job table) comes from one fixed 40 MiB array that each
+ * GPU holds until the process ends.  Limits: n <= 20, and K small enough that
+ * the job table fits beside the states (about 30000 terms at n = 20); larger
+ * calls return MQW_EINVAL.  Calls on one GPU share that GPU's array: calls
+ * that overlap on one GPU (several host threads or contexts) race. */
+mqw_status mqw_clenshaw_noalloc(mqw_ctx* ctx, int n, const float* coeffs, int K, float* psi, const float* hp,
+                                float gamma, float scale, int psi_real);
 
 /* The per-problem loop of MultiQW.cpp:296-455 for count problems of one
  * (n, m): recs holds count*n(n+1)/2 doubles.  prob and total_t receive one

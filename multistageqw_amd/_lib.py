@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmqw.so")
 CLI_PATH = os.path.join(HERE, "mqw")
 
-ABI_VERSION = 10  # MQW_ABI_VERSION of include/mqw.h this binding matches
+ABI_VERSION = 11  # MQW_ABI_VERSION of include/mqw.h this binding matches
 MAX_STAGES = 64
 DEFAULT_SAMPLES = 100
 DEFAULT_TOL = 1e-6
@@ -27,7 +27,7 @@ FP32, FP64 = 0, 1
 EXPORTS = [
     "mqw_create", "mqw_destroy", "mqw_last_error", "mqw_status_string", "mqw_abi_version", "mqw_opts_default",
     "mqw_unpack", "mqw_schedule_heuristic", "mqw_schedule_heuristic_moments", "mqw_sample_times", "mqw_chebyshev_coeffs", "mqw_generate_sk",
-    "mqw_energy", "mqw_clenshaw_step_dev", "mqw_clenshaw", "mqw_solve", "mqw_solve_batch", "mqw_tune",
+    "mqw_energy", "mqw_clenshaw_step_dev", "mqw_clenshaw", "mqw_clenshaw_noalloc", "mqw_solve", "mqw_solve_batch", "mqw_tune",
     "mqw_last_stats", "mqw_count_instances", "mqw_read_instances", "mqw_append_output", "mqw_features",
     "mqw_mlp_create", "mqw_mlp_destroy", "mqw_mlp_get_params", "mqw_mlp_set_params", "mqw_mlp_forward",
     "mqw_mlp_loss_grad", "mqw_mlp_adam_step", "mqw_diag_moments",
@@ -113,6 +113,7 @@ def load():
         "mqw_diag_moments": ([vp, C.c_int, dp, C.c_int, dp], C.c_int),
         "mqw_clenshaw_step_dev": ([vp, vp, vp, vp, vp, C.c_uint, C.c_float, C.c_float, C.c_float], C.c_int),
         "mqw_clenshaw": ([vp, C.c_int, fp, C.c_int, fp, fp, C.c_float, C.c_float, C.c_int], C.c_int),
+        "mqw_clenshaw_noalloc": ([vp, C.c_int, fp, C.c_int, fp, fp, C.c_float, C.c_float, C.c_int], C.c_int),
         "mqw_solve": ([vp, C.c_int, C.c_int, dp, C.c_int, C.POINTER(Opts), fp, fp], C.c_int),
         "mqw_solve_batch": ([vp, C.POINTER(Instance), C.c_int, C.POINTER(Opts), fp, fp], C.c_int),
         "mqw_tune": ([vp, C.c_int, C.c_char_p, ip, ip, ip, ip], C.c_int),

@@ -217,6 +217,17 @@ class Context:
                                            scale, int(psi_real)))
         return out
 
+    def clenshaw_noalloc(self, coeffs, psi, hp, gamma: float, scale: float, psi_real: bool = False) -> np.ndarray:
+        """clenshaw() through mqw_clenshaw_noalloc: no device allocation, buffers in a fixed
+        per-device array (n <= 20)."""
+        hp = np.ascontiguousarray(hp, np.float32)
+        n = int(np.log2(hp.size))
+        out = np.ascontiguousarray(psi, np.float32).copy()
+        cf = np.ascontiguousarray(coeffs, np.float32)
+        self._check(self._lib.mqw_clenshaw_noalloc(self._h, n, L.f32p(cf), cf.size, L.f32p(out), L.f32p(hp),
+                                                   gamma, scale, int(psi_real)))
+        return out
+
     # -- the per-problem loop (MultiQW.cpp:296-455) -------------------------
     def solve(self, n: int, m: int, recs, with_samples: bool = False, **opts):
         recs = np.ascontiguousarray(recs, np.float64).reshape(-1, params_per_instance(n))

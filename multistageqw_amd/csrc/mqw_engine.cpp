@@ -1738,6 +1738,9 @@ mqw_status mqw_create(int device, mqw_ctx** out)
         c->tuned_var[i] = 0;
         c->tuned_hp[i] = -1;
     }
+    // this device's copy of the no-allocation scratch (mqw_clenshaw_noalloc);
+    // a failed lookup is reported by that entry point, not here
+    (void)noalloc_scratch_init(device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return MQW_EDEVICE;
     if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
@@ -2043,6 +2046,80 @@ mqw_status mqw_clenshaw(mqw_ctx* c, int n, const float* coeffs, int K, float* ps
     if (st) return st;
     const float* outp = (const float*)plan.rjobs.back().out;
     HIPCHK(c, hipMemcpyAsync(psi, outp, sizeof(float) * 2 * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MQW_OK;
+}
+
+mqw_status mqw_clenshaw_noalloc(mqw_ctx* c, int n, const float* coeffs, int K, float* psi, const float* hp,
+                                float gamma, float scale, int psi_real)
+{
+    // mqw_clenshaw's jobs and kernels, with every device buffer carved from
+    // the fixed per-device scratch (mqw_noalloc.hip) instead of the context's
+    // workspace: no device allocation, no graph, no events.
+    (void)psi_real;
+    if (!c) return MQW_EINVAL;
+    if (n < MQW_MIN_N || n > MQW_MAX_N) return fail(c, MQW_EINVAL, "n=%d outside range", n);
+    if (n > kNoAllocMaxN)
+        return fail(c, MQW_EINVAL, "n=%d exceeds the no-allocation limit n <= %d (%zu-byte device scratch)", n,
+                    kNoAllocMaxN, kNoAllocScratchBytes);
+    if (!coeffs || K < 1 || !psi || !hp) return fail(c, MQW_EINVAL, "null argument or K < 1");
+    if (!(scale > 0.0f) || !isfinite(scale) || !isfinite(gamma)) return fail(c, MQW_EINVAL, "bad scale");
+    char* base = (char*)noalloc_scratch(c->device);
+    if (!base) {
+        const hipError_t e = noalloc_scratch_init(c->device);
+        return fail(c, MQW_EDEVICE, "no-allocation scratch unavailable on device %d: %s", c->device,
+                    hipGetErrorString(e));
+    }
+    hipSetDevice(c->device);
+    const size_t N = (size_t)1 << n;
+    const SweepCfg k = choose_sweep(c, MQW_FP32, n, 0, -1, -1, 0);
+    const uint32_t ntiles = k.parts;
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        char* p = base + off;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    float* E = (float*)carve(sizeof(float) * N);
+    float* state = (float*)carve(sizeof(float) * 8 * N);
+    double* np = (double*)carve(sizeof(double) * 2 * ntiles);
+    double* dummy = (double*)carve(sizeof(double));
+    Traj tr;
+    tr.prob = 0;
+    tr.psi_in = state;
+    Segment sg;
+    sg.gamma = gamma;
+    sg.onenorm = scale;
+    sg.c.assign(coeffs, coeffs + K);
+    sg.prob_slot = 0;
+    tr.segs.push_back(sg);
+    Slot<float> sl;
+    for (int q = 0; q < 3; q++) sl.st[q] = state + (size_t)(2 + 2 * q) * N;
+    sl.part[0] = np;
+    sl.part[1] = np + ntiles;
+    thread_local std::vector<TermJob<float>> jobs; // host table, reused across calls on this thread
+    jobs.clear();
+    ReduceJob fin{};
+    bool has = false;
+    traj_jobs<float>(n, ntiles, tr, sl, E, 0, dummy, jobs, fin, has);
+    TermJob<float>* jd = (TermJob<float>*)carve(sizeof(TermJob<float>) * jobs.size());
+    ReduceJob* rd = (ReduceJob*)carve(sizeof(ReduceJob));
+    if (off > kNoAllocScratchBytes)
+        return fail(c, MQW_EINVAL, "n=%d with K=%d needs %zu bytes of device scratch; the no-allocation limit is %zu",
+                    n, K, off, kNoAllocScratchBytes);
+    HIPCHK(c, hipMemcpyAsync(E, hp, sizeof(float) * N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(state, psi, sizeof(float) * 2 * N, hipMemcpyHostToDevice, c->stream));
+    if (!jobs.empty())
+        HIPCHK(c, hipMemcpyAsync(jd, jobs.data(), sizeof(TermJob<float>) * jobs.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+    HIPCHK(c, hipMemcpyAsync(rd, &fin, sizeof(ReduceJob), hipMemcpyHostToDevice, c->stream));
+    for (size_t q = 0; q < jobs.size(); q++) {
+        const int rc = k.half ? launch_half_term(k.tb, k.variant, jd + q, 1, n, k.lc, k.hi, c->stream)
+                              : launch_term(MQW_FP32, k.tb, k.variant, jd + q, 1, n, k.lc, c->stream);
+        if (rc) return fail(c, MQW_EDEVICE, "term launch: %s", last_kernel_error());
+    }
+    if (launch_reduce(MQW_FP32, rd, 1, c->stream)) return fail(c, MQW_EDEVICE, "reduce: %s", last_kernel_error());
+    HIPCHK(c, hipMemcpyAsync(psi, fin.out, sizeof(float) * 2 * N, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MQW_OK;
 }

@@ -351,6 +351,18 @@ struct GraphGuard {
     }
 };
 
+// The entry point that allocates nothing (mqw_clenshaw_noalloc) carves its
+// device buffers from one fixed array per device (mqw_noalloc.hip): 40 MiB,
+// which holds the diagonal and the four planar states of n <= 20 (36 MiB)
+// plus partial norms and the job table.
+constexpr size_t kNoAllocScratchBytes = (size_t)40 << 20;
+constexpr int kNoAllocMaxN = 20;
+// Looks up this device's copy of the array (once per device; the library's
+// context set-up calls it) and returns the lookup's result.
+hipError_t noalloc_scratch_init(int device);
+// Device address of this device's copy, or nullptr before a successful init.
+void* noalloc_scratch(int device);
+
 mqw_status fail(mqw_ctx* c, mqw_status s, const char* fmt, ...);
 void* dev_get(mqw_ctx* c, const std::string& name, size_t bytes);
 void release_bufs(mqw_ctx* c);
