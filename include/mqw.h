@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 11
+#define MQW_ABI_VERSION 12
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -130,6 +130,13 @@ typedef struct {
                                 (exactly two bits each; the reference README.md:41 "XY" to-do):
                                 (H_M psi)[x] = sum_k 2 psi[x ^ m_k] where bits a, b of x differ.  It
                                 keeps Hamming weight; spectral bound E_abs + gamma * 2 * count. */
+    int compress_bases;      /* 1 (default) = where the second stage is shared, its basis is built from a
+                                compressed first-stage basis: the samples' first-stage states span far
+                                fewer directions than the kc1 Chebyshev vectors T_k(H1/onenorm1)|+>, so an
+                                SVD (weighted by the vectors' Gram matrix, per parity class) keeps the
+                                smallest set of orthonormal combinations Q_r that holds every sample's
+                                state to within 1e-7 relative, and W_lr = T_l(H2/onenorm2) Q_r replaces
+                                W_lk (about 20 instead of 37 chains at n = 20); 0 = W from every V_k */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are
@@ -296,6 +303,9 @@ typedef struct {
     uint64_t overlap_batches; /* of those, batches whose recurrences ran on the second stream beside the main
                                  stream's work (overlap_bases); their event times overlap, so gpu_ms then
                                  exceeds the device's busy time */
+    uint64_t compressed_instances; /* instances whose second-stage basis came from a compressed first stage */
+    uint64_t compressed_vectors;   /* their kept first-stage vectors (2R each), summed */
+    uint64_t uncompressed_vectors; /* their first-stage vectors before compression (kc1 each), summed */
 } mqw_stats;
 mqw_status mqw_last_stats(const mqw_ctx* ctx, mqw_stats* out);
 

@@ -1,0 +1,202 @@
+// Compression of the shared first-stage basis (see GramJob in mqw_internal.h).
+//
+// For one instance and one parity class (k even: the real parts of the
+// samples' first-stage states, k odd: the imaginary parts), the samples'
+// states are the rows of A V, with A the samples' coefficients and V the
+// class's basis vectors.  With G = V V^T = Z diag(w) Z^T (cyclic Jacobi), the
+// matrix M = A Z diag(sqrt(w)) has M M^T = A G A^T, so its singular values and
+// left singular vectors are those of A V itself.  A one-sided Jacobi SVD of M
+// (column rotations, accurate for small singular values) gives columns
+// m_i = sigma_i u_i.  Keeping the top R columns reconstructs sample j to
+// within sqrt(sum_{i >= R} m_i[j]^2); R is the smallest rank that keeps every
+// sample within rel_tol of its own norm.  The kept directions are the unit
+// vectors Q_r = (1 / sigma_r^2) sum_j m_r[j] (A V)_j, i.e. B = m_r^T A / sigma_r^2,
+// and sample j's coordinates are m_r[j].
+#include <math.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "mqw_internal.h"
+
+namespace mqw {
+
+namespace {
+
+// Eigen-decomposition of the symmetric n x n matrix a (row-major, destroyed):
+// eigenvalues w, eigenvectors as the columns of z (row-major).
+void sym_eig_jacobi(int n, std::vector<double>& a, std::vector<double>& w, std::vector<double>& z)
+{
+    z.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) z[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 64; sweep++) {
+        double off = 0.0, all = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                const double v = a[(size_t)i * n + j] * a[(size_t)i * n + j];
+                all += v;
+                if (i != j) off += v;
+            }
+        if (off <= 1e-32 * all || off == 0.0) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = a[(size_t)p * n + q];
+                if (apq == 0.0) continue;
+                const double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; k++) { // columns p, q
+                    const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+                    a[(size_t)k * n + p] = c * akp - s * akq;
+                    a[(size_t)k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) { // rows p, q
+                    const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+                    a[(size_t)p * n + k] = c * apk - s * aqk;
+                    a[(size_t)q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    const double zkp = z[(size_t)k * n + p], zkq = z[(size_t)k * n + q];
+                    z[(size_t)k * n + p] = c * zkp - s * zkq;
+                    z[(size_t)k * n + q] = s * zkp + c * zkq;
+                }
+            }
+    }
+    w.resize(n);
+    for (int i = 0; i < n; i++) w[i] = a[(size_t)i * n + i];
+}
+
+// One-sided Jacobi: rotates the columns of m (rows x cols, column-major)
+// until they are mutually orthogonal.
+void orthogonalise_columns(int rows, int cols, std::vector<double>& m)
+{
+    for (int sweep = 0; sweep < 64; sweep++) {
+        bool rotated = false;
+        for (int p = 0; p < cols - 1; p++)
+            for (int q = p + 1; q < cols; q++) {
+                double* mp = &m[(size_t)p * rows];
+                double* mq = &m[(size_t)q * rows];
+                double al = 0.0, be = 0.0, ga = 0.0;
+                for (int r = 0; r < rows; r++) {
+                    al += mp[r] * mp[r];
+                    be += mq[r] * mq[r];
+                    ga += mp[r] * mq[r];
+                }
+                if (ga == 0.0 || fabs(ga) <= 1e-15 * sqrt(al * be)) continue;
+                rotated = true;
+                const double zeta = (be - al) / (2.0 * ga);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                for (int r = 0; r < rows; r++) {
+                    const double x = mp[r], y = mq[r];
+                    mp[r] = c * x - s * y;
+                    mq[r] = s * x + c * y;
+                }
+            }
+        if (!rotated) break;
+    }
+}
+
+struct ClassSvd {
+    int nc = 0;
+    std::vector<int> ks;     // basis indices of the class
+    std::vector<double> m;   // samples x nc, column-major, columns by decreasing norm
+    std::vector<double> sig; // column norms
+};
+
+void class_svd(int kc, int cls, const double* gram, const std::vector<const double*>& coefs, ClassSvd& out)
+{
+    const int S = (int)coefs.size();
+    for (int k = cls; k < kc; k += 2) out.ks.push_back(k);
+    const int nc = out.nc = (int)out.ks.size();
+    std::vector<double> g((size_t)nc * nc);
+    for (int a = 0; a < nc; a++)
+        for (int b = 0; b < nc; b++) {
+            const uint32_t k1 = (uint32_t)std::min(out.ks[a], out.ks[b]), k2 = (uint32_t)std::max(out.ks[a], out.ks[b]);
+            g[(size_t)a * nc + b] = gram[gram_pair_start(k2) + k1 / 2];
+        }
+    std::vector<double> w, z;
+    sym_eig_jacobi(nc, g, w, z);
+    // M = A Z diag(sqrt(w)), column-major
+    std::vector<double> m((size_t)S * nc, 0.0);
+    for (int i = 0; i < nc; i++) {
+        const double sw = w[i] > 0.0 ? sqrt(w[i]) : 0.0;
+        for (int j = 0; j < S; j++) {
+            double acc = 0.0;
+            for (int a = 0; a < nc; a++) acc += coefs[j][out.ks[a]] * z[(size_t)a * nc + i];
+            m[(size_t)i * S + j] = acc * sw;
+        }
+    }
+    orthogonalise_columns(S, nc, m);
+    std::vector<double> sig(nc);
+    for (int i = 0; i < nc; i++) {
+        double s2 = 0.0;
+        for (int j = 0; j < S; j++) s2 += m[(size_t)i * S + j] * m[(size_t)i * S + j];
+        sig[i] = sqrt(s2);
+    }
+    std::vector<int> order(nc);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return sig[x] > sig[y]; });
+    out.m.assign((size_t)S * nc, 0.0);
+    out.sig.resize(nc);
+    for (int i = 0; i < nc; i++) {
+        std::copy(&m[(size_t)order[i] * S], &m[(size_t)order[i] * S] + S, &out.m[(size_t)i * S]);
+        out.sig[i] = sig[order[i]];
+    }
+}
+
+// Smallest rank keeping every sample within rel_tol of its own norm.
+int class_rank(const ClassSvd& c, int S, double rel_tol)
+{
+    std::vector<double> tot(S, 0.0), tail(S, 0.0);
+    for (int i = 0; i < c.nc; i++)
+        for (int j = 0; j < S; j++) tot[j] += c.m[(size_t)i * S + j] * c.m[(size_t)i * S + j];
+    tail = tot;
+    for (int R = 0; R <= c.nc; R++) {
+        bool ok = true;
+        for (int j = 0; j < S && ok; j++) ok = tail[j] <= rel_tol * rel_tol * tot[j];
+        if (ok) return R;
+        if (R < c.nc)
+            for (int j = 0; j < S; j++) tail[j] -= c.m[(size_t)R * S + j] * c.m[(size_t)R * S + j];
+    }
+    return c.nc;
+}
+
+} // namespace
+
+bool compress_first_stage(int kc, const double* gram, const std::vector<const double*>& coefs, double rel_tol,
+                          FirstStageCompression& out)
+{
+    const int S = (int)coefs.size();
+    if (kc < 4 || S < 1) return false;
+    ClassSvd cs[2];
+    int R = 1;
+    for (int cls = 0; cls < 2; cls++) {
+        class_svd(kc, cls, gram, coefs, cs[cls]);
+        R = std::max(R, class_rank(cs[cls], S, rel_tol));
+    }
+    if (2 * R >= kc || R > cs[1].nc) return false;
+    out.R = R;
+    out.B.assign((size_t)2 * R * kc, 0.0);
+    out.u.assign((size_t)S * 2 * R, 0.0);
+    for (int cls = 0; cls < 2; cls++) {
+        const ClassSvd& c = cs[cls];
+        for (int r = 0; r < R; r++) {
+            const int row = 2 * r + cls;
+            const double s2 = c.sig[r] * c.sig[r];
+            if (!(s2 > 0.0)) continue; // a zero direction: Q_row = 0, no weight
+            const double* mr = &c.m[(size_t)r * S];
+            for (int a = 0; a < c.nc; a++) {
+                double acc = 0.0;
+                for (int j = 0; j < S; j++) acc += mr[j] * coefs[j][c.ks[a]];
+                out.B[(size_t)row * kc + c.ks[a]] = acc / s2;
+            }
+            for (int j = 0; j < S; j++) out.u[(size_t)j * 2 * R + row] = mr[j];
+        }
+    }
+    return true;
+}
+
+} // namespace mqw

@@ -205,6 +205,7 @@ struct Traj {
     bool first_done = false;      // the first stage comes from the shared basis (a combine op)
     bool second_done = false;     // the first two stages come from the shared bases (a combine2 op)
     bool adjoint = false;         // the last stage is read out by the adjoint dots (no terms)
+    std::vector<float> c1q;       // with a compressed first-stage basis: the sample's coordinates in it
 };
 
 // Chebyshev coefficients of every segment.  Segments are independent (the
@@ -504,6 +505,10 @@ struct Bases {
     std::vector<uint32_t> q0, q1;
     const float* plus = nullptr;
     bool split = true; // the GEMM on FP16 matrix ops with split operands (else FP32 matrix ops)
+    // per instance: 1 = V holds a compressed first-stage basis Q_1.. (kc = 2R
+    // vectors, Q_0 at plusv), computed before the batch; no V recurrence then
+    std::vector<char> cmp;
+    std::vector<const float*> plusv;
 };
 
 // Rows (l, k) of the second-stage GEMM in two classes, each padded to a
@@ -656,7 +661,8 @@ void combine2_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_
             for (size_t i0 = 0; i0 < mem.size(); i0 += tsz) {
                 const size_t ns = std::min<size_t>(tsz, mem.size() - i0);
                 Combine2Tile ct{};
-                ct.plus = B.plus;
+                const bool cq = !B.cmp.empty() && B.cmp[pr];
+                ct.plus = cq ? B.plusv[pr] : B.plus;
                 ct.V = B.V[pr];
                 ct.W = B.W[pr];
                 ct.qtab = B.qtab[pr];
@@ -675,7 +681,10 @@ void combine2_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_
                     it.norm_part = slots[i].part[0];
                     it.c1_off = (uint32_t)P.ccoef.size();
                     for (int k = 0; k < kc1; k++) {
-                        P.ccoef.push_back(k < (int)tr.segs[0].c.size() ? (float)tr.segs[0].c[k] : 0.0f);
+                        if (cq)
+                            P.ccoef.push_back(k < (int)tr.c1q.size() ? tr.c1q[k] : 0.0f);
+                        else
+                            P.ccoef.push_back(k < (int)tr.segs[0].c.size() ? (float)tr.segs[0].c[k] : 0.0f);
                         m1 = std::max(m1, fabsf(P.ccoef.back()));
                     }
                     it.c2_off = (uint32_t)P.ccoef.size();
@@ -1040,19 +1049,25 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
     //   V_{k+1} = 2 Ht_1 V_k - V_{k-1},  X_{k+1} = 2 Ht_m X_k - X_{k-1}
     //   W_{l+1,k} = 2 Ht_2 W_lk - W_{l-1,k},  W_1k = Ht_2 V_k,  W_0k = V_k (V_0 = |+>)
     // kc2 holds zeros for the instances (or the whole batch) without a shared second stage
-    bool has_w = false;
-    for (size_t p = 0; p < B.kc2.size(); p++) has_w |= B.kc2[p] > 0;
-    if (has_w && !B.plus) return fail(c, MQW_EINVAL, "second-stage bases without the |+> row");
+    // a compressed instance's first-stage basis Q is ready before the batch:
+    // no V recurrence, and every W chain starts at step 0 from its Q_k
+    auto cmp = [&](size_t p) { return !B.cmp.empty() && B.cmp[p] != 0; };
+    bool has_w = false, need_plus = false;
+    for (size_t p = 0; p < B.kc2.size(); p++) {
+        has_w |= B.kc2[p] > 0;
+        need_plus |= B.kc2[p] > 0 && !cmp(p);
+    }
+    if (need_plus && !B.plus) return fail(c, MQW_EINVAL, "second-stage bases without the |+> row");
     std::vector<const Segment*> seg1(ps.size(), nullptr);
     if (has_w)
         for (const Traj& tr : trajs)
             if (!seg1[tr.prob] && tr.segs.size() > 1) seg1[tr.prob] = &tr.segs[1];
     int steps = 0;
     for (size_t p = 0; p < ps.size(); p++) {
-        steps = std::max({steps, B.kc[p] - 1, B.kx[p] - 1});
-        if (has_w && B.kc2[p] > 0) steps = std::max(steps, B.kc[p] - 1 + B.kc2[p] - 1);
+        steps = std::max({steps, cmp(p) ? 0 : B.kc[p] - 1, B.kx[p] - 1});
+        if (has_w && B.kc2[p] > 0) steps = std::max(steps, (cmp(p) ? 0 : B.kc[p] - 1) + B.kc2[p] - 1);
     }
-    if (has_w) {
+    if (need_plus) {
         // |+> as a vector: the GEMM's row (l, k) = (0, 0)
         if (launch_fill(const_cast<float*>(B.plus), N, psi0, N, 0.0f, stm))
             return fail(c, MQW_EDEVICE, "fill: %s", last_kernel_error());
@@ -1071,7 +1086,7 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
     for (int q = 0; q < steps; q++) {
         Op op{4, P.jobs.size(), 0};
         for (size_t p = 0; p < ps.size(); p++) {
-            if (q < B.kc[p] - 1) {
+            if (!cmp(p) && q < B.kc[p] - 1) {
                 float* V = const_cast<float*>(B.V[p]); // V_1 .. V_{kc-1}
                 TermJob<float> j = job(p, *seg0[p], q);
                 j.out = V + (size_t)q * N;
@@ -1121,20 +1136,26 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
             float* W = const_cast<float*>(B.W[p]);
             const float* V = B.V[p];
             auto row = [&](int ll, int kk) { return W + ((size_t)kk * (kc2 - 1) + (size_t)(ll - 1)) * N; };
-            // W_{l+1,kk} with l = q - kk
-            for (int kk = 0; kk < kc1 && kk <= q; kk++) {
-                const int l = q - kk;
+            const bool cq = cmp(p);
+            // W_0kk: |+> (nullptr: the constant psi0) for kk = 0, else V_kk;
+            // compressed: Q_0 at plusv, Q_kk = V_kk
+            auto seed = [&](int kk) -> const float* {
+                return kk > 0 ? V + (size_t)(kk - 1) * N : cq ? B.plusv[p] : nullptr;
+            };
+            // W_{l+1,kk} with l = q - kk (compressed: l = q)
+            for (int kk = 0; kk < kc1 && (cq || kk <= q); kk++) {
+                const int l = cq ? q : q - kk;
                 if (l >= kc2 - 1) continue;
                 TermJob<float> j = job(p, *seg1[p], l);
                 j.out = row(l + 1, kk);
                 uint32_t flags = 0;
                 double b = 8.0; // E read, output write
                 if (l == 0) {
-                    if (kk == 0) {
+                    if (!seed(kk)) {
                         flags |= SRC1_PSI | F_PSI_CONST;
                     } else {
                         flags |= SRC1_ARRAY;
-                        j.b1 = V + (size_t)(kk - 1) * N;
+                        j.b1 = seed(kk);
                         b += 4.0;
                     }
                     flags |= SRC2_ZERO;
@@ -1144,11 +1165,11 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
                     j.w2r = 1.0f;
                     b += 4.0;
                     if (l == 1) {
-                        if (kk == 0) {
+                        if (!seed(kk)) {
                             flags |= SRC2_PSI | F_PSI_CONST;
                         } else {
                             flags |= SRC2_ARRAY;
-                            j.b2 = V + (size_t)(kk - 1) * N;
+                            j.b2 = seed(kk);
                             b += 4.0;
                         }
                     } else {
@@ -1183,6 +1204,184 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
     c->stats.gpu_ms += ms_sweep + ms_aux;
     c->stats.aux_ms += ms_sweep + ms_aux;
     c->stats.basis_ms += ms_basis;
+    return MQW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Compressed first-stage bases (see GramJob), for the instances whose second
+// stage comes from the shared basis W.  In chunks of instances whose V fit:
+// the V recurrences of every instance of the chunk in one plan, their Gram
+// pair sums on the device, on the host one SVD per instance and parity class
+// (compress_first_stage, on up to 16 threads), and a combine launch that
+// writes Q: item r of an instance holds Q_2r (even class) in its real half
+// and Q_2r+1 in its imaginary half, so its Q vectors lie in index order, N
+// floats apart.  kcq[p] = 2R (0 = not compressed), qbase[p] = Q_0, and every
+// trajectory of a compressed instance gets its coordinates in c1q.
+// ---------------------------------------------------------------------------
+constexpr double kCompressTol = 1e-7; // per-sample relative error of the kept span
+
+mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Problem*>& ps,
+                          std::vector<Traj>& trajs, const float* E, const std::vector<int>& kc1,
+                          const std::vector<int>& kc2, bool use_graph, std::vector<int>& kcq,
+                          std::vector<const float*>& qbase)
+{
+    const size_t N = (size_t)1 << n;
+    const float psi0 = (float)(1 / sqrt((double)N));
+    kcq.assign(ps.size(), 0);
+    qbase.assign(ps.size(), nullptr);
+    std::vector<size_t> cand;
+    for (size_t p = 0; p < ps.size(); p++)
+        if (kc2[p] > 0 && kc1[p] >= 8) cand.push_back(p);
+    if (cand.empty()) return MQW_OK;
+    std::vector<std::vector<size_t>> ptraj(ps.size());
+    for (size_t t = 0; t < trajs.size(); t++) ptraj[trajs[t].prob].push_back(t);
+    const size_t vcap = (size_t)16 << 30;
+    const uint32_t nchunk = (uint32_t)std::min<size_t>(64, N / 64);
+    size_t i0 = 0;
+    int chunk_id = 0;
+    while (i0 < cand.size()) {
+        size_t i1 = i0, vbytes = 0;
+        while (i1 < cand.size()) {
+            const size_t b = sizeof(float) * N * (size_t)(kc1[cand[i1]] - 1);
+            if (i1 > i0 && vbytes + b > vcap) break;
+            vbytes += b;
+            i1++;
+        }
+        char* vbuf = (char*)dev_get(c, "cmp_V", vbytes);
+        if (!vbuf) return fail(c, MQW_ENOMEM, "device allocation failed for first-stage bases");
+        // V recurrences of the chunk (no X, no W)
+        Bases vb;
+        vb.V.assign(ps.size(), nullptr);
+        vb.kc.assign(ps.size(), 0);
+        vb.X.assign(ps.size(), nullptr);
+        vb.kx.assign(ps.size(), 0);
+        vb.kc2.assign(ps.size(), 0);
+        size_t off = 0, pairs = 0;
+        uint32_t kcmax = 0;
+        std::vector<size_t> goff(ps.size(), 0);
+        for (size_t i = i0; i < i1; i++) {
+            const size_t p = cand[i];
+            vb.kc[p] = kc1[p];
+            vb.V[p] = (const float*)(vbuf + off);
+            off += sizeof(float) * N * (size_t)(kc1[p] - 1);
+            goff[p] = pairs;
+            pairs += gram_pairs((uint32_t)kc1[p]);
+            kcmax = std::max<uint32_t>(kcmax, (uint32_t)kc1[p]);
+        }
+        mqw_status st = run_bases(c, n, k, ps, trajs, E, vb, use_graph);
+        if (st) return st;
+        // Gram pair sums
+        const size_t njobs = i1 - i0;
+        const size_t pmax = gram_pairs(kcmax <= 32 ? 32u : 64u);
+        double* gram_d = (double*)dev_get(c, "cmp_gram", sizeof(double) * pairs);
+        double* part_d = (double*)dev_get(c, "cmp_gram_part", sizeof(double) * njobs * nchunk * pmax);
+        GramJob* gj_d = (GramJob*)dev_get(c, "cmp_gram_jobs", sizeof(GramJob) * njobs);
+        if (!gram_d || !part_d || !gj_d) return fail(c, MQW_ENOMEM, "device allocation failed for Gram matrices");
+        std::vector<GramJob> gj(njobs);
+        for (size_t i = i0; i < i1; i++) {
+            const size_t p = cand[i];
+            gj[i - i0] = GramJob{vb.V[p], gram_d + goff[p], (uint32_t)kc1[p], psi0};
+        }
+        EventPair ev;
+        HIPCHK(c, ev.create());
+        HIPCHK(c, hipEventRecord(ev.a, c->stream));
+        HIPCHK(c, hipMemcpyAsync(gj_d, gj.data(), sizeof(GramJob) * njobs, hipMemcpyHostToDevice, c->stream));
+        if (launch_gram(gj_d, (uint32_t)njobs, kcmax, n, nchunk, part_d, c->stream))
+            return fail(c, MQW_EDEVICE, "gram: %s", last_kernel_error());
+        std::vector<double> gram(pairs);
+        HIPCHK(c, hipMemcpyAsync(gram.data(), gram_d, sizeof(double) * pairs, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        // one SVD per instance and class
+        std::vector<FirstStageCompression> fc(njobs);
+        std::vector<char> ok(njobs, 0);
+        std::vector<std::vector<double>> rows(njobs);
+        auto work = [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; i++) {
+                const size_t p = cand[i0 + i];
+                const int kc = kc1[p];
+                const std::vector<size_t>& tl = ptraj[p];
+                rows[i].assign(tl.size() * (size_t)kc, 0.0);
+                std::vector<const double*> cp(tl.size());
+                for (size_t j = 0; j < tl.size(); j++) {
+                    const std::vector<double>& cf = trajs[tl[j]].segs[0].c;
+                    for (int q = 0; q < kc && q < (int)cf.size(); q++) rows[i][j * kc + q] = cf[q];
+                    cp[j] = &rows[i][j * kc];
+                }
+                ok[i] = compress_first_stage(kc, &gram[goff[p]], cp, kCompressTol, fc[i]);
+            }
+        };
+        {
+            const unsigned hw = std::thread::hardware_concurrency();
+            const size_t nt = std::min<size_t>({(size_t)(hw ? hw : 1), 16, njobs});
+            std::vector<std::thread> th;
+            for (size_t t = 1; t < nt; t++) th.emplace_back(work, njobs * t / nt, njobs * (t + 1) / nt);
+            work(0, njobs / nt);
+            for (std::thread& x : th) x.join();
+        }
+        // Q: one combine job per compressed instance, R items of 2N floats
+        size_t qbytes = 0, nitems = 0;
+        for (size_t i = 0; i < njobs; i++)
+            if (ok[i]) {
+                qbytes += sizeof(float) * 2 * N * (size_t)fc[i].R;
+                nitems += (size_t)fc[i].R;
+            }
+        if (nitems > 0) {
+            char* qbuf = (char*)dev_get(c, "cmp_Q" + std::to_string(chunk_id), qbytes);
+            double* dummy = (double*)dev_get(c, "cmp_norm", sizeof(double) * k.parts * nitems);
+            CombineJob* cj_d = (CombineJob*)dev_get(c, "cmp_cjobs", sizeof(CombineJob) * njobs);
+            CombineItem* ci_d = (CombineItem*)dev_get(c, "cmp_citems", sizeof(CombineItem) * nitems);
+            float* cf_d = (float*)dev_get(c, "cmp_coefs", sizeof(float) * nitems * kcmax);
+            if (!qbuf || !dummy || !cj_d || !ci_d || !cf_d)
+                return fail(c, MQW_ENOMEM, "device allocation failed for compressed bases");
+            std::vector<CombineJob> cj;
+            std::vector<CombineItem> ci;
+            std::vector<float> cf;
+            size_t qoff = 0;
+            for (size_t i = 0; i < njobs; i++) {
+                if (!ok[i]) continue;
+                const size_t p = cand[i0 + i];
+                const int kc = kc1[p], R = fc[i].R;
+                float* Q = (float*)(qbuf + qoff);
+                qoff += sizeof(float) * 2 * N * (size_t)R;
+                cj.push_back(CombineJob{vb.V[p], (uint32_t)kc, (uint32_t)R, (uint32_t)ci.size(), psi0});
+                for (int r = 0; r < R; r++) {
+                    CombineItem it{};
+                    it.out = Q + (size_t)2 * r * N;
+                    it.norm_part = dummy + ci.size() * k.parts;
+                    it.coef_off = (uint32_t)cf.size();
+                    for (int q = 0; q < kc; q++) cf.push_back((float)fc[i].B[(size_t)(2 * r + (q & 1)) * kc + q]);
+                    ci.push_back(it);
+                }
+                kcq[p] = 2 * R;
+                qbase[p] = Q;
+                const std::vector<size_t>& tl = ptraj[p];
+                for (size_t j = 0; j < tl.size(); j++) {
+                    std::vector<float>& u = trajs[tl[j]].c1q;
+                    u.resize((size_t)2 * R);
+                    for (int q = 0; q < 2 * R; q++) u[q] = (float)fc[i].u[j * 2 * R + q];
+                }
+                c->stats.compressed_instances++;
+                c->stats.compressed_vectors += (uint64_t)(2 * R);
+                c->stats.uncompressed_vectors += (uint64_t)kc;
+                c->stats.aux_bytes += (double)N * (4.0 * (kc - 1) + 8.0 * R);
+            }
+            HIPCHK(c, hipMemcpyAsync(cj_d, cj.data(), sizeof(CombineJob) * cj.size(), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(ci_d, ci.data(), sizeof(CombineItem) * ci.size(), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(cf_d, cf.data(), sizeof(float) * cf.size(), hipMemcpyHostToDevice, c->stream));
+            if (launch_combine(cj_d, (uint32_t)cj.size(), ci_d, cf_d, n, k.parts, kcmax, c->stream))
+                return fail(c, MQW_EDEVICE, "basis compression: %s", last_kernel_error());
+        }
+        HIPCHK(c, hipEventRecord(ev.b, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream)); // cmp_V is reused by the next chunk
+        float ms = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&ms, ev.a, ev.b));
+        c->stats.gpu_ms += ms;
+        c->stats.aux_ms += ms;
+        for (size_t i = i0; i < i1; i++) c->stats.aux_bytes += 4.0 * (double)N * (double)(kc1[cand[i]] - 1);
+        c->stats.aux_launches += 3;
+        i0 = i1;
+        chunk_id++;
+    }
     return MQW_OK;
 }
 
@@ -1300,11 +1499,19 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     }
     const size_t vcap = o.max_device_bytes ? o.max_device_bytes / 4 : ((size_t)16 << 30);
     const size_t wcap = o.max_device_bytes ? o.max_device_bytes / 4 : ((size_t)32 << 30);
+    // compressed first-stage bases (kcq = 2R vectors from Q, else the kc1 V_k)
+    std::vector<int> kcq(ps.size(), 0);
+    std::vector<const float*> qbase(ps.size(), nullptr);
+    auto kcw = [&](size_t p) { return kcq[p] > 0 ? kcq[p] : kc1[p]; };
     auto wbytes = [&](size_t p) -> size_t {
-        return kc2[p] ? sizeof(float) * N * (size_t)kc1[p] * (size_t)(kc2[p] - 1) : 0;
+        return kc2[p] ? sizeof(float) * N * (size_t)kcw(p) * (size_t)(kc2[p] - 1) : 0;
     };
     for (size_t p = 0; p < ps.size(); p++)
         if (wbytes(p) > wcap) kc2[p] = 0;
+    if (basis_path && o.compress_bases) {
+        st = compress_bases(c, n, k, ps, trajs, (const float*)E, kc1, kc2, o.use_graph != 0, kcq, qbase);
+        if (st) return st;
+    }
     // problem batches whose second-stage bases fit together
     std::vector<std::pair<size_t, size_t>> pbatches;
     {
@@ -1376,12 +1583,19 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             bb.offx.assign(ps.size(), 0);
             bb.offw.assign(ps.size(), 0);
             bb.qoff.assign(ps.size(), 0);
+            bases.cmp.assign(ps.size(), 0);
+            bases.plusv.assign(ps.size(), nullptr);
             size_t vbytes = 0;
             std::vector<uint32_t> rows;
             for (size_t p = pb.first; p < pb.second; p++) {
                 const size_t bv = sizeof(float) * N * (size_t)std::max(kc1[p] - 1, 0);
                 const size_t bx = sizeof(float) * N * (size_t)kx[p];
-                if (kc1[p] > 0 && vbytes + bv <= vcap) {
+                if (kcq[p] > 0) { // the compressed basis Q, computed before the batches
+                    bases.kc[p] = kcq[p];
+                    bases.cmp[p] = 1;
+                    bases.plusv[p] = qbase[p];
+                    bases.V[p] = qbase[p] + N;
+                } else if (kc1[p] > 0 && vbytes + bv <= vcap) {
                     bases.kc[p] = kc1[p];
                     bb.offv[p] = bb.total;
                     bb.total += bv;
@@ -1397,7 +1611,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
                     bases.kc2[p] = kc2[p];
                     bb.offw[p] = bb.total;
                     bb.total += wbytes(p);
-                    second_stage_rows(kc1[p], kc2[p], rows, bases.q0[p], bases.q1[p]);
+                    second_stage_rows(bases.kc[p], kc2[p], rows, bases.q0[p], bases.q1[p]);
                     bb.qoff[p] = bb.rows.size();
                     bb.rows.insert(bb.rows.end(), rows.begin(), rows.end());
                     bb.any_w = true;
@@ -1458,7 +1672,8 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
                                      bs));
         }
         for (size_t p = pb.first; p < pb.second; p++) {
-            if (bases.kc[p] > 0) bases.V[p] = (const float*)(base + bb.offv[p]); // unused when kc = 1 (V_0 only)
+            if (bases.kc[p] > 0 && !bases.cmp[p])
+                bases.V[p] = (const float*)(base + bb.offv[p]); // unused when kc = 1 (V_0 only)
             if (bases.kx[p] > 0) bases.X[p] = (const float*)(base + bb.offx[p]);
             if (bases.kc2[p] > 0) {
                 bases.W[p] = (const float*)(base + bb.offw[p]);
@@ -1716,6 +1931,7 @@ void mqw_opts_default(mqw_opts* o)
     o->overlap_bases = 1;
     o->mixer_kind = MQW_MIXER_XSTRING;
     o->second_stage_gemm = MQW_GEMM_SPLIT_FP16;
+    o->compress_bases = 1;
     o->sample_probs = nullptr;
 }
 

@@ -1,0 +1,111 @@
+// Gram matrices of the shared first-stage bases (the compressed second stage,
+// see GramJob): per instance, the sums over all amplitudes of V_k1 V_k2 for
+// k1 <= k2 of equal parity, with V_0 = psi0 * ones.  Products of FP32 values
+// are exact in FP64 and the sums are FP64: per workgroup over a chunk of
+// amplitudes, then over the chunks in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include "mqw_internal.h"
+
+using namespace mqw;
+
+namespace {
+
+constexpr int kSlab = 64; // amplitudes per LDS slab
+
+// gram_pair_start / gram_pairs of mqw_internal.h, on the device
+__device__ __forceinline__ int pair_start(int k2) { return k2 == 0 ? 0 : k2 + ((k2 - 1) * (k2 - 1)) / 4; }
+
+// One workgroup: one job, one chunk of N / nchunk amplitudes.  Thread t keeps
+// the FP64 sums of pairs t, t + 256, ... (at most PMAX / 256 of them).
+template <int KC>
+__global__ void __launch_bounds__(256)
+gram_kernel(const GramJob* __restrict__ jobs, uint32_t njobs, int n, uint32_t nchunk, double* __restrict__ partial)
+{
+    constexpr int PMAX = KC + ((KC - 1) * (KC - 1)) / 4;
+    constexpr int PT = (PMAX + 255) / 256;
+    __shared__ float s[KC][kSlab + 1];
+    __shared__ uint16_t pr[PMAX];
+    const uint32_t job = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
+    if (job >= njobs) return;
+    const GramJob J = jobs[job];
+    const int kc = (int)J.kc;
+    const int t = threadIdx.x;
+    const int P = pair_start(kc);
+    if (t < kc) { // the pairs (k1, t) of equal parity, k1 <= t
+        const int b = pair_start(t);
+        for (int k1 = t & 1, q = 0; k1 <= t; k1 += 2, q++) pr[b + q] = (uint16_t)(k1 | (t << 8));
+    }
+    const uint64_t N = 1ull << n;
+    const uint64_t C = N / nchunk;
+    const uint64_t base = (uint64_t)chunk * C;
+    double acc[PT];
+#pragma unroll
+    for (int i = 0; i < PT; i++) acc[i] = 0.0;
+    for (uint64_t a0 = 0; a0 < C; a0 += kSlab) {
+        __syncthreads();
+        for (int i = t; i < kc * kSlab; i += 256) {
+            const int k = i / kSlab, x = i % kSlab;
+            s[k][x] = k == 0 ? J.psi0 : J.V[(uint64_t)(k - 1) * N + base + a0 + (uint64_t)x];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PT; i++) {
+            const int p = t + 256 * i;
+            if (p < P) {
+                const int k1 = pr[p] & 0xff, k2 = pr[p] >> 8;
+                double a = acc[i];
+                for (int x = 0; x < kSlab; x++) a = fma((double)s[k1][x], (double)s[k2][x], a);
+                acc[i] = a;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < PT; i++) {
+        const int p = t + 256 * i;
+        if (p < P) partial[((uint64_t)job * nchunk + chunk) * PMAX + p] = acc[i];
+    }
+}
+
+template <int KC>
+__global__ void __launch_bounds__(256)
+gram_reduce_kernel(const GramJob* __restrict__ jobs, uint32_t nchunk, const double* __restrict__ partial)
+{
+    constexpr int PMAX = KC + ((KC - 1) * (KC - 1)) / 4;
+    const uint32_t job = blockIdx.x;
+    const GramJob J = jobs[job];
+    const int P = pair_start((int)J.kc);
+    for (int p = threadIdx.x; p < P; p += 256) {
+        double sum = 0.0;
+        for (uint32_t c = 0; c < nchunk; c++) sum += partial[((uint64_t)job * nchunk + c) * PMAX + p];
+        J.out[p] = sum;
+    }
+}
+
+template <int KC>
+int launch_gram_kc(const GramJob* jobs, uint32_t njobs, int n, uint32_t nchunk, double* partial, hipStream_t st)
+{
+    hipLaunchKernelGGL(gram_kernel<KC>, dim3(njobs * nchunk), dim3(256), 0, st, jobs, njobs, n, nchunk, partial);
+    if (check_launch("gram")) return -1;
+    hipLaunchKernelGGL(gram_reduce_kernel<KC>, dim3(njobs), dim3(256), 0, st, jobs, nchunk, partial);
+    return check_launch("gram_reduce");
+}
+
+} // namespace
+
+namespace mqw {
+
+int launch_gram(const GramJob* jobs_dev, uint32_t njobs, uint32_t kc_max, int n, uint32_t nchunk, double* partial,
+                void* stream)
+{
+    if (njobs == 0) return 0;
+    const uint64_t N = 1ull << n;
+    if (nchunk == 0 || N % nchunk != 0 || (N / nchunk) % kSlab != 0) return -2;
+    if ((uint64_t)njobs * nchunk > 0x7fffffffull) return -3;
+    hipStream_t st = (hipStream_t)stream;
+    if (kc_max <= 32) return launch_gram_kc<32>(jobs_dev, njobs, n, nchunk, partial, st);
+    if (kc_max <= 64) return launch_gram_kc<64>(jobs_dev, njobs, n, nchunk, partial, st);
+    return -2;
+}
+
+} // namespace mqw

@@ -117,6 +117,40 @@ struct CombineJob {
     uint32_t item_off;   // first item
     float psi0;
 };
+// Compressed first-stage basis for the shared second stage.  The samples'
+// first-stage states psi_j = sum_k c_j[k] i^(k&1) V_k span far fewer
+// directions than kc: T_k(Ht)|+> for k < kc are numerically dependent, since
+// the spectral measure of |+> under Ht is narrow.  Per parity class (k even:
+// real parts, k odd: imaginary parts) the Gram matrix G = V V^T (FP64 sums of
+// the FP32 vectors' products) and the samples' coefficients give, by an SVD
+// of A G^(1/2), orthonormal combinations Q_r = sum_k B[r][k] V_k and sample
+// coordinates u_j[r] with psi_j = sum_r u_j[r] Q_r to within a set relative
+// error per sample; the second stage's basis is then T_l(Ht_2) Q_r.
+struct GramJob {
+    const float* V;     // V_1 .. V_{kc-1}, N floats each (V_0 = psi0 * ones)
+    double* out;        // gram_pairs(kc) pair sums, written by the reduce launch
+    uint32_t kc;
+    float psi0;
+};
+// pairs (k1 <= k2, k1 = k2 mod 2) ordered by k2 then k1: pair_start(k2) + k1 / 2
+inline uint32_t gram_pair_start(uint32_t k2) { return k2 == 0 ? 0u : k2 + ((k2 - 1) * (k2 - 1)) / 4; }
+inline uint32_t gram_pairs(uint32_t kc) { return gram_pair_start(kc); }
+// Pair sums of every job's vectors: njobs x nchunk partial sums of
+// gram_pairs(kc_max) doubles in `partial`, then their fixed-order sum per job.
+int launch_gram(const GramJob* jobs_dev, uint32_t njobs, uint32_t kc_max, int n, uint32_t nchunk, double* partial,
+                void* stream);
+// The compression of one instance from its Gram pair sums and its samples'
+// first-stage coefficients (each row kc long, zero padded): rank R per class
+// (the same for both), B (2R x kc, row 2r + cls combines class cls) and u
+// (samples x 2R); false when 2R would not be below kc.
+struct FirstStageCompression {
+    int R = 0;
+    std::vector<double> B;
+    std::vector<double> u;
+};
+bool compress_first_stage(int kc, const double* gram, const std::vector<const double*>& coefs, double rel_tol,
+                          FirstStageCompression& out);
+
 int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem* items_dev, const float* coefs_dev,
                    int n, uint32_t nparts, uint32_t kc_max, void* stream);
 
@@ -277,6 +311,8 @@ int launch_lds(int n, const LdsTraj* trajs, uint32_t ntraj, const LdsSeg* segs, 
 int lds_grid(int n, int device);
 
 const char* last_kernel_error();
+// After a launch: 0, or -1 with the runtime's error kept for last_kernel_error.
+int check_launch(const char* what);
 // Counts of the shifted diagonal in `bins` equal bins over [-r, r] (counts zeroed by the caller).
 int diag_histogram(const float* hp, uint64_t N, float r, int bins, uint32_t* counts, void* stream);
 // Power sums k = 1..4 of (hp + shift - mu): 4 doubles per block into part.

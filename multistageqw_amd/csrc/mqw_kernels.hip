@@ -24,7 +24,7 @@ namespace mqw {
 static thread_local const char* g_kernel_error = "";
 const char* last_kernel_error() { return g_kernel_error; }
 
-static int check_launch(const char* what)
+int check_launch(const char* what)
 {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

@@ -51,6 +51,7 @@ def test_headline_path_vs_oracle(ctx, headline_oracle, gemm):
     # the timed path: shared bases, the second-stage GEMM, the adjoint readout, no term sweeps
     assert st["gemm_launches"] > 0 and st["basis_launches"] > 0
     assert st["sweeps"] == 0 and st["lds_trajectories"] == 0
+    assert st["compressed_instances"] == 2  # the second-stage basis from the compressed first stage
     assert np.array_equal(tt, tto)
     dp = np.abs(sp - spo)
     dd = np.abs(sp - spd)
