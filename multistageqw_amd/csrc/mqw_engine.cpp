@@ -1209,7 +1209,8 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
 
 // ---------------------------------------------------------------------------
 // Compressed first-stage bases (see GramJob), for the instances whose second
-// stage comes from the shared basis W.  In chunks of instances whose V fit:
+// stage comes from the shared basis W.  In chunks of instances whose V fit
+// in vcap bytes:
 // the V recurrences of every instance of the chunk in one plan, their Gram
 // pair sums on the device, on the host one SVD per instance and parity class
 // (compress_first_stage, on up to 16 threads), and a combine launch that
@@ -1222,7 +1223,7 @@ constexpr double kCompressTol = 1e-7; // per-sample relative error of the kept s
 
 mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Problem*>& ps,
                           std::vector<Traj>& trajs, const float* E, const std::vector<int>& kc1,
-                          const std::vector<int>& kc2, bool use_graph, std::vector<int>& kcq,
+                          const std::vector<int>& kc2, bool use_graph, size_t vcap, std::vector<int>& kcq,
                           std::vector<const float*>& qbase)
 {
     const size_t N = (size_t)1 << n;
@@ -1235,7 +1236,6 @@ mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vecto
     if (cand.empty()) return MQW_OK;
     std::vector<std::vector<size_t>> ptraj(ps.size());
     for (size_t t = 0; t < trajs.size(); t++) ptraj[trajs[t].prob].push_back(t);
-    const size_t vcap = (size_t)16 << 30;
     const uint32_t nchunk = (uint32_t)std::min<size_t>(64, N / 64);
     size_t i0 = 0;
     int chunk_id = 0;
@@ -1360,9 +1360,6 @@ mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vecto
                     u.resize((size_t)2 * R);
                     for (int q = 0; q < 2 * R; q++) u[q] = (float)fc[i].u[j * 2 * R + q];
                 }
-                c->stats.compressed_instances++;
-                c->stats.compressed_vectors += (uint64_t)(2 * R);
-                c->stats.uncompressed_vectors += (uint64_t)kc;
                 c->stats.aux_bytes += (double)N * (4.0 * (kc - 1) + 8.0 * R);
             }
             HIPCHK(c, hipMemcpyAsync(cj_d, cj.data(), sizeof(CombineJob) * cj.size(), hipMemcpyHostToDevice, c->stream));
@@ -1506,11 +1503,23 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     auto wbytes = [&](size_t p) -> size_t {
         return kc2[p] ? sizeof(float) * N * (size_t)kcw(p) * (size_t)(kc2[p] - 1) : 0;
     };
-    for (size_t p = 0; p < ps.size(); p++)
-        if (wbytes(p) > wcap) kc2[p] = 0;
     if (basis_path && o.compress_bases) {
-        st = compress_bases(c, n, k, ps, trajs, (const float*)E, kc1, kc2, o.use_graph != 0, kcq, qbase);
+        st = compress_bases(c, n, k, ps, trajs, (const float*)E, kc1, kc2, o.use_graph != 0, vcap, kcq, qbase);
         if (st) return st;
+    }
+    // second-stage bases that do not fit a batch stay per-sample terms (their
+    // first stage then comes from V, recomputed in the batch)
+    for (size_t p = 0; p < ps.size(); p++) {
+        if (wbytes(p) > wcap) {
+            kc2[p] = 0;
+            kcq[p] = 0;
+            qbase[p] = nullptr;
+        }
+        if (kcq[p] > 0) {
+            c->stats.compressed_instances++;
+            c->stats.compressed_vectors += (uint64_t)kcq[p];
+            c->stats.uncompressed_vectors += (uint64_t)kc1[p];
+        }
     }
     // problem batches whose second-stage bases fit together
     std::vector<std::pair<size_t, size_t>> pbatches;

@@ -17,25 +17,32 @@ def _say(*a):
     print("[case]", *a, flush=True)
 
 
+# The memory caps give one instance per batch of second-stage bases built from every V_k
+# (compress_bases=False); the last case compresses them, with a cap that still splits.
 @pytest.mark.parametrize("n,m,count,samples,kw", [
-    (16, 3, 3, 64, dict(max_device_bytes=2 << 30)),                     # three batches (W 336-394 MiB each)
-    (16, 3, 2, 64, dict(max_device_bytes=2 << 30, use_graph=False)),    # two batches, no graphs
-    (16, 3, 3, 70, dict(max_device_bytes=2 << 30, group_bytes=24 << 20)),  # several groups per batch
+    (16, 3, 3, 64, dict(max_device_bytes=2 << 30, compress_bases=False)),  # three batches (W 336-394 MiB each)
+    (16, 3, 2, 64, dict(max_device_bytes=2 << 30, use_graph=False, compress_bases=False)),  # no graphs
+    (16, 3, 3, 70, dict(max_device_bytes=2 << 30, group_bytes=24 << 20, compress_bases=False)),  # groups
     # middle stages as terms; the seed of the second-stage test, whose two instances both share
     # their second stage (W 203 MiB each)
-    (16, 5, 2, 48, dict(max_device_bytes=3 << 29, seed=6165)),
+    (16, 5, 2, 48, dict(max_device_bytes=3 << 29, seed=6165, compress_bases=False)),
+    # compressed bases (W about 190-250 MiB each, 336-456 MiB uncompressed): count = 0 means
+    # at least two batches of 512 MiB
+    (16, 3, 0, 64, dict(max_device_bytes=2 << 30)),
 ])
 def test_overlapped_bases_identical(ctx, oracle, n, m, count, samples, kw):
     _say("overlapped bases n", n, "m", m, "count", count, "samples", samples, kw)
     kw = dict(kw)
-    recs = M.generate_sk(n, count, kw.pop("seed", 7000 + 10 * n + m))
+    recs = M.generate_sk(n, count or 3, kw.pop("seed", 7000 + 10 * n + m))
     p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, **kw)
     st = ctx.last_stats()
-    assert st["basis_batches"] == count and st["overlap_batches"] == count
+    batches = count or st["basis_batches"]
+    assert st["basis_batches"] == batches >= 2 and st["overlap_batches"] == batches
+    assert st["compressed_instances"] == (0 if kw.get("compress_bases") is False else len(recs))
     p_off, tt_off, sp_off = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0,
                                       overlap_bases=False, **kw)
     st_off = ctx.last_stats()
-    assert st_off["basis_batches"] == count and st_off["overlap_batches"] == 0
+    assert st_off["basis_batches"] == batches and st_off["overlap_batches"] == 0
     assert st["gemm_launches"] == st_off["gemm_launches"] > 0
     assert np.array_equal(sp, sp_off) and np.array_equal(p, p_off) and np.array_equal(tt, tt_off)
     po, tto, spo = oracle.solve(n, m, recs, samples=samples, threads=8, with_samples=True)
@@ -49,9 +56,10 @@ def test_overlap_repeated_solves(ctx):
     n, m, samples = 16, 3, 64
     recs = M.generate_sk(n, 3, 7101)
     _say("overlapped bases: three solves in a row")
-    first = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, max_device_bytes=2 << 30)[2]
+    kw = dict(samples=samples, with_samples=True, small_n_max=0, max_device_bytes=2 << 30, compress_bases=False)
+    first = ctx.solve(n, m, recs, **kw)[2]
     for _ in range(2):
-        again = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, max_device_bytes=2 << 30)[2]
+        again = ctx.solve(n, m, recs, **kw)[2]
         assert ctx.last_stats()["overlap_batches"] == 3
         assert np.array_equal(first, again)
 
