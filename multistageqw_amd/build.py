@@ -68,9 +68,6 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
-    copy_src = os.path.join(CSRC, "hbm_copy.hip")
-    _run([HIPCC, "-x", "hip", f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, "-o", os.path.join(HERE, "hbm_copy"),
-          copy_src])
     cli_src = os.path.join(CSRC, "mqw_cli.cpp")
     if os.path.exists(cli_src):
         _run([HIPCC, "-x", "c++", *COMMON, "-D__HIP_PLATFORM_AMD__", "-o", CLI, cli_src, f"-L{HERE}", "-lmqw",
@@ -78,5 +75,15 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_copy_probe() -> str:
+    """The plain-copy kernel whose counter readings calibrate the HBM traffic
+    profiles (hbm_copy): measurement infrastructure, not part of libmqw."""
+    out = os.path.join(HERE, "hbm_copy")
+    _run([HIPCC, "-x", "hip", f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, "-o", out,
+          os.path.join(CSRC, "hbm_copy.hip")])
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_copy_probe())
