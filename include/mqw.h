@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 12
+#define MQW_ABI_VERSION 13
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -137,6 +137,13 @@ typedef struct {
                                 smallest set of orthonormal combinations Q_r that holds every sample's
                                 state to within 1e-7 relative, and W_lr = T_l(H2/onenorm2) Q_r replaces
                                 W_lk (about 20 instead of 37 chains at n = 20); 0 = W from every V_k */
+    int direct_m1;           /* 1 (default) = an m == 1 instance's chained samples (MultiQW.cpp:426-430: each
+                                sample continues from the previous one by the next time increment) are
+                                evaluated directly: sample j is exp(iH T_j)|+> with T_j the sum of the
+                                increments, one Chebyshev series in the instance's basis T_k(H/onenorm)|+>,
+                                its amplitude and norm from V_k[E_loc] and the basis's Gram matrix (no term
+                                sweeps).  It differs from the chain by the chain's accumulated truncation,
+                                measured below 4e-7 in probability; 0 = the chain, as the reference */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are
@@ -306,6 +313,7 @@ typedef struct {
     uint64_t compressed_instances; /* instances whose second-stage basis came from a compressed first stage */
     uint64_t compressed_vectors;   /* their kept first-stage vectors (2R each), summed */
     uint64_t uncompressed_vectors; /* their first-stage vectors before compression (kc1 each), summed */
+    uint64_t direct_chains;        /* m == 1 instances evaluated directly (direct_m1) */
 } mqw_stats;
 mqw_status mqw_last_stats(const mqw_ctx* ctx, mqw_stats* out);
 

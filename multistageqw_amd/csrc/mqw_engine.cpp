@@ -206,6 +206,7 @@ struct Traj {
     bool second_done = false;     // the first two stages come from the shared bases (a combine2 op)
     bool adjoint = false;         // the last stage is read out by the adjoint dots (no terms)
     std::vector<float> c1q;       // with a compressed first-stage basis: the sample's coordinates in it
+    bool direct_done = false;     // an m == 1 chain evaluated directly (direct_m1): no terms run
 };
 
 // Chebyshev coefficients of every segment.  Segments are independent (the
@@ -1208,6 +1209,146 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
 }
 
 // ---------------------------------------------------------------------------
+// m == 1 instances evaluated directly.  The reference chains its samples: it
+// sorts the m == 1 stage times, differences them, and propagates each sample's
+// state from the previous sample's by the increment (MultiQW.cpp:426-430,447).
+// Sample j's state is therefore exp(iH T_j)|+> with T_j the sum of the first
+// j + 1 increments, up to each step's truncation (its tail, <= tol) and
+// renormalisation.  Here it is evaluated as one series,
+//     psi_j = sum_k a_jk V_k,  V_k = T_k(Ht)|+>,  a_jk = c_k(onenorm T_j) i^(k&1),
+// from the instance's first-stage basis V: the amplitude is
+// sum_k a_jk V_k[E_loc] and the squared norm c_e^T G_e c_e + c_o^T G_o c_o with
+// G the Gram matrix per parity class (gram_kernel, FP64).  No term sweep runs.
+// In chunks of instances whose V fit in vcap bytes; the probabilities go to
+// probs (host) and probs_d.
+// ---------------------------------------------------------------------------
+mqw_status direct_m1(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Problem*>& ps, std::vector<Traj>& trajs,
+                     const float* E, double tol, bool use_graph, size_t vcap, std::vector<double>& probs,
+                     double* probs_d)
+{
+    const size_t N = (size_t)1 << n;
+    const float psi0 = (float)(1 / sqrt((double)N));
+    struct Chain {
+        size_t t;                           // its trajectory
+        int kc = 0;
+        std::vector<std::vector<double>> cf; // per sample: coefficients at onenorm * T_j (float values)
+    };
+    std::vector<Chain> ch;
+    for (size_t t = 0; t < trajs.size(); t++) {
+        const Traj& tr = trajs[t];
+        if (ps[tr.prob]->m != 1 || tr.segs.empty()) continue;
+        Chain q;
+        q.t = t;
+        q.cf.resize(tr.segs.size());
+        double T = 0.0;
+        const double onenorm = (double)tr.segs.front().onenorm;
+        for (size_t j = 0; j < tr.segs.size(); j++) {
+            T += (double)ps[tr.prob]->times[j];
+            cheb_coeffs_vec(onenorm * T, tol, q.cf[j]);
+            for (double& v : q.cf[j]) v = (double)(float)v; // the reference casts to float (:440)
+            q.kc = std::max(q.kc, (int)q.cf[j].size());
+        }
+        if (q.kc >= 2 && q.kc <= kGramMaxK) ch.push_back(std::move(q));
+    }
+    if (ch.empty()) return MQW_OK;
+    const uint32_t nchunk = (uint32_t)std::min<size_t>(64, N / 64);
+    size_t i0 = 0;
+    while (i0 < ch.size()) {
+        size_t i1 = i0, vbytes = 0;
+        while (i1 < ch.size()) {
+            const size_t b = sizeof(float) * N * (size_t)(ch[i1].kc - 1);
+            if (i1 > i0 && vbytes + b > vcap) break;
+            vbytes += b;
+            i1++;
+        }
+        char* vbuf = (char*)dev_get(c, "cmp_V", vbytes);
+        if (!vbuf) return fail(c, MQW_ENOMEM, "device allocation failed for first-stage bases");
+        Bases vb;
+        vb.V.assign(ps.size(), nullptr);
+        vb.kc.assign(ps.size(), 0);
+        vb.X.assign(ps.size(), nullptr);
+        vb.kx.assign(ps.size(), 0);
+        vb.kc2.assign(ps.size(), 0);
+        size_t off = 0, pairs = 0;
+        uint32_t kcmax = 0;
+        std::vector<size_t> goff(i1 - i0);
+        for (size_t i = i0; i < i1; i++) {
+            const size_t p = (size_t)trajs[ch[i].t].prob;
+            vb.kc[p] = ch[i].kc;
+            vb.V[p] = (const float*)(vbuf + off);
+            off += sizeof(float) * N * (size_t)(ch[i].kc - 1);
+            goff[i - i0] = pairs;
+            pairs += gram_pairs((uint32_t)ch[i].kc);
+            kcmax = std::max<uint32_t>(kcmax, (uint32_t)ch[i].kc);
+        }
+        mqw_status st = run_bases(c, n, k, ps, trajs, E, vb, use_graph);
+        if (st) return st;
+        const size_t njobs = i1 - i0;
+        const size_t pmax = gram_pmax(kcmax);
+        double* gram_d = (double*)dev_get(c, "cmp_gram", sizeof(double) * pairs);
+        double* part_d = (double*)dev_get(c, "cmp_gram_part", sizeof(double) * njobs * nchunk * pmax);
+        GramJob* gj_d = (GramJob*)dev_get(c, "cmp_gram_jobs", sizeof(GramJob) * njobs);
+        if (!gram_d || !part_d || !gj_d) return fail(c, MQW_ENOMEM, "device allocation failed for Gram matrices");
+        std::vector<GramJob> gj(njobs);
+        for (size_t i = i0; i < i1; i++) {
+            const size_t p = (size_t)trajs[ch[i].t].prob;
+            gj[i - i0] = GramJob{vb.V[p], gram_d + goff[i - i0], (uint32_t)ch[i].kc, psi0};
+        }
+        EventPair ev;
+        HIPCHK(c, ev.create());
+        HIPCHK(c, hipEventRecord(ev.a, c->stream));
+        HIPCHK(c, hipMemcpyAsync(gj_d, gj.data(), sizeof(GramJob) * njobs, hipMemcpyHostToDevice, c->stream));
+        if (launch_gram(gj_d, (uint32_t)njobs, kcmax, n, nchunk, part_d, c->stream))
+            return fail(c, MQW_EDEVICE, "gram: %s", last_kernel_error());
+        HIPCHK(c, hipEventRecord(ev.b, c->stream));
+        std::vector<double> gram(pairs);
+        HIPCHK(c, hipMemcpyAsync(gram.data(), gram_d, sizeof(double) * pairs, hipMemcpyDeviceToHost, c->stream));
+        // V_k[E_loc], k >= 1: one float per vector, N floats apart
+        std::vector<std::vector<float>> ve(njobs);
+        for (size_t i = i0; i < i1; i++) {
+            const size_t p = (size_t)trajs[ch[i].t].prob;
+            std::vector<float>& v = ve[i - i0];
+            v.assign((size_t)ch[i].kc, psi0);
+            HIPCHK(c, hipMemcpy2DAsync(v.data() + 1, sizeof(float), vb.V[p] + ps[p]->info.eloc, sizeof(float) * N,
+                                       sizeof(float), (size_t)(ch[i].kc - 1), hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream)); // cmp_V is reused by the next chunk
+        float ms = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&ms, ev.a, ev.b));
+        c->stats.gpu_ms += ms;
+        c->stats.aux_ms += ms;
+        c->stats.aux_launches += 2;
+        for (size_t i = i0; i < i1; i++) {
+            Chain& q = ch[i];
+            Traj& tr = trajs[q.t];
+            const Problem& pr = *ps[tr.prob];
+            const double* g = &gram[goff[i - i0]];
+            const std::vector<float>& v = ve[i - i0];
+            c->stats.aux_bytes += 4.0 * (double)N * (double)(q.kc - 1); // the Gram read of V
+            for (size_t j = 0; j < q.cf.size(); j++) {
+                const std::vector<double>& cf = q.cf[j];
+                const int K = (int)cf.size();
+                double re = 0.0, im = 0.0, nrm = 0.0;
+                for (int k2 = 0; k2 < K; k2++) {
+                    ((k2 & 1) ? im : re) += cf[k2] * (double)v[k2];
+                    const uint32_t b = gram_pair_start((uint32_t)k2);
+                    for (int k1 = k2 & 1; k1 <= k2; k1 += 2)
+                        nrm += (k1 == k2 ? 1.0 : 2.0) * cf[k1] * cf[k2] * g[b + (uint32_t)k1 / 2];
+                }
+                probs[pr.prob_base + j] = (re * re + im * im) / nrm;
+            }
+            HIPCHK(c, hipMemcpyAsync(probs_d + pr.prob_base, &probs[pr.prob_base], sizeof(double) * q.cf.size(),
+                                     hipMemcpyHostToDevice, c->stream));
+            tr.direct_done = true;
+            c->stats.direct_chains++;
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        i0 = i1;
+    }
+    return MQW_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Compressed first-stage bases (see GramJob), for the instances whose second
 // stage comes from the shared basis W.  In chunks of instances whose V fit
 // in vcap bytes:
@@ -1272,7 +1413,7 @@ mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vecto
         if (st) return st;
         // Gram pair sums
         const size_t njobs = i1 - i0;
-        const size_t pmax = gram_pairs(kcmax <= 32 ? 32u : 64u);
+        const size_t pmax = gram_pmax(kcmax);
         double* gram_d = (double*)dev_get(c, "cmp_gram", sizeof(double) * pairs);
         double* part_d = (double*)dev_get(c, "cmp_gram_part", sizeof(double) * njobs * nchunk * pmax);
         GramJob* gj_d = (GramJob*)dev_get(c, "cmp_gram_jobs", sizeof(GramJob) * njobs);
@@ -1469,6 +1610,11 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     // own terms) and the last stage's adjoint X (kx).  0 = not shared.
     const bool basis_path =
         std::is_same<real, float>::value && k.half && !mixer && n >= k.tb && (N / nparts) % 256 == 0;
+    const size_t vcap = o.max_device_bytes ? o.max_device_bytes / 4 : ((size_t)16 << 30);
+    if (basis_path && o.direct_m1) {
+        st = direct_m1(c, n, k, ps, trajs, (const float*)E, o.tol, o.use_graph != 0, vcap, probs, probs_d);
+        if (st) return st;
+    }
     std::vector<int> kc1(ps.size(), 0), kc2(ps.size(), 0), kx(ps.size(), 0);
     if (basis_path) {
         std::vector<double> own2(ps.size(), 0.0); // bytes per amplitude of the samples' own second-stage terms
@@ -1494,7 +1640,6 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
                 kc2[p] = 0;
         }
     }
-    const size_t vcap = o.max_device_bytes ? o.max_device_bytes / 4 : ((size_t)16 << 30);
     const size_t wcap = o.max_device_bytes ? o.max_device_bytes / 4 : ((size_t)32 << 30);
     // compressed first-stage bases (kcq = 2R vectors from Q, else the kc1 V_k)
     std::vector<int> kcq(ps.size(), 0);
@@ -1733,6 +1878,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         for (size_t p = pb.first; p < pb.second; p++)
             for (size_t t : ptraj[p]) {
                 Traj& tr = trajs[t];
+                if (tr.direct_done) continue;
                 tr.second_done = use_bases && !bases.kc2.empty() && bases.kc2[p] > 0;
                 tr.first_done = use_bases && !tr.second_done && bases.kc[p] > 0;
                 tr.adjoint = use_bases && bases.kx[p] > 0;
@@ -1941,6 +2087,7 @@ void mqw_opts_default(mqw_opts* o)
     o->mixer_kind = MQW_MIXER_XSTRING;
     o->second_stage_gemm = MQW_GEMM_SPLIT_FP16;
     o->compress_bases = 1;
+    o->direct_m1 = 1;
     o->sample_probs = nullptr;
 }
 

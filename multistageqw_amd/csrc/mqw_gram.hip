@@ -105,6 +105,7 @@ int launch_gram(const GramJob* jobs_dev, uint32_t njobs, uint32_t kc_max, int n,
     hipStream_t st = (hipStream_t)stream;
     if (kc_max <= 32) return launch_gram_kc<32>(jobs_dev, njobs, n, nchunk, partial, st);
     if (kc_max <= 64) return launch_gram_kc<64>(jobs_dev, njobs, n, nchunk, partial, st);
+    if (kc_max <= (uint32_t)kGramMaxK) return launch_gram_kc<kGramMaxK>(jobs_dev, njobs, n, nchunk, partial, st);
     return -2;
 }
 

@@ -135,6 +135,9 @@ struct GramJob {
 // pairs (k1 <= k2, k1 = k2 mod 2) ordered by k2 then k1: pair_start(k2) + k1 / 2
 inline uint32_t gram_pair_start(uint32_t k2) { return k2 == 0 ? 0u : k2 + ((k2 - 1) * (k2 - 1)) / 4; }
 inline uint32_t gram_pairs(uint32_t kc) { return gram_pair_start(kc); }
+constexpr int kGramMaxK = 128;
+// pairs per partial sum of a launch whose largest basis has kc_max vectors
+inline uint32_t gram_pmax(uint32_t kc_max) { return gram_pairs(kc_max <= 32 ? 32u : kc_max <= 64 ? 64u : 128u); }
 // Pair sums of every job's vectors: njobs x nchunk partial sums of
 // gram_pairs(kc_max) doubles in `partial`, then their fixed-order sum per job.
 int launch_gram(const GramJob* jobs_dev, uint32_t njobs, uint32_t kc_max, int n, uint32_t nchunk, double* partial,

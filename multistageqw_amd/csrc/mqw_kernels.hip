@@ -2078,12 +2078,13 @@ int launch_readout_dots(const ReadTile* tiles_dev, uint32_t ntiles, const ReadIt
 // amplitudes, multiplies by psi there and keeps its own partial dot: one
 // FP32 sum per 256 amplitudes, then FP64 across passes, the two halves of the
 // wave and the 8 waves in order.
-template <int MB>
-__global__ void __launch_bounds__(512, 2)
+template <int MB, int OCC>
+__global__ void __launch_bounds__(512, OCC)
 readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const ReadItem* __restrict__ items,
                     const float* __restrict__ coefs, int n, uint32_t nchunk, double* __restrict__ dots)
 {
     constexpr int NSP = MB * 32;
+    constexpr int SUB = kReadTile / NSP; // workgroups per tile and chunk, NSP samples each
     constexpr int KB = kReadMaxK / 32; // 16-row blocks per class
     constexpr int BLK = 2 * NSP * 16;  // halves of one block (hi and lo)
     constexpr float kXScale = 4096.0f; // 2^12: basis entries are at most 1 in magnitude
@@ -2094,16 +2095,19 @@ readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const R
     // of consecutive units, chunk-major, so the tiles of one amplitude chunk
     // (two tiles of an instance share its basis) meet in one L2
     const uint32_t u = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-    const uint32_t chunk = u / ntiles, tile = u % ntiles;
+    const uint32_t chunk = u / (ntiles * SUB), tile = (u / SUB) % ntiles;
     if (chunk >= nchunk) return;
     const ReadTile& T = tiles[tile];
-    const int kx = (int)T.kx, ns = (int)T.ns;
+    const int j0 = (int)(u % SUB) * NSP; // this workgroup's first sample of the tile
+    const int kx = (int)T.kx, ns = min((int)T.ns - j0, NSP);
+    if (ns <= 0) return; // a sub-tile past the tile's samples
+    const ReadItem* const TI = items + T.item_off + j0;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int col = lane & 31, half = lane >> 5;
     for (int i = t; i < 2 * KB * NSP * 16; i += 512) {
         const int rr = i & 15, j = (i >> 4) % NSP, cb = (i >> 4) / NSP; // cb = class * KB + block
         const int k = 2 * (16 * (cb % KB) + rr) + cb / KB;
-        const float v = (j < ns && k < kx) ? coefs[items[T.item_off + j].coef_off + k] * T.ascale : 0.0f;
+        const float v = (j < ns && k < kx) ? coefs[TI[j].coef_off + k] * T.ascale : 0.0f;
         const _Float16 hi = (_Float16)v;
         // [block][hi, lo][rows 0-7, rows 8-15][NSP samples][8 rows]: a wave's
         // lanes read consecutive 16-byte pieces (no LDS bank conflicts)
@@ -2111,7 +2115,7 @@ readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const R
         s_b[cb * BLK + pos] = hi;
         s_b[cb * BLK + NSP * 16 + pos] = (_Float16)(v - (float)hi);
     }
-    for (int i = t; i < NSP; i += 512) s_psi[i] = i < ns ? items[T.item_off + i].psi : nullptr;
+    for (int i = t; i < NSP; i += 512) s_psi[i] = i < ns ? TI[i].psi : nullptr;
     for (int i = t; i < 8 * NSP * 2; i += 512) (&s_w[0][0][0])[i] = 0.0;
     __syncthreads();
     const uint64_t N = 1ull << n;
@@ -2224,7 +2228,7 @@ readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const R
         const int j = t >> 1, q = t & 1;
         double sum = 0.0;
         for (int v = 0; v < 8; v++) sum += s_w[v][j][q];
-        dots[((uint64_t)tile * nchunk + chunk) * kReadTile * 2 + 2 * j + q] = sum;
+        dots[((uint64_t)tile * nchunk + chunk) * kReadTile * 2 + 2 * (j0 + j) + q] = sum;
     }
 }
 
@@ -2235,10 +2239,14 @@ int launch_readout_dots_mfma(const ReadTile* tiles_dev, uint32_t ntiles, const R
     static_assert(kReadTile == 64 && kReadMaxK % 32 == 0, "tiles of two 32-sample blocks, whole row blocks");
     const uint64_t N = 1ull << n;
     if (nchunk == 0 || N % nchunk != 0 || (N / nchunk) % 256 != 0) return -2;
-    const uint64_t grid = ((uint64_t)ntiles * nchunk + 7) / 8 * 8; // a multiple of the 8 XCDs
+    // 32 samples per workgroup: half the accumulators of a 64-sample tile, for
+    // more waves per SIMD; the two workgroups of a tile and chunk read its X
+    // rows through one L2
+    constexpr int SUB = 2;
+    const uint64_t grid = ((uint64_t)ntiles * nchunk * SUB + 7) / 8 * 8; // a multiple of the 8 XCDs
     if (grid > 0x7fffffffull) return -3;
-    hipLaunchKernelGGL(readout_mfma_kernel<2>, dim3((uint32_t)grid), dim3(512), 0, (hipStream_t)stream, tiles_dev,
-                       ntiles, items_dev, coefs_dev, n, nchunk, dots);
+    hipLaunchKernelGGL((readout_mfma_kernel<1, 4>), dim3((uint32_t)grid), dim3(512), 0, (hipStream_t)stream,
+                       tiles_dev, ntiles, items_dev, coefs_dev, n, nchunk, dots);
     return check_launch("readout_dot_mfma");
 }
 
