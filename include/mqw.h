@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 13
+#define MQW_ABI_VERSION 14
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -314,6 +314,9 @@ typedef struct {
     uint64_t compressed_vectors;   /* their kept first-stage vectors (2R each), summed */
     uint64_t uncompressed_vectors; /* their first-stage vectors before compression (kc1 each), summed */
     uint64_t direct_chains;        /* m == 1 instances evaluated directly (direct_m1) */
+    uint64_t first_stage_chunks;   /* chunks of the first-stage bases computed before the batches (for
+                                      compression and direct m == 1 chains), each within the V share of
+                                      max_device_bytes */
 } mqw_stats;
 mqw_status mqw_last_stats(const mqw_ctx* ctx, mqw_stats* out);
 

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmqw.so")
 CLI_PATH = os.path.join(HERE, "mqw")
 
-ABI_VERSION = 13  # MQW_ABI_VERSION of include/mqw.h this binding matches
+ABI_VERSION = 14  # MQW_ABI_VERSION of include/mqw.h this binding matches
 MAX_STAGES = 64
 DEFAULT_SAMPLES = 100
 DEFAULT_TOL = 1e-6
@@ -78,7 +78,8 @@ class Stats(C.Structure):
                 ("basis_launches", C.c_uint64), ("gemm_launches", C.c_uint64), ("gemm_bytes", C.c_double),
                 ("basis_batches", C.c_uint64), ("overlap_batches", C.c_uint64),
                 ("compressed_instances", C.c_uint64), ("compressed_vectors", C.c_uint64),
-                ("uncompressed_vectors", C.c_uint64), ("direct_chains", C.c_uint64)]
+                ("uncompressed_vectors", C.c_uint64), ("direct_chains", C.c_uint64),
+                ("first_stage_chunks", C.c_uint64)]
 
 
 _lib = None

@@ -1263,6 +1263,7 @@ mqw_status direct_m1(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
         }
         char* vbuf = (char*)dev_get(c, "cmp_V", vbytes);
         if (!vbuf) return fail(c, MQW_ENOMEM, "device allocation failed for first-stage bases");
+        c->stats.first_stage_chunks++;
         Bases vb;
         vb.V.assign(ps.size(), nullptr);
         vb.kc.assign(ps.size(), 0);
@@ -1390,6 +1391,7 @@ mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vecto
         }
         char* vbuf = (char*)dev_get(c, "cmp_V", vbytes);
         if (!vbuf) return fail(c, MQW_ENOMEM, "device allocation failed for first-stage bases");
+        c->stats.first_stage_chunks++;
         // V recurrences of the chunk (no X, no W)
         Bases vb;
         vb.V.assign(ps.size(), nullptr);

@@ -73,3 +73,63 @@ def test_compressed_in_uneven_batch_with_overrides(ctx, oracle):
                                     gamma=g[None, :], short_t=t[None, :], norm="f64")
         assert tt[k] == tto[0]
         assert np.max(np.abs(sp[k] - spo[0])) < 1e-6, (n, m)
+
+
+def test_compression_in_chunks_is_bit_identical(ctx):
+    """A device cap under which the first-stage bases of the compressed instances (computed before
+    the batches) take several chunks gives the same results bit for bit as one chunk."""
+    n, m, samples = 14, 3, 40
+    ni = 64  # their V together exceed the cap's V share while each instance's W fits its share
+    recs = M.generate_sk(n, ni, 990)
+    _say("compression in chunks,", ni, "instances n=14")
+    p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True)
+    st = ctx.last_stats()
+    assert st["compressed_instances"] == ni and st["first_stage_chunks"] == 1
+    tried = []
+    for cap_mib in (640, 384, 320, 256, 224):
+        pc, ttc, spc = ctx.solve(n, m, recs, samples=samples, with_samples=True, max_device_bytes=cap_mib << 20)
+        stc = ctx.last_stats()
+        tried.append((cap_mib, stc["compressed_instances"], stc["first_stage_chunks"]))
+        if stc["compressed_instances"] != ni:
+            continue
+        assert np.array_equal(tt, ttc)
+        assert np.array_equal(sp, spc), cap_mib
+        if stc["first_stage_chunks"] >= 2:
+            break
+    print("[chunks] (cap MiB, compressed, chunks):", tried, flush=True)
+    assert any(c == ni and k >= 2 for _, c, k in tried), tried
+
+
+def test_direct_m1_in_chunks_is_bit_identical(ctx):
+    """Direct m = 1 chains whose first-stage bases do not fit together run one chunk at a time
+    with the same results bit for bit."""
+    n, samples = 16, 40
+    recs = M.generate_sk(n, 6, 1990)
+    _say("direct m=1 in chunks, 6 instances n=16")
+    p, tt, sp = ctx.solve(n, 1, recs, samples=samples, with_samples=True)
+    st = ctx.last_stats()
+    pc, ttc, spc = ctx.solve(n, 1, recs, samples=samples, with_samples=True, max_device_bytes=64 << 20)
+    stc = ctx.last_stats()
+    print(f"[chunks] direct m=1: {st['first_stage_chunks']} -> {stc['first_stage_chunks']} chunks", flush=True)
+    assert st["direct_chains"] == stc["direct_chains"] == 6
+    assert st["first_stage_chunks"] == 1 and stc["first_stage_chunks"] >= 2
+    assert np.array_equal(tt, ttc) and np.array_equal(sp, spc)
+
+
+def test_compressed_with_few_samples(ctx, oracle):
+    """Just above the sample count where the shared second stage pays: few rows per class."""
+    n, m, samples = 16, 3, 30
+    recs = M.generate_sk(n, 2, 1201)
+    _say("compressed, 30 samples")
+    p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True)
+    st = ctx.last_stats()
+    pf, _, spf = ctx.solve(n, m, recs, samples=samples, with_samples=True, compress_bases=False)
+    print(f"[compress] 30 samples: {st['compressed_instances']} compressed, vectors "
+          f"{st['uncompressed_vectors']} -> {st['compressed_vectors']}", flush=True)
+    if st["gemm_launches"] == 0:
+        pytest.skip("the shared second stage does not pay at this sample count for these instances")
+    # an instance whose samples need 2R >= kc directions keeps its V_k
+    assert st["compressed_instances"] >= 1
+    assert np.max(np.abs(sp - spf)) < 2e-7
+    po, _, spo = oracle.solve(n, m, recs, samples=samples, threads=THREADS, with_samples=True, norm="f64")
+    assert np.max(np.abs(sp - spo)) < 1e-6

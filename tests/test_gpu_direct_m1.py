@@ -57,3 +57,16 @@ def test_direct_m1_in_uneven_batch_with_override(ctx, oracle):
                                     norm="f64", **kw)
         assert tt[k] == tto[0]
         assert np.max(np.abs(sp[k] - spo[0])) < 1e-6, (n, m)
+
+
+def test_direct_m1_fixed_times_vs_chain(ctx):
+    """fixed_times: every increment of the chain is exactly short_t, so sample j sits at (j+1) short_t
+    (two samples keep the chain's series within the 128 terms of the direct path, as the
+    Monte-Carlo times, whose chain ends below 2 short_t, do)."""
+    n = 15
+    recs = M.generate_sk(n, 2, 961)
+    _say("direct m=1, fixed times")
+    p, _, sp = ctx.solve(n, 1, recs, samples=2, with_samples=True, fixed_times=True)
+    assert ctx.last_stats()["direct_chains"] == 2
+    pc, _, spc = ctx.solve(n, 1, recs, samples=2, with_samples=True, fixed_times=True, direct_m1=False)
+    assert np.max(np.abs(sp - spc)) < 1e-6
