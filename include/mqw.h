@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 14
+#define MQW_ABI_VERSION 15
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -144,6 +144,16 @@ typedef struct {
                                 its amplitude and norm from V_k[E_loc] and the basis's Gram matrix (no term
                                 sweeps).  It differs from the chain by the chain's accumulated truncation,
                                 measured below 4e-7 in probability; 0 = the chain, as the reference */
+    int adjoint_two_stages;  /* 1 (default) = for m >= 4, where it costs less than the samples' own stage
+                                m - 1 terms, the last two stages are read out together: the samples'
+                                adjoint states phi_j = P_m^T e_Eloc (P_i the stage propagators, sums of
+                                T_k(H_i/onenorm_i) with the sample's coefficients; T_k is symmetric) lie
+                                in the span of the last stage's basis T_k(H_m/onenorm_m) e_Eloc, which is
+                                compressed like the first stage's (1e-7 per sample); P_{m-1}^T phi_j is a
+                                second-stage GEMM over T_l(H_{m-1}/onenorm_{m-1}) Z_r, and the amplitude
+                                psi_m[E_loc] = (P_{m-1}^T phi_j)^T psi_{m-2} a dot with the input of stage
+                                m - 1, whose norm stands for the output's (two truncated series: at most
+                                4 tol relative in probability); 0 = stage m - 1 as per-sample terms */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are
@@ -317,6 +327,9 @@ typedef struct {
     uint64_t first_stage_chunks;   /* chunks of the first-stage bases computed before the batches (for
                                       compression and direct m == 1 chains), each within the V share of
                                       max_device_bytes */
+    uint64_t adjoint2_instances;   /* instances whose last two stages were read out together (adjoint_two_stages) */
+    uint64_t adjoint2_vectors;     /* their kept adjoint directions (2R each), summed */
+    uint64_t adjoint2_uncompressed; /* their last-stage basis vectors before compression (kx each), summed */
 } mqw_stats;
 mqw_status mqw_last_stats(const mqw_ctx* ctx, mqw_stats* out);
 

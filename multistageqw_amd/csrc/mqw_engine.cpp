@@ -43,6 +43,32 @@ mqw_status mqw::fail(mqw_ctx* c, mqw_status s, const char* fmt, ...)
     return s;
 }
 
+namespace {
+// Host phase times on stderr with MQW_TRACE=1: wall clock since the current
+// solve_group began (per host thread).
+thread_local std::chrono::steady_clock::time_point g_trace_t0;
+bool trace_on()
+{
+    static const bool on = [] {
+        const char* v = getenv("MQW_TRACE");
+        return v && *v == '1';
+    }();
+    return on;
+}
+void trace(const char* fmt, ...)
+{
+    if (!trace_on()) return;
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - g_trace_t0).count();
+    va_list ap;
+    va_start(ap, fmt);
+    fprintf(stderr, "[mqw %9.3f ms] ", ms);
+    vfprintf(stderr, fmt, ap);
+    fputc('\n', stderr);
+    va_end(ap);
+}
+} // namespace
+
 // Grow-only named device buffer.  Growing synchronises both streams first.
 void* mqw::dev_get(mqw_ctx* c, const std::string& name, size_t bytes)
 {
@@ -61,6 +87,7 @@ void* mqw::dev_get(mqw_ctx* c, const std::string& name, size_t bytes)
         return nullptr;
     }
     b.cap = want;
+    trace("device buffer %s: %zu bytes", name.c_str(), want);
     return b.ptr;
 }
 
@@ -205,7 +232,9 @@ struct Traj {
     bool first_done = false;      // the first stage comes from the shared basis (a combine op)
     bool second_done = false;     // the first two stages come from the shared bases (a combine2 op)
     bool adjoint = false;         // the last stage is read out by the adjoint dots (no terms)
+    bool adjoint2 = false;        // the last two stages are read out by the two-stage adjoint (no terms)
     std::vector<float> c1q;       // with a compressed first-stage basis: the sample's coordinates in it
+    std::vector<float> cxq;       // with a compressed last-stage adjoint basis: the sample's coordinates in it
     bool direct_done = false;     // an m == 1 chain evaluated directly (direct_m1): no terms run
 };
 
@@ -302,11 +331,12 @@ mqw_status group_energies(mqw_ctx* c, int n, std::vector<Problem*>& ps, int mode
 struct Op {
     int kind; // 0 = term launch, 1 = reduce launch, 2 = high-bit pass over the jobs of the next term launch,
               // 3 = combine launch (shared first stage), 4 = real-half term launch (basis recurrences),
-              // 5 = adjoint readout dots, 6 = adjoint readout finalize, 7 = combine2 launch (second stage)
+              // 5 = adjoint readout dots, 6 = adjoint readout finalize, 7 = combine2 launch (second stage),
+              // 8 = pair dots of the two-stage adjoint
     size_t off;
     uint32_t count;
     uint32_t aux = 0;  // kinds 3, 5: largest basis size of its jobs / tiles; kind 7: largest tile
-    uint32_t aux2 = 0; // kinds 5, 6: amplitude chunks of the dots; kind 7: 1 = split-FP16 kernel
+    uint32_t aux2 = 0; // kinds 5, 6, 8: amplitude chunks of the dots; kind 7: 1 = split-FP16 kernel
     uint32_t aux3 = 0; // kind 5: 1 = dots on the matrix cores (split FP16)
 };
 
@@ -343,10 +373,14 @@ struct Slot {
     real* hs = nullptr; // high-bit sums of the current term's s1 (planar), when the pass is on
 };
 
-// The last stage's input of an adjoint trajectory: its state and partial norms.
+// The input of an adjoint trajectory's read-out stages: its state and partial
+// norms, and the slot's other state and partial-sum array, free once the
+// trajectory's terms have run (the two-stage adjoint writes phi' there).
 struct AdjointIn {
     const void* psi = nullptr;
     const double* norm_part = nullptr;
+    void* free_st = nullptr;
+    double* free_part = nullptr;
 };
 
 // Event pairs and graphs of launched op runs, with each run's timing class
@@ -391,8 +425,9 @@ void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, cons
         prev_slot = tr.segs[s0 - 1].prob_slot;
     }
     auto rc = [](double v) { return (real)v; };
-    // an adjoint trajectory stops before its last stage
-    const size_t s1 = tr.adjoint ? tr.segs.size() - 1 : tr.segs.size();
+    // an adjoint trajectory stops before its last stage, a two-stage adjoint
+    // one before its last two
+    const size_t s1 = tr.segs.size() - (tr.adjoint2 ? 2 : tr.adjoint ? 1 : 0);
     for (size_t s = s0; s < s1; s++) {
         const Segment& sg = tr.segs[s];
         const std::vector<double>& cc = sg.c;
@@ -471,11 +506,13 @@ void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, cons
         par ^= 1;
         prev_slot = sg.prob_slot;
     }
-    if (tr.adjoint) {
+    if (tr.adjoint || tr.adjoint2) {
         has_fin = false;
         if (adj) {
             adj->psi = sl.st[p];
             adj->norm_part = sl.part[par ^ 1];
+            adj->free_st = sl.st[a];
+            adj->free_part = sl.part[par];
         }
         return;
     }
@@ -510,6 +547,16 @@ struct Bases {
     // vectors, Q_0 at plusv), computed before the batch; no V recurrence then
     std::vector<char> cmp;
     std::vector<const float*> plusv;
+    // two-stage adjoint (m >= 4): the compressed last-stage adjoint basis Z
+    // (kz = 2R vectors, Z_0 .. Z_{kz-1} N floats apart, computed before the
+    // batch), Y_lr = T_l(Ht_{m-1}) Z_r (l >= 1 stored like W) and its GEMM
+    // row table; kz = 0 = none
+    std::vector<const float*> Z;
+    std::vector<int> kz;
+    std::vector<const float*> Y;
+    std::vector<int> ky;
+    std::vector<const uint32_t*> ytab;
+    std::vector<uint32_t> y0, y1;
 };
 
 // Rows (l, k) of the second-stage GEMM in two classes, each padded to a
@@ -594,6 +641,51 @@ void readout_ops(int n, uint32_t parts, const std::vector<Traj>& trajs, const st
     (void)parts;
 }
 
+// Pair-dot tiles of the two-stage adjoint members of one group, after their
+// phi' GEMM: per problem, tiles of up to kReadTile samples whose items pair
+// phi' (the slot's free state) with the input psi of stage m - 1; the dots are
+// split over amplitude chunks like the readout's, and the readout's finalize
+// divides |amplitude|^2 by psi's norm.
+template <typename real>
+void pair_dot_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_t>& grp,
+                  const std::vector<AdjointIn>& adj, double* probs, Plan<real>& P)
+{
+    const size_t N = (size_t)1 << n;
+    std::map<int, std::vector<size_t>> byprob;
+    for (size_t i = 0; i < grp.size(); i++)
+        if (trajs[grp[i]].adjoint2) byprob[trajs[grp[i]].prob].push_back(i);
+    if (byprob.empty()) return;
+    const size_t t0 = P.rtiles.size();
+    for (const auto& kv : byprob) {
+        const std::vector<size_t>& mem = kv.second;
+        for (size_t i0 = 0; i0 < mem.size(); i0 += kReadTile) {
+            const size_t ns = std::min<size_t>(kReadTile, mem.size() - i0);
+            ReadTile rt{};
+            rt.ns = (uint32_t)ns;
+            rt.item_off = (uint32_t)P.ritems.size();
+            for (size_t q = 0; q < ns; q++) {
+                const size_t i = mem[i0 + q];
+                ReadItem it{};
+                it.psi = (const float*)adj[i].psi;
+                it.phi = (const float*)adj[i].free_st;
+                it.norm_part = adj[i].norm_part;
+                it.prob = probs + trajs[grp[i]].segs.back().prob_slot;
+                P.ritems.push_back(it);
+            }
+            P.rtiles.push_back(rt);
+            // reads both planar states of every item once
+            P.aux_bytes += 16.0 * (double)N * (double)ns;
+        }
+    }
+    const uint32_t ntiles = (uint32_t)(P.rtiles.size() - t0);
+    uint32_t nchunk = 1;
+    while ((uint64_t)nchunk * ntiles * kReadTile < 4096 && (N / (nchunk * 2)) >= 4096) nchunk *= 2;
+    P.dots_doubles = std::max<size_t>(P.dots_doubles, (size_t)ntiles * nchunk * kReadTile * 2);
+    P.ops.push_back(Op{8, t0, ntiles, 0, nchunk});
+    P.ops.push_back(Op{6, t0, ntiles, 0, nchunk});
+    P.aux_launches += 2;
+}
+
 // Combine jobs for the first_done members of one group: per problem, batches
 // of up to kCombineBatch samples, each writing its slot's st[0] and part[0].
 template <typename real>
@@ -642,33 +734,47 @@ void combine_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_t
     }
 }
 
-// Combine2 tiles for the second_done members of one group: per problem,
-// tiles of up to kC2Tile samples, each writing its slot's st[0] and part[0].
+// Combine2 tiles of one group, per problem in tiles of up to kC2Tile
+// (kC2TileH) samples: the second_done members' second stage, written to their
+// slot's st[0] and part[0]; or, with adj, the two-stage adjoint members'
+// phi'_j = P_{m-1}^T phi_j (basis Y from Z, coefficients: the sample's
+// coordinates in Z and its stage m - 1 coefficients), written to the free
+// state of their slot.
 template <typename real>
 void combine2_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_t>& grp, const Bases& B,
-                  const std::vector<Slot<real>>& slots, Plan<real>& P)
+                  const std::vector<Slot<real>>& slots, Plan<real>& P, const std::vector<AdjointIn>* adj = nullptr)
 {
     if constexpr (std::is_same<real, float>::value) {
         const size_t N = (size_t)1 << n;
         std::map<int, std::vector<size_t>> byprob;
         for (size_t i = 0; i < grp.size(); i++)
-            if (trajs[grp[i]].second_done) byprob[trajs[grp[i]].prob].push_back(i);
+            if (adj ? trajs[grp[i]].adjoint2 : trajs[grp[i]].second_done) byprob[trajs[grp[i]].prob].push_back(i);
         if (byprob.empty()) return;
         Op op{7, P.c2tiles.size(), 0, 0, B.split ? 1u : 0u};
         const size_t tsz = B.split ? kC2TileH : kC2Tile;
         for (const auto& kv : byprob) {
-            const int pr = kv.first, kc1 = B.kc[pr], kc2 = B.kc2[pr];
+            const int pr = kv.first;
+            const int kc1 = adj ? B.kz[pr] : B.kc[pr], kc2 = adj ? B.ky[pr] : B.kc2[pr];
             const std::vector<size_t>& mem = kv.second;
             for (size_t i0 = 0; i0 < mem.size(); i0 += tsz) {
                 const size_t ns = std::min<size_t>(tsz, mem.size() - i0);
                 Combine2Tile ct{};
-                const bool cq = !B.cmp.empty() && B.cmp[pr];
-                ct.plus = cq ? B.plusv[pr] : B.plus;
-                ct.V = B.V[pr];
-                ct.W = B.W[pr];
-                ct.qtab = B.qtab[pr];
-                ct.q0 = B.q0[pr];
-                ct.q1 = B.q1[pr];
+                const bool cq = adj || (!B.cmp.empty() && B.cmp[pr]);
+                if (adj) {
+                    ct.plus = B.Z[pr];
+                    ct.V = B.Z[pr] + N;
+                    ct.W = B.Y[pr];
+                    ct.qtab = B.ytab[pr];
+                    ct.q0 = B.y0[pr];
+                    ct.q1 = B.y1[pr];
+                } else {
+                    ct.plus = cq ? B.plusv[pr] : B.plus;
+                    ct.V = B.V[pr];
+                    ct.W = B.W[pr];
+                    ct.qtab = B.qtab[pr];
+                    ct.q0 = B.q0[pr];
+                    ct.q1 = B.q1[pr];
+                }
                 ct.kc1 = (uint32_t)kc1;
                 ct.kc2 = (uint32_t)kc2;
                 ct.item_off = (uint32_t)P.c2items.size();
@@ -678,19 +784,21 @@ void combine2_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_
                     const size_t i = mem[i0 + q];
                     const Traj& tr = trajs[grp[i]];
                     Combine2Item it{};
-                    it.out = slots[i].st[0];
-                    it.norm_part = slots[i].part[0];
+                    it.out = adj ? (float*)(*adj)[i].free_st : slots[i].st[0];
+                    it.norm_part = adj ? (*adj)[i].free_part : slots[i].part[0];
+                    const std::vector<float>& u = adj ? tr.cxq : tr.c1q;
+                    const std::vector<double>& c2 = adj ? tr.segs[tr.segs.size() - 2].c : tr.segs[1].c;
                     it.c1_off = (uint32_t)P.ccoef.size();
                     for (int k = 0; k < kc1; k++) {
                         if (cq)
-                            P.ccoef.push_back(k < (int)tr.c1q.size() ? tr.c1q[k] : 0.0f);
+                            P.ccoef.push_back(k < (int)u.size() ? u[k] : 0.0f);
                         else
                             P.ccoef.push_back(k < (int)tr.segs[0].c.size() ? (float)tr.segs[0].c[k] : 0.0f);
                         m1 = std::max(m1, fabsf(P.ccoef.back()));
                     }
                     it.c2_off = (uint32_t)P.ccoef.size();
                     for (int l = 0; l < kc2; l++) {
-                        P.ccoef.push_back(l < (int)tr.segs[1].c.size() ? (float)tr.segs[1].c[l] : 0.0f);
+                        P.ccoef.push_back(l < (int)c2.size() ? (float)c2[l] : 0.0f);
                         m2 = std::max(m2, fabsf(P.ccoef.back()));
                     }
                     P.c2items.push_back(it);
@@ -769,6 +877,10 @@ void build_plan(int n, uint32_t parts, const std::vector<Traj>& trajs, const std
             if (any_hsum) P.ops.push_back(Op{2, op.off, op.count});
             P.ops.push_back(op);
             P.sweeps++;
+        }
+        if (B && !B->Z.empty()) {
+            combine2_ops<real>(n, trajs, grp, *B, slots, P, &adj);
+            pair_dot_ops<real>(n, trajs, grp, adj, probs, P);
         }
         if (B && !B->X.empty()) readout_ops<real>(n, parts, trajs, grp, adj, *B, probs, P);
         if (!fins.empty()) {
@@ -869,6 +981,8 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
                 rc = launch_readout_dots(rtd + op.off, op.count, rid, cf, n, op.aux2, op.aux, dots, stm);
             else if (op.kind == 6)
                 rc = launch_readout_finalize(rtd + op.off, op.count, rid, op.aux2, k.parts, dots, stm);
+            else if (op.kind == 8)
+                rc = launch_pair_dots(rtd + op.off, op.count, rid, n, op.aux2, dots, stm);
             else if (op.kind == 7 && op.aux2)
                 rc = launch_combine2h(c2t + op.off, op.count, c2i, cf, c2a, n, k.parts, op.aux, stm);
             else if (op.kind == 7)
@@ -1059,14 +1173,24 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
         need_plus |= B.kc2[p] > 0 && !cmp(p);
     }
     if (need_plus && !B.plus) return fail(c, MQW_EINVAL, "second-stage bases without the |+> row");
-    std::vector<const Segment*> seg1(ps.size(), nullptr);
+    std::vector<const Segment*> seg1(ps.size(), nullptr), segy(ps.size(), nullptr);
     if (has_w)
         for (const Traj& tr : trajs)
             if (!seg1[tr.prob] && tr.segs.size() > 1) seg1[tr.prob] = &tr.segs[1];
+    // two-stage adjoint chains: the operator of stage m - 1
+    bool has_y = false;
+    for (size_t p = 0; p < B.kz.size(); p++) has_y |= B.kz[p] > 0 && B.ky[p] > 0;
+    if (has_y)
+        for (const Traj& tr : trajs)
+            if (!segy[tr.prob] && tr.segs.size() > 2) segy[tr.prob] = &tr.segs[tr.segs.size() - 2];
     int steps = 0;
     for (size_t p = 0; p < ps.size(); p++) {
         steps = std::max({steps, cmp(p) ? 0 : B.kc[p] - 1, B.kx[p] - 1});
         if (has_w && B.kc2[p] > 0) steps = std::max(steps, (cmp(p) ? 0 : B.kc[p] - 1) + B.kc2[p] - 1);
+        if (has_y && B.kz[p] > 0 && B.ky[p] > 0) {
+            if (!segy[p]) return fail(c, MQW_EINVAL, "two-stage adjoint basis of an instance with m < 3");
+            steps = std::max(steps, B.ky[p] - 1);
+        }
     }
     if (need_plus) {
         // |+> as a vector: the GEMM's row (l, k) = (0, 0)
@@ -1084,8 +1208,55 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
         j.beta = (float)((double)sg.gamma / sc);
         return j;
     };
+    // Chains from seeds: row (l, kk) of rows at W + (kk (kc2 - 1) + l - 1) N for
+    // l >= 1, row (0, kk) = seed(kk) (nullptr: the constant psi0); step q holds
+    // l = q - kk, or l = q when every seed is ready before the batch (cq).
+    Op op{4, 0, 0};
+    auto chains = [&](size_t p, float* W, int kc1, int kc2, const Segment& sg, bool cq, int q, auto seed) {
+        auto row = [&](int ll, int kk) { return W + ((size_t)kk * (kc2 - 1) + (size_t)(ll - 1)) * N; };
+        for (int kk = 0; kk < kc1 && (cq || kk <= q); kk++) {
+            const int l = cq ? q : q - kk;
+            if (l >= kc2 - 1) continue;
+            TermJob<float> j = job(p, sg, l);
+            j.out = row(l + 1, kk);
+            uint32_t flags = 0;
+            double b = 8.0; // E read, output write
+            if (l == 0) {
+                if (!seed(kk)) {
+                    flags |= SRC1_PSI | F_PSI_CONST;
+                } else {
+                    flags |= SRC1_ARRAY;
+                    j.b1 = seed(kk);
+                    b += 4.0;
+                }
+                flags |= SRC2_ZERO;
+            } else {
+                flags |= SRC1_ARRAY;
+                j.b1 = row(l, kk);
+                j.w2r = 1.0f;
+                b += 4.0;
+                if (l == 1) {
+                    if (!seed(kk)) {
+                        flags |= SRC2_PSI | F_PSI_CONST;
+                    } else {
+                        flags |= SRC2_ARRAY;
+                        j.b2 = seed(kk);
+                        b += 4.0;
+                    }
+                } else {
+                    flags |= SRC2_ARRAY;
+                    j.b2 = row(l - 1, kk);
+                    b += 4.0;
+                }
+            }
+            j.flags = flags;
+            P.jobs.push_back(j);
+            op.count++;
+            bytes += b * N;
+        }
+    };
     for (int q = 0; q < steps; q++) {
-        Op op{4, P.jobs.size(), 0};
+        op = Op{4, P.jobs.size(), 0};
         for (size_t p = 0; p < ps.size(); p++) {
             if (!cmp(p) && q < B.kc[p] - 1) {
                 float* V = const_cast<float*>(B.V[p]); // V_1 .. V_{kc-1}
@@ -1132,57 +1303,20 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
                 op.count++;
             }
             const int kc2 = has_w ? B.kc2[p] : 0;
-            if (kc2 <= 0) continue;
-            const int kc1 = B.kc[p];
-            float* W = const_cast<float*>(B.W[p]);
-            const float* V = B.V[p];
-            auto row = [&](int ll, int kk) { return W + ((size_t)kk * (kc2 - 1) + (size_t)(ll - 1)) * N; };
-            const bool cq = cmp(p);
-            // W_0kk: |+> (nullptr: the constant psi0) for kk = 0, else V_kk;
-            // compressed: Q_0 at plusv, Q_kk = V_kk
-            auto seed = [&](int kk) -> const float* {
-                return kk > 0 ? V + (size_t)(kk - 1) * N : cq ? B.plusv[p] : nullptr;
-            };
-            // W_{l+1,kk} with l = q - kk (compressed: l = q)
-            for (int kk = 0; kk < kc1 && (cq || kk <= q); kk++) {
-                const int l = cq ? q : q - kk;
-                if (l >= kc2 - 1) continue;
-                TermJob<float> j = job(p, *seg1[p], l);
-                j.out = row(l + 1, kk);
-                uint32_t flags = 0;
-                double b = 8.0; // E read, output write
-                if (l == 0) {
-                    if (!seed(kk)) {
-                        flags |= SRC1_PSI | F_PSI_CONST;
-                    } else {
-                        flags |= SRC1_ARRAY;
-                        j.b1 = seed(kk);
-                        b += 4.0;
-                    }
-                    flags |= SRC2_ZERO;
-                } else {
-                    flags |= SRC1_ARRAY;
-                    j.b1 = row(l, kk);
-                    j.w2r = 1.0f;
-                    b += 4.0;
-                    if (l == 1) {
-                        if (!seed(kk)) {
-                            flags |= SRC2_PSI | F_PSI_CONST;
-                        } else {
-                            flags |= SRC2_ARRAY;
-                            j.b2 = seed(kk);
-                            b += 4.0;
-                        }
-                    } else {
-                        flags |= SRC2_ARRAY;
-                        j.b2 = row(l - 1, kk);
-                        b += 4.0;
-                    }
-                }
-                j.flags = flags;
-                P.jobs.push_back(j);
-                op.count++;
-                bytes += b * N;
+            if (kc2 > 0) {
+                const float* V = B.V[p];
+                const bool cq = cmp(p);
+                // W_0kk: |+> (nullptr: the constant psi0) for kk = 0, else V_kk;
+                // compressed: Q_0 at plusv, Q_kk = V_kk
+                chains(p, const_cast<float*>(B.W[p]), B.kc[p], kc2, *seg1[p], cq, q, [&](int kk) -> const float* {
+                    return kk > 0 ? V + (size_t)(kk - 1) * N : cq ? B.plusv[p] : nullptr;
+                });
+            }
+            // Y_{l+1,r} = 2 Ht_{m-1} Y_lr - Y_{l-1,r} from Y_0r = Z_r (ready before the batch)
+            if (has_y && B.kz[p] > 0 && B.ky[p] > 0) {
+                const float* Z = B.Z[p];
+                chains(p, const_cast<float*>(B.Y[p]), B.kz[p], B.ky[p], *segy[p], true, q,
+                       [&](int kk) -> const float* { return Z + (size_t)kk * N; });
             }
         }
         P.ops.push_back(op);
@@ -1350,41 +1484,55 @@ mqw_status direct_m1(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
 }
 
 // ---------------------------------------------------------------------------
-// Compressed first-stage bases (see GramJob), for the instances whose second
-// stage comes from the shared basis W.  In chunks of instances whose V fit
-// in vcap bytes:
-// the V recurrences of every instance of the chunk in one plan, their Gram
-// pair sums on the device, on the host one SVD per instance and parity class
-// (compress_first_stage, on up to 16 threads), and a combine launch that
-// writes Q: item r of an instance holds Q_2r (even class) in its real half
-// and Q_2r+1 in its imaginary half, so its Q vectors lie in index order, N
-// floats apart.  kcq[p] = 2R (0 = not compressed), qbase[p] = Q_0, and every
-// trajectory of a compressed instance gets its coordinates in c1q.
+// Compressed bases computed before the batches, in chunks of instances whose
+// bases fit in vcap bytes:
+// - the first-stage basis V (see GramJob), for the instances whose second
+//   stage comes from the shared basis W (kc2 > 0);
+// - the last-stage adjoint basis X (X_0 = e_eloc), for the instances whose
+//   last two stages are read out by the two-stage adjoint (kxa > 0): the
+//   samples' adjoint states phi_j = sum_k a_jk X_k span few directions too.
+// The recurrences of every instance of a chunk run in one plan, their Gram
+// pair sums on the device, then on the host one SVD per instance, basis and
+// parity class (compress_first_stage, on up to 16 threads), and combine
+// launches write the kept directions: item r of an instance holds the even
+// class's direction 2r in its real half and the odd class's 2r + 1 in its
+// imaginary half, so the vectors lie in index order, N floats apart.
+// kcq[p] = 2R and qbase[p] = Q_0 for V (every trajectory gets its coordinates
+// in c1q), kzq[p] = 2R and zbase[p] = Z_0 for X (coordinates in cxq); 0 = not
+// compressed.
 // ---------------------------------------------------------------------------
 constexpr double kCompressTol = 1e-7; // per-sample relative error of the kept span
 
 mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Problem*>& ps,
                           std::vector<Traj>& trajs, const float* E, const std::vector<int>& kc1,
-                          const std::vector<int>& kc2, bool use_graph, size_t vcap, std::vector<int>& kcq,
-                          std::vector<const float*>& qbase)
+                          const std::vector<int>& kc2, const std::vector<int>& kxa, bool use_graph, size_t vcap,
+                          std::vector<int>& kcq, std::vector<const float*>& qbase, std::vector<int>& kzq,
+                          std::vector<const float*>& zbase)
 {
     const size_t N = (size_t)1 << n;
     const float psi0 = (float)(1 / sqrt((double)N));
     kcq.assign(ps.size(), 0);
     qbase.assign(ps.size(), nullptr);
+    kzq.assign(ps.size(), 0);
+    zbase.assign(ps.size(), nullptr);
+    auto fwd = [&](size_t p) { return kc2[p] > 0 && kc1[p] >= 8; };
+    auto adj = [&](size_t p) { return kxa[p] >= 8 && kxa[p] <= kAdjMaxK && kxa[p] <= kGramMaxK; };
     std::vector<size_t> cand;
     for (size_t p = 0; p < ps.size(); p++)
-        if (kc2[p] > 0 && kc1[p] >= 8) cand.push_back(p);
+        if (fwd(p) || adj(p)) cand.push_back(p);
     if (cand.empty()) return MQW_OK;
     std::vector<std::vector<size_t>> ptraj(ps.size());
     for (size_t t = 0; t < trajs.size(); t++) ptraj[trajs[t].prob].push_back(t);
+    auto pbytes = [&](size_t p) {
+        return sizeof(float) * N * (size_t)((fwd(p) ? kc1[p] - 1 : 0) + (adj(p) ? kxa[p] : 0));
+    };
     const uint32_t nchunk = (uint32_t)std::min<size_t>(64, N / 64);
     size_t i0 = 0;
     int chunk_id = 0;
     while (i0 < cand.size()) {
         size_t i1 = i0, vbytes = 0;
         while (i1 < cand.size()) {
-            const size_t b = sizeof(float) * N * (size_t)(kc1[cand[i1]] - 1);
+            const size_t b = pbytes(cand[i1]);
             if (i1 > i0 && vbytes + b > vcap) break;
             vbytes += b;
             i1++;
@@ -1392,65 +1540,98 @@ mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vecto
         char* vbuf = (char*)dev_get(c, "cmp_V", vbytes);
         if (!vbuf) return fail(c, MQW_ENOMEM, "device allocation failed for first-stage bases");
         c->stats.first_stage_chunks++;
-        // V recurrences of the chunk (no X, no W)
+        // V and X recurrences of the chunk (no W)
         Bases vb;
         vb.V.assign(ps.size(), nullptr);
         vb.kc.assign(ps.size(), 0);
         vb.X.assign(ps.size(), nullptr);
         vb.kx.assign(ps.size(), 0);
         vb.kc2.assign(ps.size(), 0);
+        // the compressions of the chunk: (instance, 0 = V / 1 = X)
+        struct Cjob {
+            size_t p;
+            int which, kc;
+            const float* V; // basis vectors 1 .. kc - 1
+            size_t goff;    // its Gram pairs in gram
+        };
+        std::vector<Cjob> jobs;
         size_t off = 0, pairs = 0;
         uint32_t kcmax = 0;
-        std::vector<size_t> goff(ps.size(), 0);
         for (size_t i = i0; i < i1; i++) {
             const size_t p = cand[i];
-            vb.kc[p] = kc1[p];
-            vb.V[p] = (const float*)(vbuf + off);
-            off += sizeof(float) * N * (size_t)(kc1[p] - 1);
-            goff[p] = pairs;
-            pairs += gram_pairs((uint32_t)kc1[p]);
-            kcmax = std::max<uint32_t>(kcmax, (uint32_t)kc1[p]);
+            if (fwd(p)) {
+                vb.kc[p] = kc1[p];
+                vb.V[p] = (const float*)(vbuf + off);
+                off += sizeof(float) * N * (size_t)(kc1[p] - 1);
+                jobs.push_back(Cjob{p, 0, kc1[p], vb.V[p], pairs});
+                pairs += gram_pairs((uint32_t)kc1[p]);
+                kcmax = std::max<uint32_t>(kcmax, (uint32_t)kc1[p]);
+            }
+            if (adj(p)) {
+                vb.kx[p] = kxa[p];
+                vb.X[p] = (const float*)(vbuf + off);
+                off += sizeof(float) * N * (size_t)kxa[p];
+                jobs.push_back(Cjob{p, 1, kxa[p], vb.X[p] + N, pairs});
+                pairs += gram_pairs((uint32_t)kxa[p]);
+                kcmax = std::max<uint32_t>(kcmax, (uint32_t)kxa[p]);
+            }
         }
+        trace("compression chunk: %zu instances, %zu bytes", i1 - i0, vbytes);
         mqw_status st = run_bases(c, n, k, ps, trajs, E, vb, use_graph);
         if (st) return st;
+        trace("compression chunk: recurrences");
         // Gram pair sums
-        const size_t njobs = i1 - i0;
+        const size_t njobs = jobs.size();
         const size_t pmax = gram_pmax(kcmax);
         double* gram_d = (double*)dev_get(c, "cmp_gram", sizeof(double) * pairs);
         double* part_d = (double*)dev_get(c, "cmp_gram_part", sizeof(double) * njobs * nchunk * pmax);
         GramJob* gj_d = (GramJob*)dev_get(c, "cmp_gram_jobs", sizeof(GramJob) * njobs);
         if (!gram_d || !part_d || !gj_d) return fail(c, MQW_ENOMEM, "device allocation failed for Gram matrices");
-        std::vector<GramJob> gj(njobs);
-        for (size_t i = i0; i < i1; i++) {
-            const size_t p = cand[i];
-            gj[i - i0] = GramJob{vb.V[p], gram_d + goff[p], (uint32_t)kc1[p], psi0};
+        // one launch per size class of the kernel (32, 64, 128, 256 vectors): a wide
+        // basis would otherwise set the pair count and occupancy of every job
+        auto gclass = [](int kc) { return kc <= 32 ? 0 : kc <= 64 ? 1 : kc <= 128 ? 2 : 3; };
+        std::vector<GramJob> gj;
+        std::vector<size_t> cls_end(4, 0);
+        std::vector<uint32_t> cls_kc(4, 0);
+        for (int g = 0; g < 4; g++) {
+            for (const Cjob& J : jobs)
+                if (gclass(J.kc) == g) {
+                    gj.push_back(GramJob{J.V, gram_d + J.goff, (uint32_t)J.kc, J.which ? 0.0f : psi0,
+                                         J.which ? ps[J.p]->info.eloc + 1u : 0u, 0u});
+                    cls_kc[g] = std::max<uint32_t>(cls_kc[g], (uint32_t)J.kc);
+                }
+            cls_end[g] = gj.size();
         }
         EventPair ev;
         HIPCHK(c, ev.create());
         HIPCHK(c, hipEventRecord(ev.a, c->stream));
         HIPCHK(c, hipMemcpyAsync(gj_d, gj.data(), sizeof(GramJob) * njobs, hipMemcpyHostToDevice, c->stream));
-        if (launch_gram(gj_d, (uint32_t)njobs, kcmax, n, nchunk, part_d, c->stream))
-            return fail(c, MQW_EDEVICE, "gram: %s", last_kernel_error());
+        for (int g = 0, a = 0; g < 4; a = (int)cls_end[g], g++)
+            if (cls_end[g] > (size_t)a && launch_gram(gj_d + a, (uint32_t)(cls_end[g] - a), cls_kc[g], n, nchunk, part_d,
+                                                       c->stream))
+                return fail(c, MQW_EDEVICE, "gram: %s", last_kernel_error());
         std::vector<double> gram(pairs);
         HIPCHK(c, hipMemcpyAsync(gram.data(), gram_d, sizeof(double) * pairs, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        // one SVD per instance and class
+        trace("compression chunk: Gram matrices");
+        // one SVD per instance, basis and class; the samples' coefficients of
+        // the first stage (V) or of the last (X)
         std::vector<FirstStageCompression> fc(njobs);
         std::vector<char> ok(njobs, 0);
         std::vector<std::vector<double>> rows(njobs);
         auto work = [&](size_t b, size_t e) {
             for (size_t i = b; i < e; i++) {
-                const size_t p = cand[i0 + i];
-                const int kc = kc1[p];
-                const std::vector<size_t>& tl = ptraj[p];
+                const Cjob& J = jobs[i];
+                const int kc = J.kc;
+                const std::vector<size_t>& tl = ptraj[J.p];
                 rows[i].assign(tl.size() * (size_t)kc, 0.0);
                 std::vector<const double*> cp(tl.size());
                 for (size_t j = 0; j < tl.size(); j++) {
-                    const std::vector<double>& cf = trajs[tl[j]].segs[0].c;
+                    const std::vector<double>& cf = J.which ? trajs[tl[j]].segs.back().c : trajs[tl[j]].segs[0].c;
                     for (int q = 0; q < kc && q < (int)cf.size(); q++) rows[i][j * kc + q] = cf[q];
                     cp[j] = &rows[i][j * kc];
                 }
-                ok[i] = compress_first_stage(kc, &gram[goff[p]], cp, kCompressTol, fc[i]);
+                ok[i] = compress_first_stage(kc, &gram[J.goff], cp, kCompressTol, fc[i]);
             }
         };
         {
@@ -1461,45 +1642,67 @@ mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vecto
             work(0, njobs / nt);
             for (std::thread& x : th) x.join();
         }
-        // Q: one combine job per compressed instance, R items of 2N floats
-        size_t qbytes = 0, nitems = 0;
-        for (size_t i = 0; i < njobs; i++)
-            if (ok[i]) {
-                qbytes += sizeof(float) * 2 * N * (size_t)fc[i].R;
-                nitems += (size_t)fc[i].R;
-            }
-        if (nitems > 0) {
-            char* qbuf = (char*)dev_get(c, "cmp_Q" + std::to_string(chunk_id), qbytes);
-            double* dummy = (double*)dev_get(c, "cmp_norm", sizeof(double) * k.parts * nitems);
-            CombineJob* cj_d = (CombineJob*)dev_get(c, "cmp_cjobs", sizeof(CombineJob) * njobs);
-            CombineItem* ci_d = (CombineItem*)dev_get(c, "cmp_citems", sizeof(CombineItem) * nitems);
-            float* cf_d = (float*)dev_get(c, "cmp_coefs", sizeof(float) * nitems * kcmax);
+        trace("compression chunk: %zu SVDs", njobs);
+        // the kept directions: one combine job per compression, R items of 2N floats
+        for (int which = 0; which < 2; which++) {
+            size_t bytes = 0, nitems = 0;
+            uint32_t kcm = 0;
+            for (size_t i = 0; i < njobs; i++)
+                if (ok[i] && jobs[i].which == which) {
+                    bytes += sizeof(float) * 2 * N * (size_t)fc[i].R;
+                    nitems += (size_t)fc[i].R;
+                    kcm = std::max<uint32_t>(kcm, (uint32_t)jobs[i].kc);
+                }
+            if (nitems == 0) continue;
+            const std::string tag = which ? "cmp_Z" : "cmp_Q";
+            char* qbuf = (char*)dev_get(c, tag + std::to_string(chunk_id), bytes);
+            double* dummy = (double*)dev_get(c, tag + "_norm", sizeof(double) * k.parts * nitems);
+            CombineJob* cj_d = (CombineJob*)dev_get(c, tag + "_cjobs", sizeof(CombineJob) * njobs);
+            CombineItem* ci_d = (CombineItem*)dev_get(c, tag + "_citems", sizeof(CombineItem) * nitems);
+            float* cf_d = (float*)dev_get(c, tag + "_coefs", sizeof(float) * nitems * (kcm + 2));
             if (!qbuf || !dummy || !cj_d || !ci_d || !cf_d)
                 return fail(c, MQW_ENOMEM, "device allocation failed for compressed bases");
-            std::vector<CombineJob> cj;
-            std::vector<CombineItem> ci;
+            // a basis wider than kCombineMaxKX: a first job over its vectors 0..127 and a
+            // second that adds 128..kc-1 (as its vectors 2.. of V + 127 N, so the parity
+            // classes stay; its vectors 0 and 1 have no weight), launched after the first
+            std::vector<CombineJob> cj, cj2;
+            std::vector<CombineItem> ci, ci2;
             std::vector<float> cf;
             size_t qoff = 0;
             for (size_t i = 0; i < njobs; i++) {
-                if (!ok[i]) continue;
-                const size_t p = cand[i0 + i];
-                const int kc = kc1[p], R = fc[i].R;
+                if (!ok[i] || jobs[i].which != which) continue;
+                const size_t p = jobs[i].p;
+                const int kc = jobs[i].kc, R = fc[i].R;
+                const int kca = std::min(kc, kCombineMaxKX), kcb = kc > kCombineMaxKX ? kc - (kCombineMaxKX - 2) : 0;
                 float* Q = (float*)(qbuf + qoff);
                 qoff += sizeof(float) * 2 * N * (size_t)R;
-                cj.push_back(CombineJob{vb.V[p], (uint32_t)kc, (uint32_t)R, (uint32_t)ci.size(), psi0});
+                cj.push_back(CombineJob{jobs[i].V, (uint32_t)kca, (uint32_t)R, (uint32_t)ci.size(),
+                                        which ? 0.0f : psi0, which ? ps[p]->info.eloc + 1u : 0u, 0u});
+                if (kcb > 0)
+                    cj2.push_back(CombineJob{jobs[i].V + (size_t)(kCombineMaxKX - 2) * N, (uint32_t)kcb, (uint32_t)R,
+                                             (uint32_t)ci2.size(), 0.0f, 0u, 1u});
                 for (int r = 0; r < R; r++) {
+                    const double* Br = &fc[i].B[0];
                     CombineItem it{};
                     it.out = Q + (size_t)2 * r * N;
                     it.norm_part = dummy + ci.size() * k.parts;
                     it.coef_off = (uint32_t)cf.size();
-                    for (int q = 0; q < kc; q++) cf.push_back((float)fc[i].B[(size_t)(2 * r + (q & 1)) * kc + q]);
+                    for (int q = 0; q < kca; q++) cf.push_back((float)Br[(size_t)(2 * r + (q & 1)) * kc + q]);
                     ci.push_back(it);
+                    if (kcb > 0) {
+                        it.coef_off = (uint32_t)cf.size();
+                        for (int q = 0; q < kcb; q++) {
+                            const int kk = q + kCombineMaxKX - 2; // basis index of column q
+                            cf.push_back(q < 2 ? 0.0f : (float)Br[(size_t)(2 * r + (kk & 1)) * kc + kk]);
+                        }
+                        ci2.push_back(it);
+                    }
                 }
-                kcq[p] = 2 * R;
-                qbase[p] = Q;
+                (which ? kzq : kcq)[p] = 2 * R;
+                (which ? zbase : qbase)[p] = Q;
                 const std::vector<size_t>& tl = ptraj[p];
                 for (size_t j = 0; j < tl.size(); j++) {
-                    std::vector<float>& u = trajs[tl[j]].c1q;
+                    std::vector<float>& u = which ? trajs[tl[j]].cxq : trajs[tl[j]].c1q;
                     u.resize((size_t)2 * R);
                     for (int q = 0; q < 2 * R; q++) u[q] = (float)fc[i].u[j * 2 * R + q];
                 }
@@ -1508,17 +1711,34 @@ mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vecto
             HIPCHK(c, hipMemcpyAsync(cj_d, cj.data(), sizeof(CombineJob) * cj.size(), hipMemcpyHostToDevice, c->stream));
             HIPCHK(c, hipMemcpyAsync(ci_d, ci.data(), sizeof(CombineItem) * ci.size(), hipMemcpyHostToDevice, c->stream));
             HIPCHK(c, hipMemcpyAsync(cf_d, cf.data(), sizeof(float) * cf.size(), hipMemcpyHostToDevice, c->stream));
-            if (launch_combine(cj_d, (uint32_t)cj.size(), ci_d, cf_d, n, k.parts, kcmax, c->stream))
+            if (launch_combine(cj_d, (uint32_t)cj.size(), ci_d, cf_d, n, k.parts,
+                               std::min<uint32_t>(kcm, kCombineMaxKX), c->stream))
                 return fail(c, MQW_EDEVICE, "basis compression: %s", last_kernel_error());
+            c->stats.aux_launches++;
+            if (!cj2.empty()) {
+                CombineJob* cj2_d = (CombineJob*)dev_get(c, tag + "_cjobs2", sizeof(CombineJob) * cj2.size());
+                CombineItem* ci2_d = (CombineItem*)dev_get(c, tag + "_citems2", sizeof(CombineItem) * ci2.size());
+                if (!cj2_d || !ci2_d) return fail(c, MQW_ENOMEM, "device allocation failed for compressed bases");
+                uint32_t kcm2 = 0;
+                for (const CombineJob& j : cj2) kcm2 = std::max(kcm2, j.kc);
+                HIPCHK(c, hipMemcpyAsync(cj2_d, cj2.data(), sizeof(CombineJob) * cj2.size(), hipMemcpyHostToDevice,
+                                         c->stream));
+                HIPCHK(c, hipMemcpyAsync(ci2_d, ci2.data(), sizeof(CombineItem) * ci2.size(), hipMemcpyHostToDevice,
+                                         c->stream));
+                if (launch_combine(cj2_d, (uint32_t)cj2.size(), ci2_d, cf_d, n, k.parts, kcm2, c->stream))
+                    return fail(c, MQW_EDEVICE, "basis compression: %s", last_kernel_error());
+                c->stats.aux_launches++;
+            }
         }
         HIPCHK(c, hipEventRecord(ev.b, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream)); // cmp_V is reused by the next chunk
+        trace("compression chunk: combines");
         float ms = 0.0f;
         HIPCHK(c, hipEventElapsedTime(&ms, ev.a, ev.b));
         c->stats.gpu_ms += ms;
         c->stats.aux_ms += ms;
-        for (size_t i = i0; i < i1; i++) c->stats.aux_bytes += 4.0 * (double)N * (double)(kc1[cand[i]] - 1);
-        c->stats.aux_launches += 3;
+        for (const Cjob& J : jobs) c->stats.aux_bytes += 4.0 * (double)N * (double)(J.kc - 1); // the Gram reads
+        c->stats.aux_launches += 2;
         i0 = i1;
         chunk_id++;
     }
@@ -1534,6 +1754,8 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     const size_t N = (size_t)1 << n;
     const int precision = o.precision;
     const int samples = o.samples;
+    g_trace_t0 = std::chrono::steady_clock::now();
+    trace("solve_group n=%d, %zu problems", n, ps.size());
     SweepCfg k = choose_sweep(c, precision, n, o.tile_bits, o.chunk_bits, o.variant, o.high_pass);
     const bool mixer = o.mixer_count > 0;
     if (mixer) {
@@ -1590,7 +1812,9 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             }
         }
     }
+    trace("energies");
     fill_coeffs(trajs, o.tol, precision == MQW_FP32);
+    trace("coefficients of %zu trajectories", trajs.size());
 
     double* probs_d = (double*)dev_get(c, "probs", sizeof(double) * probs.size());
     if (!probs_d) return fail(c, MQW_ENOMEM, "device allocation failed for probabilities");
@@ -1616,8 +1840,10 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     if (basis_path && o.direct_m1) {
         st = direct_m1(c, n, k, ps, trajs, (const float*)E, o.tol, o.use_graph != 0, vcap, probs, probs_d);
         if (st) return st;
+        trace("direct m = 1 chains");
     }
-    std::vector<int> kc1(ps.size(), 0), kc2(ps.size(), 0), kx(ps.size(), 0);
+    std::vector<int> kc1(ps.size(), 0), kc2(ps.size(), 0), kx(ps.size(), 0), ky(ps.size(), 0);
+    std::vector<double> own_pen(ps.size(), 0.0); // bytes per amplitude of the samples' own stage m - 1 terms
     if (basis_path) {
         std::vector<double> own2(ps.size(), 0.0); // bytes per amplitude of the samples' own second-stage terms
         for (const Traj& tr : trajs) {
@@ -1629,35 +1855,85 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
                 kc2[tr.prob] = std::max(kc2[tr.prob], (int)tr.segs[1].c.size());
                 own2[tr.prob] += 36.0 * (double)std::max<size_t>(tr.segs[1].c.size() - 1, 1);
             }
+            if (pm >= 4 && o.final_readout && o.adjoint_two_stages) {
+                const size_t K = tr.segs[(size_t)pm - 2].c.size();
+                ky[tr.prob] = std::max(ky[tr.prob], (int)K);
+                own_pen[tr.prob] += 36.0 * (double)std::max<size_t>(K - 1, 1);
+            }
         }
+        const bool split = o.second_stage_gemm == MQW_GEMM_SPLIT_FP16;
+        const int kc2max = split ? kC2MaxK2 : kC2MaxK; // the GEMM's second-stage coefficients
         for (size_t p = 0; p < ps.size(); p++) {
-            if (kc1[p] > kCombineMaxK) kc1[p] = 0;
-            if (kx[p] > kReadMaxK) kx[p] = 0;
+            if (kc1[p] > kCombineMaxKX) kc1[p] = 0;
+            if (kx[p] > (split ? kReadMaxK : kReadMaxKF32)) kx[p] = 0;
+            // the two-stage adjoint: a second-stage GEMM over Y (limits as W's)
+            if (ky[p] < 2 || ky[p] > kc2max || kx[p] < 8 || kx[p] > kAdjMaxK ||
+                (o.second_stage_gemm == MQW_GEMM_SPLIT_FP16 && (N / nparts) % 512 != 0))
+                ky[p] = 0;
             // W: recurrences (16 B per vector and amplitude) plus one GEMM read (4 B), against the
             // samples' own terms, with a margin for the matrix work
             const double shared2 = 20.0 * (double)kc1[p] * (double)kc2[p];
             // (the split-FP16 kernel takes amplitude chunks of 512: always so with half tiles)
-            if (kc1[p] < 1 || kc1[p] > kC2MaxK || kc2[p] < 2 || kc2[p] > kC2MaxK || shared2 > 0.8 * own2[p] ||
+            // (kc1 beyond the GEMM's limit only where the compression may bring it within)
+            if (kc1[p] < 1 || (kc1[p] > kC2MaxK && !o.compress_bases) || kc2[p] < 2 || kc2[p] > kc2max ||
+                shared2 > 0.8 * own2[p] ||
                 (o.second_stage_gemm == MQW_GEMM_SPLIT_FP16 && (N / nparts) % 512 != 0))
                 kc2[p] = 0;
         }
     }
+    // second-stage and two-stage adjoint bases: batches within wcap; an instance
+    // whose own bases exceed it (Y at n = 22 is about 40 GiB) runs alone in a
+    // batch of up to twice that
     const size_t wcap = o.max_device_bytes ? o.max_device_bytes / 4 : ((size_t)32 << 30);
+    const size_t wcap1 = 2 * wcap;
     // compressed first-stage bases (kcq = 2R vectors from Q, else the kc1 V_k)
-    std::vector<int> kcq(ps.size(), 0);
-    std::vector<const float*> qbase(ps.size(), nullptr);
+    // and compressed last-stage adjoint bases of the two-stage adjoint (kzq = 2R from Z)
+    std::vector<int> kcq(ps.size(), 0), kzq(ps.size(), 0);
+    std::vector<const float*> qbase(ps.size(), nullptr), zbase(ps.size(), nullptr);
     auto kcw = [&](size_t p) { return kcq[p] > 0 ? kcq[p] : kc1[p]; };
     auto wbytes = [&](size_t p) -> size_t {
         return kc2[p] ? sizeof(float) * N * (size_t)kcw(p) * (size_t)(kc2[p] - 1) : 0;
     };
-    if (basis_path && o.compress_bases) {
-        st = compress_bases(c, n, k, ps, trajs, (const float*)E, kc1, kc2, o.use_graph != 0, vcap, kcq, qbase);
-        if (st) return st;
+    auto ybytes = [&](size_t p) -> size_t {
+        return kzq[p] ? sizeof(float) * N * (size_t)kzq[p] * (size_t)(ky[p] - 1) : 0;
+    };
+    if (basis_path) {
+        std::vector<int> kxa(ps.size(), 0);
+        bool any = o.compress_bases != 0;
+        for (size_t p = 0; p < ps.size(); p++) {
+            // not even a quarter of the last-stage basis would pay: no compression
+            kxa[p] = ky[p] > 0 && 20.0 * (kx[p] / 4) * ky[p] < 0.8 * own_pen[p] ? kx[p] : 0;
+            any |= kxa[p] > 0;
+        }
+        if (any) {
+            const std::vector<int> none(ps.size(), 0);
+            st = compress_bases(c, n, k, ps, trajs, (const float*)E, kc1, o.compress_bases ? kc2 : none, kxa,
+                                o.use_graph != 0, vcap, kcq, qbase, kzq, zbase);
+            if (st) return st;
+            trace("compressed bases");
+        }
+        // the two-stage adjoint where its Y recurrences and GEMM read (20 B per
+        // vector and amplitude) cost less than the samples' own stage m - 1 terms
+        for (size_t p = 0; p < ps.size(); p++)
+            if (kzq[p] > 0 && (kzq[p] > kC2MaxK || 20.0 * kzq[p] * ky[p] >= 0.8 * own_pen[p])) kzq[p] = 0;
     }
     // second-stage bases that do not fit a batch stay per-sample terms (their
-    // first stage then comes from V, recomputed in the batch)
+    // first stage then comes from V, recomputed in the batch); so do the
+    // stage m - 1 terms of a two-stage adjoint that does not fit beside them
     for (size_t p = 0; p < ps.size(); p++) {
-        if (wbytes(p) > wcap) {
+        // W's first-stage directions within the GEMM's limit (compressed, or kc1 itself)
+        if (kc2[p] > 0 && kcw(p) > kC2MaxK) {
+            kc2[p] = 0;
+            kcq[p] = 0;
+            qbase[p] = nullptr;
+        }
+        if (kzq[p] > 0 && wbytes(p) + ybytes(p) > wcap1) kzq[p] = 0;
+        if (kzq[p] > 0) {
+            c->stats.adjoint2_instances++;
+            c->stats.adjoint2_vectors += (uint64_t)kzq[p];
+            c->stats.adjoint2_uncompressed += (uint64_t)kx[p];
+        }
+        if (wbytes(p) + ybytes(p) > wcap1) {
             kc2[p] = 0;
             kcq[p] = 0;
             qbase[p] = nullptr;
@@ -1668,17 +1944,28 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             c->stats.uncompressed_vectors += (uint64_t)kc1[p];
         }
     }
-    // problem batches whose second-stage bases fit together
+    // problem batches whose bases fit together: second-stage and two-stage
+    // adjoint bases within wcap, first-stage and last-stage bases made in the
+    // batch within vcap (so no instance's shared stage falls back to terms for
+    // lack of room beside the others)
+    auto vxbytes = [&](size_t p) -> size_t {
+        if (!basis_path) return 0;
+        const size_t bv = kcq[p] > 0 ? 0 : sizeof(float) * N * (size_t)std::max(kc1[p] - 1, 0);
+        const size_t bx = kzq[p] > 0 ? 0 : sizeof(float) * N * (size_t)kx[p];
+        return bv + bx;
+    };
     std::vector<std::pair<size_t, size_t>> pbatches;
     {
-        size_t b0 = 0, wsum = 0;
+        size_t b0 = 0, wsum = 0, vsum = 0;
         for (size_t p = 0; p < ps.size(); p++) {
-            if (p > b0 && wsum + wbytes(p) > wcap) {
+            if (p > b0 && (wsum + wbytes(p) + ybytes(p) > wcap || vsum + vxbytes(p) > vcap)) {
                 pbatches.emplace_back(b0, p);
                 b0 = p;
                 wsum = 0;
+                vsum = 0;
             }
-            wsum += wbytes(p);
+            wsum += wbytes(p) + ybytes(p);
+            vsum += vxbytes(p);
         }
         pbatches.emplace_back(b0, ps.size());
     }
@@ -1698,6 +1985,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     size_t G = std::max<size_t>(1, budget / per_traj);
     if (o.max_device_bytes) G = std::min(G, std::max<size_t>(1, o.max_device_bytes / 2 / per_traj));
     G = std::min(G, std::max<size_t>(1, max_batch_trajs));
+    trace("%zu batches of bases", pbatches.size());
     real* state = (real*)dev_get(c, "state", sizeof(real) * nst * N * G);
     double* parts = (double*)dev_get(c, "norm_part", sizeof(double) * 2 * nparts * G);
     if (!state || !parts) return fail(c, MQW_ENOMEM, "device allocation failed for %zu trajectory slots at n=%d", G, n);
@@ -1713,10 +2001,10 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     // a buffer.
     struct BatchBases {
         Bases b;
-        std::vector<size_t> offv, offx, offw, qoff;
+        std::vector<size_t> offv, offx, offw, qoff, offy, yoff;
         std::vector<uint32_t> rows;
         size_t total = 0, plus_off = 0;
-        bool any_w = false, use = false;
+        bool any_w = false, any_y = false, use = false;
     };
     std::vector<BatchBases> bbs(pbatches.size());
     size_t max_total = 0, max_rows = 0;
@@ -1741,6 +2029,15 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             bb.qoff.assign(ps.size(), 0);
             bases.cmp.assign(ps.size(), 0);
             bases.plusv.assign(ps.size(), nullptr);
+            bases.Z.assign(ps.size(), nullptr);
+            bases.kz.assign(ps.size(), 0);
+            bases.Y.assign(ps.size(), nullptr);
+            bases.ky.assign(ps.size(), 0);
+            bases.ytab.assign(ps.size(), nullptr);
+            bases.y0.assign(ps.size(), 0);
+            bases.y1.assign(ps.size(), 0);
+            bb.offy.assign(ps.size(), 0);
+            bb.yoff.assign(ps.size(), 0);
             size_t vbytes = 0;
             std::vector<uint32_t> rows;
             for (size_t p = pb.first; p < pb.second; p++) {
@@ -1757,7 +2054,17 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
                     bb.total += bv;
                     vbytes += bv;
                 }
-                if (kx[p] > 0 && vbytes + bx <= vcap) {
+                if (kzq[p] > 0) { // the two-stage adjoint: Z from before the batch, Y in it, no X
+                    bases.kz[p] = kzq[p];
+                    bases.ky[p] = ky[p];
+                    bases.Z[p] = zbase[p];
+                    bb.offy[p] = bb.total;
+                    bb.total += ybytes(p);
+                    second_stage_rows(kzq[p], ky[p], rows, bases.y0[p], bases.y1[p]);
+                    bb.yoff[p] = bb.rows.size();
+                    bb.rows.insert(bb.rows.end(), rows.begin(), rows.end());
+                    bb.any_y = true;
+                } else if (kx[p] > 0 && vbytes + bx <= vcap) {
                     bases.kx[p] = kx[p];
                     bb.offx[p] = bb.total;
                     bb.total += bx;
@@ -1772,7 +2079,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
                     bb.rows.insert(bb.rows.end(), rows.begin(), rows.end());
                     bb.any_w = true;
                 }
-                bb.use |= bases.kc[p] > 0 || bases.kx[p] > 0;
+                bb.use |= bases.kc[p] > 0 || bases.kx[p] > 0 || bases.kz[p] > 0;
             }
             bb.plus_off = bb.total;
             if (bb.any_w) bb.total += sizeof(float) * N;
@@ -1787,6 +2094,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     const int nbuf = overlap ? 2 : 1;
     char* bbase[2] = {nullptr, nullptr};
     uint32_t* bq[2] = {nullptr, nullptr};
+    trace("trajectory slots: %zu", G);
     for (int b = 0; b < nbuf && max_total > 0; b++) {
         bbase[b] = (char*)dev_get(c, "basis" + std::to_string(b), max_total);
         if (!bbase[b]) return fail(c, MQW_ENOMEM, "device allocation failed for the shared bases");
@@ -1822,11 +2130,10 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         const hipStream_t bs = overlap ? c->stream2 : c->stream;
         char* base = bbase[b];
         const auto& pb = pbatches[q];
-        if (bb.any_w) {
-            bases.plus = (const float*)(base + bb.plus_off);
+        if (bb.any_w) bases.plus = (const float*)(base + bb.plus_off);
+        if (!bb.rows.empty())
             HIPCHK(c, hipMemcpyAsync(bq[b], bb.rows.data(), sizeof(uint32_t) * bb.rows.size(), hipMemcpyHostToDevice,
                                      bs));
-        }
         for (size_t p = pb.first; p < pb.second; p++) {
             if (bases.kc[p] > 0 && !bases.cmp[p])
                 bases.V[p] = (const float*)(base + bb.offv[p]); // unused when kc = 1 (V_0 only)
@@ -1834,6 +2141,10 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             if (bases.kc2[p] > 0) {
                 bases.W[p] = (const float*)(base + bb.offw[p]);
                 bases.qtab[p] = bq[b] + bb.qoff[p];
+            }
+            if (bases.kz[p] > 0) {
+                bases.Y[p] = (const float*)(base + bb.offy[p]);
+                bases.ytab[p] = bq[b] + bb.yoff[p];
             }
         }
         mqw_status rs = MQW_OK;
@@ -1853,6 +2164,12 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             bases.W.clear();
             bases.kc2.clear();
         }
+        if (!bb.any_y) {
+            bases.Z.clear();
+            bases.kz.clear();
+            bases.Y.clear();
+            bases.ky.clear();
+        }
         return MQW_OK;
     };
     if (overlap) {
@@ -1871,6 +2188,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             st = bind(q);
             if (st) return st;
         }
+        trace("batch %zu of %zu: bases bound", q, pbatches.size());
         Bases& bases = bbs[q].b;
         const bool use_bases = bbs[q].use;
         c->stats.basis_batches += use_bases;
@@ -1883,13 +2201,14 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
                 if (tr.direct_done) continue;
                 tr.second_done = use_bases && !bases.kc2.empty() && bases.kc2[p] > 0;
                 tr.first_done = use_bases && !tr.second_done && bases.kc[p] > 0;
-                tr.adjoint = use_bases && bases.kx[p] > 0;
+                tr.adjoint2 = use_bases && !bases.kz.empty() && bases.kz[p] > 0;
+                tr.adjoint = use_bases && !tr.adjoint2 && bases.kx[p] > 0;
                 order.push_back(t);
             }
         std::vector<size_t> cost(trajs.size(), 0);
         for (size_t t : order) {
             const Traj& tr = trajs[t];
-            const size_t s1 = tr.segs.size() - (tr.adjoint ? 1 : 0);
+            const size_t s1 = tr.segs.size() - (tr.adjoint2 ? 2 : tr.adjoint ? 1 : 0);
             for (size_t s = tr.second_done ? 2 : tr.first_done ? 1 : 0; s < s1; s++)
                 cost[t] += std::max<size_t>(tr.segs[s].c.size() - 1, 1);
         }
@@ -1913,9 +2232,11 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             Plan<real> plan;
             build_plan<real>(n, nparts, trajs, batch, ps, E, slots, probs_d, plan, k.hi,
                              use_bases ? &bases : nullptr);
+            trace("batch %zu: plan of %zu ops", q, plan.ops.size());
             double ms_sweep = 0.0, ms_aux = 0.0, ms_basis = 0.0, ms_gemm = 0.0;
             st = run_plan<real>(c, precision, n, k, plan, o.use_graph != 0, &ms_sweep, &ms_aux, &ms_basis, &ms_gemm);
             if (st) return st;
+            trace("batch %zu: plan ran", q);
             c->stats.gpu_ms += ms_sweep + ms_aux;
             c->stats.sweep_ms += ms_sweep;
             c->stats.aux_ms += ms_aux;
@@ -2090,6 +2411,7 @@ void mqw_opts_default(mqw_opts* o)
     o->second_stage_gemm = MQW_GEMM_SPLIT_FP16;
     o->compress_bases = 1;
     o->direct_m1 = 1;
+    o->adjoint_two_stages = 1;
     o->sample_probs = nullptr;
 }
 

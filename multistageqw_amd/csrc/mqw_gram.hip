@@ -1,6 +1,7 @@
 // Gram matrices of the shared first-stage bases (the compressed second stage,
-// see GramJob): per instance, the sums over all amplitudes of V_k1 V_k2 for
-// k1 <= k2 of equal parity, with V_0 = psi0 * ones.  Products of FP32 values
+// see GramJob) and of the last-stage adjoint bases X: per instance, the sums
+// over all amplitudes of V_k1 V_k2 for k1 <= k2 of equal parity, with
+// V_0 = psi0 * ones or (X_0) a unit vector.  Products of FP32 values
 // are exact in FP64 and the sums are FP64: per workgroup over a chunk of
 // amplitudes, then over the chunks in a fixed order.
 #include <hip/hip_runtime.h>
@@ -46,7 +47,8 @@ gram_kernel(const GramJob* __restrict__ jobs, uint32_t njobs, int n, uint32_t nc
         __syncthreads();
         for (int i = t; i < kc * kSlab; i += 256) {
             const int k = i / kSlab, x = i % kSlab;
-            s[k][x] = k == 0 ? J.psi0 : J.V[(uint64_t)(k - 1) * N + base + a0 + (uint64_t)x];
+            const uint64_t a = base + a0 + (uint64_t)x;
+            s[k][x] = k > 0 ? J.V[(uint64_t)(k - 1) * N + a] : J.unit1 ? (a + 1 == J.unit1 ? 1.0f : 0.0f) : J.psi0;
         }
         __syncthreads();
 #pragma unroll
@@ -105,6 +107,7 @@ int launch_gram(const GramJob* jobs_dev, uint32_t njobs, uint32_t kc_max, int n,
     hipStream_t st = (hipStream_t)stream;
     if (kc_max <= 32) return launch_gram_kc<32>(jobs_dev, njobs, n, nchunk, partial, st);
     if (kc_max <= 64) return launch_gram_kc<64>(jobs_dev, njobs, n, nchunk, partial, st);
+    if (kc_max <= 128) return launch_gram_kc<128>(jobs_dev, njobs, n, nchunk, partial, st);
     if (kc_max <= (uint32_t)kGramMaxK) return launch_gram_kc<kGramMaxK>(jobs_dev, njobs, n, nchunk, partial, st);
     return -2;
 }

@@ -103,6 +103,8 @@ struct ReduceJob {
 // of its own Jacobi-Anger coefficients: the quantity the Clenshaw terms of
 // that stage evaluate, by the forward recurrence instead of the backward one.
 constexpr int kCombineMaxK = 64;   // basis vectors per instance, V_0 included
+constexpr int kCombineMaxKX = 128; // the same for the compression of a last-stage adjoint basis X
+constexpr int kAdjMaxK = 254;      // a compressed adjoint basis: two combine jobs of at most 128 vectors
 constexpr int kCombineBatch = 64;  // samples per combine job
 struct CombineItem {
     float* out;          // planar output (2N floats)
@@ -112,10 +114,13 @@ struct CombineItem {
 };
 struct CombineJob {
     const float* V;      // V_1 .. V_{kc-1}, N floats each, contiguous
-    uint32_t kc;         // basis vectors including V_0 = psi0 * ones (<= kCombineMaxK)
+    uint32_t kc;         // basis vectors including V_0 (<= kCombineMaxKX)
     uint32_t ns;         // samples (items) of this job (<= kCombineBatch)
     uint32_t item_off;   // first item
-    float psi0;
+    float psi0;          // V_0 = psi0 * ones ...
+    uint32_t unit1;      // ... unless unit1 > 0: V_0 = the unit vector at index unit1 - 1 (an adjoint basis X)
+    uint32_t accum;      // 1 = add to the outputs (the second job of a basis wider than kCombineMaxKX);
+                         // its norm partials are not written
 };
 // Compressed first-stage basis for the shared second stage.  The samples'
 // first-stage states psi_j = sum_k c_j[k] i^(k&1) V_k span far fewer
@@ -127,17 +132,22 @@ struct CombineJob {
 // coordinates u_j[r] with psi_j = sum_r u_j[r] Q_r to within a set relative
 // error per sample; the second stage's basis is then T_l(Ht_2) Q_r.
 struct GramJob {
-    const float* V;     // V_1 .. V_{kc-1}, N floats each (V_0 = psi0 * ones)
+    const float* V;     // V_1 .. V_{kc-1}, N floats each
     double* out;        // gram_pairs(kc) pair sums, written by the reduce launch
     uint32_t kc;
-    float psi0;
+    float psi0;         // V_0 = psi0 * ones, unless unit1 > 0: the unit vector at index unit1 - 1
+    uint32_t unit1;
+    uint32_t pad;
 };
 // pairs (k1 <= k2, k1 = k2 mod 2) ordered by k2 then k1: pair_start(k2) + k1 / 2
 inline uint32_t gram_pair_start(uint32_t k2) { return k2 == 0 ? 0u : k2 + ((k2 - 1) * (k2 - 1)) / 4; }
 inline uint32_t gram_pairs(uint32_t kc) { return gram_pair_start(kc); }
-constexpr int kGramMaxK = 128;
+constexpr int kGramMaxK = 256;
 // pairs per partial sum of a launch whose largest basis has kc_max vectors
-inline uint32_t gram_pmax(uint32_t kc_max) { return gram_pairs(kc_max <= 32 ? 32u : kc_max <= 64 ? 64u : 128u); }
+inline uint32_t gram_pmax(uint32_t kc_max)
+{
+    return gram_pairs(kc_max <= 32 ? 32u : kc_max <= 64 ? 64u : kc_max <= 128 ? 128u : 256u);
+}
 // Pair sums of every job's vectors: njobs x nchunk partial sums of
 // gram_pairs(kc_max) doubles in `partial`, then their fixed-order sum per job.
 int launch_gram(const GramJob* jobs_dev, uint32_t njobs, uint32_t kc_max, int n, uint32_t nchunk, double* partial,
@@ -165,11 +175,12 @@ int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem
 // The rows (l, k) split into a real class ((l&1) == (k&1); sign -1 when both
 // are odd) and an imaginary class.  The scale of psi2 is irrelevant: every
 // later stage and the readout normalise their input.
-constexpr int kC2MaxK = 64;   // kc1 and kc2 limits
+constexpr int kC2MaxK = 64;   // kc1 limit; kc2 limit of the FP32 kernel
+constexpr int kC2MaxK2 = 128; // kc2 limit of the split-FP16 kernel
 constexpr int kC2Tile = 64;   // samples per workgroup of the FP32 kernel (2 MFMA row blocks of 32)
 constexpr int kC2TileH = 128; // samples per workgroup of the split-FP16 kernel (4 row blocks of 32)
 constexpr int kC2RowPad = 16; // each class's rows padded to a multiple of this (one FP16 MFMA k block)
-constexpr int kC2MaxRows = kC2MaxK * kC2MaxK + 2 * kC2RowPad;
+constexpr int kC2MaxRows = kC2MaxK * kC2MaxK2 + 2 * kC2RowPad;
 struct Combine2Item {
     float* out;           // planar output (2N floats)
     double* norm_part;    // nparts partial sums of |out|^2, one per workgroup
@@ -211,9 +222,11 @@ inline uint32_t combine2h_nsp(uint32_t ns_max) { return ns_max <= 32 ? 32u : ns_
 // within the series' truncation (the tail sum of |c_k| beyond K, at most the
 // coefficient tolerance), so the probability is |sum_k a_k <X_k, psi>|^2 / ||psi||^2.
 constexpr int kReadTile = 64;   // samples per dot workgroup
-constexpr int kReadMaxK = 128;  // basis vectors per instance (X_0 included)
+constexpr int kReadMaxK = 256;    // basis vectors per instance (X_0 included), matrix-core dots
+constexpr int kReadMaxKF32 = 128; // the same for the FP32 vector dots (values in registers)
 struct ReadItem {
     const float* psi;         // the last stage's input state (planar 2N, unnormalised)
+    const float* phi;         // pair dots only: the other factor (planar 2N)
     const double* norm_part;  // its nparts partial sums of |psi|^2
     double* prob;             // receives the probability
     uint32_t coef_off;        // kx coefficients (zero padded) of the last stage
@@ -221,7 +234,7 @@ struct ReadItem {
 };
 struct ReadTile {
     const float* X;           // X_0 .. X_{kx-1}, N floats each
-    uint32_t kx;              // basis size (<= kReadMaxK)
+    uint32_t kx;              // basis size (<= kReadMaxK; kReadMaxKF32 for the FP32 dots)
     uint32_t ns;              // items of the tile (<= kReadTile)
     uint32_t item_off;
     uint32_t pad;
@@ -237,6 +250,20 @@ int launch_readout_dots(const ReadTile* tiles_dev, uint32_t ntiles, const ReadIt
 // phi = X a as a product over the basis index, then sum_a phi psi per lane.
 int launch_readout_dots_mfma(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev,
                              const float* coefs_dev, int n, uint32_t nchunk, double* dots, void* stream);
+// Two-stage adjoint readout (m >= 4).  For the last two stages, with P_i the
+// stage propagators and e = e_eloc,
+//     psi_m[eloc] = e^T P_m P_{m-1} psi = (P_{m-1}^T P_m^T e)^T psi,
+// and P^T = sum_k a_k T_k(Ht) (T_k real symmetric), so the adjoint state
+// phi_j = P_m^T e = sum_k a_jk X_k lies in the span of the last stage's basis
+// X, which compresses like the first stage's V (the samples' phi_j span few
+// directions Z_r).  phi'_j = P_{m-1}^T phi_j is then a second-stage GEMM
+// over Y_lr = T_l(Ht_{m-1}) Z_r, and the amplitude the bilinear dot
+// sum_a phi'_j[a] psi_j[a] with psi_j the input of stage m - 1; its norm
+// stands for the output's, as for the one-stage readout.
+// Partial amplitudes sum_a phi[a] psi[a] (complex, no conjugate) of every
+// item's pair, per amplitude chunk, in the layout of launch_readout_dots.
+int launch_pair_dots(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev, int n, uint32_t nchunk,
+                     double* dots, void* stream);
 // Per item: sum the chunks and divide |amplitude|^2 by the norm.
 int launch_readout_finalize(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev, uint32_t nchunk,
                             uint32_t nparts, const double* dots, void* stream);

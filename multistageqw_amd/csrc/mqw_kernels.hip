@@ -1443,7 +1443,7 @@ combine_kernel(const CombineJob* __restrict__ jobs, uint32_t njobs, const Combin
         const uint64_t a = base + a0 + (uint64_t)t;
         const MQW_GLOBAL float* V = gview(J.V);
         float v[KR];
-        v[0] = J.psi0;
+        v[0] = J.unit1 ? (a + 1 == J.unit1 ? 1.0f : 0.0f) : J.psi0;
 #pragma unroll
         for (int k = 1; k < KR; k++) v[k] = k < kc ? V[(uint64_t)(k - 1) * N + a] : 0.0f;
         // keep the column in registers for the whole sample loop (the loads
@@ -1459,6 +1459,10 @@ combine_kernel(const CombineJob* __restrict__ jobs, uint32_t njobs, const Combin
                 im += c[k + 1] * v[k + 1];
             }
             MQW_GLOBAL float* o = gview(items[J.item_off + j].out);
+            if (J.accum) {
+                re += o[a];
+                im += o[N + a];
+            }
             o[a] = re;
             o[N + a] = im;
             double s = (double)re * (double)re + (double)im * (double)im;
@@ -1468,7 +1472,8 @@ combine_kernel(const CombineJob* __restrict__ jobs, uint32_t njobs, const Combin
         }
     }
     __syncthreads();
-    if (t < ns) items[J.item_off + t].norm_part[part] = ((s_w[0][t] + s_w[1][t]) + s_w[2][t]) + s_w[3][t];
+    if (t < ns && !J.accum)
+        items[J.item_off + t].norm_part[part] = ((s_w[0][t] + s_w[1][t]) + s_w[2][t]) + s_w[3][t];
 }
 
 int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem* items_dev, const float* coefs_dev,
@@ -1478,7 +1483,7 @@ int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem
     const uint64_t N = 1ull << n;
     // whole 256-amplitude rows per workgroup
     if (nparts == 0 || N % nparts != 0 || (N / nparts) % 256 != 0) return -2;
-    if (kc_max < 1 || kc_max > (uint32_t)kCombineMaxK) return -2;
+    if (kc_max < 1 || kc_max > (uint32_t)kCombineMaxKX) return -2;
     const uint64_t grid = (uint64_t)njobs * nparts;
     if (grid > 0x7fffffffull) return -3;
     hipStream_t st = (hipStream_t)stream;
@@ -1491,8 +1496,14 @@ int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem
     else if (kc_max <= 48)
         hipLaunchKernelGGL(combine_kernel<48>, dim3((uint32_t)grid), dim3(256), 0, st, jobs_dev, njobs, items_dev,
                            coefs_dev, n, nparts);
-    else
+    else if (kc_max <= 64)
         hipLaunchKernelGGL(combine_kernel<64>, dim3((uint32_t)grid), dim3(256), 0, st, jobs_dev, njobs, items_dev,
+                           coefs_dev, n, nparts);
+    else if (kc_max <= 96)
+        hipLaunchKernelGGL(combine_kernel<96>, dim3((uint32_t)grid), dim3(256), 0, st, jobs_dev, njobs, items_dev,
+                           coefs_dev, n, nparts);
+    else
+        hipLaunchKernelGGL(combine_kernel<128>, dim3((uint32_t)grid), dim3(256), 0, st, jobs_dev, njobs, items_dev,
                            coefs_dev, n, nparts);
     return check_launch("combine");
 }
@@ -1792,7 +1803,7 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
     const int col = lane & 31, half = lane >> 5;
     const uint64_t N = 1ull << n;
     // row codes: 0 for |+>, k for V_k (k < 64), 64 + k (kc2 - 1) + l - 1 for
-    // W_lk (below 4096 with kc1, kc2 <= 64)
+    // W_lk (below 8192 with kc1 <= 64, kc2 <= 128)
     for (int i = t; i < nq; i += NT) {
         const uint32_t e = T.qtab[i];
         const int l = (int)(e & 255u), k = (int)((e >> 8) & 255u);
@@ -2049,7 +2060,7 @@ int launch_readout_dots(const ReadTile* tiles_dev, uint32_t ntiles, const ReadIt
     if (ntiles == 0) return 0;
     const uint64_t N = 1ull << n;
     if (nchunk == 0 || N % nchunk != 0 || (N / nchunk) % 256 != 0) return -2;
-    if (kx_max < 1 || kx_max > (uint32_t)kReadMaxK) return -2;
+    if (kx_max < 1 || kx_max > (uint32_t)kReadMaxKF32) return -2;
     const uint64_t grid = (uint64_t)ntiles * nchunk;
     if (grid > 0x7fffffffull) return -3;
     hipStream_t st = (hipStream_t)stream;
@@ -2248,6 +2259,72 @@ int launch_readout_dots_mfma(const ReadTile* tiles_dev, uint32_t ntiles, const R
     hipLaunchKernelGGL((readout_mfma_kernel<1, 4>), dim3((uint32_t)grid), dim3(512), 0, (hipStream_t)stream,
                        tiles_dev, ntiles, items_dev, coefs_dev, n, nchunk, dots);
     return check_launch("readout_dot_mfma");
+}
+
+// Pair dots of the two-stage adjoint readout: one workgroup per (tile, chunk,
+// item), sum_a phi[a] psi[a] over the chunk (complex, no conjugate).  The
+// products of FP32 values are exact in FP64 and summed in FP64 per thread,
+// then by a cross-lane tree and the 4 waves in order (deterministic).  The
+// chunk is a multiple of 1024 amplitudes (readout_ops sizes chunks of at
+// least 4096).
+__global__ void __launch_bounds__(256)
+pair_dot_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const ReadItem* __restrict__ items, int n,
+                uint32_t nchunk, double* __restrict__ dots)
+{
+    __shared__ double s_w[4][2];
+    const uint32_t j = blockIdx.x % kReadTile;
+    const uint32_t tc = blockIdx.x / kReadTile;
+    const uint32_t chunk = tc % nchunk, tile = tc / nchunk;
+    if (tile >= ntiles) return;
+    const ReadTile& T = tiles[tile];
+    if (j >= T.ns) return;
+    const ReadItem& I = items[T.item_off + j];
+    const uint64_t N = 1ull << n;
+    const uint64_t C = N / nchunk;
+    const uint64_t base = (uint64_t)chunk * C;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const MQW_GLOBAL f4* pr = gview(reinterpret_cast<const f4*>(I.phi + base));
+    const MQW_GLOBAL f4* pi = gview(reinterpret_cast<const f4*>(I.phi + N + base));
+    const MQW_GLOBAL f4* sr = gview(reinterpret_cast<const f4*>(I.psi + base));
+    const MQW_GLOBAL f4* si = gview(reinterpret_cast<const f4*>(I.psi + N + base));
+    double re = 0.0, im = 0.0;
+    for (uint64_t q = (uint64_t)t; q < C / 4; q += 256) {
+        const f4 a = __builtin_nontemporal_load(pr + q), b = __builtin_nontemporal_load(pi + q);
+        const f4 x = __builtin_nontemporal_load(sr + q), y = __builtin_nontemporal_load(si + q);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            re = fma((double)a[e], (double)x[e], re);
+            re = fma(-(double)b[e], (double)y[e], re);
+            im = fma((double)a[e], (double)y[e], im);
+            im = fma((double)b[e], (double)x[e], im);
+        }
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        re += __shfl_xor(re, d, 64);
+        im += __shfl_xor(im, d, 64);
+    }
+    if (lane == 0) {
+        s_w[w][0] = re;
+        s_w[w][1] = im;
+    }
+    __syncthreads();
+    if (t < 2) dots[((uint64_t)tile * nchunk + chunk) * kReadTile * 2 + 2 * j + t] =
+        ((s_w[0][t] + s_w[1][t]) + s_w[2][t]) + s_w[3][t];
+}
+
+int launch_pair_dots(const ReadTile* tiles_dev, uint32_t ntiles, const ReadItem* items_dev, int n, uint32_t nchunk,
+                     double* dots, void* stream)
+{
+    if (ntiles == 0) return 0;
+    const uint64_t N = 1ull << n;
+    if (nchunk == 0 || N % nchunk != 0 || (N / nchunk) % 1024 != 0) return -2;
+    const uint64_t grid = (uint64_t)ntiles * nchunk * kReadTile;
+    if (grid > 0x7fffffffull) return -3;
+    hipLaunchKernelGGL(pair_dot_kernel, dim3((uint32_t)grid), dim3(256), 0, (hipStream_t)stream, tiles_dev, ntiles,
+                       items_dev, n, nchunk, dots);
+    return check_launch("pair_dots");
 }
 
 // One wave per item: lane l sums the chunk partials l, l + 64, ... and the

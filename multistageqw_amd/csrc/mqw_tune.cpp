@@ -32,11 +32,14 @@ struct Cand {
     int hp; // high_pass option: -1 by n, 0 off, 1..3 on
 };
 
+// From n = 14 only the real-half kernel: the shared bases (first and second
+// stage, adjoint readouts, direct m = 1 chains) run on it, so a complex-tile
+// choice would turn them off for every m > 1 instance of that n.
 std::vector<Cand> candidates(int n)
 {
     std::vector<Cand> v;
     for (int tb : {11, 12}) {
-        if (tb > n) continue;
+        if (tb > n || n >= 14) continue;
         for (int var = 0; var < term_variants(); var++)
             for (int lc : {0, 3, 6, 8})
                 if (lc <= n - tb) v.push_back({tb, var, lc, 0});
