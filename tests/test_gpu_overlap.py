@@ -23,9 +23,9 @@ def _say(*a):
     (16, 3, 3, 64, dict(max_device_bytes=2 << 30, compress_bases=False)),  # three batches (W 336-394 MiB each)
     (16, 3, 2, 64, dict(max_device_bytes=2 << 30, use_graph=False, compress_bases=False)),  # no graphs
     (16, 3, 3, 70, dict(max_device_bytes=2 << 30, group_bytes=24 << 20, compress_bases=False)),  # groups
-    # middle stages as terms; the seed of the second-stage test, whose two instances both share
-    # their second stage (W 203 MiB each)
-    (16, 5, 2, 48, dict(max_device_bytes=3 << 29, seed=6165, compress_bases=False)),
+    # middle stages as terms (no two-stage adjoint, whose bases would not fit the cap twice); the
+    # seed of the second-stage test, whose two instances both share their second stage (W 203 MiB each)
+    (16, 5, 2, 48, dict(max_device_bytes=3 << 29, seed=6165, compress_bases=False, adjoint_two_stages=False)),
     # compressed bases (W about 190-250 MiB each, 336-456 MiB uncompressed): count = 0 means
     # at least two batches of 512 MiB
     (16, 3, 0, 64, dict(max_device_bytes=2 << 30)),
@@ -87,7 +87,9 @@ def test_overlap_in_uneven_batch(ctx):
         (16, 3, M.generate_sk(16, 1, 7305)[0]),
         (14, 3, M.generate_sk(14, 1, 7306)[0]),
     ]
-    kw = dict(samples=64, with_samples=True, small_n_max=0, max_device_bytes=2 << 30)
+    # (the m = 4 instance's two-stage adjoint bases would not fit the cap twice: off here; its
+    # overlapped batches are tested with the two-stage adjoint)
+    kw = dict(samples=64, with_samples=True, small_n_max=0, max_device_bytes=2 << 30, adjoint_two_stages=False)
     p, tt, sp = ctx.solve_batch(inst, **kw)
     st = ctx.last_stats()
     assert st["overlap_batches"] >= 2
