@@ -135,8 +135,9 @@ typedef struct {
                                 fewer directions than the kc1 Chebyshev vectors T_k(H1/onenorm1)|+>, so an
                                 SVD (weighted by the vectors' Gram matrix, per parity class) keeps the
                                 smallest set of orthonormal combinations Q_r that holds every sample's
-                                state to within 1e-7 relative, and W_lr = T_l(H2/onenorm2) Q_r replaces
-                                W_lk (about 20 instead of 37 chains at n = 20); 0 = W from every V_k */
+                                state to within 2e-7 relative (a quarter of the FP32 rounding of the
+                                basis itself), and W_lr = T_l(H2/onenorm2) Q_r replaces W_lk (about 18
+                                instead of 37 chains at n = 20); 0 = W from every V_k */
     int direct_m1;           /* 1 (default) = an m == 1 instance's chained samples (MultiQW.cpp:426-430: each
                                 sample continues from the previous one by the next time increment) are
                                 evaluated directly: sample j is exp(iH T_j)|+> with T_j the sum of the
@@ -149,7 +150,7 @@ typedef struct {
                                 adjoint states phi_j = P_m^T e_Eloc (P_i the stage propagators, sums of
                                 T_k(H_i/onenorm_i) with the sample's coefficients; T_k is symmetric) lie
                                 in the span of the last stage's basis T_k(H_m/onenorm_m) e_Eloc, which is
-                                compressed like the first stage's (1e-7 per sample); P_{m-1}^T phi_j is a
+                                compressed like the first stage's (2e-7 per sample); P_{m-1}^T phi_j is a
                                 second-stage GEMM over T_l(H_{m-1}/onenorm_{m-1}) Z_r, and the amplitude
                                 psi_m[E_loc] = (P_{m-1}^T phi_j)^T psi_{m-2} a dot with the input of stage
                                 m - 1, whose norm stands for the output's (two truncated series: at most

@@ -8,8 +8,9 @@
 // left singular vectors are those of A V itself.  A one-sided Jacobi SVD of M
 // (column rotations, accurate for small singular values) gives columns
 // m_i = sigma_i u_i.  Keeping the top R columns reconstructs sample j to
-// within sqrt(sum_{i >= R} m_i[j]^2); R is the smallest rank that keeps every
-// sample within rel_tol of its own norm.  The kept directions are the unit
+// within sqrt(sum_{i >= R} m_i[j]^2) per class; R is the smallest rank that
+// keeps every sample's discarded part, both classes together, within rel_tol
+// of its whole norm.  The kept directions are the unit
 // vectors Q_r = (1 / sigma_r^2) sum_j m_r[j] (A V)_j, i.e. B = m_r^T A / sigma_r^2,
 // and sample j's coordinates are m_r[j].
 #include <math.h>
@@ -147,21 +148,27 @@ void class_svd(int kc, int cls, const double* gram, const std::vector<const doub
     }
 }
 
-// Smallest rank keeping every sample within rel_tol of its own norm.
-int class_rank(const ClassSvd& c, int S, double rel_tol)
+// Smallest rank R (the same for both classes) keeping every sample's
+// discarded part, over both classes, within rel_tol of its whole norm.
+int joint_rank(const ClassSvd (&c)[2], int S, double rel_tol)
 {
-    std::vector<double> tot(S, 0.0), tail(S, 0.0);
-    for (int i = 0; i < c.nc; i++)
-        for (int j = 0; j < S; j++) tot[j] += c.m[(size_t)i * S + j] * c.m[(size_t)i * S + j];
-    tail = tot;
-    for (int R = 0; R <= c.nc; R++) {
-        bool ok = true;
-        for (int j = 0; j < S && ok; j++) ok = tail[j] <= rel_tol * rel_tol * tot[j];
-        if (ok) return R;
-        if (R < c.nc)
-            for (int j = 0; j < S; j++) tail[j] -= c.m[(size_t)R * S + j] * c.m[(size_t)R * S + j];
+    std::vector<double> tot(S, 0.0), tail[2];
+    for (int cls = 0; cls < 2; cls++) {
+        tail[cls].assign(S, 0.0);
+        for (int i = 0; i < c[cls].nc; i++)
+            for (int j = 0; j < S; j++) tail[cls][j] += c[cls].m[(size_t)i * S + j] * c[cls].m[(size_t)i * S + j];
+        for (int j = 0; j < S; j++) tot[j] += tail[cls][j];
     }
-    return c.nc;
+    const int nmax = std::max(c[0].nc, c[1].nc);
+    for (int R = 0; R <= nmax; R++) {
+        bool ok = true;
+        for (int j = 0; j < S && ok; j++) ok = tail[0][j] + tail[1][j] <= rel_tol * rel_tol * tot[j];
+        if (ok) return R;
+        for (int cls = 0; cls < 2; cls++)
+            if (R < c[cls].nc)
+                for (int j = 0; j < S; j++) tail[cls][j] -= c[cls].m[(size_t)R * S + j] * c[cls].m[(size_t)R * S + j];
+    }
+    return nmax;
 }
 
 } // namespace
@@ -172,12 +179,9 @@ bool compress_first_stage(int kc, const double* gram, const std::vector<const do
     const int S = (int)coefs.size();
     if (kc < 4 || S < 1) return false;
     ClassSvd cs[2];
-    int R = 1;
-    for (int cls = 0; cls < 2; cls++) {
-        class_svd(kc, cls, gram, coefs, cs[cls]);
-        R = std::max(R, class_rank(cs[cls], S, rel_tol));
-    }
-    if (2 * R >= kc || R > cs[1].nc) return false;
+    for (int cls = 0; cls < 2; cls++) class_svd(kc, cls, gram, coefs, cs[cls]);
+    const int R = std::max(1, joint_rank(cs, S, rel_tol));
+    if (2 * R >= kc || R > cs[1].nc || R > cs[0].nc) return false;
     out.R = R;
     out.B.assign((size_t)2 * R * kc, 0.0);
     out.u.assign((size_t)S * 2 * R, 0.0);
