@@ -1427,7 +1427,7 @@ mqw_status direct_m1(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
         std::vector<GramJob> gj(njobs);
         for (size_t i = i0; i < i1; i++) {
             const size_t p = (size_t)trajs[ch[i].t].prob;
-            gj[i - i0] = GramJob{vb.V[p], gram_d + goff[i - i0], (uint32_t)ch[i].kc, psi0};
+            gj[i - i0] = GramJob{vb.V[p], nullptr, gram_d + goff[i - i0], (uint32_t)ch[i].kc, psi0, 0u, 0u};
         }
         EventPair ev;
         HIPCHK(c, ev.create());
@@ -1484,27 +1484,195 @@ mqw_status direct_m1(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
 }
 
 // ---------------------------------------------------------------------------
-// Compressed bases computed before the batches, in chunks of instances whose
-// bases fit in vcap bytes:
-// - the first-stage basis V (see GramJob), for the instances whose second
-//   stage comes from the shared basis W (kc2 > 0);
-// - the last-stage adjoint basis X (X_0 = e_eloc), for the instances whose
-//   last two stages are read out by the two-stage adjoint (kxa > 0): the
-//   samples' adjoint states phi_j = sum_k a_jk X_k span few directions too.
-// The recurrences of every instance of a chunk run in one plan, their Gram
-// pair sums on the device, then on the host one SVD per instance, basis and
-// parity class (compress_first_stage, on up to 16 threads), and combine
-// launches write the kept directions: item r of an instance holds the even
-// class's direction 2r in its real half and the odd class's 2r + 1 in its
-// imaginary half, so the vectors lie in index order, N floats apart.
-// kcq[p] = 2R and qbase[p] = Q_0 for V (every trajectory gets its coordinates
-// in c1q), kzq[p] = 2R and zbase[p] = Z_0 for X (coordinates in cxq); 0 = not
-// compressed.
+// Compression of a shared basis (see GramJob): a basis of kc real vectors
+// (V_0 stored at V0, else psi0 * ones or the unit vector at unit1 - 1; V_1 ..
+// at V, N floats apart) and the samples' coefficient rows over it.  The Gram
+// pair sums run on the device, one launch per size class of the kernel (a
+// wide basis would otherwise set the pair count and occupancy of every job);
+// one SVD per job and parity class on the host, on up to 16 threads
+// (compress_first_stage); then combine launches write the kept directions:
+// item r holds the even class's direction 2r in its real half and the odd
+// class's 2r + 1 in its imaginary half, so the 2R vectors lie in index order,
+// N floats apart.  A basis wider than kCombineMaxKX takes two combine jobs,
+// the second adding its vectors 127.. to the first's outputs.
 // ---------------------------------------------------------------------------
 // per-sample relative error of the kept span: a quarter of the FP32 rounding of
 // the basis itself (7.4e-7 relative, measured on bench instances)
 constexpr double kCompressTol = 2e-7;
 
+struct Compression {
+    int kc = 0;
+    const float* V = nullptr;
+    const float* V0 = nullptr;
+    float psi0 = 0.0f;
+    uint32_t unit1 = 0;
+    size_t S = 0;
+    std::vector<double> rows; // S x kc coefficients
+    size_t goff = 0;          // its pair sums in the Gram array
+    bool ok = false;
+    FirstStageCompression fc;
+};
+
+// Gram pair sums and SVDs of the jobs, on c->stream (synchronised on return).
+mqw_status compress_svd(mqw_ctx* c, int n, std::vector<Compression>& cj, const std::string& tag)
+{
+    const size_t N = (size_t)1 << n;
+    const uint32_t nchunk = (uint32_t)std::min<size_t>(64, N / 64);
+    const size_t njobs = cj.size();
+    if (njobs == 0) return MQW_OK;
+    size_t pairs = 0;
+    uint32_t kcmax = 0;
+    for (Compression& j : cj) {
+        j.goff = pairs;
+        pairs += gram_pairs((uint32_t)j.kc);
+        kcmax = std::max<uint32_t>(kcmax, (uint32_t)j.kc);
+    }
+    const size_t pmax = gram_pmax(kcmax);
+    double* gram_d = (double*)dev_get(c, tag + "_gram", sizeof(double) * pairs);
+    double* part_d = (double*)dev_get(c, tag + "_gram_part", sizeof(double) * njobs * nchunk * pmax);
+    GramJob* gj_d = (GramJob*)dev_get(c, tag + "_gram_jobs", sizeof(GramJob) * njobs);
+    if (!gram_d || !part_d || !gj_d) return fail(c, MQW_ENOMEM, "device allocation failed for Gram matrices");
+    auto gclass = [](int kc) { return kc <= 32 ? 0 : kc <= 64 ? 1 : kc <= 128 ? 2 : 3; };
+    std::vector<GramJob> gj;
+    std::vector<size_t> cls_end(4, 0);
+    std::vector<uint32_t> cls_kc(4, 0);
+    for (int g = 0; g < 4; g++) {
+        for (const Compression& j : cj)
+            if (gclass(j.kc) == g) {
+                gj.push_back(GramJob{j.V, j.V0, gram_d + j.goff, (uint32_t)j.kc, j.psi0, j.unit1, 0u});
+                cls_kc[g] = std::max<uint32_t>(cls_kc[g], (uint32_t)j.kc);
+            }
+        cls_end[g] = gj.size();
+    }
+    EventPair ev;
+    HIPCHK(c, ev.create());
+    HIPCHK(c, hipEventRecord(ev.a, c->stream));
+    HIPCHK(c, hipMemcpyAsync(gj_d, gj.data(), sizeof(GramJob) * njobs, hipMemcpyHostToDevice, c->stream));
+    for (int g = 0, a = 0; g < 4; a = (int)cls_end[g], g++)
+        if (cls_end[g] > (size_t)a &&
+            launch_gram(gj_d + a, (uint32_t)(cls_end[g] - a), cls_kc[g], n, nchunk, part_d, c->stream))
+            return fail(c, MQW_EDEVICE, "gram: %s", last_kernel_error());
+    HIPCHK(c, hipEventRecord(ev.b, c->stream));
+    std::vector<double> gram(pairs);
+    HIPCHK(c, hipMemcpyAsync(gram.data(), gram_d, sizeof(double) * pairs, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, ev.a, ev.b));
+    c->stats.gpu_ms += ms;
+    c->stats.aux_ms += ms;
+    c->stats.aux_launches += 2 * (uint64_t)std::count_if(cls_end.begin(), cls_end.end(), [&](size_t e) {
+        return e > 0;
+    });
+    for (const Compression& j : cj) c->stats.aux_bytes += 4.0 * (double)N * (double)(j.kc - (j.V0 ? 0 : 1));
+    trace("compression: %zu Gram matrices", njobs);
+    auto work = [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; i++) {
+            Compression& j = cj[i];
+            std::vector<const double*> cp(j.S);
+            for (size_t s = 0; s < j.S; s++) cp[s] = &j.rows[s * (size_t)j.kc];
+            j.ok = compress_first_stage(j.kc, &gram[j.goff], cp, kCompressTol, j.fc);
+        }
+    };
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t nt = std::min<size_t>({(size_t)(hw ? hw : 1), 16, njobs});
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; t++) th.emplace_back(work, njobs * t / nt, njobs * (t + 1) / nt);
+    work(0, njobs / nt);
+    for (std::thread& x : th) x.join();
+    trace("compression: %zu SVDs", njobs);
+    return MQW_OK;
+}
+
+// The kept directions of every successful job into out[i] (2R vectors of N
+// floats), on c->stream (not synchronised).
+mqw_status compress_combine(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Compression>& cj,
+                            const std::vector<float*>& out, const std::string& tag)
+{
+    const size_t N = (size_t)1 << n;
+    size_t nitems = 0;
+    uint32_t kcm = 0;
+    for (const Compression& j : cj)
+        if (j.ok) {
+            nitems += (size_t)j.fc.R;
+            kcm = std::max<uint32_t>(kcm, (uint32_t)j.kc);
+        }
+    if (nitems == 0) return MQW_OK;
+    double* dummy = (double*)dev_get(c, tag + "_norm", sizeof(double) * k.parts * nitems);
+    CombineJob* cj_d = (CombineJob*)dev_get(c, tag + "_cjobs", sizeof(CombineJob) * cj.size());
+    CombineJob* cj2_d = (CombineJob*)dev_get(c, tag + "_cjobs2", sizeof(CombineJob) * cj.size());
+    CombineItem* ci_d = (CombineItem*)dev_get(c, tag + "_citems", sizeof(CombineItem) * nitems);
+    CombineItem* ci2_d = (CombineItem*)dev_get(c, tag + "_citems2", sizeof(CombineItem) * nitems);
+    float* cf_d = (float*)dev_get(c, tag + "_coefs", sizeof(float) * nitems * (kcm + 2));
+    if (!dummy || !cj_d || !cj2_d || !ci_d || !ci2_d || !cf_d)
+        return fail(c, MQW_ENOMEM, "device allocation failed for compressed bases");
+    // the second job of a wide basis: its vectors 0 and 1 (basis vectors 126
+    // and 127, in the first job) have no weight, so the parity classes stay
+    std::vector<CombineJob> jb, jb2;
+    std::vector<CombineItem> ci, ci2;
+    std::vector<float> cf;
+    uint32_t kcb_max = 0;
+    for (size_t i = 0; i < cj.size(); i++) {
+        const Compression& j = cj[i];
+        if (!j.ok) continue;
+        const int kc = j.kc, R = j.fc.R;
+        const int kca = std::min(kc, kCombineMaxKX), kcb = kc > kCombineMaxKX ? kc - (kCombineMaxKX - 2) : 0;
+        jb.push_back(CombineJob{j.V, j.V0, (uint32_t)kca, (uint32_t)R, (uint32_t)ci.size(), j.psi0, j.unit1, 0u});
+        if (kcb > 0) {
+            jb2.push_back(CombineJob{j.V + (size_t)(kCombineMaxKX - 2) * N, nullptr, (uint32_t)kcb, (uint32_t)R,
+                                     (uint32_t)ci2.size(), 0.0f, 0u, 1u});
+            kcb_max = std::max<uint32_t>(kcb_max, (uint32_t)kcb);
+        }
+        const double* B = j.fc.B.data();
+        for (int r = 0; r < R; r++) {
+            CombineItem it{};
+            it.out = out[i] + (size_t)2 * r * N;
+            it.norm_part = dummy + ci.size() * k.parts;
+            it.coef_off = (uint32_t)cf.size();
+            for (int q = 0; q < kca; q++) cf.push_back((float)B[(size_t)(2 * r + (q & 1)) * kc + q]);
+            ci.push_back(it);
+            if (kcb > 0) {
+                it.coef_off = (uint32_t)cf.size();
+                for (int q = 0; q < kcb; q++) {
+                    const int kk = q + kCombineMaxKX - 2; // basis index of column q
+                    cf.push_back(q < 2 ? 0.0f : (float)B[(size_t)(2 * r + (kk & 1)) * kc + kk]);
+                }
+                ci2.push_back(it);
+            }
+        }
+        c->stats.aux_bytes += (double)N * (4.0 * (kc - (j.V0 ? 0 : 1)) + 8.0 * R);
+    }
+    HIPCHK(c, hipMemcpyAsync(cj_d, jb.data(), sizeof(CombineJob) * jb.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ci_d, ci.data(), sizeof(CombineItem) * ci.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cf_d, cf.data(), sizeof(float) * cf.size(), hipMemcpyHostToDevice, c->stream));
+    if (launch_combine(cj_d, (uint32_t)jb.size(), ci_d, cf_d, n, k.parts, std::min<uint32_t>(kcm, kCombineMaxKX),
+                       c->stream))
+        return fail(c, MQW_EDEVICE, "basis compression: %s", last_kernel_error());
+    c->stats.aux_launches++;
+    if (!jb2.empty()) {
+        HIPCHK(c, hipMemcpyAsync(cj2_d, jb2.data(), sizeof(CombineJob) * jb2.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(ci2_d, ci2.data(), sizeof(CombineItem) * ci2.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        if (launch_combine(cj2_d, (uint32_t)jb2.size(), ci2_d, cf_d, n, k.parts, kcb_max, c->stream))
+            return fail(c, MQW_EDEVICE, "basis compression: %s", last_kernel_error());
+        c->stats.aux_launches++;
+    }
+    return MQW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Compressed bases computed before the batches, in chunks of instances whose
+// bases fit in vcap bytes:
+// - the first-stage basis V, for the instances whose second stage comes from
+//   the shared basis W (kc2 > 0);
+// - the last-stage adjoint basis X (X_0 = e_eloc), for the instances whose
+//   last two stages are read out by the two-stage adjoint (kxa > 0): the
+//   samples' adjoint states phi_j = sum_k a_jk X_k span few directions too.
+// The recurrences of every instance of a chunk run in one plan, then the
+// compressions above.  kcq[p] = 2R and qbase[p] = Q_0 for V (every
+// trajectory gets its coordinates in c1q), kzq[p] = 2R and zbase[p] = Z_0 for
+// X (coordinates in cxq); 0 = not compressed.
+// ---------------------------------------------------------------------------
 mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Problem*>& ps,
                           std::vector<Traj>& trajs, const float* E, const std::vector<int>& kc1,
                           const std::vector<int>& kc2, const std::vector<int>& kxa, bool use_graph, size_t vcap,
@@ -1528,7 +1696,6 @@ mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vecto
     auto pbytes = [&](size_t p) {
         return sizeof(float) * N * (size_t)((fwd(p) ? kc1[p] - 1 : 0) + (adj(p) ? kxa[p] : 0));
     };
-    const uint32_t nchunk = (uint32_t)std::min<size_t>(64, N / 64);
     size_t i0 = 0;
     int chunk_id = 0;
     while (i0 < cand.size()) {
@@ -1549,198 +1716,79 @@ mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vecto
         vb.X.assign(ps.size(), nullptr);
         vb.kx.assign(ps.size(), 0);
         vb.kc2.assign(ps.size(), 0);
-        // the compressions of the chunk: (instance, 0 = V / 1 = X)
-        struct Cjob {
-            size_t p;
-            int which, kc;
-            const float* V; // basis vectors 1 .. kc - 1
-            size_t goff;    // its Gram pairs in gram
-        };
-        std::vector<Cjob> jobs;
-        size_t off = 0, pairs = 0;
-        uint32_t kcmax = 0;
+        std::vector<Compression> cj;
+        std::vector<std::pair<size_t, int>> who; // (instance, 0 = V / 1 = X)
+        size_t off = 0;
         for (size_t i = i0; i < i1; i++) {
             const size_t p = cand[i];
-            if (fwd(p)) {
-                vb.kc[p] = kc1[p];
-                vb.V[p] = (const float*)(vbuf + off);
-                off += sizeof(float) * N * (size_t)(kc1[p] - 1);
-                jobs.push_back(Cjob{p, 0, kc1[p], vb.V[p], pairs});
-                pairs += gram_pairs((uint32_t)kc1[p]);
-                kcmax = std::max<uint32_t>(kcmax, (uint32_t)kc1[p]);
-            }
-            if (adj(p)) {
-                vb.kx[p] = kxa[p];
-                vb.X[p] = (const float*)(vbuf + off);
-                off += sizeof(float) * N * (size_t)kxa[p];
-                jobs.push_back(Cjob{p, 1, kxa[p], vb.X[p] + N, pairs});
-                pairs += gram_pairs((uint32_t)kxa[p]);
-                kcmax = std::max<uint32_t>(kcmax, (uint32_t)kxa[p]);
+            for (int which = 0; which < 2; which++) {
+                if (which ? !adj(p) : !fwd(p)) continue;
+                Compression j;
+                j.kc = which ? kxa[p] : kc1[p];
+                if (which) {
+                    vb.kx[p] = kxa[p];
+                    vb.X[p] = (const float*)(vbuf + off);
+                    off += sizeof(float) * N * (size_t)kxa[p];
+                    j.V = vb.X[p] + N;
+                    j.unit1 = ps[p]->info.eloc + 1u;
+                } else {
+                    vb.kc[p] = kc1[p];
+                    vb.V[p] = (const float*)(vbuf + off);
+                    off += sizeof(float) * N * (size_t)(kc1[p] - 1);
+                    j.V = vb.V[p];
+                    j.psi0 = psi0;
+                }
+                // the samples' coefficients of the first stage (V) or of the last (X)
+                const std::vector<size_t>& tl = ptraj[p];
+                j.S = tl.size();
+                j.rows.assign(j.S * (size_t)j.kc, 0.0);
+                for (size_t s = 0; s < j.S; s++) {
+                    const std::vector<double>& cf = which ? trajs[tl[s]].segs.back().c : trajs[tl[s]].segs[0].c;
+                    for (int q = 0; q < j.kc && q < (int)cf.size(); q++) j.rows[s * (size_t)j.kc + q] = cf[q];
+                }
+                cj.push_back(std::move(j));
+                who.emplace_back(p, which);
             }
         }
         trace("compression chunk: %zu instances, %zu bytes", i1 - i0, vbytes);
         mqw_status st = run_bases(c, n, k, ps, trajs, E, vb, use_graph);
         if (st) return st;
         trace("compression chunk: recurrences");
-        // Gram pair sums
-        const size_t njobs = jobs.size();
-        const size_t pmax = gram_pmax(kcmax);
-        double* gram_d = (double*)dev_get(c, "cmp_gram", sizeof(double) * pairs);
-        double* part_d = (double*)dev_get(c, "cmp_gram_part", sizeof(double) * njobs * nchunk * pmax);
-        GramJob* gj_d = (GramJob*)dev_get(c, "cmp_gram_jobs", sizeof(GramJob) * njobs);
-        if (!gram_d || !part_d || !gj_d) return fail(c, MQW_ENOMEM, "device allocation failed for Gram matrices");
-        // one launch per size class of the kernel (32, 64, 128, 256 vectors): a wide
-        // basis would otherwise set the pair count and occupancy of every job
-        auto gclass = [](int kc) { return kc <= 32 ? 0 : kc <= 64 ? 1 : kc <= 128 ? 2 : 3; };
-        std::vector<GramJob> gj;
-        std::vector<size_t> cls_end(4, 0);
-        std::vector<uint32_t> cls_kc(4, 0);
-        for (int g = 0; g < 4; g++) {
-            for (const Cjob& J : jobs)
-                if (gclass(J.kc) == g) {
-                    gj.push_back(GramJob{J.V, gram_d + J.goff, (uint32_t)J.kc, J.which ? 0.0f : psi0,
-                                         J.which ? ps[J.p]->info.eloc + 1u : 0u, 0u});
-                    cls_kc[g] = std::max<uint32_t>(cls_kc[g], (uint32_t)J.kc);
-                }
-            cls_end[g] = gj.size();
-        }
-        EventPair ev;
-        HIPCHK(c, ev.create());
-        HIPCHK(c, hipEventRecord(ev.a, c->stream));
-        HIPCHK(c, hipMemcpyAsync(gj_d, gj.data(), sizeof(GramJob) * njobs, hipMemcpyHostToDevice, c->stream));
-        for (int g = 0, a = 0; g < 4; a = (int)cls_end[g], g++)
-            if (cls_end[g] > (size_t)a && launch_gram(gj_d + a, (uint32_t)(cls_end[g] - a), cls_kc[g], n, nchunk, part_d,
-                                                       c->stream))
-                return fail(c, MQW_EDEVICE, "gram: %s", last_kernel_error());
-        std::vector<double> gram(pairs);
-        HIPCHK(c, hipMemcpyAsync(gram.data(), gram_d, sizeof(double) * pairs, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        trace("compression chunk: Gram matrices");
-        // one SVD per instance, basis and class; the samples' coefficients of
-        // the first stage (V) or of the last (X)
-        std::vector<FirstStageCompression> fc(njobs);
-        std::vector<char> ok(njobs, 0);
-        std::vector<std::vector<double>> rows(njobs);
-        auto work = [&](size_t b, size_t e) {
-            for (size_t i = b; i < e; i++) {
-                const Cjob& J = jobs[i];
-                const int kc = J.kc;
-                const std::vector<size_t>& tl = ptraj[J.p];
-                rows[i].assign(tl.size() * (size_t)kc, 0.0);
-                std::vector<const double*> cp(tl.size());
-                for (size_t j = 0; j < tl.size(); j++) {
-                    const std::vector<double>& cf = J.which ? trajs[tl[j]].segs.back().c : trajs[tl[j]].segs[0].c;
-                    for (int q = 0; q < kc && q < (int)cf.size(); q++) rows[i][j * kc + q] = cf[q];
-                    cp[j] = &rows[i][j * kc];
-                }
-                ok[i] = compress_first_stage(kc, &gram[J.goff], cp, kCompressTol, fc[i]);
-            }
-        };
-        {
-            const unsigned hw = std::thread::hardware_concurrency();
-            const size_t nt = std::min<size_t>({(size_t)(hw ? hw : 1), 16, njobs});
-            std::vector<std::thread> th;
-            for (size_t t = 1; t < nt; t++) th.emplace_back(work, njobs * t / nt, njobs * (t + 1) / nt);
-            work(0, njobs / nt);
-            for (std::thread& x : th) x.join();
-        }
-        trace("compression chunk: %zu SVDs", njobs);
-        // the kept directions: one combine job per compression, R items of 2N floats
+        st = compress_svd(c, n, cj, "cmp");
+        if (st) return st;
+        // the kept directions, kept for the whole solve
+        std::vector<float*> out(cj.size(), nullptr);
         for (int which = 0; which < 2; which++) {
-            size_t bytes = 0, nitems = 0;
-            uint32_t kcm = 0;
-            for (size_t i = 0; i < njobs; i++)
-                if (ok[i] && jobs[i].which == which) {
-                    bytes += sizeof(float) * 2 * N * (size_t)fc[i].R;
-                    nitems += (size_t)fc[i].R;
-                    kcm = std::max<uint32_t>(kcm, (uint32_t)jobs[i].kc);
-                }
-            if (nitems == 0) continue;
-            const std::string tag = which ? "cmp_Z" : "cmp_Q";
-            char* qbuf = (char*)dev_get(c, tag + std::to_string(chunk_id), bytes);
-            double* dummy = (double*)dev_get(c, tag + "_norm", sizeof(double) * k.parts * nitems);
-            CombineJob* cj_d = (CombineJob*)dev_get(c, tag + "_cjobs", sizeof(CombineJob) * njobs);
-            CombineItem* ci_d = (CombineItem*)dev_get(c, tag + "_citems", sizeof(CombineItem) * nitems);
-            float* cf_d = (float*)dev_get(c, tag + "_coefs", sizeof(float) * nitems * (kcm + 2));
-            if (!qbuf || !dummy || !cj_d || !ci_d || !cf_d)
-                return fail(c, MQW_ENOMEM, "device allocation failed for compressed bases");
-            // a basis wider than kCombineMaxKX: a first job over its vectors 0..127 and a
-            // second that adds 128..kc-1 (as its vectors 2.. of V + 127 N, so the parity
-            // classes stay; its vectors 0 and 1 have no weight), launched after the first
-            std::vector<CombineJob> cj, cj2;
-            std::vector<CombineItem> ci, ci2;
-            std::vector<float> cf;
+            size_t bytes = 0;
+            for (size_t i = 0; i < cj.size(); i++)
+                if (cj[i].ok && who[i].second == which) bytes += sizeof(float) * 2 * N * (size_t)cj[i].fc.R;
+            if (bytes == 0) continue;
+            char* buf = (char*)dev_get(c, (which ? "cmp_Z" : "cmp_Q") + std::to_string(chunk_id), bytes);
+            if (!buf) return fail(c, MQW_ENOMEM, "device allocation failed for compressed bases");
             size_t qoff = 0;
-            for (size_t i = 0; i < njobs; i++) {
-                if (!ok[i] || jobs[i].which != which) continue;
-                const size_t p = jobs[i].p;
-                const int kc = jobs[i].kc, R = fc[i].R;
-                const int kca = std::min(kc, kCombineMaxKX), kcb = kc > kCombineMaxKX ? kc - (kCombineMaxKX - 2) : 0;
-                float* Q = (float*)(qbuf + qoff);
-                qoff += sizeof(float) * 2 * N * (size_t)R;
-                cj.push_back(CombineJob{jobs[i].V, (uint32_t)kca, (uint32_t)R, (uint32_t)ci.size(),
-                                        which ? 0.0f : psi0, which ? ps[p]->info.eloc + 1u : 0u, 0u});
-                if (kcb > 0)
-                    cj2.push_back(CombineJob{jobs[i].V + (size_t)(kCombineMaxKX - 2) * N, (uint32_t)kcb, (uint32_t)R,
-                                             (uint32_t)ci2.size(), 0.0f, 0u, 1u});
-                for (int r = 0; r < R; r++) {
-                    const double* Br = &fc[i].B[0];
-                    CombineItem it{};
-                    it.out = Q + (size_t)2 * r * N;
-                    it.norm_part = dummy + ci.size() * k.parts;
-                    it.coef_off = (uint32_t)cf.size();
-                    for (int q = 0; q < kca; q++) cf.push_back((float)Br[(size_t)(2 * r + (q & 1)) * kc + q]);
-                    ci.push_back(it);
-                    if (kcb > 0) {
-                        it.coef_off = (uint32_t)cf.size();
-                        for (int q = 0; q < kcb; q++) {
-                            const int kk = q + kCombineMaxKX - 2; // basis index of column q
-                            cf.push_back(q < 2 ? 0.0f : (float)Br[(size_t)(2 * r + (kk & 1)) * kc + kk]);
-                        }
-                        ci2.push_back(it);
-                    }
+            for (size_t i = 0; i < cj.size(); i++)
+                if (cj[i].ok && who[i].second == which) {
+                    out[i] = (float*)(buf + qoff);
+                    qoff += sizeof(float) * 2 * N * (size_t)cj[i].fc.R;
                 }
-                (which ? kzq : kcq)[p] = 2 * R;
-                (which ? zbase : qbase)[p] = Q;
-                const std::vector<size_t>& tl = ptraj[p];
-                for (size_t j = 0; j < tl.size(); j++) {
-                    std::vector<float>& u = which ? trajs[tl[j]].cxq : trajs[tl[j]].c1q;
-                    u.resize((size_t)2 * R);
-                    for (int q = 0; q < 2 * R; q++) u[q] = (float)fc[i].u[j * 2 * R + q];
-                }
-                c->stats.aux_bytes += (double)N * (4.0 * (kc - 1) + 8.0 * R);
-            }
-            HIPCHK(c, hipMemcpyAsync(cj_d, cj.data(), sizeof(CombineJob) * cj.size(), hipMemcpyHostToDevice, c->stream));
-            HIPCHK(c, hipMemcpyAsync(ci_d, ci.data(), sizeof(CombineItem) * ci.size(), hipMemcpyHostToDevice, c->stream));
-            HIPCHK(c, hipMemcpyAsync(cf_d, cf.data(), sizeof(float) * cf.size(), hipMemcpyHostToDevice, c->stream));
-            if (launch_combine(cj_d, (uint32_t)cj.size(), ci_d, cf_d, n, k.parts,
-                               std::min<uint32_t>(kcm, kCombineMaxKX), c->stream))
-                return fail(c, MQW_EDEVICE, "basis compression: %s", last_kernel_error());
-            c->stats.aux_launches++;
-            if (!cj2.empty()) {
-                CombineJob* cj2_d = (CombineJob*)dev_get(c, tag + "_cjobs2", sizeof(CombineJob) * cj2.size());
-                CombineItem* ci2_d = (CombineItem*)dev_get(c, tag + "_citems2", sizeof(CombineItem) * ci2.size());
-                if (!cj2_d || !ci2_d) return fail(c, MQW_ENOMEM, "device allocation failed for compressed bases");
-                uint32_t kcm2 = 0;
-                for (const CombineJob& j : cj2) kcm2 = std::max(kcm2, j.kc);
-                HIPCHK(c, hipMemcpyAsync(cj2_d, cj2.data(), sizeof(CombineJob) * cj2.size(), hipMemcpyHostToDevice,
-                                         c->stream));
-                HIPCHK(c, hipMemcpyAsync(ci2_d, ci2.data(), sizeof(CombineItem) * ci2.size(), hipMemcpyHostToDevice,
-                                         c->stream));
-                if (launch_combine(cj2_d, (uint32_t)cj2.size(), ci2_d, cf_d, n, k.parts, kcm2, c->stream))
-                    return fail(c, MQW_EDEVICE, "basis compression: %s", last_kernel_error());
-                c->stats.aux_launches++;
-            }
         }
-        HIPCHK(c, hipEventRecord(ev.b, c->stream));
+        st = compress_combine(c, n, k, cj, out, "cmp");
+        if (st) return st;
         HIPCHK(c, hipStreamSynchronize(c->stream)); // cmp_V is reused by the next chunk
         trace("compression chunk: combines");
-        float ms = 0.0f;
-        HIPCHK(c, hipEventElapsedTime(&ms, ev.a, ev.b));
-        c->stats.gpu_ms += ms;
-        c->stats.aux_ms += ms;
-        for (const Cjob& J : jobs) c->stats.aux_bytes += 4.0 * (double)N * (double)(J.kc - 1); // the Gram reads
-        c->stats.aux_launches += 2;
+        for (size_t i = 0; i < cj.size(); i++) {
+            if (!cj[i].ok) continue;
+            const size_t p = who[i].first;
+            const int which = who[i].second, R = cj[i].fc.R;
+            (which ? kzq : kcq)[p] = 2 * R;
+            (which ? zbase : qbase)[p] = out[i];
+            const std::vector<size_t>& tl = ptraj[p];
+            for (size_t s = 0; s < tl.size(); s++) {
+                std::vector<float>& u = which ? trajs[tl[s]].cxq : trajs[tl[s]].c1q;
+                u.resize((size_t)2 * R);
+                for (int q = 0; q < 2 * R; q++) u[q] = (float)cj[i].fc.u[s * 2 * R + q];
+            }
+        }
         i0 = i1;
         chunk_id++;
     }

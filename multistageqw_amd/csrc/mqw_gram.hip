@@ -1,7 +1,7 @@
 // Gram matrices of the shared first-stage bases (the compressed second stage,
 // see GramJob) and of the last-stage adjoint bases X: per instance, the sums
 // over all amplitudes of V_k1 V_k2 for k1 <= k2 of equal parity, with
-// V_0 = psi0 * ones or (X_0) a unit vector.  Products of FP32 values
+// V_0 = psi0 * ones, (X_0) a unit vector or a stored vector.  Products of FP32 values
 // are exact in FP64 and the sums are FP64: per workgroup over a chunk of
 // amplitudes, then over the chunks in a fixed order.
 #include <hip/hip_runtime.h>
@@ -48,7 +48,10 @@ gram_kernel(const GramJob* __restrict__ jobs, uint32_t njobs, int n, uint32_t nc
         for (int i = t; i < kc * kSlab; i += 256) {
             const int k = i / kSlab, x = i % kSlab;
             const uint64_t a = base + a0 + (uint64_t)x;
-            s[k][x] = k > 0 ? J.V[(uint64_t)(k - 1) * N + a] : J.unit1 ? (a + 1 == J.unit1 ? 1.0f : 0.0f) : J.psi0;
+            s[k][x] = k > 0     ? J.V[(uint64_t)(k - 1) * N + a]
+                      : J.V0    ? J.V0[a]
+                      : J.unit1 ? (a + 1 == J.unit1 ? 1.0f : 0.0f)
+                                : J.psi0;
         }
         __syncthreads();
 #pragma unroll

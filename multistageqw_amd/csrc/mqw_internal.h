@@ -114,6 +114,7 @@ struct CombineItem {
 };
 struct CombineJob {
     const float* V;      // V_1 .. V_{kc-1}, N floats each, contiguous
+    const float* V0;     // V_0 when stored (nullptr: psi0 or unit1 below)
     uint32_t kc;         // basis vectors including V_0 (<= kCombineMaxKX)
     uint32_t ns;         // samples (items) of this job (<= kCombineBatch)
     uint32_t item_off;   // first item
@@ -133,6 +134,7 @@ struct CombineJob {
 // error per sample; the second stage's basis is then T_l(Ht_2) Q_r.
 struct GramJob {
     const float* V;     // V_1 .. V_{kc-1}, N floats each
+    const float* V0;    // V_0 when stored (nullptr: psi0 or unit1 below)
     double* out;        // gram_pairs(kc) pair sums, written by the reduce launch
     uint32_t kc;
     float psi0;         // V_0 = psi0 * ones, unless unit1 > 0: the unit vector at index unit1 - 1
@@ -175,8 +177,8 @@ int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem
 // The rows (l, k) split into a real class ((l&1) == (k&1); sign -1 when both
 // are odd) and an imaginary class.  The scale of psi2 is irrelevant: every
 // later stage and the readout normalise their input.
-constexpr int kC2MaxK = 64;   // kc1 limit; kc2 limit of the FP32 kernel
-constexpr int kC2MaxK2 = 128; // kc2 limit of the split-FP16 kernel
+constexpr int kC2MaxK = 64;   // kc1 and kc2 limit of the FP32 kernel
+constexpr int kC2MaxK2 = 128; // kc1 and kc2 limit of the split-FP16 kernel (rows at most kC2MaxRows)
 constexpr int kC2Tile = 64;   // samples per workgroup of the FP32 kernel (2 MFMA row blocks of 32)
 constexpr int kC2TileH = 128; // samples per workgroup of the split-FP16 kernel (4 row blocks of 32)
 constexpr int kC2RowPad = 16; // each class's rows padded to a multiple of this (one FP16 MFMA k block)

@@ -1443,7 +1443,7 @@ combine_kernel(const CombineJob* __restrict__ jobs, uint32_t njobs, const Combin
         const uint64_t a = base + a0 + (uint64_t)t;
         const MQW_GLOBAL float* V = gview(J.V);
         float v[KR];
-        v[0] = J.unit1 ? (a + 1 == J.unit1 ? 1.0f : 0.0f) : J.psi0;
+        v[0] = J.V0 ? gview(J.V0)[a] : J.unit1 ? (a + 1 == J.unit1 ? 1.0f : 0.0f) : J.psi0;
 #pragma unroll
         for (int k = 1; k < KR; k++) v[k] = k < kc ? V[(uint64_t)(k - 1) * N + a] : 0.0f;
         // keep the column in registers for the whole sample loop (the loads
@@ -1802,12 +1802,12 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int col = lane & 31, half = lane >> 5;
     const uint64_t N = 1ull << n;
-    // row codes: 0 for |+>, k for V_k (k < 64), 64 + k (kc2 - 1) + l - 1 for
-    // W_lk (below 8192 with kc1 <= 64, kc2 <= 128)
+    // row codes: 0 for |+>, k for V_k (k < 128), 128 + k (kc2 - 1) + l - 1
+    // for W_lk (below 16384 with kc1, kc2 <= 128)
     for (int i = t; i < nq; i += NT) {
         const uint32_t e = T.qtab[i];
         const int l = (int)(e & 255u), k = (int)((e >> 8) & 255u);
-        s_rc[i] = (uint16_t)(l == 0 ? k : 64 + k * (kc2 - 1) + (l - 1));
+        s_rc[i] = (uint16_t)(l == 0 ? k : 128 + k * (kc2 - 1) + (l - 1));
     }
     for (int i = t; i < NSP; i += NT) s_out[i] = i < ns ? items[T.item_off + i].out : nullptr;
     for (int i = t; i < NW * NSP; i += NT) (&s_n[0][0])[i] = 0.0;
@@ -1824,9 +1824,9 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
     const float* const rV = T.V;
     const float* const rP = T.plus;
     auto row_ptr = [&](uint32_t code) -> const float* {
-        const bool isw = code >= 64u;
+        const bool isw = code >= 128u;
         const float* const rb = isw ? rW : (code ? rV : rP);
-        const uint32_t idx = isw ? code - 64u : (code ? code - 1u : 0u);
+        const uint32_t idx = isw ? code - 128u : (code ? code - 1u : 0u);
         return rb + (uint64_t)idx * N;
     };
     // block b of A into buffer bf: slot s (1 KiB) by wave s, so this wave
