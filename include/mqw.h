@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 15
+#define MQW_ABI_VERSION 16
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -155,6 +155,11 @@ typedef struct {
                                 psi_m[E_loc] = (P_{m-1}^T phi_j)^T psi_{m-2} a dot with the input of stage
                                 m - 1, whose norm stands for the output's (two truncated series: at most
                                 4 tol relative in probability); 0 = stage m - 1 as per-sample terms */
+    int shared_third_stage;  /* 1 (default) = for m >= 4, where the second stage is shared, the third stage is
+                                neither read out (last, or one of the two above) and it pays: the samples'
+                                second-stage states are compressed (their real parts and imaginary parts,
+                                2e-7 per sample) to directions U_r, and the third stage is a GEMM over
+                                T_l(H3/onenorm3) U_r like the second; 0 = per-sample terms */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are
@@ -331,6 +336,9 @@ typedef struct {
     uint64_t adjoint2_instances;   /* instances whose last two stages were read out together (adjoint_two_stages) */
     uint64_t adjoint2_vectors;     /* their kept adjoint directions (2R each), summed */
     uint64_t adjoint2_uncompressed; /* their last-stage basis vectors before compression (kx each), summed */
+    uint64_t third_stage_instances; /* instances whose third stage came from compressed second-stage states */
+    uint64_t third_stage_vectors;   /* their kept directions (2R each), summed */
+    uint64_t third_stage_states;    /* their second-stage state halves before compression (2 per sample), summed */
 } mqw_stats;
 mqw_status mqw_last_stats(const mqw_ctx* ctx, mqw_stats* out);
 

@@ -26,53 +26,126 @@ namespace mqw {
 namespace {
 
 // Eigen-decomposition of the symmetric n x n matrix a (row-major, destroyed):
-// eigenvalues w, eigenvectors as the columns of z (row-major).
-void sym_eig_jacobi(int n, std::vector<double>& a, std::vector<double>& w, std::vector<double>& z)
+// eigenvalues w, eigenvectors as the columns of z (row-major).  Householder
+// reduction to tridiagonal form T = Q^T A Q, then implicit symmetric QR steps
+// with Wilkinson shifts on T, every Givens rotation also applied to Q (Golub
+// and Van Loan, Matrix Computations, algorithms 8.3.1 and 8.3.3): O(n^3) in
+// all, where cyclic Jacobi takes O(n^3) per sweep.
+void sym_eig(int n, std::vector<double>& a, std::vector<double>& w, std::vector<double>& z)
 {
+    auto A = [&](int i, int j) -> double& { return a[(size_t)i * n + j]; };
+    auto Z = [&](int i, int j) -> double& { return z[(size_t)i * n + j]; };
     z.assign((size_t)n * n, 0.0);
-    for (int i = 0; i < n; i++) z[(size_t)i * n + i] = 1.0;
-    for (int sweep = 0; sweep < 64; sweep++) {
-        double off = 0.0, all = 0.0;
-        for (int i = 0; i < n; i++)
-            for (int j = 0; j < n; j++) {
-                const double v = a[(size_t)i * n + j] * a[(size_t)i * n + j];
-                all += v;
-                if (i != j) off += v;
+    for (int i = 0; i < n; i++) Z(i, i) = 1.0;
+    std::vector<double> v(n), p(n);
+    // 1. Householder: column k below the subdiagonal to zero, k = 0 .. n - 3
+    for (int k = 0; k + 2 < n; k++) {
+        double sig = 0.0;
+        for (int i = k + 1; i < n; i++) sig += A(i, k) * A(i, k);
+        const double xn = sqrt(sig);
+        if (xn == 0.0) continue;
+        const double alpha = A(k + 1, k) >= 0.0 ? -xn : xn;
+        double vn = 0.0;
+        for (int i = k + 1; i < n; i++) {
+            v[i] = A(i, k) - (i == k + 1 ? alpha : 0.0);
+            vn += v[i] * v[i];
+        }
+        vn = sqrt(vn);
+        if (vn == 0.0) continue;
+        for (int i = k + 1; i < n; i++) v[i] /= vn;
+        // trailing block B <- H B H, H = I - 2 v v^T: with p = B v and
+        // q = p - (v^T p) v, B <- B - 2 v q^T - 2 q v^T
+        double kk = 0.0;
+        for (int i = k + 1; i < n; i++) {
+            double acc = 0.0;
+            for (int j = k + 1; j < n; j++) acc += A(i, j) * v[j];
+            p[i] = acc;
+            kk += v[i] * acc;
+        }
+        for (int i = k + 1; i < n; i++) p[i] -= kk * v[i];
+        for (int i = k + 1; i < n; i++)
+            for (int j = k + 1; j < n; j++) A(i, j) -= 2.0 * (v[i] * p[j] + p[i] * v[j]);
+        A(k + 1, k) = A(k, k + 1) = alpha;
+        for (int i = k + 2; i < n; i++) A(i, k) = A(k, i) = 0.0;
+        // Q <- Q H
+        for (int r = 0; r < n; r++) {
+            double acc = 0.0;
+            for (int j = k + 1; j < n; j++) acc += Z(r, j) * v[j];
+            for (int j = k + 1; j < n; j++) Z(r, j) -= 2.0 * acc * v[j];
+        }
+    }
+    // 2. implicit QR steps on the unreduced trailing block [lo, hi]; a
+    //    rotation in the plane (k, k + 1) touches rows and columns k - 1 .. k + 2
+    const double eps = 2.220446049250313e-16;
+    auto negligible = [&](int i) { return fabs(A(i + 1, i)) <= eps * (fabs(A(i, i)) + fabs(A(i + 1, i + 1))); };
+    int hi = n - 1, steps = 0;
+    while (hi > 0 && steps < 64 * n) {
+        if (negligible(hi - 1)) {
+            A(hi, hi - 1) = A(hi - 1, hi) = 0.0;
+            hi--;
+            continue;
+        }
+        int lo = hi - 1;
+        while (lo > 0 && !negligible(lo - 1)) lo--;
+        // Wilkinson shift: the eigenvalue of the trailing 2 x 2 closer to T(hi, hi)
+        const double dd = 0.5 * (A(hi - 1, hi - 1) - A(hi, hi)), ee = A(hi, hi - 1);
+        const double den = dd + (dd >= 0.0 ? 1.0 : -1.0) * sqrt(dd * dd + ee * ee);
+        const double mu = A(hi, hi) - (den != 0.0 ? ee * ee / den : 0.0);
+        double x = A(lo, lo) - mu, y = A(lo + 1, lo);
+        for (int k = lo; k < hi; k++) {
+            // [c s; -s c]^T [x; y] = [r; 0]
+            double c = 1.0, s = 0.0;
+            if (y != 0.0) {
+                if (fabs(y) > fabs(x)) {
+                    const double tau = -x / y;
+                    s = 1.0 / sqrt(1.0 + tau * tau);
+                    c = s * tau;
+                } else {
+                    const double tau = -y / x;
+                    c = 1.0 / sqrt(1.0 + tau * tau);
+                    s = c * tau;
+                }
             }
-        if (off <= 1e-32 * all || off == 0.0) break;
-        for (int p = 0; p < n - 1; p++)
-            for (int q = p + 1; q < n; q++) {
-                const double apq = a[(size_t)p * n + q];
-                if (apq == 0.0) continue;
-                const double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
-                const double theta = (aqq - app) / (2.0 * apq);
-                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-                for (int k = 0; k < n; k++) { // columns p, q
-                    const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
-                    a[(size_t)k * n + p] = c * akp - s * akq;
-                    a[(size_t)k * n + q] = s * akp + c * akq;
-                }
-                for (int k = 0; k < n; k++) { // rows p, q
-                    const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
-                    a[(size_t)p * n + k] = c * apk - s * aqk;
-                    a[(size_t)q * n + k] = s * apk + c * aqk;
-                }
-                for (int k = 0; k < n; k++) {
-                    const double zkp = z[(size_t)k * n + p], zkq = z[(size_t)k * n + q];
-                    z[(size_t)k * n + p] = c * zkp - s * zkq;
-                    z[(size_t)k * n + q] = s * zkp + c * zkq;
-                }
+            const int j0 = std::max(lo, k - 1), j1 = std::min(hi, k + 2);
+            for (int j = j0; j <= j1; j++) { // rows k, k + 1: G^T T
+                const double t1 = A(k, j), t2 = A(k + 1, j);
+                A(k, j) = c * t1 - s * t2;
+                A(k + 1, j) = s * t1 + c * t2;
             }
+            for (int i = j0; i <= j1; i++) { // columns k, k + 1: T G
+                const double t1 = A(i, k), t2 = A(i, k + 1);
+                A(i, k) = c * t1 - s * t2;
+                A(i, k + 1) = s * t1 + c * t2;
+            }
+            for (int r = 0; r < n; r++) { // Q G
+                const double t1 = Z(r, k), t2 = Z(r, k + 1);
+                Z(r, k) = c * t1 - s * t2;
+                Z(r, k + 1) = s * t1 + c * t2;
+            }
+            if (k + 1 < hi) {
+                x = A(k + 1, k);
+                y = A(k + 2, k);
+            }
+        }
+        steps++;
     }
     w.resize(n);
-    for (int i = 0; i < n; i++) w[i] = a[(size_t)i * n + i];
+    for (int i = 0; i < n; i++) w[i] = A(i, i);
 }
 
 // One-sided Jacobi: rotates the columns of m (rows x cols, column-major)
-// until they are mutually orthogonal.
+// until they are mutually orthogonal.  Columns below 1e-12 of the largest
+// column's norm (far below any kept direction) are left alone: the Gram sums
+// give them no accurate direction, and rotating them only costs sweeps.
 void orthogonalise_columns(int rows, int cols, std::vector<double>& m)
 {
+    double big = 0.0;
+    for (int p = 0; p < cols; p++) {
+        double s = 0.0;
+        for (int r = 0; r < rows; r++) s += m[(size_t)p * rows + r] * m[(size_t)p * rows + r];
+        big = std::max(big, s);
+    }
+    const double floor2 = 1e-24 * big;
     for (int sweep = 0; sweep < 64; sweep++) {
         bool rotated = false;
         for (int p = 0; p < cols - 1; p++)
@@ -85,6 +158,7 @@ void orthogonalise_columns(int rows, int cols, std::vector<double>& m)
                     be += mq[r] * mq[r];
                     ga += mp[r] * mq[r];
                 }
+                if (al < floor2 || be < floor2) continue;
                 if (ga == 0.0 || fabs(ga) <= 1e-15 * sqrt(al * be)) continue;
                 rotated = true;
                 const double zeta = (be - al) / (2.0 * ga);
@@ -119,7 +193,7 @@ void class_svd(int kc, int cls, const double* gram, const std::vector<const doub
             g[(size_t)a * nc + b] = gram[gram_pair_start(k2) + k1 / 2];
         }
     std::vector<double> w, z;
-    sym_eig_jacobi(nc, g, w, z);
+    sym_eig(nc, g, w, z);
     // M = A Z diag(sqrt(w)), column-major
     std::vector<double> m((size_t)S * nc, 0.0);
     for (int i = 0; i < nc; i++) {

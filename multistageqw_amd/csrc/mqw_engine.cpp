@@ -70,6 +70,9 @@ void trace(const char* fmt, ...)
 } // namespace
 
 // Grow-only named device buffer.  Growing synchronises both streams first.
+// Beyond 8 GiB a buffer grows in steps of 8 GiB: allocating tens of GiB takes
+// about a second, and nearby sizes (the bases of different instances) then
+// share one allocation.
 void* mqw::dev_get(mqw_ctx* c, const std::string& name, size_t bytes)
 {
     if (bytes == 0) bytes = 256;
@@ -80,7 +83,8 @@ void* mqw::dev_get(mqw_ctx* c, const std::string& name, size_t bytes)
     if (b.ptr) hipFree(b.ptr);
     b.ptr = nullptr;
     b.cap = 0;
-    const size_t want = (bytes + 4095) & ~(size_t)4095;
+    const size_t step = bytes > ((size_t)8 << 30) ? ((size_t)8 << 30) : 4096;
+    const size_t want = (bytes + step - 1) / step * step;
     if (hipMalloc(&b.ptr, want) != hipSuccess) {
         hipGetLastError();
         b.ptr = nullptr;
@@ -236,6 +240,10 @@ struct Traj {
     std::vector<float> c1q;       // with a compressed first-stage basis: the sample's coordinates in it
     std::vector<float> cxq;       // with a compressed last-stage adjoint basis: the sample's coordinates in it
     bool direct_done = false;     // an m == 1 chain evaluated directly (direct_m1): no terms run
+    bool mid_done = false;        // the first three stages from shared bases (the third from the
+                                  // compressed basis of the second-stage states, a combine2 op)
+    std::vector<float> cmq;       // the sample's coordinates in that basis
+    int seg0 = 0;                 // with no combine op: the first segment to run, psi_in its input
 };
 
 // Chebyshev coefficients of every segment.  Segments are independent (the
@@ -414,15 +422,17 @@ void traj_jobs(int n, uint32_t parts, const Traj& tr, const Slot<real>& sl, cons
     int p = -1, a = 0, b = 1, par = 0;
     int64_t prev_slot = -1;
     size_t s0 = 0;
-    if (tr.first_done || tr.second_done) {
-        // a combine op left the first (combine2: second) stage's output in st[0]
-        // and its partial norms in part[0]
+    if (tr.first_done || tr.second_done || tr.mid_done) {
+        // a combine op left the first (combine2: second, the middle combine2:
+        // third) stage's output in st[0] and its partial norms in part[0]
         p = 0;
         a = 1;
         b = 2;
         par = 1;
-        s0 = tr.second_done ? 2 : 1;
+        s0 = tr.mid_done ? 3 : tr.second_done ? 2 : 1;
         prev_slot = tr.segs[s0 - 1].prob_slot;
+    } else {
+        s0 = (size_t)tr.seg0;
     }
     auto rc = [](double v) { return (real)v; };
     // an adjoint trajectory stops before its last stage, a two-stage adjoint
@@ -557,6 +567,15 @@ struct Bases {
     std::vector<int> ky;
     std::vector<const uint32_t*> ytab;
     std::vector<uint32_t> y0, y1;
+    // the compressed middle stage (stage 3): U (ku = 2R directions of the
+    // samples' second-stage states, N floats apart, made in the batch), W3_lr =
+    // T_l(Ht_3) U_r (l >= 1 stored like W) and its GEMM row table; ku = 0 = none
+    std::vector<const float*> U;
+    std::vector<int> ku;
+    std::vector<const float*> W3;
+    std::vector<int> k3;
+    std::vector<const uint32_t*> mtab;
+    std::vector<uint32_t> m0, m1;
 };
 
 // Rows (l, k) of the second-stage GEMM in two classes, each padded to a
@@ -742,24 +761,27 @@ void combine_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_t
 // state of their slot.
 template <typename real>
 void combine2_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_t>& grp, const Bases& B,
-                  const std::vector<Slot<real>>& slots, Plan<real>& P, const std::vector<AdjointIn>* adj = nullptr)
+                  const std::vector<Slot<real>>& slots, Plan<real>& P, const std::vector<AdjointIn>* adj = nullptr,
+                  bool mid = false)
 {
     if constexpr (std::is_same<real, float>::value) {
         const size_t N = (size_t)1 << n;
         std::map<int, std::vector<size_t>> byprob;
         for (size_t i = 0; i < grp.size(); i++)
-            if (adj ? trajs[grp[i]].adjoint2 : trajs[grp[i]].second_done) byprob[trajs[grp[i]].prob].push_back(i);
+            if (adj ? trajs[grp[i]].adjoint2 : mid ? trajs[grp[i]].mid_done : trajs[grp[i]].second_done)
+                byprob[trajs[grp[i]].prob].push_back(i);
         if (byprob.empty()) return;
         Op op{7, P.c2tiles.size(), 0, 0, B.split ? 1u : 0u};
         const size_t tsz = B.split ? kC2TileH : kC2Tile;
         for (const auto& kv : byprob) {
             const int pr = kv.first;
-            const int kc1 = adj ? B.kz[pr] : B.kc[pr], kc2 = adj ? B.ky[pr] : B.kc2[pr];
+            const int kc1 = adj ? B.kz[pr] : mid ? B.ku[pr] : B.kc[pr];
+            const int kc2 = adj ? B.ky[pr] : mid ? B.k3[pr] : B.kc2[pr];
             const std::vector<size_t>& mem = kv.second;
             for (size_t i0 = 0; i0 < mem.size(); i0 += tsz) {
                 const size_t ns = std::min<size_t>(tsz, mem.size() - i0);
                 Combine2Tile ct{};
-                const bool cq = adj || (!B.cmp.empty() && B.cmp[pr]);
+                const bool cq = adj || mid || (!B.cmp.empty() && B.cmp[pr]);
                 if (adj) {
                     ct.plus = B.Z[pr];
                     ct.V = B.Z[pr] + N;
@@ -767,6 +789,13 @@ void combine2_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_
                     ct.qtab = B.ytab[pr];
                     ct.q0 = B.y0[pr];
                     ct.q1 = B.y1[pr];
+                } else if (mid) {
+                    ct.plus = B.U[pr];
+                    ct.V = B.U[pr] + N;
+                    ct.W = B.W3[pr];
+                    ct.qtab = B.mtab[pr];
+                    ct.q0 = B.m0[pr];
+                    ct.q1 = B.m1[pr];
                 } else {
                     ct.plus = cq ? B.plusv[pr] : B.plus;
                     ct.V = B.V[pr];
@@ -786,8 +815,9 @@ void combine2_ops(int n, const std::vector<Traj>& trajs, const std::vector<size_
                     Combine2Item it{};
                     it.out = adj ? (float*)(*adj)[i].free_st : slots[i].st[0];
                     it.norm_part = adj ? (*adj)[i].free_part : slots[i].part[0];
-                    const std::vector<float>& u = adj ? tr.cxq : tr.c1q;
-                    const std::vector<double>& c2 = adj ? tr.segs[tr.segs.size() - 2].c : tr.segs[1].c;
+                    const std::vector<float>& u = adj ? tr.cxq : mid ? tr.cmq : tr.c1q;
+                    const std::vector<double>& c2 =
+                        adj ? tr.segs[tr.segs.size() - 2].c : mid ? tr.segs[2].c : tr.segs[1].c;
                     it.c1_off = (uint32_t)P.ccoef.size();
                     for (int k = 0; k < kc1; k++) {
                         if (cq)
@@ -849,6 +879,7 @@ void build_plan(int n, uint32_t parts, const std::vector<Traj>& trajs, const std
     for (const auto& grp : groups) {
         if (B) combine_ops<real>(n, trajs, grp, *B, slots, P);
         if (B && !B->W.empty()) combine2_ops<real>(n, trajs, grp, *B, slots, P);
+        if (B && !B->U.empty()) combine2_ops<real>(n, trajs, grp, *B, slots, P, nullptr, true);
         std::vector<std::vector<TermJob<real>>> seq(grp.size());
         std::vector<ReduceJob> fins;
         std::vector<AdjointIn> adj(grp.size());
@@ -1183,8 +1214,19 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
     if (has_y)
         for (const Traj& tr : trajs)
             if (!segy[tr.prob] && tr.segs.size() > 2) segy[tr.prob] = &tr.segs[tr.segs.size() - 2];
+    // middle-stage chains: the operator of stage 3
+    bool has_m = false;
+    std::vector<const Segment*> segm(ps.size(), nullptr);
+    for (size_t p = 0; p < B.ku.size(); p++) has_m |= B.ku[p] > 0 && B.k3[p] > 0;
+    if (has_m)
+        for (const Traj& tr : trajs)
+            if (!segm[tr.prob] && tr.segs.size() > 3) segm[tr.prob] = &tr.segs[2];
     int steps = 0;
     for (size_t p = 0; p < ps.size(); p++) {
+        if (has_m && B.ku[p] > 0 && B.k3[p] > 0) {
+            if (!segm[p]) return fail(c, MQW_EINVAL, "middle-stage basis of an instance with m < 4");
+            steps = std::max(steps, B.k3[p] - 1);
+        }
         steps = std::max({steps, cmp(p) ? 0 : B.kc[p] - 1, B.kx[p] - 1});
         if (has_w && B.kc2[p] > 0) steps = std::max(steps, (cmp(p) ? 0 : B.kc[p] - 1) + B.kc2[p] - 1);
         if (has_y && B.kz[p] > 0 && B.ky[p] > 0) {
@@ -1317,6 +1359,12 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
                 const float* Z = B.Z[p];
                 chains(p, const_cast<float*>(B.Y[p]), B.kz[p], B.ky[p], *segy[p], true, q,
                        [&](int kk) -> const float* { return Z + (size_t)kk * N; });
+            }
+            // W3_{l+1,r} = 2 Ht_3 W3_lr - W3_{l-1,r} from W3_0r = U_r
+            if (has_m && B.ku[p] > 0 && B.k3[p] > 0) {
+                const float* U = B.U[p];
+                chains(p, const_cast<float*>(B.W3[p]), B.ku[p], B.k3[p], *segm[p], true, q,
+                       [&](int kk) -> const float* { return U + (size_t)kk * N; });
             }
         }
         P.ops.push_back(op);
@@ -1796,6 +1844,191 @@ mqw_status compress_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vecto
 }
 
 // ---------------------------------------------------------------------------
+// The compressed third stage, per batch of bases (m >= 4, FP32, split-FP16
+// GEMM), for an instance whose third stage is neither read out (the last
+// stage, or one of the two-stage adjoint's) nor preceded by an unshared
+// second stage.  The samples' second-stage states psi2_j span few
+// directions: their real parts and their imaginary parts, each a parity class
+// as in the first-stage compression (vector 2j = Re psi2_j, 2j + 1 =
+// Im psi2_j; sample j's coefficients are ones at 2j and 2j + 1, so its state
+// is sum_k c_jk i^(k&1) V_k).  So:
+// 1. the second-stage GEMM writes psi2_j into one contiguous buffer per
+//    instance instead of the trajectory slots;
+// 2. their Gram pair sums, the SVDs and the combines give U (2R directions)
+//    and the samples' coordinates (compress_svd, compress_combine);
+// 3. the recurrences W3_lr = T_l(Ht_3) U_r run (run_bases);
+// 4. the groups' plans take the third stage as a GEMM over W3 (mid_done),
+//    exactly as the second stage over W.
+// An instance whose compression fails or does not pay continues from its
+// psi2 in the buffer with the third stage as terms (seg0 = 2).  cap bounds
+// the W3 bases of the batch.
+// ---------------------------------------------------------------------------
+mqw_status middle_stage(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Problem*>& ps,
+                        std::vector<Traj>& trajs, const std::vector<std::vector<size_t>>& ptraj,
+                        const std::vector<size_t>& cand, const float* E, Bases& B, const std::vector<int>& k3,
+                        const std::vector<double>& own3, bool use_graph, size_t cap)
+{
+    const size_t N = (size_t)1 << n;
+    const uint32_t nparts = k.parts;
+    size_t ntr = 0;
+    for (size_t p : cand) ntr += ptraj[p].size();
+    float* buf = (float*)dev_get(c, "mid_psi2", sizeof(float) * 2 * N * ntr);
+    double* part = (double*)dev_get(c, "mid_psi2_norm", sizeof(double) * nparts * ntr);
+    if (!buf || !part) return MQW_OK; // no room: the third stage stays as terms
+    trace("third stage: %zu instances, %zu states", cand.size(), ntr);
+    // 1. the second stage's outputs into the buffer, sample j of an instance at
+    //    its base + j (every sample of an instance is in this batch)
+    std::vector<size_t> grp, base(ps.size(), 0);
+    std::vector<Slot<float>> sl;
+    for (size_t p : cand) {
+        base[p] = grp.size();
+        for (size_t t : ptraj[p]) {
+            Slot<float> s{};
+            s.st[0] = buf + grp.size() * 2 * N;
+            s.part[0] = part + grp.size() * nparts;
+            sl.push_back(s);
+            grp.push_back(t);
+        }
+    }
+    {
+        Plan<float> P;
+        combine2_ops<float>(n, trajs, grp, B, sl, P);
+        double ms_sweep = 0.0, ms_aux = 0.0, ms_basis = 0.0, ms_gemm = 0.0;
+        mqw_status st = run_plan<float>(c, MQW_FP32, n, k, P, use_graph, &ms_sweep, &ms_aux, &ms_basis, &ms_gemm,
+                                        nullptr, c->stream, "mid");
+        if (st) return st;
+        c->stats.gpu_ms += ms_sweep + ms_aux;
+        c->stats.aux_ms += ms_aux;
+        c->stats.gemm_ms += ms_gemm;
+        c->stats.gemm_flops += P.gemm_flops;
+        c->stats.gemm_useful_flops += P.gemm_useful_flops;
+        c->stats.gemm_bytes += P.gemm_bytes;
+        c->stats.gemm_launches += 1;
+        c->stats.aux_bytes += P.aux_bytes;
+        c->stats.aux_launches += P.aux_launches;
+    }
+    trace("third stage: second-stage states");
+    // 2. compression of each instance's states
+    std::vector<Compression> cj(cand.size());
+    for (size_t i = 0; i < cand.size(); i++) {
+        const size_t p = cand[i];
+        Compression& j = cj[i];
+        j.S = ptraj[p].size();
+        j.kc = (int)(2 * j.S);
+        j.V0 = buf + base[p] * 2 * N;
+        j.V = j.V0 + N;
+        j.rows.assign(j.S * (size_t)j.kc, 0.0);
+        for (size_t s = 0; s < j.S; s++) j.rows[s * (size_t)j.kc + 2 * s] = j.rows[s * (size_t)j.kc + 2 * s + 1] = 1.0;
+    }
+    mqw_status st = compress_svd(c, n, cj, "mid");
+    if (st) return st;
+    // where it pays and fits: the W3 recurrences and GEMM read (20 B per vector
+    // and amplitude) against 0.8 x the samples' own third-stage terms
+    std::vector<float*> out(cj.size(), nullptr);
+    std::vector<size_t> w3off(cj.size(), 0);
+    size_t ubytes = 0, wbytes = 0;
+    for (size_t i = 0; i < cj.size(); i++) {
+        const size_t p = cand[i];
+        if (!cj[i].ok) continue;
+        const int ku = 2 * cj[i].fc.R;
+        const size_t w = sizeof(float) * N * (size_t)ku * (size_t)(k3[p] - 1);
+        const bool pays = ku <= kC2MaxK2 && (size_t)ku * (size_t)k3[p] + 2 * kC2RowPad <= (size_t)kC2MaxRows &&
+                          20.0 * ku * k3[p] < 0.8 * own3[p] && wbytes + w <= cap;
+        if (!pays) {
+            cj[i].ok = false;
+            continue;
+        }
+        w3off[i] = wbytes;
+        wbytes += w;
+        ubytes += sizeof(float) * N * (size_t)ku;
+    }
+    char* ub = ubytes ? (char*)dev_get(c, "mid_U", ubytes) : nullptr;
+    char* wb = wbytes ? (char*)dev_get(c, "mid_W3", wbytes) : nullptr;
+    if (ubytes && (!ub || !wb))
+        for (Compression& j : cj) j.ok = false; // no room: the third stage stays as terms
+    size_t uo = 0;
+    for (size_t i = 0; i < cj.size(); i++)
+        if (cj[i].ok) {
+            out[i] = (float*)(ub + uo);
+            uo += sizeof(float) * N * (size_t)(2 * cj[i].fc.R);
+        }
+    st = compress_combine(c, n, k, cj, out, "mid");
+    if (st) return st;
+    // 3. the W3 recurrences, and the row tables of the third-stage GEMM
+    Bases mb;
+    mb.V.assign(ps.size(), nullptr);
+    mb.kc.assign(ps.size(), 0);
+    mb.X.assign(ps.size(), nullptr);
+    mb.kx.assign(ps.size(), 0);
+    mb.kc2.assign(ps.size(), 0);
+    mb.U.assign(ps.size(), nullptr);
+    mb.ku.assign(ps.size(), 0);
+    mb.W3.assign(ps.size(), nullptr);
+    mb.k3.assign(ps.size(), 0);
+    std::vector<uint32_t> rows, all;
+    std::vector<size_t> roff(cj.size(), 0);
+    B.U.assign(ps.size(), nullptr);
+    B.ku.assign(ps.size(), 0);
+    B.W3.assign(ps.size(), nullptr);
+    B.k3.assign(ps.size(), 0);
+    B.mtab.assign(ps.size(), nullptr);
+    B.m0.assign(ps.size(), 0);
+    B.m1.assign(ps.size(), 0);
+    bool any = false;
+    for (size_t i = 0; i < cj.size(); i++) {
+        if (!cj[i].ok) continue;
+        const size_t p = cand[i];
+        const int ku = 2 * cj[i].fc.R;
+        mb.U[p] = B.U[p] = out[i];
+        mb.ku[p] = B.ku[p] = ku;
+        mb.W3[p] = B.W3[p] = (const float*)(wb + w3off[i]);
+        mb.k3[p] = B.k3[p] = k3[p];
+        second_stage_rows(ku, k3[p], rows, B.m0[p], B.m1[p]);
+        roff[i] = all.size();
+        all.insert(all.end(), rows.begin(), rows.end());
+        any = true;
+    }
+    if (any) {
+        uint32_t* rd = (uint32_t*)dev_get(c, "mid_rows", sizeof(uint32_t) * all.size());
+        if (!rd) return fail(c, MQW_ENOMEM, "device allocation failed for third-stage rows");
+        HIPCHK(c, hipMemcpyAsync(rd, all.data(), sizeof(uint32_t) * all.size(), hipMemcpyHostToDevice, c->stream));
+        for (size_t i = 0; i < cj.size(); i++)
+            if (cj[i].ok) B.mtab[cand[i]] = rd + roff[i];
+        st = run_bases(c, n, k, ps, trajs, E, mb, use_graph);
+        if (st) return st;
+    } else {
+        B.U.clear();
+        B.ku.clear();
+    }
+    trace("third stage: bases");
+    // 4. the trajectories continue from the third stage's GEMM, or from their
+    //    second-stage states with the third stage as terms
+    for (size_t i = 0; i < cj.size(); i++) {
+        const size_t p = cand[i];
+        const int R = cj[i].ok ? cj[i].fc.R : 0;
+        for (size_t s = 0; s < ptraj[p].size(); s++) {
+            Traj& tr = trajs[ptraj[p][s]];
+            tr.second_done = false;
+            tr.first_done = false;
+            if (cj[i].ok) {
+                tr.mid_done = true;
+                tr.cmq.resize((size_t)2 * R);
+                for (int q = 0; q < 2 * R; q++) tr.cmq[q] = (float)cj[i].fc.u[s * 2 * R + q];
+            } else {
+                tr.seg0 = 2;
+                tr.psi_in = buf + (base[p] + s) * 2 * N;
+            }
+        }
+        if (cj[i].ok) {
+            c->stats.third_stage_instances++;
+            c->stats.third_stage_vectors += (uint64_t)(2 * R);
+            c->stats.third_stage_states += (uint64_t)(2 * ptraj[p].size());
+        }
+    }
+    return MQW_OK;
+}
+
+// ---------------------------------------------------------------------------
 // One group of problems of the same n.
 // ---------------------------------------------------------------------------
 template <typename real>
@@ -1892,8 +2125,9 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         if (st) return st;
         trace("direct m = 1 chains");
     }
-    std::vector<int> kc1(ps.size(), 0), kc2(ps.size(), 0), kx(ps.size(), 0), ky(ps.size(), 0);
+    std::vector<int> kc1(ps.size(), 0), kc2(ps.size(), 0), kx(ps.size(), 0), ky(ps.size(), 0), k3(ps.size(), 0);
     std::vector<double> own_pen(ps.size(), 0.0); // bytes per amplitude of the samples' own stage m - 1 terms
+    std::vector<double> own3(ps.size(), 0.0);    // and of their own third-stage terms
     if (basis_path) {
         std::vector<double> own2(ps.size(), 0.0); // bytes per amplitude of the samples' own second-stage terms
         for (const Traj& tr : trajs) {
@@ -1909,6 +2143,11 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
                 const size_t K = tr.segs[(size_t)pm - 2].c.size();
                 ky[tr.prob] = std::max(ky[tr.prob], (int)K);
                 own_pen[tr.prob] += 36.0 * (double)std::max<size_t>(K - 1, 1);
+            }
+            if (pm >= 4 && o.shared_third_stage) {
+                const size_t K = tr.segs[2].c.size();
+                k3[tr.prob] = std::max(k3[tr.prob], (int)K);
+                own3[tr.prob] += 36.0 * (double)std::max<size_t>(K - 1, 1);
             }
         }
         const bool split = o.second_stage_gemm == MQW_GEMM_SPLIT_FP16;
@@ -2139,8 +2378,11 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     }
     // Several batches: two basis buffers, the next batch's recurrences on the
     // second stream while this batch's plans run on the main one.
+    // (two buffers of bases within a third of the 288 GB part: the n = 22
+    // instances with m >= 4, about 60 GB of bases each, run one buffer)
     const bool overlap = o.overlap_bases && c->stream2 && pbatches.size() > 1 && max_total > 0 &&
-                         (!o.max_device_bytes || 2 * max_total <= o.max_device_bytes / 2);
+                         (!o.max_device_bytes || 2 * max_total <= o.max_device_bytes / 2) &&
+                         2 * max_total <= ((size_t)96 << 30);
     const int nbuf = overlap ? 2 : 1;
     char* bbase[2] = {nullptr, nullptr};
     uint32_t* bq[2] = {nullptr, nullptr};
@@ -2255,11 +2497,34 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
                 tr.adjoint = use_bases && !tr.adjoint2 && bases.kx[p] > 0;
                 order.push_back(t);
             }
+        // the compressed third stage of the batch's instances where it applies
+        if (use_bases && o.shared_third_stage && !bases.kc2.empty() &&
+            o.second_stage_gemm == MQW_GEMM_SPLIT_FP16) {
+            std::vector<size_t> cand;
+            for (size_t p = pb.first; p < pb.second; p++) {
+                const int pm = ps[p]->m;
+                const size_t S = ptraj[p].size();
+                const bool adj3 = !bases.kz.empty() && bases.kz[p] > 0 && pm == 4; // stage 3 read out by the adjoint
+                if (pm < 4 || adj3 || bases.kc2[p] <= 0 || S < 2 || 2 * S > (size_t)kGramMaxK || k3[p] < 2 ||
+                    k3[p] > kC2MaxK2)
+                    continue;
+                bool all = true; // every sample still to run (none evaluated otherwise)
+                for (size_t t : ptraj[p]) all &= trajs[t].second_done;
+                // at most 2S directions (and at most the GEMM's 128)
+                const double ku = (double)std::min<size_t>(2 * S, (size_t)kC2MaxK2);
+                if (all && 20.0 * 0.5 * ku * k3[p] < 0.8 * own3[p]) cand.push_back(p);
+            }
+            if (!cand.empty()) {
+                st = middle_stage(c, n, k, ps, trajs, ptraj, cand, (const float*)E, bases, k3, own3,
+                                  o.use_graph != 0, wcap1);
+                if (st) return st;
+            }
+        }
         std::vector<size_t> cost(trajs.size(), 0);
         for (size_t t : order) {
             const Traj& tr = trajs[t];
             const size_t s1 = tr.segs.size() - (tr.adjoint2 ? 2 : tr.adjoint ? 1 : 0);
-            for (size_t s = tr.second_done ? 2 : tr.first_done ? 1 : 0; s < s1; s++)
+            for (size_t s = tr.mid_done ? 3 : tr.second_done ? 2 : tr.first_done ? 1 : (size_t)tr.seg0; s < s1; s++)
                 cost[t] += std::max<size_t>(tr.segs[s].c.size() - 1, 1);
         }
         std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
@@ -2462,6 +2727,7 @@ void mqw_opts_default(mqw_opts* o)
     o->compress_bases = 1;
     o->direct_m1 = 1;
     o->adjoint_two_stages = 1;
+    o->shared_third_stage = 1;
     o->sample_probs = nullptr;
 }
 
