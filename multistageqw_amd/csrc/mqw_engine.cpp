@@ -88,6 +88,7 @@ void* mqw::dev_get(mqw_ctx* c, const std::string& name, size_t bytes)
     if (hipMalloc(&b.ptr, want) != hipSuccess) {
         hipGetLastError();
         b.ptr = nullptr;
+        trace("device buffer %s: %zu bytes failed", name.c_str(), want);
         return nullptr;
     }
     b.cap = want;
@@ -1874,7 +1875,10 @@ mqw_status middle_stage(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<
     for (size_t p : cand) ntr += ptraj[p].size();
     float* buf = (float*)dev_get(c, "mid_psi2", sizeof(float) * 2 * N * ntr);
     double* part = (double*)dev_get(c, "mid_psi2_norm", sizeof(double) * nparts * ntr);
-    if (!buf || !part) return MQW_OK; // no room: the third stage stays as terms
+    if (!buf || !part) { // no room: the third stage stays as terms
+        trace("third stage: no room for %zu states", ntr);
+        return MQW_OK;
+    }
     trace("third stage: %zu instances, %zu states", cand.size(), ntr);
     // 1. the second stage's outputs into the buffer, sample j of an instance at
     //    its base + j (every sample of an instance is in this batch)
@@ -1934,6 +1938,8 @@ mqw_status middle_stage(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<
         const size_t w = sizeof(float) * N * (size_t)ku * (size_t)(k3[p] - 1);
         const bool pays = ku <= kC2MaxK2 && (size_t)ku * (size_t)k3[p] + 2 * kC2RowPad <= (size_t)kC2MaxRows &&
                           20.0 * ku * k3[p] < 0.8 * own3[p] && wbytes + w <= cap;
+        trace("third stage: instance %zu, %d of %d directions, k3 %d, W3 %zu bytes%s", p, ku, cj[i].kc, k3[p], w,
+              pays ? "" : ": as terms");
         if (!pays) {
             cj[i].ok = false;
             continue;
@@ -1944,8 +1950,10 @@ mqw_status middle_stage(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<
     }
     char* ub = ubytes ? (char*)dev_get(c, "mid_U", ubytes) : nullptr;
     char* wb = wbytes ? (char*)dev_get(c, "mid_W3", wbytes) : nullptr;
-    if (ubytes && (!ub || !wb))
+    if (ubytes && (!ub || !wb)) {
+        trace("third stage: no room for %zu + %zu bytes of bases", ubytes, wbytes);
         for (Compression& j : cj) j.ok = false; // no room: the third stage stays as terms
+    }
     size_t uo = 0;
     for (size_t i = 0; i < cj.size(); i++)
         if (cj[i].ok) {
