@@ -1167,9 +1167,12 @@ mqw_status run_lds(mqw_ctx* c, int n, const std::vector<Traj>& trajs, const std:
 // With pending, the launches go to `stream` and are not waited for: the
 // plans (whose host tables the copies read) move into keep, the events into
 // pending, and the caller synchronises and collects the times.
+// tag names the device buffers of the plan's tables: work on the main stream
+// while a pending plan runs on the second stream must use another.
 mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Problem*>& ps,
                      const std::vector<Traj>& trajs, const float* E, const Bases& B, bool use_graph,
-                     Timed* pending = nullptr, hipStream_t stream = nullptr, std::deque<Plan<float>>* keep = nullptr)
+                     Timed* pending = nullptr, hipStream_t stream = nullptr, std::deque<Plan<float>>* keep = nullptr,
+                     const std::string& tag = "bases")
 {
     const hipStream_t stm = stream ? stream : c->stream;
     const size_t N = (size_t)1 << n;
@@ -1377,13 +1380,13 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
     c->stats.aux_launches += P.ops.size();
     if (pending) {
         mqw_status st = run_plan<float>(c, MQW_FP32, n, k, P, use_graph, nullptr, nullptr, nullptr, nullptr, pending,
-                                        stm, "bases");
+                                        stm, tag);
         keep->push_back(std::move(P)); // the copies read its host tables
         return st;
     }
     double ms_sweep = 0.0, ms_aux = 0.0, ms_basis = 0.0; // every op here is a basis launch
     mqw_status st = run_plan<float>(c, MQW_FP32, n, k, P, use_graph, &ms_sweep, &ms_aux, &ms_basis, nullptr, nullptr,
-                                    stm, "bases");
+                                    stm, tag);
     if (st) return st;
     c->stats.gpu_ms += ms_sweep + ms_aux;
     c->stats.aux_ms += ms_sweep + ms_aux;
@@ -1434,7 +1437,6 @@ mqw_status direct_m1(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
         if (q.kc >= 2 && q.kc <= kGramMaxK) ch.push_back(std::move(q));
     }
     if (ch.empty()) return MQW_OK;
-    const uint32_t nchunk = (uint32_t)std::min<size_t>(64, N / 64);
     size_t i0 = 0;
     while (i0 < ch.size()) {
         size_t i1 = i0, vbytes = 0;
@@ -1469,6 +1471,7 @@ mqw_status direct_m1(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
         if (st) return st;
         const size_t njobs = i1 - i0;
         const size_t pmax = gram_pmax(kcmax);
+        const uint32_t nchunk = gram_nchunk(n);
         double* gram_d = (double*)dev_get(c, "cmp_gram", sizeof(double) * pairs);
         double* part_d = (double*)dev_get(c, "cmp_gram_part", sizeof(double) * njobs * nchunk * pmax);
         GramJob* gj_d = (GramJob*)dev_get(c, "cmp_gram_jobs", sizeof(GramJob) * njobs);
@@ -1566,9 +1569,9 @@ struct Compression {
 mqw_status compress_svd(mqw_ctx* c, int n, std::vector<Compression>& cj, const std::string& tag)
 {
     const size_t N = (size_t)1 << n;
-    const uint32_t nchunk = (uint32_t)std::min<size_t>(64, N / 64);
     const size_t njobs = cj.size();
     if (njobs == 0) return MQW_OK;
+    const uint32_t nchunk = gram_nchunk(n);
     size_t pairs = 0;
     uint32_t kcmax = 0;
     for (Compression& j : cj) {
@@ -2002,7 +2005,9 @@ mqw_status middle_stage(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<
         HIPCHK(c, hipMemcpyAsync(rd, all.data(), sizeof(uint32_t) * all.size(), hipMemcpyHostToDevice, c->stream));
         for (size_t i = 0; i < cj.size(); i++)
             if (cj[i].ok) B.mtab[cand[i]] = rd + roff[i];
-        st = run_bases(c, n, k, ps, trajs, E, mb, use_graph);
+        // (the next batch's recurrences may be running on the second stream
+        // with the "bases" tables)
+        st = run_bases(c, n, k, ps, trajs, E, mb, use_graph, nullptr, nullptr, nullptr, "mid_bases");
         if (st) return st;
     } else {
         B.U.clear();

@@ -72,6 +72,8 @@ gram_kernel(const GramJob* __restrict__ jobs, uint32_t njobs, int n, uint32_t nc
     }
 }
 
+// One thread per pair (workgroups: jobs x pair blocks of 256), the chunks in
+// order: consecutive threads read consecutive partial sums.
 template <int KC>
 __global__ void __launch_bounds__(256)
 gram_reduce_kernel(const GramJob* __restrict__ jobs, uint32_t nchunk, const double* __restrict__ partial)
@@ -80,11 +82,11 @@ gram_reduce_kernel(const GramJob* __restrict__ jobs, uint32_t nchunk, const doub
     const uint32_t job = blockIdx.x;
     const GramJob J = jobs[job];
     const int P = pair_start((int)J.kc);
-    for (int p = threadIdx.x; p < P; p += 256) {
-        double sum = 0.0;
-        for (uint32_t c = 0; c < nchunk; c++) sum += partial[((uint64_t)job * nchunk + c) * PMAX + p];
-        J.out[p] = sum;
-    }
+    const int p = (int)(blockIdx.y * 256 + threadIdx.x);
+    if (p >= P) return;
+    double sum = 0.0;
+    for (uint32_t c = 0; c < nchunk; c++) sum += partial[((uint64_t)job * nchunk + c) * PMAX + p];
+    J.out[p] = sum;
 }
 
 // Wide bases: the same pair sums in register blocks.  Per parity class cls,
@@ -189,16 +191,18 @@ gram_blocked_kernel(const GramJob* __restrict__ jobs, uint32_t njobs, int n, uin
 template <int KC>
 int launch_gram_kc(const GramJob* jobs, uint32_t njobs, int n, uint32_t nchunk, double* partial, hipStream_t st)
 {
+    constexpr int PMAX = KC + ((KC - 1) * (KC - 1)) / 4;
+    const dim3 rgrid(njobs, (PMAX + 255) / 256);
     if constexpr (KC > 64) {
         hipLaunchKernelGGL(gram_blocked_kernel<KC>, dim3(njobs * nchunk), dim3(512), 0, st, jobs, njobs, n, nchunk,
                            partial);
         if (check_launch("gram")) return -1;
-        hipLaunchKernelGGL(gram_reduce_kernel<KC>, dim3(njobs), dim3(256), 0, st, jobs, nchunk, partial);
+        hipLaunchKernelGGL(gram_reduce_kernel<KC>, rgrid, dim3(256), 0, st, jobs, nchunk, partial);
         return check_launch("gram_reduce");
     }
     hipLaunchKernelGGL(gram_kernel<KC>, dim3(njobs * nchunk), dim3(256), 0, st, jobs, njobs, n, nchunk, partial);
     if (check_launch("gram")) return -1;
-    hipLaunchKernelGGL(gram_reduce_kernel<KC>, dim3(njobs), dim3(256), 0, st, jobs, nchunk, partial);
+    hipLaunchKernelGGL(gram_reduce_kernel<KC>, rgrid, dim3(256), 0, st, jobs, nchunk, partial);
     return check_launch("gram_reduce");
 }
 
@@ -212,7 +216,7 @@ int launch_gram(const GramJob* jobs_dev, uint32_t njobs, uint32_t kc_max, int n,
     if (njobs == 0) return 0;
     const uint64_t N = 1ull << n;
     if (nchunk == 0 || N % nchunk != 0 || (N / nchunk) % kSlab != 0) return -2;
-    if ((uint64_t)njobs * nchunk > 0x7fffffffull) return -3;
+    if ((uint64_t)njobs * nchunk > 0x7fffffffull || njobs > 0x7fffffffu) return -3;
     hipStream_t st = (hipStream_t)stream;
     if (kc_max <= 32) return launch_gram_kc<32>(jobs_dev, njobs, n, nchunk, partial, st);
     if (kc_max <= 64) return launch_gram_kc<64>(jobs_dev, njobs, n, nchunk, partial, st);

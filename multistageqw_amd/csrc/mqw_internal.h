@@ -5,6 +5,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -151,7 +152,17 @@ inline uint32_t gram_pmax(uint32_t kc_max)
     return gram_pairs(kc_max <= 32 ? 32u : kc_max <= 64 ? 64u : kc_max <= 128 ? 128u : 256u);
 }
 // Pair sums of every job's vectors: njobs x nchunk partial sums of
-// gram_pairs(kc_max) doubles in `partial`, then their fixed-order sum per job.
+// gram_pmax(kc_max) doubles in `partial`, then their fixed-order sum per job.
+// gram_nchunk: amplitude chunks of 8192 amplitudes, at least 64 chunks (where
+// N allows) and at most 512, so a single large basis still spreads over the
+// device.  It depends on n alone: the sums, and so the compressions, are the
+// same however the jobs are grouped into launches.
+inline uint32_t gram_nchunk(int n)
+{
+    const uint64_t N = 1ull << n;
+    const uint64_t lo = std::min<uint64_t>(64, N / 64);
+    return (uint32_t)std::max<uint64_t>(lo, std::min<uint64_t>(512, N / 8192));
+}
 int launch_gram(const GramJob* jobs_dev, uint32_t njobs, uint32_t kc_max, int n, uint32_t nchunk, double* partial,
                 void* stream);
 // The compression of one instance from its Gram pair sums and its samples'

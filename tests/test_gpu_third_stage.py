@@ -122,3 +122,27 @@ def test_third_stage_overlapped_batches_bit_identical(ctx):
     print(f"[mid] batches {st['basis_batches']} (overlapped {st['overlap_batches']})", flush=True)
     assert st["third_stage_instances"] == 4 and st["basis_batches"] >= 2 and st["overlap_batches"] >= 1
     assert np.array_equal(tt, tt2) and np.array_equal(sp, sp2)
+
+
+def test_third_stage_while_next_batch_runs_n21(ctx):
+    """n = 21, m = 5: a batch's third-stage recurrences run on the main stream while the next
+    batch's second-stage recurrences are still running on the second stream, each on its own job
+    tables (the mixed-size benchmark's draws, whose n = 21 instances once read the other's)."""
+    rng = np.random.default_rng(3)
+    inst = []
+    for k in range(512):
+        nn, mm = int(rng.integers(12, 23)), int(rng.integers(1, 6))
+        if (nn, mm) == (21, 5) and len(inst) < 3:
+            inst.append((nn, mm, M.generate_sk(nn, 1, 3 * 100003 + k)[0]))
+    assert len(inst) == 3
+    _say("third stage n=21 m=5 beside the next batch's recurrences")
+    p, tt, sp = ctx.solve_batch(inst, samples=100, with_samples=True)
+    st = ctx.last_stats()
+    p2, tt2, sp2 = ctx.solve_batch(inst, samples=100, with_samples=True, overlap_bases=False)
+    p3, tt3, sp3 = ctx.solve_batch(inst, samples=100, with_samples=True, shared_third_stage=False)
+    d = np.max(np.abs(sp - sp3))
+    print(f"[mid] n=21: batches {st['basis_batches']} (overlapped {st['overlap_batches']}), third stages "
+          f"{st['third_stage_instances']}, max|dp| vs terms {d:.3e}", flush=True)
+    assert st["third_stage_instances"] == 3 and st["overlap_batches"] >= 1
+    assert np.array_equal(tt, tt2) and np.array_equal(sp, sp2)
+    assert np.array_equal(tt, tt3) and d < 2e-6
