@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 16
+#define MQW_ABI_VERSION 17
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -160,6 +160,14 @@ typedef struct {
                                 second-stage states are compressed (their real parts and imaginary parts,
                                 2e-7 per sample) to directions U_r, and the third stage is a GEMM over
                                 T_l(H3/onenorm3) U_r like the second; 0 = per-sample terms */
+    int chain_recurrences;   /* 1 (default) = for n = 15..20 on a 256-CU device, the shared bases' Chebyshev
+                                recurrences keep their state on chip: one launch runs every step of a
+                                chain on a group of workgroups of one XCD, each holding 2^15 amplitudes of
+                                s_k and of the diagonal in registers, the partner amplitudes of the top
+                                n - 15 bits read from the XCD's L2 (8 instead of about 20 bytes of HBM
+                                traffic per vector and amplitude).  Results are bit-identical to 0 = one
+                                launch per recurrence step.  Batches of bases then run one after the
+                                other (no second stream: two such launches must not share the device). */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are
@@ -339,6 +347,8 @@ typedef struct {
     uint64_t third_stage_instances; /* instances whose third stage came from compressed second-stage states */
     uint64_t third_stage_vectors;   /* their kept directions (2R each), summed */
     uint64_t third_stage_states;    /* their second-stage state halves before compression (2 per sample), summed */
+    uint64_t chain_launches;        /* on-chip recurrence launches (chain_recurrences) */
+    uint64_t chain_steps;           /* recurrence steps (vectors) they made */
 } mqw_stats;
 mqw_status mqw_last_stats(const mqw_ctx* ctx, mqw_stats* out);
 

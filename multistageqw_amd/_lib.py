@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmqw.so")
 CLI_PATH = os.path.join(HERE, "mqw")
 
-ABI_VERSION = 16  # MQW_ABI_VERSION of include/mqw.h this binding matches
+ABI_VERSION = 17  # MQW_ABI_VERSION of include/mqw.h this binding matches
 MAX_STAGES = 64
 DEFAULT_SAMPLES = 100
 DEFAULT_TOL = 1e-6
@@ -61,7 +61,8 @@ class Opts(C.Structure):
                 ("shared_first_stage", C.c_int), ("final_readout", C.c_int), ("shared_second_stage", C.c_int),
                 ("overlap_bases", C.c_int), ("second_stage_gemm", C.c_int), ("mixer_kind", C.c_int),
                 ("compress_bases", C.c_int), ("direct_m1", C.c_int), ("adjoint_two_stages", C.c_int),
-                ("shared_third_stage", C.c_int)]
+                ("shared_third_stage", C.c_int),
+                ("chain_recurrences", C.c_int)]
 
 
 class Instance(C.Structure):
@@ -83,7 +84,8 @@ class Stats(C.Structure):
                 ("first_stage_chunks", C.c_uint64), ("adjoint2_instances", C.c_uint64),
                 ("adjoint2_vectors", C.c_uint64), ("adjoint2_uncompressed", C.c_uint64),
                 ("third_stage_instances", C.c_uint64), ("third_stage_vectors", C.c_uint64),
-                ("third_stage_states", C.c_uint64)]
+                ("third_stage_states", C.c_uint64),
+                ("chain_launches", C.c_uint64), ("chain_steps", C.c_uint64)]
 
 
 _lib = None

@@ -117,7 +117,8 @@ def make_opts(samples=L.DEFAULT_SAMPLES, tol=L.DEFAULT_TOL, precision=L.FP32, en
               group_bytes=0, variant=-1, high_pass=-1, mixer_masks=None, fixed_times=False,
               shared_first_stage=True, final_readout=True, shared_second_stage=True,
               overlap_bases=True, gemm="split_fp16", mixer="xstring", compress_bases=True,
-              direct_m1=True, adjoint_two_stages=True, shared_third_stage=True) -> L.Opts:
+              direct_m1=True, adjoint_two_stages=True, shared_third_stage=True,
+              chain_recurrences=True) -> L.Opts:
     o = L.Opts()
     L.load().mqw_opts_default(C.byref(o))
     o.samples = samples
@@ -141,6 +142,7 @@ def make_opts(samples=L.DEFAULT_SAMPLES, tol=L.DEFAULT_TOL, precision=L.FP32, en
     o.direct_m1 = int(bool(direct_m1))
     o.adjoint_two_stages = int(bool(adjoint_two_stages))
     o.shared_third_stage = int(bool(shared_third_stage))
+    o.chain_recurrences = int(bool(chain_recurrences))
     gemms = {"split_fp16": 0, "fp32": 1}
     if gemm not in gemms:
         raise ValueError(f"gemm must be one of {sorted(gemms)}, got {gemm!r}")

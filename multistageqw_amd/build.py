@@ -28,7 +28,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", "-Wno-unused-value", f"-I{INCLUDE}", f"-I{CSRC}",
           f"-I{ROCM}/include"]
 
-KERNEL_SOURCES = ["mqw_kernels.hip", "mqw_lds.hip", "mqw_mlp.hip", "mqw_noalloc.hip", "mqw_gram.hip"]
+KERNEL_SOURCES = ["mqw_kernels.hip", "mqw_lds.hip", "mqw_mlp.hip", "mqw_noalloc.hip", "mqw_gram.hip", "mqw_chain.hip"]
 HOST_SOURCES = ["mqw_engine.cpp", "mqw_host.cpp", "mqw_tune.cpp", "mqw_io.cpp", "mqw_learn.cpp", "mqw_basis.cpp"]
 
 

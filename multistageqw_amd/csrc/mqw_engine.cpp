@@ -341,7 +341,7 @@ struct Op {
     int kind; // 0 = term launch, 1 = reduce launch, 2 = high-bit pass over the jobs of the next term launch,
               // 3 = combine launch (shared first stage), 4 = real-half term launch (basis recurrences),
               // 5 = adjoint readout dots, 6 = adjoint readout finalize, 7 = combine2 launch (second stage),
-              // 8 = pair dots of the two-stage adjoint
+              // 8 = pair dots of the two-stage adjoint, 9 = on-chip recurrences (the plan's chains)
     size_t off;
     uint32_t count;
     uint32_t aux = 0;  // kinds 3, 5: largest basis size of its jobs / tiles; kind 7: largest tile
@@ -368,6 +368,8 @@ struct Plan {
     double gemm_bytes = 0.0;        // their algorithmic bytes (also in aux_bytes)
     uint64_t aux_launches = 0;
     std::vector<Op> ops;
+    std::vector<ChainJob> chains;        // on-chip recurrences (op kind 9): the chains,
+    std::vector<uint32_t> corder, cgoff; // their order by group and each group's first
     uint64_t amp_terms = 0;
     uint64_t psi_const_amp_terms = 0;
     uint64_t terms = 0;
@@ -994,6 +996,22 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
     if (!P.jobs.empty())
         HIPCHK(c, hipMemcpyAsync(jd, P.jobs.data(), sizeof(TermJob<real>) * P.jobs.size(), hipMemcpyHostToDevice,
                                  stm));
+    ChainJob* chd = nullptr;
+    uint32_t *cord = nullptr, *cgo = nullptr, *csync = nullptr;
+    if (!P.chains.empty()) {
+        chd = (ChainJob*)dev_get(c, tag + "_chains", sizeof(ChainJob) * P.chains.size());
+        cord = (uint32_t*)dev_get(c, tag + "_corder", sizeof(uint32_t) * P.corder.size());
+        cgo = (uint32_t*)dev_get(c, tag + "_cgoff", sizeof(uint32_t) * P.cgoff.size());
+        csync = (uint32_t*)dev_get(c, tag + "_csync", sizeof(uint32_t) * chain_sync_words());
+        if (!chd || !cord || !cgo || !csync || !c->chain_err)
+            return fail(c, MQW_ENOMEM, "device allocation failed for %zu chains", P.chains.size());
+        HIPCHK(c, hipMemcpyAsync(chd, P.chains.data(), sizeof(ChainJob) * P.chains.size(), hipMemcpyHostToDevice,
+                                 stm));
+        HIPCHK(c, hipMemcpyAsync(cord, P.corder.data(), sizeof(uint32_t) * P.corder.size(), hipMemcpyHostToDevice,
+                                 stm));
+        HIPCHK(c, hipMemcpyAsync(cgo, P.cgoff.data(), sizeof(uint32_t) * P.cgoff.size(), hipMemcpyHostToDevice,
+                                 stm));
+    }
     if (!P.rjobs.empty())
         HIPCHK(c, hipMemcpyAsync(rd, P.rjobs.data(), sizeof(ReduceJob) * P.rjobs.size(), hipMemcpyHostToDevice,
                                  stm));
@@ -1007,6 +1025,8 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
                 rc = launch_combine(cd + op.off, op.count, ci, cf, n, k.parts, op.aux, stm);
             else if (op.kind == 4)
                 rc = launch_half_real(k.tb, k.variant, jd + op.off, op.count, n, k.lc, stm);
+            else if (op.kind == 9)
+                rc = launch_chains(chd, cord, cgo, csync, c->chain_err, n, stm);
             else if (op.kind == 5 && op.aux3)
                 rc = launch_readout_dots_mfma(rtd + op.off, op.count, rid, cf, n, op.aux2, dots, stm);
             else if (op.kind == 5)
@@ -1033,7 +1053,7 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
     };
     // timing classes: 0 sweep (terms, reduce, high-bit pass), 1 basis recurrences,
     // 2 second-stage GEMM, 3 other auxiliary launches (combine, readout)
-    auto op_class = [](int kind) { return kind <= 2 ? 0 : kind == 4 ? 1 : kind == 7 ? 2 : 3; };
+    auto op_class = [](int kind) { return kind <= 2 ? 0 : kind == 4 || kind == 9 ? 1 : kind == 7 ? 2 : 3; };
     struct Run {
         size_t b, e;
         int cls;
@@ -1254,6 +1274,110 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
         j.beta = (float)((double)sg.gamma / sc);
         return j;
     };
+    // On-chip recurrences (launch_chains): every chain runs all its steps in
+    // one launch, so every chain must start from a vector ready before it
+    // (no uncompressed W chain, which waits for its V_k).  The same rows, steps
+    // and scalars as the per-step plan below, and bit-identical outputs.
+    bool use_chains = k.chains && !pending;
+    for (size_t p = 0; use_chains && p < ps.size(); p++)
+        if (has_w && B.kc2[p] > 0 && !cmp(p)) use_chains = false;
+    if (use_chains) {
+        double cbytes = 0.0;
+        uint64_t csteps = 0;
+        auto add = [&](size_t p, const Segment& sg, const float* seed, float* out, int nsteps) {
+            if (nsteps <= 0) return;
+            const TermJob<float> j0 = job(p, sg, 0), j1 = job(p, sg, 1);
+            // the kernel takes alpha0 E as (alpha E) / 2
+            if (j1.alpha != 2.0f * j0.alpha) use_chains = false;
+            ChainJob cj{};
+            cj.E = E + p * N;
+            cj.seed = seed;
+            cj.out = out;
+            cj.alpha0 = j0.alpha;
+            cj.beta0 = j0.beta;
+            cj.alpha = j1.alpha;
+            cj.beta = j1.beta;
+            cj.a1 = 1.0f;
+            cj.a2 = 1.0f;
+            cj.psi0 = psi0;
+            cj.steps = (uint32_t)nsteps;
+            P.chains.push_back(cj);
+            // HBM: the diagonal and the seed once, each output written, and
+            // s_{k-1} read back at every step k >= 1 (at k = 1 the seed, if an
+            // array; |+> is a constant)
+            const double back = nsteps >= 2 ? (double)(nsteps - 2) + (seed ? 1.0 : 0.0) : 0.0;
+            cbytes += 4.0 * N * (1.0 + (seed ? 1.0 : 0.0) + nsteps + back);
+            csteps += (uint64_t)nsteps;
+        };
+        for (size_t p = 0; p < ps.size(); p++) {
+            if (!cmp(p) && B.kc[p] > 1) add(p, *seg0[p], nullptr, const_cast<float*>(B.V[p]), B.kc[p] - 1);
+            if (B.kx[p] > 1)
+                add(p, *segl[p], B.X[p], const_cast<float*>(B.X[p]) + N, B.kx[p] - 1);
+            const int kc2 = has_w ? B.kc2[p] : 0;
+            if (kc2 > 0) // compressed: Q_0 at plusv, Q_kk = V_kk
+                for (int kk = 0; kk < B.kc[p]; kk++)
+                    add(p, *seg1[p], kk > 0 ? B.V[p] + (size_t)(kk - 1) * N : B.plusv[p],
+                        const_cast<float*>(B.W[p]) + (size_t)kk * (kc2 - 1) * N, kc2 - 1);
+            if (has_y && B.kz[p] > 0 && B.ky[p] > 0)
+                for (int kk = 0; kk < B.kz[p]; kk++)
+                    add(p, *segy[p], B.Z[p] + (size_t)kk * N,
+                        const_cast<float*>(B.Y[p]) + (size_t)kk * (B.ky[p] - 1) * N, B.ky[p] - 1);
+            if (has_m && B.ku[p] > 0 && B.k3[p] > 0)
+                for (int kk = 0; kk < B.ku[p]; kk++)
+                    add(p, *segm[p], B.U[p] + (size_t)kk * N,
+                        const_cast<float*>(B.W3[p]) + (size_t)kk * (B.k3[p] - 1) * N, B.k3[p] - 1);
+        }
+        if (use_chains && !P.chains.empty()) {
+            // longest chains first, each to the group with the fewest steps so far
+            const uint32_t ngroups = (uint32_t)kChainWorkgroups >> (n - kChainMinN);
+            std::vector<uint32_t> idx(P.chains.size());
+            for (uint32_t i = 0; i < idx.size(); i++) idx[i] = i;
+            std::stable_sort(idx.begin(), idx.end(),
+                             [&](uint32_t a, uint32_t b) { return P.chains[a].steps > P.chains[b].steps; });
+            std::vector<uint64_t> load(ngroups, 0);
+            std::vector<std::vector<uint32_t>> lists(ngroups);
+            for (uint32_t i : idx) {
+                const uint32_t g = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+                load[g] += P.chains[i].steps;
+                lists[g].push_back(i);
+            }
+            P.cgoff.assign(1, 0);
+            for (uint32_t g = 0; g < ngroups; g++) {
+                P.corder.insert(P.corder.end(), lists[g].begin(), lists[g].end());
+                P.cgoff.push_back((uint32_t)P.corder.size());
+            }
+            P.ops.push_back(Op{9, 0, (uint32_t)P.chains.size()});
+            double ms_sweep = 0.0, ms_aux = 0.0, ms_basis = 0.0;
+            mqw_status st = run_plan<float>(c, MQW_FP32, n, k, P, use_graph, &ms_sweep, &ms_aux, &ms_basis, nullptr,
+                                            nullptr, stm, tag);
+            if (st) return st;
+            uint32_t ev = 0;
+            HIPCHK(c, hipMemcpy(&ev, c->chain_err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+            if (ev == 0) {
+                c->stats.aux_bytes += bytes + cbytes;
+                c->stats.basis_bytes += bytes + cbytes;
+                c->stats.basis_launches += 1;
+                c->stats.aux_launches += 1;
+                c->stats.chain_launches += 1;
+                c->stats.chain_steps += csteps;
+                c->stats.gpu_ms += ms_sweep + ms_aux;
+                c->stats.aux_ms += ms_sweep + ms_aux;
+                c->stats.basis_ms += ms_basis;
+                return MQW_OK;
+            }
+            // a group's workgroups on several XCDs, or a wait that ran out: this
+            // context runs its recurrences one launch per step from now on,
+            // starting with these (they rewrite every row)
+            (void)hipMemset(c->chain_err, 0, sizeof(uint32_t));
+            c->chain_capable = false;
+            trace("on-chip recurrences aborted (%s): one launch per step from now on",
+                  (ev & 2u) ? "workgroups of a group on several XCDs" : "a wait ran out");
+            SweepCfg k1 = k;
+            k1.chains = false;
+            return run_bases(c, n, k1, ps, trajs, E, B, use_graph, pending, stream, keep, tag);
+        }
+        P.chains.clear();
+    }
     // Chains from seeds: row (l, kk) of rows at W + (kk (kc2 - 1) + l - 1) N for
     // l >= 1, row (0, kk) = seed(kk) (nullptr: the constant psi0); step q holds
     // l = q - kk, or l = q when every seed is ready before the batch (cq).
@@ -2053,6 +2177,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     g_trace_t0 = std::chrono::steady_clock::now();
     trace("solve_group n=%d, %zu problems", n, ps.size());
     SweepCfg k = choose_sweep(c, precision, n, o.tile_bits, o.chunk_bits, o.variant, o.high_pass);
+    k.chains = o.chain_recurrences && c->chain_capable && k.half && n >= kChainMinN && n <= kChainMaxN;
     const bool mixer = o.mixer_count > 0;
     if (mixer) {
         // X-string mixer: its own sweep kernel, masks on the device
@@ -2393,7 +2518,9 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     // second stream while this batch's plans run on the main one.
     // (two buffers of bases within a third of the 288 GB part: the n = 22
     // instances with m >= 4, about 60 GB of bases each, run one buffer)
-    const bool overlap = o.overlap_bases && c->stream2 && pbatches.size() > 1 && max_total > 0 &&
+    // (not with on-chip recurrences: a launch of them needs every CU of the
+    // device, and two on two streams could wait for each other's workgroups)
+    const bool overlap = o.overlap_bases && !k.chains && c->stream2 && pbatches.size() > 1 && max_total > 0 &&
                          (!o.max_device_bytes || 2 * max_total <= o.max_device_bytes / 2) &&
                          2 * max_total <= ((size_t)96 << 30);
     const int nbuf = overlap ? 2 : 1;
@@ -2741,6 +2868,7 @@ void mqw_opts_default(mqw_opts* o)
     o->direct_m1 = 1;
     o->adjoint_two_stages = 1;
     o->shared_third_stage = 1;
+    o->chain_recurrences = 1;
     o->sample_probs = nullptr;
 }
 
@@ -2766,6 +2894,13 @@ mqw_status mqw_create(int device, mqw_ctx** out)
     // this device's copy of the no-allocation scratch (mqw_clenshaw_noalloc);
     // a failed lookup is reported by that entry point, not here
     (void)noalloc_scratch_init(device);
+    // the on-chip recurrences take one workgroup per CU of a 256-CU (8 x 32) part
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        cus == kChainWorkgroups && hipMalloc((void**)&c->chain_err, sizeof(uint32_t)) == hipSuccess) {
+        c->chain_capable = hipMemset(c->chain_err, 0, sizeof(uint32_t)) == hipSuccess;
+    }
+    hipGetLastError();
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return MQW_EDEVICE;
     if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
@@ -2781,6 +2916,7 @@ void mqw_destroy(mqw_ctx* c)
     hipSetDevice(c->device);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     release_bufs(c);
+    if (c->chain_err) (void)hipFree(c->chain_err);
     if (c->stream) hipStreamDestroy(c->stream);
     if (c->stream2) hipStreamDestroy(c->stream2);
     delete c;

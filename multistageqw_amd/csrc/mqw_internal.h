@@ -165,6 +165,31 @@ inline uint32_t gram_nchunk(int n)
 }
 int launch_gram(const GramJob* jobs_dev, uint32_t njobs, uint32_t kc_max, int n, uint32_t nchunk, double* partial,
                 void* stream);
+
+// Basis recurrences with the state on chip (mqw_chain.hip): a chain
+//   s_1 = a1 (alpha0 E s_0 - beta0 M s_0),
+//   s_{k+1} = a1 (alpha E s_k - beta M s_k) - a2 s_{k-1}  (k >= 1)
+// (M = sum_i X_i; a1 = a2 = 1, kept as values so the arithmetic is that of
+// the per-step jobs) runs all its steps in one launch on one group of
+// 2^(n-15) workgroups of one XCD.  kChainWorkgroups workgroups in all, one
+// per CU: groups g = xcd * (32 >> (n - 15)) + slot / 2^(n-15) for workgroup
+// b = 8 slot + xcd; order[goff[g] .. goff[g + 1]) lists group g's chains.
+struct ChainJob {
+    const float* E;     // the instance's diagonal (N floats)
+    const float* seed;  // s_0, or nullptr: the constant psi0 (|+>)
+    float* out;         // s_{k+1} at out + k N
+    float alpha0, beta0, alpha, beta;
+    float a1, a2;
+    float psi0;
+    uint32_t steps;     // outputs, >= 1
+};
+constexpr int kChainMinN = 15, kChainMaxN = 20;
+constexpr int kChainWorkgroups = 256; // 8 XCDs x 32 CUs
+uint32_t chain_sync_words();
+// sync: chain_sync_words() words the launch clears first; err: a word the
+// kernel ORs 1 (a wait ran out) or 2 (a group's members on several XCDs) into
+int launch_chains(const ChainJob* jobs_dev, const uint32_t* order_dev, const uint32_t* goff_dev, uint32_t* sync,
+                  uint32_t* err, int n, void* stream);
 // The compression of one instance from its Gram pair sums and its samples'
 // first-stage coefficients (each row kc long, zero padded): rank R per class
 // (the same for both), B (2R x kc, row 2r + cls combines class cls) and u
@@ -375,6 +400,7 @@ struct SweepCfg {
     int tb = 12;       // tile bits of that kernel
     int lc = 0;        // XCD chunk bits
     int variant = 0;   // loads-in-flight variant of the real-half kernel
+    bool chains = false; // basis recurrences on chip (launch_chains) where they apply
     uint32_t parts = 1; // partial norm sums per output
     int hi = 0;        // top bits summed by the high-bit pass (0 = none)
     int hwb = 0;       // its pencil rows are 64 * 2^hwb bytes
@@ -397,6 +423,8 @@ struct mqw_ctx {
     int tuned_var[MQW_MAX_N + 1];
     int tuned_hp[MQW_MAX_N + 1]; // high_pass option chosen by the tuner, -1 = none
     std::string arch;
+    bool chain_capable = false;   // kChainWorkgroups CUs: the on-chip recurrences can run
+    uint32_t* chain_err = nullptr; // their error word (device), 0 when all went well
 };
 
 namespace mqw {

@@ -104,10 +104,12 @@ def test_two_stage_adjoint_overlapped_batches_bit_identical(ctx):
     recs = M.generate_sk(n, 6, 515)
     cap = 4 << 30  # a quarter of it per batch of bases: one instance (W and Y about 0.6 GiB)
     _say("two-stage adjoint, overlapped batches")
-    p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True, max_device_bytes=cap)
+    # (one launch per recurrence step: the on-chip recurrences do not overlap batches)
+    p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True, max_device_bytes=cap,
+                          chain_recurrences=False)
     st = ctx.last_stats()
     p2, tt2, sp2 = ctx.solve(n, m, recs, samples=samples, with_samples=True, max_device_bytes=cap,
-                             overlap_bases=False)
+                             overlap_bases=False, chain_recurrences=False)
     st2 = ctx.last_stats()
     print(f"[adjoint2] batches {st['basis_batches']} (overlapped {st['overlap_batches']}), "
           f"{st2['basis_batches']} without overlap", flush=True)

@@ -2,7 +2,9 @@
 batches, the next batch's basis recurrences run on a second stream while the
 current batch's GEMM, readout and sweeps run on the main one.  The kernels and
 their inputs are the same as one batch after the other, so the results must be
-bit-identical with overlap_bases off; also checked against the oracle."""
+bit-identical with overlap_bases off; also checked against the oracle.  The
+on-chip recurrences (n = 15..20) take the whole device, so batches do not
+overlap with them: these tests run the recurrences one launch per step."""
 import numpy as np
 import pytest
 
@@ -34,6 +36,7 @@ def test_overlapped_bases_identical(ctx, oracle, n, m, count, samples, kw):
     _say("overlapped bases n", n, "m", m, "count", count, "samples", samples, kw)
     kw = dict(kw)
     recs = M.generate_sk(n, count or 3, kw.pop("seed", 7000 + 10 * n + m))
+    kw["chain_recurrences"] = False
     p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True, small_n_max=0, **kw)
     st = ctx.last_stats()
     batches = count or st["basis_batches"]
@@ -56,7 +59,8 @@ def test_overlap_repeated_solves(ctx):
     n, m, samples = 16, 3, 64
     recs = M.generate_sk(n, 3, 7101)
     _say("overlapped bases: three solves in a row")
-    kw = dict(samples=samples, with_samples=True, small_n_max=0, max_device_bytes=2 << 30, compress_bases=False)
+    kw = dict(samples=samples, with_samples=True, small_n_max=0, max_device_bytes=2 << 30, compress_bases=False,
+              chain_recurrences=False)
     first = ctx.solve(n, m, recs, **kw)[2]
     for _ in range(2):
         again = ctx.solve(n, m, recs, **kw)[2]
@@ -89,7 +93,8 @@ def test_overlap_in_uneven_batch(ctx):
     ]
     # (the m = 4 instance's two-stage adjoint bases would not fit the cap twice: off here; its
     # overlapped batches are tested with the two-stage adjoint)
-    kw = dict(samples=64, with_samples=True, small_n_max=0, max_device_bytes=2 << 30, adjoint_two_stages=False)
+    kw = dict(samples=64, with_samples=True, small_n_max=0, max_device_bytes=2 << 30, adjoint_two_stages=False,
+              chain_recurrences=False)
     p, tt, sp = ctx.solve_batch(inst, **kw)
     st = ctx.last_stats()
     assert st["overlap_batches"] >= 2

@@ -115,10 +115,12 @@ def test_third_stage_overlapped_batches_bit_identical(ctx):
     recs = M.generate_sk(n, 4, 715)
     cap = 6 << 30  # a quarter per batch of bases: one instance each
     _say("third stage, overlapped batches")
-    p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True, max_device_bytes=cap)
+    # (one launch per recurrence step: the on-chip recurrences do not overlap batches)
+    p, tt, sp = ctx.solve(n, m, recs, samples=samples, with_samples=True, max_device_bytes=cap,
+                          chain_recurrences=False)
     st = ctx.last_stats()
     p2, tt2, sp2 = ctx.solve(n, m, recs, samples=samples, with_samples=True, max_device_bytes=cap,
-                             overlap_bases=False)
+                             overlap_bases=False, chain_recurrences=False)
     print(f"[mid] batches {st['basis_batches']} (overlapped {st['overlap_batches']})", flush=True)
     assert st["third_stage_instances"] == 4 and st["basis_batches"] >= 2 and st["overlap_batches"] >= 1
     assert np.array_equal(tt, tt2) and np.array_equal(sp, sp2)
