@@ -182,7 +182,7 @@ def main() -> int:
         step(k)
     keys = ("amp_terms", "gpu_ms", "sweep_ms", "sweeps", "term_units", "algorithmic_bytes", "required_bytes",
             "aux_ms", "aux_bytes", "aux_launches", "basis_ms", "basis_bytes", "basis_launches", "gemm_ms",
-            "gemm_flops", "gemm_useful_flops", "gemm_launches", "gemm_bytes")
+            "gemm_flops", "gemm_useful_flops", "gemm_launches", "gemm_bytes", "chain_launches", "chain_steps")
     agg = {key: 0.0 for key in keys}
     t0 = time.perf_counter()
     probs = []
@@ -202,7 +202,10 @@ def main() -> int:
     def roofline_of(agg):
         # Each kernel class, event-timed on the stream it runs on:
         #  - term sweeps: algorithmic bytes 36 B per amplitude per complex H-apply;
-        #  - basis recurrences (the same term kernel on real halves): 16 B per vector and amplitude;
+        #  - basis recurrences: on chip (chain_kernel, one launch per batch of chains: each output
+        #    written, the diagonal and the seed read once per chain: 4 B per vector and amplitude
+        #    and 4-8 B per chain), or the term kernel on real halves, one launch per step (16 B per
+        #    vector and amplitude: s_k, s_{k-1}, the diagonal read, s_{k+1} written);
         #  - second-stage GEMM: basis rows read once per tile and outputs written (HBM), and the
         #    matrix-core flops it executes (three FP16 products per FP32 product when split).
         split = args.gemm == "split_fp16"
@@ -216,10 +219,15 @@ def main() -> int:
                                work=agg["algorithmic_bytes"],
                                achieved=rate(agg["algorithmic_bytes"], agg["sweep_ms"]),
                                profile="half_term_kernel_n20_m3_traffic.md"),
-            "basis": dict(kernel="half_term_kernel (mode 3, real halves)", bound="hbm", unit="GB/s",
-                          peak=HBM_PEAK_GBS, ms=agg["basis_ms"], launches=int(agg["basis_launches"]),
-                          work=agg["basis_bytes"], achieved=rate(agg["basis_bytes"], agg["basis_ms"]),
-                          profile="basis_recurrence_n20_m3_traffic.md"),
+            "basis": (dict(kernel="chain_kernel (on-chip recurrences, a group of one XCD per chain)", bound="hbm",
+                           unit="GB/s", peak=HBM_PEAK_GBS, ms=agg["basis_ms"], launches=int(agg["basis_launches"]),
+                           work=agg["basis_bytes"], achieved=rate(agg["basis_bytes"], agg["basis_ms"]),
+                           profile="chain_kernel_n20_m3_traffic.md")
+                      if agg["chain_launches"] > 0 else
+                      dict(kernel="half_term_kernel (mode 3, real halves)", bound="hbm", unit="GB/s",
+                           peak=HBM_PEAK_GBS, ms=agg["basis_ms"], launches=int(agg["basis_launches"]),
+                           work=agg["basis_bytes"], achieved=rate(agg["basis_bytes"], agg["basis_ms"]),
+                           profile="basis_recurrence_n20_m3_traffic.md")),
             "gemm": dict(kernel=("combine2h_kernel (v_mfma_f32_32x32x16_f16, split FP16 operands)" if split
                                  else "combine2_kernel (v_mfma_f32_32x32x2f32)"),
                          bound="hbm" if gemm_hbm_bound else "mfma", unit="GB/s" if gemm_hbm_bound else "TFLOP/s",

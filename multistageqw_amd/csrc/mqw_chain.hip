@@ -38,33 +38,36 @@ constexpr int kWgBits = 15;             // amplitudes per workgroup, log2
 constexpr int kPer = 1 << (kWgBits - kTBits); // per thread: 64
 constexpr int kPlane = 4 << kTBits;     // element c * 2048 + 4 t + i of thread t is its value 4 c + i
 constexpr int kHalf = kPer / 8;          // float4 loads of half a thread's values
+constexpr int kParked = 4;               // planes of s_{k-1} kept in LDS beside the exchange buffer
+constexpr int kKept = kPer / 4 - kParked; // planes of s_{k-1} kept in registers
 constexpr uint32_t kSpinMax = 1u << 21; // polls of a wait (each sleeps about 64 clocks and reads L2)
 constexpr uint32_t kSyncStride = 32;    // words between groups' counters (128 bytes)
 
-__device__ __forceinline__ f4 ld4(const float* p) { return *(const MQW_GLOBAL f4*)p; }
-__device__ __forceinline__ f4 ld4_nt(const float* p) { return __builtin_nontemporal_load((const MQW_GLOBAL f4*)p); }
-__device__ __forceinline__ void st4(float* p, f4 v) { *(MQW_GLOBAL f4*)p = v; }
+// Global accesses are buffer loads and stores on one workgroup's region of
+// 2^15 floats (a uniform descriptor; the lane's byte offset in one register,
+// the plane's in a scalar one).  The hardware checks every access against the
+// region: out of it, a load returns 0 and a store is dropped.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000; // gfx9 raw buffer: 32-bit data, no format conversion
+constexpr int kNonTemporal = 2;        // cache policy bit of a streamed access
+__device__ __forceinline__ rsrc_t region(const float* base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)(4u << kWgBits), kRsrcWord3);
+}
 
 // Planes [C0, C0 + NC) of a thread's values (NC float4 loads in flight,
 // static indices): v[.] = p[.] (MODE 0), v[.] += p[.] (1), v[.] -= a * p[.]
-// (2).  Global memory: base is uniform (a scalar register pair) and lo = 4 t
-// the lane's element offset, so every load is a scalar base plus one 32-bit
-// lane offset; NT: non-temporal.  LDS: base is the lane's own address.  The
-// callers put scheduling barriers between parts, so a later part's loads are
-// not hoisted above an earlier part's uses.
-template <int C0, int NC, int MODE, bool NT, bool LDS>
-__device__ __forceinline__ void part_op(float (&v)[kPer], const float* base, uint32_t lo, float a)
+// (2).  Global: the region r at the lane's byte offset boff (NT: streamed).
+// LDS: base is the lane's own address.  The callers put scheduling barriers
+// between parts, so a later part's loads are not hoisted above an earlier
+// part's uses.
+template <int C0, int NC, int MODE, bool NT>
+__device__ __forceinline__ void part_glob(float (&v)[kPer], rsrc_t r, uint32_t boff, float a)
 {
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int cc = C0 + c;
-        f4 w;
-        if (LDS) {
-            w = *reinterpret_cast<const f4*>(base + cc * kPlane);
-        } else {
-            const float* pc = base + cc * kPlane;
-            w = NT ? ld4_nt(pc + lo) : ld4(pc + lo);
-        }
+        const f4 w = __builtin_amdgcn_raw_buffer_load_b128(r, boff, cc * kPlane * 4, NT ? kNonTemporal : 0);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             if (MODE == 0) v[4 * cc + i] = w[i];
@@ -73,36 +76,85 @@ __device__ __forceinline__ void part_op(float (&v)[kPer], const float* base, uin
         }
     }
 }
-template <int MODE, bool NT, bool LDS>
-__device__ __forceinline__ void both_halves(float (&v)[kPer], const float* base, uint32_t lo = 0, float a = 0.0f)
+template <int C0, int NC>
+__device__ __forceinline__ void part_lds_add(float (&v)[kPer], const float* base)
 {
-    part_op<0, kHalf, MODE, NT, LDS>(v, base, lo, a);
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int cc = C0 + c;
+        const f4 w = *reinterpret_cast<const f4*>(base + cc * kPlane);
+#pragma unroll
+        for (int i = 0; i < 4; i++) v[4 * cc + i] += w[i];
+    }
+}
+template <int MODE, bool NT>
+__device__ __forceinline__ void glob_halves(float (&v)[kPer], rsrc_t r, uint32_t boff, float a = 0.0f)
+{
+    part_glob<0, kHalf, MODE, NT>(v, r, boff, a);
     __builtin_amdgcn_sched_barrier(0);
-    part_op<kHalf, kHalf, MODE, NT, LDS>(v, base, lo, a);
+    part_glob<kHalf, kHalf, MODE, NT>(v, r, boff, a);
     __builtin_amdgcn_sched_barrier(0);
 }
 // The same in quarters (16 registers in flight), where registers are short.
-template <int MODE, bool NT, bool LDS>
-__device__ __forceinline__ void four_quarters(float (&v)[kPer], const float* base, uint32_t lo = 0, float a = 0.0f)
+template <int MODE, bool NT>
+__device__ __forceinline__ void glob_quarters(float (&v)[kPer], rsrc_t r, uint32_t boff, float a = 0.0f)
 {
     constexpr int Q = kPer / 16;
-    part_op<0, Q, MODE, NT, LDS>(v, base, lo, a);
+    part_glob<0, Q, MODE, NT>(v, r, boff, a);
     __builtin_amdgcn_sched_barrier(0);
-    part_op<Q, Q, MODE, NT, LDS>(v, base, lo, a);
+    part_glob<Q, Q, MODE, NT>(v, r, boff, a);
     __builtin_amdgcn_sched_barrier(0);
-    part_op<2 * Q, Q, MODE, NT, LDS>(v, base, lo, a);
+    part_glob<2 * Q, Q, MODE, NT>(v, r, boff, a);
     __builtin_amdgcn_sched_barrier(0);
-    part_op<3 * Q, Q, MODE, NT, LDS>(v, base, lo, a);
+    part_glob<3 * Q, Q, MODE, NT>(v, r, boff, a);
+    __builtin_amdgcn_sched_barrier(0);
+}
+__device__ __forceinline__ void lds_quarters_add(float (&v)[kPer], const float* base)
+{
+    constexpr int Q = kPer / 16;
+    part_lds_add<0, Q>(v, base);
+    __builtin_amdgcn_sched_barrier(0);
+    part_lds_add<Q, Q>(v, base);
+    __builtin_amdgcn_sched_barrier(0);
+    part_lds_add<2 * Q, Q>(v, base);
+    __builtin_amdgcn_sched_barrier(0);
+    part_lds_add<3 * Q, Q>(v, base);
     __builtin_amdgcn_sched_barrier(0);
 }
 
-// Half H (planes [8 H, 8 H + 8)) of the step's end, in place in acc: bit 11
-// from this thread's own s_k (in LDS at own), then a1 (ae s_k - be sum) with
-// ae = (alpha E) hf = al E, the per-step kernel's expression and values, then
-// - a2 s_{k-1}: s2[q] (S2 = 1), the constant c2 (S2 = 2) or nothing (S2 = 0).
-template <int H, int S2>
-__device__ __forceinline__ void own_final(float (&acc)[kPer], const float (&e)[kPer], const float (&s2)[kPer],
-                                          const float* own, float a1, float hf, float be, float a2, float c2)
+// Amplitude bits 11 then 14 (planes c ^ 1, c ^ 8) for the four planes with
+// bits 1, 2 equal to B (amplitude bits 12, 13), from this thread's own s_k in
+// LDS at own: 16 values in registers.
+template <int B>
+__device__ __forceinline__ void own_bits(float (&acc)[kPer], const float* own)
+{
+    float x[16]; // plane c(m) = (m & 1) | B << 1 | (m >> 1) << 3 at 4 m
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int c = (m & 1) | (B << 1) | ((m >> 1) << 3);
+        const f4 w = *reinterpret_cast<const f4*>(own + c * kPlane);
+#pragma unroll
+        for (int i = 0; i < 4; i++) x[4 * m + i] = w[i];
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int c = (m & 1) | (B << 1) | ((m >> 1) << 3);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            acc[4 * c + i] += x[4 * (m ^ 1) + i]; // plane c ^ 1
+            acc[4 * c + i] += x[4 * (m ^ 2) + i]; // plane c ^ 8
+        }
+    }
+}
+
+// Half H (planes [8 H, 8 H + 8)) of the step's end, in place in acc:
+// a1 (ae s_k - be sum) with ae = (alpha E) hf = al E, the per-step kernel's
+// expression and values, then - a2 s_{k-1}: planes below kKept from s2, the
+// others from this thread's parked planes in LDS (park).
+template <int H>
+__device__ __forceinline__ void own_final(float (&acc)[kPer], const float (&e)[kPer], const float (&s2)[4 * kKept],
+                                          const float* own, const float* park, float a1, float hf, float be,
+                                          float a2)
 {
     float x[4 * kHalf];
 #pragma unroll
@@ -112,17 +164,17 @@ __device__ __forceinline__ void own_final(float (&acc)[kPer], const float (&e)[k
         for (int i = 0; i < 4; i++) x[4 * c + i] = w[i];
     }
 #pragma unroll
-    for (int r = 0; r < 4 * kHalf; r++) {
-        const int q = 4 * H * kHalf + r;
-        acc[q] += x[r ^ 4]; // value q ^ 4: the same half
-    }
+    for (int c = 0; c < kHalf; c++) {
+        const int cc = H * kHalf + c;
+        f4 y;
+        if (cc >= kKept) y = *reinterpret_cast<const f4*>(park + (cc - kKept) * kPlane);
 #pragma unroll
-    for (int r = 0; r < 4 * kHalf; r++) {
-        const int q = 4 * H * kHalf + r;
-        float o = a1 * (e[q] * hf * x[r] - be * acc[q]);
-        if (S2 == 1) o -= a2 * s2[q];
-        if (S2 == 2) o -= a2 * c2;
-        acc[q] = o;
+        for (int i = 0; i < 4; i++) {
+            const int q = 4 * cc;
+            float o = a1 * (e[q + i] * hf * x[4 * c + i] - be * acc[q + i]);
+            o -= a2 * (cc < kKept ? s2[q + i] : y[i]);
+            acc[q + i] = o;
+        }
     }
 }
 
@@ -146,25 +198,29 @@ __device__ __forceinline__ void group_signal(uint32_t* cnt)
     if (threadIdx.x == 0) __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Until the group counter reaches target; false when the launch aborts.
-// Every wave polls for itself (one request per wave) with uniform values, so
-// the loop is no divergent region inside the kernel's large live ranges.
-__device__ __forceinline__ bool group_wait(uint32_t* sync, uint32_t* err, const uint32_t* cnt, uint32_t target)
+// Until the group counter reaches target, or the launch aborts (then every
+// later wait returns at once: the workgroup runs out its steps without
+// waiting, every access inside its regions, and the host discards the
+// launch's outputs).  Every wave polls for itself (one request per wave) with
+// uniform values, so the loop is no divergent region inside the kernel's
+// large live ranges, and no exit leaves the step loop early.
+__device__ __forceinline__ void group_wait(uint32_t* sync, uint32_t* err, const uint32_t* cnt, uint32_t target)
 {
     for (uint32_t spins = 0;; spins++) {
-        if (__builtin_amdgcn_readfirstlane(ld_agent(cnt)) >= target) return true;
-        if (__builtin_amdgcn_readfirstlane(ld_agent(sync)) != 0u) return false;
+        if (__builtin_amdgcn_readfirstlane(ld_agent(cnt)) >= target) return;
+        if (__builtin_amdgcn_readfirstlane(ld_agent(sync)) != 0u) return;
         if (spins >= kSpinMax) {
             abort_launch(sync, err, 1u);
-            return false;
+            return;
         }
         __builtin_amdgcn_s_sleep(1);
     }
 }
 
-// The in-thread flips of the planes c of parity P (amplitude bit 11 = P):
-// acc[q] = (((v[q^1] + v[q^2]) + v[q^8]) + v[q^16]) + v[q^32] from 0, with v
-// this thread's s_k read from LDS at own (all operands have the same parity).
+// The in-thread flips of amplitude bits 0, 1, 12, 13 for the planes c of
+// parity P (amplitude bit 11 = P): acc[q] = ((v[q^1] + v[q^2]) + v[q^8]) +
+// v[q^16] from 0, with v this thread's s_k read from LDS at own (all operands
+// have the same parity).
 template <int P>
 __device__ __forceinline__ void in_thread(float (&acc)[kPer], const float* own)
 {
@@ -180,33 +236,33 @@ __device__ __forceinline__ void in_thread(float (&acc)[kPer], const float* own)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int q = 4 * (2 * m + P) + i, r = 4 * m + i;
-            // q ^ 1, q ^ 2: r ^ 1, r ^ 2; q ^ 8, 16, 32 (planes c ^ 2, 4, 8): r ^ 4, 8, 16
+            // q ^ 1, q ^ 2: r ^ 1, r ^ 2; q ^ 8, 16 (planes c ^ 2, 4): r ^ 4, 8
             acc[q] = 0.0f;
             acc[q] += x[r ^ 1];
             acc[q] += x[r ^ 2];
             acc[q] += x[r ^ 4];
             acc[q] += x[r ^ 8];
-            acc[q] += x[r ^ 16];
         }
 }
 
-// One step's output: plane c of the lane at base + c * kPlane + lo (cached
+// One step's output: plane c of the lane at base + c * kPlane + 4 t (cached
 // stores: the other members read them from L2 as the next step's partners).
-__device__ __forceinline__ void store_step(const float (&v)[kPer], float* base, uint32_t lo)
+__device__ __forceinline__ void store_step(const float (&v)[kPer], float* base, uint32_t boff)
 {
+    const rsrc_t r = region(base);
 #pragma unroll
     for (int c = 0; c < kPer / 4; c++) {
         f4 w;
 #pragma unroll
         for (int i = 0; i < 4; i++) w[i] = v[4 * c + i];
-        st4(base + c * kPlane + lo, w);
+        __builtin_amdgcn_raw_buffer_store_b128(w, r, boff, c * kPlane * 4, 0);
     }
 }
 
 // The end of a step: the group's signal (which also orders this step's LDS
 // reads before the next step's writes), or a workgroup barrier alone.
 template <int G>
-__device__ __forceinline__ bool step_done(uint32_t* cnt, uint32_t& gen)
+__device__ __forceinline__ void step_done(uint32_t* cnt, uint32_t& gen)
 {
     if constexpr (G > 1) {
         group_signal(cnt);
@@ -214,7 +270,6 @@ __device__ __forceinline__ bool step_done(uint32_t* cnt, uint32_t& gen)
     } else {
         __syncthreads();
     }
-    return true;
 }
 
 template <int GB>
@@ -225,13 +280,14 @@ chain_kernel(const ChainJob* __restrict__ jobs, const uint32_t* __restrict__ ord
     constexpr int G = 1 << GB;
     constexpr int n = kWgBits + GB;
     __shared__ __attribute__((aligned(16))) float s_x[1 << kWgBits];
+    __shared__ __attribute__((aligned(16))) float s_park[kParked * kPlane]; // 32 KiB: 160 KiB in all
     const uint32_t b = blockIdx.x, xcd = b & 7u, slot = b >> 3;
     const uint32_t grp = xcd * (32u >> GB) + (slot >> GB), mem = slot & (G - 1);
     uint32_t* cnt = sync + kSyncStride * (1u + grp);
     const int t = threadIdx.x;
     const uint64_t N = 1ull << n;
     const uint64_t mbase = (uint64_t)mem << kWgBits;
-    const uint32_t lo = 4u * (uint32_t)t; // the lane's element offset within a plane
+    const uint32_t boff = 16u * (uint32_t)t; // the lane's byte offset within a plane
     uint32_t gen = 0; // signals of every member so far
     if constexpr (G > 1) {
         // every member on one XCD: the XCD ids of the group, one bit each
@@ -242,7 +298,8 @@ chain_kernel(const ChainJob* __restrict__ jobs, const uint32_t* __restrict__ ord
         }
         group_signal(cnt);
         gen++;
-        if (!group_wait(sync, err, cnt, G * gen)) return;
+        group_wait(sync, err, cnt, G * gen);
+        if (__builtin_amdgcn_readfirstlane(ld_agent(sync)) != 0u) return;
         if (__popc(__builtin_amdgcn_readfirstlane(ld_agent(cnt + 1))) != 1) {
             abort_launch(sync, err, 2u);
             return;
@@ -258,7 +315,7 @@ chain_kernel(const ChainJob* __restrict__ jobs, const uint32_t* __restrict__ ord
         // where this thread reads its own s_{k-1} back from it before storing
         // s_{k+1}: HBM sees the outputs written and nothing read per step.
         float e[kPer], v[kPer];
-        both_halves<0, true, false>(e, J.E + mbase, lo);
+        glob_halves<0, true>(e, region(J.E + mbase), boff);
 #pragma unroll
         for (int q = 0; q < kPer; q++) e[q] *= J.alpha;
         uint32_t k = 0;
@@ -267,31 +324,41 @@ chain_kernel(const ChainJob* __restrict__ jobs, const uint32_t* __restrict__ ord
             const float base = J.beta0 * (float)n * J.psi0;
 #pragma unroll
             for (int q = 0; q < kPer; q++) v[q] = J.a1 * (e[q] * 0.5f * J.psi0 - base);
-            store_step(v, J.out + mbase, lo);
-            if (!step_done<G>(cnt, gen)) return;
+            store_step(v, J.out + mbase, boff);
+            step_done<G>(cnt, gen);
             k = 1;
         } else {
-            both_halves<0, false, false>(v, J.seed + mbase, lo);
+            glob_halves<0, false>(v, region(J.seed + mbase), boff);
         }
         float* const own = s_x + 4 * t;
+        float* const park = s_park + 4 * t;
         for (; k < J.steps; k++) {
             const float hf = k == 0 ? 0.5f : 1.0f, be = k == 0 ? J.beta0 : J.beta;
             const float* src = k == 0 ? J.seed : J.out + (uint64_t)(k - 1) * N;
             // s_{k-1}: this thread's values in LDS since the previous step (the
-            // seed at k = 1; |+> is the constant psi0).  At k = 0 there is none:
-            // subtracting a2 * 0 leaves every value (signed zeros too) as it is.
-            float s2[kPer];
+            // seed at k = 1; |+> is the constant psi0), planes below kKept to
+            // registers, the others parked in LDS beside the exchange buffer.
+            // At k = 0 there is none: subtracting a2 * 0 leaves every value
+            // (signed zeros too) as it is.
+            float s2[4 * kKept];
             if (k > 0 && !(k == 1 && cst)) {
 #pragma unroll
                 for (int c = 0; c < kPer / 4; c++) {
                     const f4 w = *reinterpret_cast<const f4*>(own + c * kPlane);
+                    if (c < kKept) {
 #pragma unroll
-                    for (int i = 0; i < 4; i++) s2[4 * c + i] = w[i];
+                        for (int i = 0; i < 4; i++) s2[4 * c + i] = w[i];
+                    } else {
+                        *reinterpret_cast<f4*>(park + (c - kKept) * kPlane) = w;
+                    }
                 }
             } else {
                 const float c2 = k == 1 ? J.psi0 : 0.0f;
 #pragma unroll
-                for (int q = 0; q < kPer; q++) s2[q] = c2;
+                for (int q = 0; q < 4 * kKept; q++) s2[q] = c2;
+                const f4 w = {c2, c2, c2, c2};
+#pragma unroll
+                for (int c = 0; c < kParked; c++) *reinterpret_cast<f4*>(park + c * kPlane) = w;
             }
             // s_k to LDS (own positions: no other thread reads them before the barrier)
 #pragma unroll
@@ -303,34 +370,46 @@ chain_kernel(const ChainJob* __restrict__ jobs, const uint32_t* __restrict__ ord
             }
             asm volatile("" ::: "memory");
             float acc[kPer];
-            // register bits (value index q = 4 c + i: q bits 0, 1 are amplitude
-            // bits 0, 1; q bits 2 .. 5 are amplitude bits 11 .. 14), in the order
-            // of the per-step kernel: bits 0, 1, 12, 13, 14 here, bit 11 last.
-            // Per parity of the plane c (bit 11), from this thread's s_k in LDS.
+            // The flips in the order of the per-step kernel with 2^14-amplitude
+            // tiles (its own flips, then its partner tiles in bit order):
+            // amplitude bits 0, 1, 12, 13 (registers; value index q = 4 c + i:
+            // q bits 0, 1 are amplitude bits 0, 1, q bits 2 .. 5 amplitude bits
+            // 11 .. 14), 2 .. 10 (thread bits, through LDS), 11 and 14 (own
+            // planes), then the other members' 15 .. n - 1.  Per parity of the
+            // plane c (bit 11), from this thread's s_k in LDS.
             in_thread<0>(acc, own);
             __builtin_amdgcn_sched_barrier(0);
             in_thread<1>(acc, own);
 #pragma unroll
             for (int q = 0; q < kPer; q++) asm volatile("" : "+v"(acc[q]));
             __builtin_amdgcn_sched_barrier(0);
-            // the other members' s_k for bits 15 .. n - 1, in bit order
-            if (G > 1 && k > 0)
-                if (!group_wait(sync, err, cnt, G * gen)) return;
-#pragma unroll
-            for (int j = 0; j < GB; j++) four_quarters<1, false, false>(acc, src + (mbase ^ (1ull << (kWgBits + j))), lo);
             __syncthreads();
-            // thread bits: amplitude bits 2 .. 10, through LDS
 #pragma unroll 1
-            for (int bb = 0; bb < kTBits; bb++) both_halves<1, false, true>(acc, s_x + 4 * (t ^ (1 << bb)));
-            // bit 11, the output and - a2 s_{k-1}, half by half, in place in acc
-            own_final<0, 1>(acc, e, s2, own, J.a1, hf, be, J.a2, 0.0f);
+            for (int bb = 0; bb < kTBits; bb++) lds_quarters_add(acc, s_x + 4 * (t ^ (1 << bb)));
+            own_bits<0>(acc, own);
             __builtin_amdgcn_sched_barrier(0);
-            own_final<1, 1>(acc, e, s2, own, J.a1, hf, be, J.a2, 0.0f);
+            own_bits<1>(acc, own);
+            __builtin_amdgcn_sched_barrier(0);
+            own_bits<2>(acc, own);
+            __builtin_amdgcn_sched_barrier(0);
+            own_bits<3>(acc, own);
+            __builtin_amdgcn_sched_barrier(0);
+            // the other members' s_k: their signals have had the LDS phase to arrive
+            // (memory clobbers: no LDS access moves across the wait)
+            asm volatile("" ::: "memory");
+            if (G > 1 && k > 0) group_wait(sync, err, cnt, G * gen);
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int j = 0; j < GB; j++) glob_quarters<1, false>(acc, region(src + (mbase ^ (1ull << (kWgBits + j)))), boff);
+            // the output and - a2 s_{k-1}, half by half, in place in acc
+            own_final<0>(acc, e, s2, own, park, J.a1, hf, be, J.a2);
+            __builtin_amdgcn_sched_barrier(0);
+            own_final<1>(acc, e, s2, own, park, J.a1, hf, be, J.a2);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < kPer; q++) v[q] = acc[q];
-            store_step(v, J.out + (uint64_t)k * N + mbase, lo);
-            if (!step_done<G>(cnt, gen)) return;
+            store_step(v, J.out + (uint64_t)k * N + mbase, boff);
+            step_done<G>(cnt, gen);
         }
     }
 }

@@ -1264,6 +1264,8 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
             return fail(c, MQW_EDEVICE, "fill: %s", last_kernel_error());
     }
     Plan<float> P;
+    // one launch per step where the on-chip recurrences apply sums in their order
+    const uint32_t late = n >= kChainMinN && n <= kChainMaxN ? (uint32_t)F_LATE : 0u;
     auto job = [&](size_t p, const Segment& sg, int q) {
         TermJob<float> j{};
         j.E = E + p * N;
@@ -1419,7 +1421,7 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
                     b += 4.0;
                 }
             }
-            j.flags = flags;
+            j.flags = flags | late;
             P.jobs.push_back(j);
             op.count++;
             bytes += b * N;
@@ -1449,7 +1451,7 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
                         bytes += 16.0 * N;
                     }
                 }
-                j.flags = flags;
+                j.flags = flags | late;
                 P.jobs.push_back(j);
                 op.count++;
             }
@@ -1468,7 +1470,7 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
                     j.w2r = 1.0f;
                     bytes += 16.0 * N;
                 }
-                j.flags = flags;
+                j.flags = flags | late;
                 P.jobs.push_back(j);
                 op.count++;
             }

@@ -65,6 +65,8 @@ enum : uint32_t {
     F_NORM_IN = 1u << 6,   // Psi's norm from norm_in (ntiles partial sums)
     F_READOUT = 1u << 7,   // tile 0 writes |Psi[eloc]|^2 / ||Psi||^2 to *prob
     F_HSUM = 1u << 8,      // s1's flips over its top `hi` bits are read from hsum
+    F_LATE = 1u << 9,      // real halves (mode 3): partner tiles after the tile's own flips,
+                           // the order of the on-chip recurrences (launch_chains)
 };
 
 template <typename real>

@@ -832,25 +832,32 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
 #pragma unroll
                 for (int q = 0; q < R; q++) acc[q] += x[q ^ (1 << b)];
             // partner tiles in bit order, UNR loads in flight per thread; with
-            // F_HSUM the top hi bits come from the high-bit pass instead
+            // F_HSUM the top hi bits come from the high-bit pass instead.  Mode 3
+            // jobs flagged F_LATE (the basis recurrences where the on-chip ones
+            // apply) add them after the tile's own flips, in their order
+            // (mqw_chain.hip), so the two give bit-identical outputs.
             const int pl = (flags & F_HSUM) ? pb - hi : pb;
-            int j = 0;
-            for (; j + UNR <= pl; j += UNR) {
-                float y[UNR][R];
+            auto partners = [&]() {
+                int j = 0;
+                for (; j + UNR <= pl; j += UNR) {
+                    float y[UNR][R];
 #pragma unroll
-                for (int u = 0; u < UNR; u++)
-                    vload<float, R>(src + ((tile ^ (1ull << (j + u))) << HB) + (uint64_t)t * 4, y[u], PS);
+                    for (int u = 0; u < UNR; u++)
+                        vload<float, R>(src + ((tile ^ (1ull << (j + u))) << HB) + (uint64_t)t * 4, y[u], PS);
 #pragma unroll
-                for (int u = 0; u < UNR; u++)
+                    for (int u = 0; u < UNR; u++)
 #pragma unroll
-                    for (int q = 0; q < R; q++) acc[q] += y[u][q];
-            }
-            for (; j < pl; j++) {
-                float y[R];
-                vload<float, R>(src + ((tile ^ (1ull << j)) << HB) + (uint64_t)t * 4, y, PS);
+                        for (int q = 0; q < R; q++) acc[q] += y[u][q];
+                }
+                for (; j < pl; j++) {
+                    float y[R];
+                    vload<float, R>(src + ((tile ^ (1ull << j)) << HB) + (uint64_t)t * 4, y, PS);
 #pragma unroll
-                for (int q = 0; q < R; q++) acc[q] += y[q];
-            }
+                    for (int q = 0; q < R; q++) acc[q] += y[q];
+                }
+            };
+            const bool late = MODE == 3 && (flags & F_LATE) != 0;
+            if (!late) partners();
             if (flags & F_HSUM) {
                 float y[R];
                 vload_nt<float, R>(J.hsum + (uint64_t)h1 * N + off, y, PS);
@@ -880,6 +887,7 @@ half_term_kernel(const TermJob<float>* __restrict__ jobs, uint32_t njobs, int n,
 #pragma unroll
                 for (int q = 0; q < R; q++) acc[q] += y[q];
             }
+            if (late) partners();
 #pragma unroll
             for (int q = 0; q < R; q++) o[q] = a1 * (J.alpha * e[q] * x[q] - J.beta * acc[q]);
         }
