@@ -204,8 +204,8 @@ def main() -> int:
         #  - term sweeps: algorithmic bytes 36 B per amplitude per complex H-apply;
         #  - basis recurrences: on chip (chain_kernel, one launch per batch of chains: each output
         #    written, the diagonal and the seed read once per chain: 4 B per vector and amplitude
-        #    and 4-8 B per chain), or the term kernel on real halves, one launch per step (16 B per
-        #    vector and amplitude: s_k, s_{k-1}, the diagonal read, s_{k+1} written);
+        #    and 4-8 B per chain and amplitude), or the term kernel on real halves, one launch per
+        #    step (16 B per vector and amplitude: s_k, s_{k-1}, the diagonal read, s_{k+1} written);
         #  - second-stage GEMM: basis rows read once per tile and outputs written (HBM), and the
         #    matrix-core flops it executes (three FP16 products per FP32 product when split).
         split = args.gemm == "split_fp16"

@@ -1304,11 +1304,9 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
             cj.psi0 = psi0;
             cj.steps = (uint32_t)nsteps;
             P.chains.push_back(cj);
-            // HBM: the diagonal and the seed once, each output written, and
-            // s_{k-1} read back at every step k >= 1 (at k = 1 the seed, if an
-            // array; |+> is a constant)
-            const double back = nsteps >= 2 ? (double)(nsteps - 2) + (seed ? 1.0 : 0.0) : 0.0;
-            cbytes += 4.0 * N * (1.0 + (seed ? 1.0 : 0.0) + nsteps + back);
+            // HBM: the diagonal and the seed read once, each output written
+            // (s_{k-1} comes back from LDS, the partners from the XCD's L2)
+            cbytes += 4.0 * N * (1.0 + (seed ? 1.0 : 0.0) + nsteps);
             csteps += (uint64_t)nsteps;
         };
         for (size_t p = 0; p < ps.size(); p++) {
@@ -1788,6 +1786,13 @@ mqw_status compress_combine(mqw_ctx* c, int n, const SweepCfg& k, const std::vec
     std::vector<CombineJob> jb, jb2;
     std::vector<CombineItem> ci, ci2;
     std::vector<float> cf;
+    // These tables are copied to the device from pageable memory, which the
+    // runtime may read when it carries the copies out, after this call:
+    // the stream is drained on every way out, before they are freed.
+    struct Drain {
+        hipStream_t s;
+        ~Drain() { (void)hipStreamSynchronize(s); }
+    } drain{c->stream};
     uint32_t kcb_max = 0;
     for (size_t i = 0; i < cj.size(); i++) {
         const Compression& j = cj[i];
