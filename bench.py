@@ -263,6 +263,11 @@ def main() -> int:
                                 for c, d in classes.items()}}
         for c in roofline["classes"].values():
             c["frac"] = c["achieved"] / c["peak"] if c["peak"] else 0.0
+        # the counted HBM traffic per launch over the same launch time: what the
+        # memory system moved, where `achieved` counts the algorithm's bytes
+        if traffic is not None and dom["unit"] == "GB/s" and roofline["avg_launch_ms"] > 0:
+            roofline["traffic_gbs"] = traffic / (roofline["avg_launch_ms"] * 1e-3) / 1e9
+            roofline["traffic_frac"] = roofline["traffic_gbs"] / dom["peak"]
         return roofline
 
     roofline = roofline_of(agg)
