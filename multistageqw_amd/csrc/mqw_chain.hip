@@ -10,15 +10,19 @@
 // top n - 15 bits are the other members' outputs, which the XCD's L2 still
 // holds (4 MiB at n = 20).  Per step and amplitude HBM sees the output written
 // and nothing read: 4 bytes, against 16 for a launch per step (s_k, s_{k-1},
-// E read, s_{k+1} written) and the L2 misses of its partner tiles.
+// E read, s_{k+1} written) and the L2 misses of its partner tiles.  (At n = 20
+// a step's s_k and s_{k+1} are twice the L2, and partners that miss it add
+// about 5 bytes of reads.)
 //
 // Workgroup b runs on XCD b % 8 (the round-robin dispatch of a 256-CU part):
 // the 32 of one XCD form 32 / G groups of consecutive slots.  Every member
 // signals a group counter after its step's stores have completed (they are in
 // the XCD's L2) and waits for all G signals before it reads partners; the
 // waits are bounded.  A wait that runs out, or a group whose members report
-// different XCD ids, sets the launch's abort flag (every member then stops)
-// and the error word the host reads after the plan.  The arithmetic and its
+// different XCD ids, sets the launch's abort flag (every member then runs out
+// its steps without waiting, all accesses inside its regions) and the error
+// word the host reads after the plan; the host then discards the outputs and
+// runs the recurrences one launch per step.  The arithmetic and its
 // order are the per-step kernel's with 2^14-amplitude tiles (half_term_kernel
 // mode 3): the outputs are bit-identical to it.
 #include <hip/hip_runtime.h>
