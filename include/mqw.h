@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MQW_ABI_VERSION 17
+#define MQW_ABI_VERSION 18
 #define MQW_DEFAULT_SAMPLES 100 /* MultiQW.cpp:257 */
 #define MQW_DEFAULT_TOL 1e-6    /* MultiQW.cpp:440 */
 #define MQW_DEFAULT_PROBLEMS 2000 /* MultiQW.cpp:252 */
@@ -168,6 +168,11 @@ typedef struct {
                                 traffic per vector and amplitude).  Results are bit-identical to 0 = one
                                 launch per recurrence step.  Batches of bases then run one after the
                                 other (no second stream: two such launches must not share the device). */
+    int chain_wait_polls;    /* 0 = default (2^21): reads of a group counter by one wait between the
+                                workgroups of an on-chip recurrence launch before it gives up; the launch
+                                then ends without waiting, and the context runs its recurrences one launch
+                                per step from then on.  -1 = no read: every such wait gives up at once,
+                                which forces that fallback (its tests); below -1 = MQW_EINVAL. */
 } mqw_opts;
 
 /* One problem of a batch; gamma/short_t override the heuristic when both are

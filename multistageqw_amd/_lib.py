@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmqw.so")
 CLI_PATH = os.path.join(HERE, "mqw")
 
-ABI_VERSION = 17  # MQW_ABI_VERSION of include/mqw.h this binding matches
+ABI_VERSION = 18  # MQW_ABI_VERSION of include/mqw.h this binding matches
 MAX_STAGES = 64
 DEFAULT_SAMPLES = 100
 DEFAULT_TOL = 1e-6
@@ -62,7 +62,8 @@ class Opts(C.Structure):
                 ("overlap_bases", C.c_int), ("second_stage_gemm", C.c_int), ("mixer_kind", C.c_int),
                 ("compress_bases", C.c_int), ("direct_m1", C.c_int), ("adjoint_two_stages", C.c_int),
                 ("shared_third_stage", C.c_int),
-                ("chain_recurrences", C.c_int)]
+                ("chain_recurrences", C.c_int),
+                ("chain_wait_polls", C.c_int)]
 
 
 class Instance(C.Structure):

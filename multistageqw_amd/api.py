@@ -118,7 +118,7 @@ def make_opts(samples=L.DEFAULT_SAMPLES, tol=L.DEFAULT_TOL, precision=L.FP32, en
               shared_first_stage=True, final_readout=True, shared_second_stage=True,
               overlap_bases=True, gemm="split_fp16", mixer="xstring", compress_bases=True,
               direct_m1=True, adjoint_two_stages=True, shared_third_stage=True,
-              chain_recurrences=True) -> L.Opts:
+              chain_recurrences=True, chain_wait_polls=0) -> L.Opts:
     o = L.Opts()
     L.load().mqw_opts_default(C.byref(o))
     o.samples = samples
@@ -143,6 +143,7 @@ def make_opts(samples=L.DEFAULT_SAMPLES, tol=L.DEFAULT_TOL, precision=L.FP32, en
     o.adjoint_two_stages = int(bool(adjoint_two_stages))
     o.shared_third_stage = int(bool(shared_third_stage))
     o.chain_recurrences = int(bool(chain_recurrences))
+    o.chain_wait_polls = int(chain_wait_polls)
     gemms = {"split_fp16": 0, "fp32": 1}
     if gemm not in gemms:
         raise ValueError(f"gemm must be one of {sorted(gemms)}, got {gemm!r}")

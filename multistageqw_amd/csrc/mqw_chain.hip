@@ -44,7 +44,7 @@ constexpr int kPlane = 4 << kTBits;     // element c * 2048 + 4 t + i of thread 
 constexpr int kHalf = kPer / 8;          // float4 loads of half a thread's values
 constexpr int kParked = 4;               // planes of s_{k-1} kept in LDS beside the exchange buffer
 constexpr int kKept = kPer / 4 - kParked; // planes of s_{k-1} kept in registers
-constexpr uint32_t kSpinMax = 1u << 21; // polls of a wait (each sleeps about 64 clocks and reads L2)
+constexpr uint32_t kSpinMax = 1u << 21; // default polls of a wait (each sleeps about 64 clocks and reads L2)
 constexpr uint32_t kSyncStride = 32;    // words between groups' counters (128 bytes)
 
 // Global accesses are buffer loads and stores on one workgroup's region of
@@ -205,18 +205,20 @@ __device__ __forceinline__ void group_signal(uint32_t* cnt)
 // Until the group counter reaches target, or the launch aborts (then every
 // later wait returns at once: the workgroup runs out its steps without
 // waiting, every access inside its regions, and the host discards the
-// launch's outputs).  Every wave polls for itself (one request per wave) with
+// launch's outputs), or `polls` reads found it short (then this wait aborts
+// the launch; polls = 0 gives up before the first read).  Every wave polls for itself (one request per wave) with
 // uniform values, so the loop is no divergent region inside the kernel's
 // large live ranges, and no exit leaves the step loop early.
-__device__ __forceinline__ void group_wait(uint32_t* sync, uint32_t* err, const uint32_t* cnt, uint32_t target)
+__device__ __forceinline__ void group_wait(uint32_t* sync, uint32_t* err, const uint32_t* cnt, uint32_t target,
+                                           uint32_t polls)
 {
     for (uint32_t spins = 0;; spins++) {
-        if (__builtin_amdgcn_readfirstlane(ld_agent(cnt)) >= target) return;
-        if (__builtin_amdgcn_readfirstlane(ld_agent(sync)) != 0u) return;
-        if (spins >= kSpinMax) {
+        if (spins >= polls) {
             abort_launch(sync, err, 1u);
             return;
         }
+        if (__builtin_amdgcn_readfirstlane(ld_agent(cnt)) >= target) return;
+        if (__builtin_amdgcn_readfirstlane(ld_agent(sync)) != 0u) return;
         __builtin_amdgcn_s_sleep(1);
     }
 }
@@ -279,7 +281,7 @@ __device__ __forceinline__ void step_done(uint32_t* cnt, uint32_t& gen)
 template <int GB>
 __global__ void __launch_bounds__(kThreads)
 chain_kernel(const ChainJob* __restrict__ jobs, const uint32_t* __restrict__ order, const uint32_t* __restrict__ goff,
-             uint32_t* __restrict__ sync, uint32_t* __restrict__ err)
+             uint32_t* __restrict__ sync, uint32_t* __restrict__ err, uint32_t polls)
 {
     constexpr int G = 1 << GB;
     constexpr int n = kWgBits + GB;
@@ -302,7 +304,7 @@ chain_kernel(const ChainJob* __restrict__ jobs, const uint32_t* __restrict__ ord
         }
         group_signal(cnt);
         gen++;
-        group_wait(sync, err, cnt, G * gen);
+        group_wait(sync, err, cnt, G * gen, polls);
         if (__builtin_amdgcn_readfirstlane(ld_agent(sync)) != 0u) return;
         if (__popc(__builtin_amdgcn_readfirstlane(ld_agent(cnt + 1))) != 1) {
             abort_launch(sync, err, 2u);
@@ -401,7 +403,7 @@ chain_kernel(const ChainJob* __restrict__ jobs, const uint32_t* __restrict__ ord
             // the other members' s_k: their signals have had the LDS phase to arrive
             // (memory clobbers: no LDS access moves across the wait)
             asm volatile("" ::: "memory");
-            if (G > 1 && k > 0) group_wait(sync, err, cnt, G * gen);
+            if (G > 1 && k > 0) group_wait(sync, err, cnt, G * gen, polls);
             asm volatile("" ::: "memory");
 #pragma unroll
             for (int j = 0; j < GB; j++) glob_quarters<1, false>(acc, region(src + (mbase ^ (1ull << (kWgBits + j)))), boff);
@@ -425,9 +427,10 @@ __global__ void __launch_bounds__(256) chain_sync_clear(uint32_t* sync, uint32_t
 
 template <int GB>
 void launch_gb(const ChainJob* jobs, const uint32_t* order, const uint32_t* goff, uint32_t* sync, uint32_t* err,
-               hipStream_t st)
+               uint32_t polls, hipStream_t st)
 {
-    hipLaunchKernelGGL((chain_kernel<GB>), dim3(kChainWorkgroups), dim3(kThreads), 0, st, jobs, order, goff, sync, err);
+    hipLaunchKernelGGL((chain_kernel<GB>), dim3(kChainWorkgroups), dim3(kThreads), 0, st, jobs, order, goff, sync, err,
+                       polls);
 }
 
 } // namespace
@@ -437,19 +440,20 @@ namespace mqw {
 uint32_t chain_sync_words() { return kSyncStride * (1u + kChainWorkgroups); }
 
 int launch_chains(const ChainJob* jobs_dev, const uint32_t* order_dev, const uint32_t* goff_dev, uint32_t* sync,
-                  uint32_t* err, int n, void* stream)
+                  uint32_t* err, int n, int wait_polls, void* stream)
 {
     if (n < kChainMinN || n > kChainMaxN) return -2;
+    const uint32_t polls = wait_polls == 0 ? kSpinMax : wait_polls < 0 ? 0u : (uint32_t)wait_polls;
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(chain_sync_clear, dim3(16), dim3(256), 0, st, sync, chain_sync_words());
     if (check_launch("chain_sync_clear")) return -1;
     switch (n - kWgBits) {
-    case 0: launch_gb<0>(jobs_dev, order_dev, goff_dev, sync, err, st); break;
-    case 1: launch_gb<1>(jobs_dev, order_dev, goff_dev, sync, err, st); break;
-    case 2: launch_gb<2>(jobs_dev, order_dev, goff_dev, sync, err, st); break;
-    case 3: launch_gb<3>(jobs_dev, order_dev, goff_dev, sync, err, st); break;
-    case 4: launch_gb<4>(jobs_dev, order_dev, goff_dev, sync, err, st); break;
-    default: launch_gb<5>(jobs_dev, order_dev, goff_dev, sync, err, st); break;
+    case 0: launch_gb<0>(jobs_dev, order_dev, goff_dev, sync, err, polls, st); break;
+    case 1: launch_gb<1>(jobs_dev, order_dev, goff_dev, sync, err, polls, st); break;
+    case 2: launch_gb<2>(jobs_dev, order_dev, goff_dev, sync, err, polls, st); break;
+    case 3: launch_gb<3>(jobs_dev, order_dev, goff_dev, sync, err, polls, st); break;
+    case 4: launch_gb<4>(jobs_dev, order_dev, goff_dev, sync, err, polls, st); break;
+    default: launch_gb<5>(jobs_dev, order_dev, goff_dev, sync, err, polls, st); break;
     }
     return check_launch("chain");
 }

@@ -1026,7 +1026,7 @@ mqw_status run_plan(mqw_ctx* c, int precision, int n, const SweepCfg& k, Plan<re
             else if (op.kind == 4)
                 rc = launch_half_real(k.tb, k.variant, jd + op.off, op.count, n, k.lc, stm);
             else if (op.kind == 9)
-                rc = launch_chains(chd, cord, cgo, csync, c->chain_err, n, stm);
+                rc = launch_chains(chd, cord, cgo, csync, c->chain_err, n, k.chain_polls, stm);
             else if (op.kind == 5 && op.aux3)
                 rc = launch_readout_dots_mfma(rtd + op.off, op.count, rid, cf, n, op.aux2, dots, stm);
             else if (op.kind == 5)
@@ -1280,7 +1280,8 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
     // one launch, so every chain must start from a vector ready before it
     // (no uncompressed W chain, which waits for its V_k).  The same rows, steps
     // and scalars as the per-step plan below, and bit-identical outputs.
-    bool use_chains = k.chains && !pending;
+    // (c->chain_capable: not after a launch of this context aborted)
+    bool use_chains = k.chains && c->chain_capable && !pending;
     for (size_t p = 0; use_chains && p < ps.size(); p++)
         if (has_w && B.kc2[p] > 0 && !cmp(p)) use_chains = false;
     if (use_chains) {
@@ -2185,6 +2186,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
     trace("solve_group n=%d, %zu problems", n, ps.size());
     SweepCfg k = choose_sweep(c, precision, n, o.tile_bits, o.chunk_bits, o.variant, o.high_pass);
     k.chains = o.chain_recurrences && c->chain_capable && k.half && n >= kChainMinN && n <= kChainMaxN;
+    k.chain_polls = o.chain_wait_polls;
     const bool mixer = o.mixer_count > 0;
     if (mixer) {
         // X-string mixer: its own sweep kernel, masks on the device
@@ -2754,6 +2756,8 @@ mqw_status check_opts(mqw_ctx* c, const mqw_opts& o)
         if (o.mixer_masks[q] == 0u) return fail(c, MQW_EINVAL, "mixer mask %d is empty", q);
     if (o.mixer_kind != MQW_MIXER_XSTRING && o.mixer_kind != MQW_MIXER_XY)
         return fail(c, MQW_EINVAL, "bad mixer_kind %d", o.mixer_kind);
+    if (o.chain_wait_polls < -1)
+        return fail(c, MQW_EINVAL, "chain_wait_polls must be >= -1 (got %d)", o.chain_wait_polls);
     if (o.mixer_kind == MQW_MIXER_XY)
         for (int q = 0; q < o.mixer_count; q++)
             if (__builtin_popcount(o.mixer_masks[q]) != 2)
@@ -2876,6 +2880,7 @@ void mqw_opts_default(mqw_opts* o)
     o->adjoint_two_stages = 1;
     o->shared_third_stage = 1;
     o->chain_recurrences = 1;
+    o->chain_wait_polls = 0;
     o->sample_probs = nullptr;
 }
 

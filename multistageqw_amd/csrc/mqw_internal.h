@@ -189,9 +189,11 @@ constexpr int kChainMinN = 15, kChainMaxN = 20;
 constexpr int kChainWorkgroups = 256; // 8 XCDs x 32 CUs
 uint32_t chain_sync_words();
 // sync: chain_sync_words() words the launch clears first; err: a word the
-// kernel ORs 1 (a wait ran out) or 2 (a group's members on several XCDs) into
+// kernel ORs 1 (a wait ran out) or 2 (a group's members on several XCDs) into;
+// wait_polls: reads of a group counter before a wait runs out (0: 2^21; < 0:
+// none, every wait between workgroups runs out and the launch aborts)
 int launch_chains(const ChainJob* jobs_dev, const uint32_t* order_dev, const uint32_t* goff_dev, uint32_t* sync,
-                  uint32_t* err, int n, void* stream);
+                  uint32_t* err, int n, int wait_polls, void* stream);
 // The compression of one instance from its Gram pair sums and its samples'
 // first-stage coefficients (each row kc long, zero padded): rank R per class
 // (the same for both), B (2R x kc, row 2r + cls combines class cls) and u
@@ -403,6 +405,7 @@ struct SweepCfg {
     int lc = 0;        // XCD chunk bits
     int variant = 0;   // loads-in-flight variant of the real-half kernel
     bool chains = false; // basis recurrences on chip (launch_chains) where they apply
+    int chain_polls = 0; // reads per wait of theirs (0: the kernel's default; < 0: none)
     uint32_t parts = 1; // partial norm sums per output
     int hi = 0;        // top bits summed by the high-bit pass (0 = none)
     int hwb = 0;       // its pencil rows are 64 * 2^hwb bytes

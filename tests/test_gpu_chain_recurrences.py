@@ -95,3 +95,35 @@ def test_chains_several_batches_and_uneven(ctx, oracle):
                                     gamma=g[None, :], short_t=t[None, :], norm="f64")
         assert tt[k] == tto[0]
         assert np.max(np.abs(sp[k] - spo[0])) < 1e-6, (nn, mm)
+
+
+def test_chains_fall_back_when_a_wait_runs_out(ctx):
+    """The fallback, forced from the host: with no read per wait (chain_wait_polls=-1) the first wait
+    between a group's workgroups gives up, the launch ends without waiting (every access inside its
+    regions) and reports it; the context then runs the recurrences one launch per step, bit for bit
+    those of chain_recurrences=False, and keeps doing so."""
+    n, m, samples = 16, 3, 64
+    recs = M.generate_sk(n, 2, 1777)
+    _say("chains: forced fallback")
+    c = M.Context(0)
+    try:
+        p, tt, sp = c.solve(n, m, recs, samples=samples, with_samples=True, tile_bits=14, chain_wait_polls=-1)
+        st = c.last_stats()
+        p2, tt2, sp2 = c.solve(n, m, recs, samples=samples, with_samples=True, tile_bits=14)
+        st2 = c.last_stats()
+    finally:
+        c.close()
+    p3, tt3, sp3 = ctx.solve(n, m, recs, samples=samples, with_samples=True, tile_bits=14, chain_recurrences=False)
+    print(f"[chain] forced fallback: {st['chain_launches']} and then {st2['chain_launches']} on-chip launches, "
+          f"{st['basis_launches']} basis launches", flush=True)
+    assert st["chain_launches"] == 0 and st2["chain_launches"] == 0
+    assert st["basis_launches"] > 2  # one per step
+    assert np.array_equal(sp, sp3) and np.array_equal(sp2, sp3)
+    assert np.array_equal(tt, tt3) and np.array_equal(p, p3)
+
+
+def test_chain_wait_polls_below_minus_one_is_an_error(ctx):
+    recs = M.generate_sk(16, 1, 5)
+    with pytest.raises(M.MQWError) as e:
+        ctx.solve(16, 3, recs, samples=8, chain_wait_polls=-2)
+    assert e.value.status == 1  # MQW_EINVAL
