@@ -2088,31 +2088,67 @@ int launch_readout_dots(const ReadTile* tiles_dev, uint32_t ntiles, const ReadIt
 }
 
 // The same dots on the FP16 matrix cores (operands split as in combine2h):
-// phi_j[a] = sum_k a_jk X_k[a] is D[a][j] = sum_k X_k[a] a_jk with the basis
+// phi_j[a] = sum_k a_jk X_k[a] is D[j][a] = sum_k a_jk X_k[a] with the basis
 // index as the inner dimension, even k (real part of a_jk X_k) and odd k
-// (imaginary part) as two classes of rows k = 2 q + class.  Per 256
-// amplitudes, wave w takes 32 of them: lane (col, half) loads rows 8 half + i
-// of each 16-row block at amplitude col (A), the tile's coefficients come
-// from LDS (B), and afterwards the lane holds phi of sample col at 16
-// amplitudes, multiplies by psi there and keeps its own partial dot: one
-// FP32 sum per 256 amplitudes, then FP64 across passes, the two halves of the
-// wave and the 8 waves in order.
-template <int MB, int OCC>
+// (imaginary part) as two classes of rows k = 2 q + class.  A workgroup takes
+// 32 samples of a tile and one amplitude chunk; per pass of 256 amplitudes,
+// wave w takes 32 of them.  Lane (col, half) gives the coefficients of sample
+// col (A, from LDS) and rows 8 half + i of each 16-row block at amplitude col
+// (B), and afterwards holds phi of samples 8 (r / 4) + 4 half + r % 4 (r < 16)
+// at amplitude col, so it reads those samples' psi there: 32 lanes, one
+// 128-byte row.
+//
+// Each lane's loads come in "chunks" of 8: the X blocks of a pass, then psi
+// of 4 samples (real and imaginary parts) four times.  They stream through a
+// ring of four chunks, issued three chunks before their use and in the order
+// of use, so no wait for one chunk waits for a later one (vector loads
+// complete in order) and 24 loads stay in flight.  The blocks are padded to a
+// multiple of 4, so a pass has a multiple of 4 chunks and every pass uses the
+// ring the same way.  Every load is issued (no masked lanes), so the
+// compiler's waits count exactly: a row past the basis reads row kx - 1, a
+// sample past the tile reads sample 0, and the last pass's look-ahead reads its
+// own amplitudes again; their coefficients are zero, so they add exact zeros
+// (every value read is finite).  The products phi psi of a pass are summed over
+// the wave's 32 amplitudes in FP32 by a reduce-scatter across the 32 lanes of
+// each half (lane col ends with the real (col even) or imaginary part of
+// sample r = col / 2), added to FP64 per lane, and the 8 waves' sums are
+// added in order.
+// One step of a reduce-scatter over the 32 lanes of each half of a wave: the
+// lane keeps the S of its 2 S values whose index has bit log2(S) equal to
+// that bit of its lane (the bits above already fixed by the earlier steps),
+// plus those of the lane at distance S.  After the steps 16, 8, 4, 2, 1, v[0]
+// of lane col is the half's sum of the values of index col.
+template <int S>
+__device__ __forceinline__ void reduce_scatter_step(float (&v)[32], int col)
+{
+    const bool up = (col & S) != 0;
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+        const float send = up ? v[i] : v[i + S];
+        const float keep = up ? v[i + S] : v[i];
+        // bit-mask swizzle within 32 lanes: lane ^ S
+        const int got = __builtin_amdgcn_ds_swizzle(__float_as_int(send), 0x1F | (S << 10));
+        v[i] = keep + __int_as_float(got);
+    }
+}
+
+template <int OCC>
 __global__ void __launch_bounds__(512, OCC)
 readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const ReadItem* __restrict__ items,
                     const float* __restrict__ coefs, int n, uint32_t nchunk, double* __restrict__ dots)
 {
-    constexpr int NSP = MB * 32;
+    constexpr int NSP = 32;
     constexpr int SUB = kReadTile / NSP; // workgroups per tile and chunk, NSP samples each
-    constexpr int KB = kReadMaxK / 32; // 16-row blocks per class
-    constexpr int BLK = 2 * NSP * 16;  // halves of one block (hi and lo)
+    constexpr int KB = kReadMaxK / 32;   // 16-row blocks per class
+    constexpr int BLK = 2 * NSP * 16;    // halves of one block (hi and lo)
+    constexpr int RING = 4, AHEAD = 3; // chunks of 8 loads per lane in the ring, issued ahead of use
     constexpr float kXScale = 4096.0f; // 2^12: basis entries are at most 1 in magnitude
     __shared__ __attribute__((aligned(16))) _Float16 s_b[2 * KB * BLK];
     __shared__ const float* s_psi[NSP];
     __shared__ double s_w[8][NSP][2];
     // XCD-aware order: workgroup b runs on XCD b % 8 and each XCD takes a run
     // of consecutive units, chunk-major, so the tiles of one amplitude chunk
-    // (two tiles of an instance share its basis) meet in one L2
+    // (the workgroups of an instance share its basis) meet in one L2
     const uint32_t u = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
     const uint32_t chunk = u / (ntiles * SUB), tile = (u / SUB) % ntiles;
     if (chunk >= nchunk) return;
@@ -2125,7 +2161,7 @@ readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const R
     const int col = lane & 31, half = lane >> 5;
     for (int i = t; i < 2 * KB * NSP * 16; i += 512) {
         const int rr = i & 15, j = (i >> 4) % NSP, cb = (i >> 4) / NSP; // cb = class * KB + block
-        const int k = 2 * (16 * (cb % KB) + rr) + cb / KB;
+        const int k = 32 * (cb % KB) + 4 * (rr & 7) + 2 * (rr >> 3) + cb / KB;
         const float v = (j < ns && k < kx) ? coefs[TI[j].coef_off + k] * T.ascale : 0.0f;
         const _Float16 hi = (_Float16)v;
         // [block][hi, lo][rows 0-7, rows 8-15][NSP samples][8 rows]: a wave's
@@ -2134,113 +2170,137 @@ readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const R
         s_b[cb * BLK + pos] = hi;
         s_b[cb * BLK + NSP * 16 + pos] = (_Float16)(v - (float)hi);
     }
-    for (int i = t; i < NSP; i += 512) s_psi[i] = i < ns ? TI[i].psi : nullptr;
+    for (int i = t; i < NSP; i += 512) s_psi[i] = TI[i < ns ? i : 0].psi;
     for (int i = t; i < 8 * NSP * 2; i += 512) (&s_w[0][0][0])[i] = 0.0;
     __syncthreads();
-    const uint64_t N = 1ull << n;
-    const uint64_t C = N / nchunk;
-    const uint64_t base = (uint64_t)chunk * C;
-    const MQW_GLOBAL float* X = gview(T.X);
+    // byte offsets below 2^32: 12 N bytes for n <= 28 (the launcher checks)
+    const uint32_t N = 1u << n;
+    const uint32_t C = N / nchunk;
+    const uint32_t base = chunk * C + (uint32_t)(w * 32 + col); // this lane's amplitude in pass 0
+    const MQW_GLOBAL char* const X = reinterpret_cast<const MQW_GLOBAL char*>(gview(T.X));
+    const uint64_t row = 4ull * N; // bytes per basis vector
     const float osc = T.oscale;
-    // 16-row blocks: class 0 (even k) then class 1 (odd k), the total padded to
-    // an even count (a padding block has no rows: zero loads, zero products)
-    const int nb0 = ((kx + 1) / 2 + 15) / 16;
-    const int nbt = (nb0 + (kx / 2 + 15) / 16 + 1) & ~1;
-    // block g of this lane's rows at amplitude a
-    auto load_x = [&](float (&xv)[8], int g, uint64_t a) {
-        const int cls = g >= nb0 ? 1 : 0, b = g - (cls ? nb0 : 0);
+    // 16-row blocks, the classes alternating: block c has class c & 1 and
+    // index c >> 1 within it, both classes padded to the same even count (a
+    // padding row reads row kx - 1), so a pass has a multiple of 4 blocks
+    const int nbc = (((kx + 1) / 2 + 15) / 16 + 1) & ~1;
+    const int nbt = 2 * nbc;
+    // X block c (< nbt) of the pass at offset p0 into dst: load i of half h
+    // reads row 32 b + 4 i + 2 h + cls of block b of class cls, a uniform row
+    // base plus this lane's byte offset (the second half's row two further on)
+    auto issue_x = [&](float (&dst)[8], int c, uint32_t p0) {
+        const uint32_t oa = 4u * (base + p0);          // this lane's amplitude
+        const uint32_t o2 = oa + (half ? 8u * N : 0u); // and two rows on for the second half
+        const int cls = c & 1, b = c >> 1;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            const int k = 2 * (16 * b + 8 * half + i) + cls;
-            xv[i] = k < kx ? X[(uint64_t)k * N + a] : 0.0f;
+            const int k0 = 32 * b + 4 * i + cls;
+            const MQW_GLOBAL char* xr = X + (uint64_t)min(k0, kx - 1) * row;
+            dst[i] = *reinterpret_cast<const MQW_GLOBAL float*>(xr + (k0 + 2 < kx ? o2 : oa));
         }
     };
-    double dre[MB], dim[MB];
+    // psi of samples 8 q + 4 h + e (e < 4) of the pass at offset p0: real
+    // part into dst[2 e], imaginary part into dst[2 e + 1]
+    auto issue_psi = [&](float (&dst)[8], int q, uint32_t p0) {
+        const uint32_t oa = 4u * (base + p0);
 #pragma unroll
-    for (int mb = 0; mb < MB; mb++) dre[mb] = dim[mb] = 0.0;
-    // one block ahead: the next block of the pass, or the first of the next pass
-    float xv[2][8];
-    load_x(xv[0], 0, base + (uint64_t)(w * 32 + col));
-    for (uint64_t a0 = 0; a0 < C; a0 += 256) {
-        const uint64_t a = base + a0 + (uint64_t)(w * 32 + col);
-        mqw_f32x16 acc[2][MB];
+        for (int e = 0; e < 4; e++) {
+            const MQW_GLOBAL char* pr = reinterpret_cast<const MQW_GLOBAL char*>(gview(s_psi[8 * q + 4 * half + e]));
+            dst[2 * e] = *reinterpret_cast<const MQW_GLOBAL float*>(pr + oa);
+            dst[2 * e + 1] = *reinterpret_cast<const MQW_GLOBAL float*>(pr + (oa + 4u * N));
+        }
+    };
+    float ring[RING][8];
+    mqw_f32x16 acc[2];
+    // X block gg (class q & 1 = gg & 1) from ring[q] into the products
+    auto consume = [&](int q, int gg) {
+        mqw_h8 xh, xl;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float xs = ring[q][i] * kXScale;
+            const _Float16 h = (_Float16)xs;
+            xh[i] = h;
+            xl[i] = (_Float16)(xs - (float)h);
+        }
+        const int cls = q & 1, b = gg >> 1; // b < nbc <= KB
+        const _Float16* Bb = s_b + (cls * KB + b) * BLK;
+        const mqw_h8 ah = *reinterpret_cast<const mqw_h8*>(Bb + (half * NSP + col) * 8);
+        const mqw_h8 al = *reinterpret_cast<const mqw_h8*>(Bb + NSP * 16 + (half * NSP + col) * 8);
+        acc[cls] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, xh, acc[cls], 0, 0, 0);
+        acc[cls] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, xh, acc[cls], 0, 0, 0);
+        acc[cls] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, xl, acc[cls], 0, 0, 0);
+    };
+    // the loads stay where they are written (in the order of their use)
+    auto fence = [] { asm volatile("" ::: "memory"); };
+    // chunk c of a pass sits in ring[c % 4] (a pass has nbt + 4 chunks, a
+    // multiple of 4) and is issued while chunk c - 3 is used
+    issue_x(ring[0], 0, 0);
+    issue_x(ring[1], 1, 0);
+    issue_x(ring[2], 2, 0);
+    fence();
+    double dsum = 0.0;
+    for (uint32_t p0 = 0; p0 < C; p0 += 256) {
+        const uint32_t pn = p0 + 256 < C ? p0 + 256 : p0; // the next pass (the last: this one again)
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
-            for (int mb = 0; mb < MB; mb++)
+            for (int r = 0; r < 16; r++) acc[c][r] = 0.0f;
+        // every group of blocks but the last looks ahead to X blocks only
+        int g = 0;
+        for (; g + RING < nbt; g += RING) {
 #pragma unroll
-                for (int r = 0; r < 16; r++) acc[c][mb][r] = 0.0f;
-        for (int g = 0; g < nbt; g += 2) {
-#pragma unroll
-            for (int p = 0; p < 2; p++) {
-                const int gg = g + p;
-                if (gg + 1 < nbt)
-                    load_x(xv[p ^ 1], gg + 1, a);
-                else if (a0 + 256 < C)
-                    load_x(xv[p ^ 1], 0, a + 256);
-                mqw_h8 xh, xl;
-#pragma unroll
-                for (int i = 0; i < 8; i++) {
-                    const float xs = xv[p][i] * kXScale;
-                    const _Float16 h = (_Float16)xs;
-                    xh[i] = h;
-                    xl[i] = (_Float16)(xs - (float)h);
-                }
-                const int cls = gg >= nb0 ? 1 : 0, b = gg - (cls ? nb0 : 0);
-                if (b >= KB) continue; // a padding block past the last class's blocks
-                const _Float16* Bb = s_b + (cls * KB + b) * BLK;
-#pragma unroll
-                for (int mb = 0; mb < MB; mb++) {
-                    const int j = mb * 32 + col;
-                    const mqw_h8 bh = *reinterpret_cast<const mqw_h8*>(Bb + (half * NSP + j) * 8);
-                    const mqw_h8 bl = *reinterpret_cast<const mqw_h8*>(Bb + NSP * 16 + (half * NSP + j) * 8);
-                    if (cls == 0) {
-                        acc[0][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bh, acc[0][mb], 0, 0, 0);
-                        acc[0][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bl, acc[0][mb], 0, 0, 0);
-                        acc[0][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xl, bh, acc[0][mb], 0, 0, 0);
-                    } else {
-                        acc[1][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bh, acc[1][mb], 0, 0, 0);
-                        acc[1][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bl, acc[1][mb], 0, 0, 0);
-                        acc[1][mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xl, bh, acc[1][mb], 0, 0, 0);
-                    }
-                }
+            for (int q = 0; q < RING; q++) {
+                fence();
+                issue_x(ring[(q + AHEAD) % RING], g + q + AHEAD, p0);
+                fence();
+                consume(q, g + q);
             }
         }
-        // lane (col, half) holds rows 8 (r / 4) + 4 half + r % 4 of column col:
-        // phi of sample mb * 32 + col at amplitudes a0 + 32 w + 8 g + 4 half + e
+        // the last group: X block nbt - 1, then psi of samples 0-7, 8-15, 16-23
+        fence();
+        issue_x(ring[3], nbt - 1, p0);
+        fence();
+        consume(0, g);
+        issue_psi(ring[0], 0, p0);
+        fence();
+        consume(1, g + 1);
+        issue_psi(ring[1], 1, p0);
+        fence();
+        consume(2, g + 2);
+        issue_psi(ring[2], 2, p0);
+        fence();
+        consume(3, g + 3);
+        // psi of samples 8 q + 4 half + e (e < 4) from ring[q] (q = 3 issued
+        // here, then the next pass's first three X blocks): phi psi (complex,
+        // no conjugate) as v[2 r + {0: Re, 1: Im}], r = 4 q + e
+        float v[32];
 #pragma unroll
-        for (int mb = 0; mb < MB; mb++) {
-            const int j = mb * 32 + col;
-            if (j >= ns) continue;
-            const MQW_GLOBAL float* psi = gview(s_psi[j]);
-            float sr = 0.0f, si = 0.0f;
+        for (int q = 0; q < RING; q++) {
+            fence();
+            if (q == 0)
+                issue_psi(ring[3], 3, p0);
+            else
+                issue_x(ring[q - 1], q - 1, pn);
+            fence();
 #pragma unroll
-            for (int g = 0; g < 4; g++) {
-                const uint64_t am = base + a0 + (uint64_t)(w * 32 + 8 * g + 4 * half);
-                typedef float f4 __attribute__((ext_vector_type(4)));
-                const f4 yr = *reinterpret_cast<const MQW_GLOBAL f4*>(psi + am);
-                const f4 yi = *reinterpret_cast<const MQW_GLOBAL f4*>(psi + N + am);
-                const float yre[4] = {yr[0], yr[1], yr[2], yr[3]};
-                const float yim[4] = {yi[0], yi[1], yi[2], yi[3]};
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const float pr = acc[0][mb][4 * g + e] * osc, pi = acc[1][mb][4 * g + e] * osc;
-                    sr += pr * yre[e] - pi * yim[e];
-                    si += pr * yim[e] + pi * yre[e];
-                }
+            for (int e = 0; e < 4; e++) {
+                const int r = 4 * q + e;
+                const float pr = acc[0][r] * osc, pi = acc[1][r] * osc;
+                const float yr = ring[q][2 * e], yi = ring[q][2 * e + 1];
+                v[2 * r] = pr * yr - pi * yi;
+                v[2 * r + 1] = pr * yi + pi * yr;
             }
-            dre[mb] += (double)sr;
-            dim[mb] += (double)si;
         }
+        reduce_scatter_step<16>(v, col);
+        reduce_scatter_step<8>(v, col);
+        reduce_scatter_step<4>(v, col);
+        reduce_scatter_step<2>(v, col);
+        reduce_scatter_step<1>(v, col);
+        dsum += (double)v[0];
     }
-#pragma unroll
-    for (int mb = 0; mb < MB; mb++) {
-        const double r2 = __shfl_xor(dre[mb], 32, 64), i2 = __shfl_xor(dim[mb], 32, 64);
-        const int j = mb * 32 + col;
-        if (half == 0 && j < ns) {
-            s_w[w][j][0] = dre[mb] + r2;
-            s_w[w][j][1] = dim[mb] + i2;
-        }
+    {
+        const int r = col >> 1;
+        s_w[w][8 * (r >> 2) + 4 * half + (r & 3)][col & 1] = dsum;
     }
     __syncthreads();
     if (t < 2 * ns) {
@@ -2258,13 +2318,13 @@ int launch_readout_dots_mfma(const ReadTile* tiles_dev, uint32_t ntiles, const R
     static_assert(kReadTile == 64 && kReadMaxK % 32 == 0, "tiles of two 32-sample blocks, whole row blocks");
     const uint64_t N = 1ull << n;
     if (nchunk == 0 || N % nchunk != 0 || (N / nchunk) % 256 != 0) return -2;
-    // 32 samples per workgroup: half the accumulators of a 64-sample tile, for
-    // more waves per SIMD; the two workgroups of a tile and chunk read its X
+    if (n > 28) return -2; // 32-bit byte offsets within a state and a basis row pair
+    // 32 samples per workgroup; the workgroups of a tile and chunk read its X
     // rows through one L2
     constexpr int SUB = 2;
     const uint64_t grid = ((uint64_t)ntiles * nchunk * SUB + 7) / 8 * 8; // a multiple of the 8 XCDs
     if (grid > 0x7fffffffull) return -3;
-    hipLaunchKernelGGL((readout_mfma_kernel<1, 4>), dim3((uint32_t)grid), dim3(512), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL((readout_mfma_kernel<4>), dim3((uint32_t)grid), dim3(512), 0, (hipStream_t)stream,
                        tiles_dev, ntiles, items_dev, coefs_dev, n, nchunk, dots);
     return check_launch("readout_dot_mfma");
 }
