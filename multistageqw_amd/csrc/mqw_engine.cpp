@@ -655,7 +655,8 @@ void readout_ops(int n, uint32_t parts, const std::vector<Traj>& trajs, const st
     uint32_t nchunk = 1;
     while ((uint64_t)nchunk * ntiles < 2048 && (N / (nchunk * 2)) >= 4096) nchunk *= 2;
     P.dots_doubles = std::max<size_t>(P.dots_doubles, (size_t)ntiles * nchunk * kReadTile * 2);
-    Op d{5, t0, ntiles, kx_max, nchunk, B.split ? 1u : 0u};
+    // the matrix-core dots up to n = kReadMfmaMaxN, the FP32 vector dots beyond
+    Op d{5, t0, ntiles, kx_max, nchunk, B.split && n <= kReadMfmaMaxN ? 1u : 0u};
     Op f{6, t0, ntiles, kx_max, nchunk};
     P.ops.push_back(d);
     P.ops.push_back(f);
@@ -2301,7 +2302,7 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         const int kc2max = split ? kC2MaxK2 : kC2MaxK; // the GEMM's second-stage coefficients
         for (size_t p = 0; p < ps.size(); p++) {
             if (kc1[p] > kCombineMaxKX) kc1[p] = 0;
-            if (kx[p] > (split ? kReadMaxK : kReadMaxKF32)) kx[p] = 0;
+            if (kx[p] > (split && n <= kReadMfmaMaxN ? kReadMaxK : kReadMaxKF32)) kx[p] = 0;
             // the two-stage adjoint: a second-stage GEMM over Y (limits as W's)
             if (ky[p] < 2 || ky[p] > kc2max || kx[p] < 8 || kx[p] > kAdjMaxK ||
                 (o.second_stage_gemm == MQW_GEMM_SPLIT_FP16 && (N / nparts) % 512 != 0))

@@ -266,6 +266,7 @@ inline uint32_t combine2h_nsp(uint32_t ns_max) { return ns_max <= 32 ? 32u : ns_
 constexpr int kReadTile = 64;   // samples per dot workgroup
 constexpr int kReadMaxK = 256;    // basis vectors per instance (X_0 included), matrix-core dots
 constexpr int kReadMaxKF32 = 128; // the same for the FP32 vector dots (values in registers)
+constexpr int kReadMfmaMaxN = 25; // matrix-core dots: byte offsets below 2^32 within 32 basis rows
 struct ReadItem {
     const float* psi;         // the last stage's input state (planar 2N, unnormalised)
     const float* phi;         // pair dots only: the other factor (planar 2N)

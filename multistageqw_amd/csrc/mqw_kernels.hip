@@ -2105,10 +2105,11 @@ int launch_readout_dots(const ReadTile* tiles_dev, uint32_t ntiles, const ReadIt
 // complete in order) and 24 loads stay in flight.  The blocks are padded to a
 // multiple of 4, so a pass has a multiple of 4 chunks and every pass uses the
 // ring the same way.  Every load is issued (no masked lanes), so the
-// compiler's waits count exactly: a row past the basis reads row kx - 1, a
-// sample past the tile reads sample 0, and the last pass's look-ahead reads its
-// own amplitudes again; their coefficients are zero, so they add exact zeros
-// (every value read is finite).  The products phi psi of a pass are summed over
+// compiler's waits count exactly, and every address lies in its buffer: a row
+// past the basis reads the basis's row rb (below), a sample past the tile
+// reads sample 0, and the last pass's look-ahead reads its own amplitudes
+// again; their coefficients are zero, so they add exact zeros (every value
+// read is finite).  The products phi psi of a pass are summed over
 // the wave's 32 amplitudes in FP32 by a reduce-scatter across the 32 lanes of
 // each half (lane col ends with the real (col even) or imaginary part of
 // sample r = col / 2), added to FP64 per lane, and the 8 waves' sums are
@@ -2173,12 +2174,12 @@ readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const R
     for (int i = t; i < NSP; i += 512) s_psi[i] = TI[i < ns ? i : 0].psi;
     for (int i = t; i < 8 * NSP * 2; i += 512) (&s_w[0][0][0])[i] = 0.0;
     __syncthreads();
-    // byte offsets below 2^32: 12 N bytes for n <= 28 (the launcher checks)
+    // byte offsets below 2^32: 124 N bytes for n <= 25 (the launcher checks)
     const uint32_t N = 1u << n;
     const uint32_t C = N / nchunk;
     const uint32_t base = chunk * C + (uint32_t)(w * 32 + col); // this lane's amplitude in pass 0
-    const MQW_GLOBAL char* const X = reinterpret_cast<const MQW_GLOBAL char*>(gview(T.X));
-    const uint64_t row = 4ull * N; // bytes per basis vector
+    const uint32_t oh = 4u * base + (half ? 8u * N : 0u);       // and its row pair's second row
+    const float* const X = T.X;
     const float osc = T.oscale;
     // 16-row blocks, the classes alternating: block c has class c & 1 and
     // index c >> 1 within it, both classes padded to the same even count (a
@@ -2186,17 +2187,26 @@ readout_mfma_kernel(const ReadTile* __restrict__ tiles, uint32_t ntiles, const R
     const int nbc = (((kx + 1) / 2 + 15) / 16 + 1) & ~1;
     const int nbt = 2 * nbc;
     // X block c (< nbt) of the pass at offset p0 into dst: load i of half h
-    // reads row 32 b + 4 i + 2 h + cls of block b of class cls, a uniform row
-    // base plus this lane's byte offset (the second half's row two further on)
+    // reads row 32 b + 4 i + 2 h + cls of block b of class cls, at a uniform
+    // base row rb (the block's first row, or the basis's last) plus the lane's
+    // byte offset, so the rows need no address arithmetic per load.  Every
+    // address lies in the basis: a row past it reads row rb instead.
     auto issue_x = [&](float (&dst)[8], int c, uint32_t p0) {
-        const uint32_t oa = 4u * (base + p0);          // this lane's amplitude
-        const uint32_t o2 = oa + (half ? 8u * N : 0u); // and two rows on for the second half
-        const int cls = c & 1, b = c >> 1;
+        const int cls = c & 1, r0 = 32 * (c >> 1) + cls;
+        const int rb = r0 < kx ? r0 : kx - 1;
+        const int lim = r0 < kx ? kx - r0 - 2 * half : 0; // row rb + 4 i + 2 half in the basis iff 4 i < lim
+        // (a uniform base, so the loads take it from scalar registers; the
+        // builtin returns int, so each half is zero-extended through uint32_t)
+        const uint64_t bp = (uint64_t)(uintptr_t)(X + (uint64_t)rb * N);
+        const uint64_t bu = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)bp) |
+                            ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(bp >> 32)) << 32);
+        const MQW_GLOBAL char* const xb = reinterpret_cast<const MQW_GLOBAL char*>(bu);
+        const uint32_t oa = 4u * (base + p0);
+        const uint32_t o = oh + 4u * p0;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            const int k0 = 32 * b + 4 * i + cls;
-            const MQW_GLOBAL char* xr = X + (uint64_t)min(k0, kx - 1) * row;
-            dst[i] = *reinterpret_cast<const MQW_GLOBAL float*>(xr + (k0 + 2 < kx ? o2 : oa));
+            const uint32_t off = 4 * i < lim ? o + 16u * N * (uint32_t)i : oa;
+            dst[i] = *reinterpret_cast<const MQW_GLOBAL float*>(xb + off);
         }
     };
     // psi of samples 8 q + 4 h + e (e < 4) of the pass at offset p0: real
@@ -2318,7 +2328,7 @@ int launch_readout_dots_mfma(const ReadTile* tiles_dev, uint32_t ntiles, const R
     static_assert(kReadTile == 64 && kReadMaxK % 32 == 0, "tiles of two 32-sample blocks, whole row blocks");
     const uint64_t N = 1ull << n;
     if (nchunk == 0 || N % nchunk != 0 || (N / nchunk) % 256 != 0) return -2;
-    if (n > 28) return -2; // 32-bit byte offsets within a state and a basis row pair
+    if (n > kReadMfmaMaxN) return -2; // 32-bit byte offsets within a block of 32 basis rows
     // 32 samples per workgroup; the workgroups of a tile and chunk read its X
     // rows through one L2
     constexpr int SUB = 2;
