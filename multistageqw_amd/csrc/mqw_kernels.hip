@@ -1914,14 +1914,28 @@ combine2h_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const 
                     // the slot's reads above complete before the copy
                     if (b + 3 < nb) copy_b(sl, qb + (b + 3) * 16, aw);
                     const _Float16* Ab = s_a + sl * BLK;
+                    // two sample tiles at a time, their products alternating, so
+                    // no product waits for the one before it on the same tile
+                    // (each tile's sums keep their order)
+                    constexpr int MS = MB >= 2 ? 2 : 1;
 #pragma unroll
-                    for (int mb = 0; mb < MB; mb++) {
-                        const int j = mb * 32 + col;
-                        const mqw_h8 ah = *reinterpret_cast<const mqw_h8*>(Ab + (half * NSP + j) * 8);
-                        const mqw_h8 al = *reinterpret_cast<const mqw_h8*>(Ab + NSP * 16 + (half * NSP + j) * 8);
-                        acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[mb], 0, 0, 0);
-                        acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[mb], 0, 0, 0);
-                        acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[mb], 0, 0, 0);
+                    for (int m0 = 0; m0 < MB; m0 += MS) {
+                        mqw_h8 ah[MS], al[MS];
+#pragma unroll
+                        for (int u = 0; u < MS; u++) {
+                            const int j = (m0 + u) * 32 + col;
+                            ah[u] = *reinterpret_cast<const mqw_h8*>(Ab + (half * NSP + j) * 8);
+                            al[u] = *reinterpret_cast<const mqw_h8*>(Ab + NSP * 16 + (half * NSP + j) * 8);
+                        }
+#pragma unroll
+                        for (int u = 0; u < MS; u++)
+                            acc[m0 + u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[u], bh, acc[m0 + u], 0, 0, 0);
+#pragma unroll
+                        for (int u = 0; u < MS; u++)
+                            acc[m0 + u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[u], bl, acc[m0 + u], 0, 0, 0);
+#pragma unroll
+                        for (int u = 0; u < MS; u++)
+                            acc[m0 + u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[u], bh, acc[m0 + u], 0, 0, 0);
                     }
                 }
             }
