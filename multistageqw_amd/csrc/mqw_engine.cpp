@@ -1190,10 +1190,65 @@ mqw_status run_lds(mqw_ctx* c, int n, const std::vector<Traj>& trajs, const std:
 // pending, and the caller synchronises and collects the times.
 // tag names the device buffers of the plan's tables: work on the main stream
 // while a pending plan runs on the second stream must use another.
+// A launch of on-chip recurrences left running (run_bases with a ChainDefer),
+// so the host can prepare the next plan meanwhile: its plan (whose host tables
+// are the sources of its copies) stays here until finish_chains has
+// synchronised the stream, which the destructor also does on an early return.
+struct ChainDefer {
+    bool active = false;
+    hipStream_t stm = nullptr;
+    Timed tm;
+    std::deque<Plan<float>> keep;
+    double bytes = 0.0; // algorithmic bytes of the launch
+    uint64_t csteps = 0;
+    ~ChainDefer()
+    {
+        if (active) (void)hipStreamSynchronize(stm);
+    }
+};
+
+// Waits for a deferred launch of on-chip recurrences.  *ok: its outputs stand;
+// false: it aborted (a group's workgroups on several XCDs, or a wait that ran
+// out), this context runs its recurrences one launch per step from now on,
+// and the caller reruns these (they rewrite every row).
+mqw_status finish_chains(mqw_ctx* c, ChainDefer& d, bool* ok)
+{
+    *ok = true;
+    if (!d.active) return MQW_OK;
+    d.active = false;
+    HIPCHK(c, hipStreamSynchronize(d.stm));
+    double ms_sweep = 0.0, ms_aux = 0.0, ms_basis = 0.0;
+    collect(d.tm, &ms_sweep, &ms_aux, &ms_basis, nullptr);
+    d.tm = Timed{};
+    d.keep.clear();
+    uint32_t ev = 0;
+    HIPCHK(c, hipMemcpy(&ev, c->chain_err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (ev == 0) {
+        c->stats.aux_bytes += d.bytes;
+        c->stats.basis_bytes += d.bytes;
+        c->stats.basis_launches += 1;
+        c->stats.aux_launches += 1;
+        c->stats.chain_launches += 1;
+        c->stats.chain_steps += d.csteps;
+        c->stats.gpu_ms += ms_sweep + ms_aux;
+        c->stats.aux_ms += ms_sweep + ms_aux;
+        c->stats.basis_ms += ms_basis;
+        return MQW_OK;
+    }
+    (void)hipMemset(c->chain_err, 0, sizeof(uint32_t));
+    c->chain_capable = false;
+    trace("on-chip recurrences aborted (%s): one launch per step from now on",
+          (ev & 2u) ? "workgroups of a group on several XCDs" : "a wait ran out");
+    *ok = false;
+    return MQW_OK;
+}
+
+// defer: on-chip recurrences are left running (see ChainDefer); otherwise the
+// call returns when its recurrences are done.
 mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Problem*>& ps,
                      const std::vector<Traj>& trajs, const float* E, const Bases& B, bool use_graph,
                      Timed* pending = nullptr, hipStream_t stream = nullptr, std::deque<Plan<float>>* keep = nullptr,
-                     const std::string& tag = "bases")
+                     const std::string& tag = "bases", ChainDefer* defer = nullptr)
 {
     const hipStream_t stm = stream ? stream : c->stream;
     const size_t N = (size_t)1 << n;
@@ -1350,6 +1405,15 @@ mqw_status run_bases(mqw_ctx* c, int n, const SweepCfg& k, const std::vector<Pro
             }
             P.ops.push_back(Op{9, 0, (uint32_t)P.chains.size()});
             double ms_sweep = 0.0, ms_aux = 0.0, ms_basis = 0.0;
+            if (defer) {
+                defer->keep.push_back(std::move(P));
+                defer->active = true; // (from here the destructor waits for the stream)
+                defer->stm = stm;
+                defer->bytes = bytes + cbytes;
+                defer->csteps = csteps;
+                return run_plan<float>(c, MQW_FP32, n, k, defer->keep.back(), use_graph, &ms_sweep, &ms_aux, &ms_basis,
+                                       nullptr, &defer->tm, stm, tag);
+            }
             mqw_status st = run_plan<float>(c, MQW_FP32, n, k, P, use_graph, &ms_sweep, &ms_aux, &ms_basis, nullptr,
                                             nullptr, stm, tag);
             if (st) return st;
@@ -2565,6 +2629,19 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
         for (int b = 0; b < 2; b++) HIPCHK(c, hipEventCreateWithFlags(&drain.ev[b], hipEventDisableTiming));
     // binds batch q's bases to its buffer and runs their recurrences: at once
     // on the main stream, or queued on the second stream when overlapping
+    // On-chip recurrences of a batch run while the host prepares its plan; the
+    // plan runs once they are checked (settle).  (Declared after the host
+    // buffers of bind's copies: on an early return it waits for the stream
+    // before they go.)
+    ChainDefer cdefer;
+    auto settle = [&](size_t q) -> mqw_status {
+        bool ok = true;
+        mqw_status s = finish_chains(c, cdefer, &ok);
+        if (s || ok) return s;
+        SweepCfg k1 = k;
+        k1.chains = false;
+        return run_bases(c, n, k1, ps, trajs, (const float*)E, bbs[q].b, o.use_graph != 0);
+    };
     auto bind = [&](size_t q) -> mqw_status {
         BatchBases& bb = bbs[q];
         Bases& bases = bb.b;
@@ -2598,7 +2675,8 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
                 HIPCHK(c, hipEventRecord(drain.ev[b], c->stream2));
                 recorded[b] = true;
             } else {
-                rs = run_bases(c, n, k, ps, trajs, (const float*)E, bases, o.use_graph != 0);
+                rs = run_bases(c, n, k, ps, trajs, (const float*)E, bases, o.use_graph != 0, nullptr, nullptr, nullptr,
+                               "bases", &cdefer);
                 if (rs) return rs;
             }
         }
@@ -2665,6 +2743,8 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
                 if (all && 20.0 * 0.5 * ku * k3[p] < 0.8 * own3[p]) cand.push_back(p);
             }
             if (!cand.empty()) {
+                st = settle(q); // (the third stage reads results back to the host)
+                if (st) return st;
                 st = middle_stage(c, n, k, ps, trajs, ptraj, cand, (const float*)E, bases, k3, own3,
                                   o.use_graph != 0, wcap1);
                 if (st) return st;
@@ -2698,6 +2778,8 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             build_plan<real>(n, nparts, trajs, batch, ps, E, slots, probs_d, plan, k.hi,
                              use_bases ? &bases : nullptr);
             trace("batch %zu: plan of %zu ops", q, plan.ops.size());
+            st = settle(q);
+            if (st) return st;
             double ms_sweep = 0.0, ms_aux = 0.0, ms_basis = 0.0, ms_gemm = 0.0;
             st = run_plan<real>(c, precision, n, k, plan, o.use_graph != 0, &ms_sweep, &ms_aux, &ms_basis, &ms_gemm);
             if (st) return st;
@@ -2718,6 +2800,8 @@ mqw_status solve_group(mqw_ctx* c, int n, std::vector<Problem*>& ps, const mqw_o
             c->stats.psi_const_amp_terms += plan.psi_const_amp_terms;
             g0 = g1;
         }
+        st = settle(q); // (a batch that ran no plan)
+        if (st) return st;
         if (recorded[q & 1]) {
             // the batch's recurrences ran on the second stream (their event
             // times overlap the main stream's)
