@@ -285,28 +285,42 @@ mqw_status group_energies(mqw_ctx* c, int n, std::vector<Problem*>& ps, int mode
     const size_t N = (size_t)1 << n;
     const size_t cnt = ps.size();
     const size_t nb = energy_blocks(n);
+    // FP32: every problem in one set of launches (block tables per problem);
+    // FP64: one problem after another through a shared float buffer
+    const bool batched = precision == MQW_FP32;
+    const size_t nt = batched ? cnt : 1; // problems whose block tables coexist
     float* Jh = (float*)dev_get(c, "energy_Jh", sizeof(float) * (size_t)(n * n + n) * cnt);
-    double* scr = (double*)dev_get(c, "energy_scratch", sizeof(double) * 4 * nb);
+    double* scr = (double*)dev_get(c, "energy_scratch", sizeof(double) * 4 * nb * nt);
+    double* e_inits = (double*)dev_get(c, "energy_e_init", sizeof(double) * cnt);
     EnergyDev* info = (EnergyDev*)dev_get(c, "energy_info", sizeof(EnergyDev) * cnt);
     float* hp_tmp = nullptr;
     if (precision == MQW_FP64) hp_tmp = (float*)dev_get(c, "energy_hp_tmp", sizeof(float) * N);
-    if (!Jh || !scr || !info || (precision == MQW_FP64 && !hp_tmp))
+    if (!Jh || !scr || !e_inits || !info || (precision == MQW_FP64 && !hp_tmp))
         return fail(c, MQW_ENOMEM, "device allocation failed in energy build (n=%d, %zu problems)", n, cnt);
     std::vector<float> hJh((size_t)(n * n + n) * cnt);
+    std::vector<double> he(cnt);
     for (size_t p = 0; p < cnt; p++) {
         memcpy(&hJh[p * (n * n + n)], ps[p]->J, sizeof(float) * n * n);
         memcpy(&hJh[p * (n * n + n) + n * n], ps[p]->h, sizeof(float) * n);
+        he[p] = mode == MQW_ENERGY_EXACT ? e_init_exact(n, ps[p]->J, ps[p]->h) : e_init_compat(n, ps[p]->J, ps[p]->h);
     }
+    // (the copies read pageable host memory: the stream drains before it goes)
+    struct Drain {
+        hipStream_t s;
+        ~Drain() { (void)hipStreamSynchronize(s); }
+    } drain{c->stream};
     HIPCHK(c, hipMemcpyAsync(Jh, hJh.data(), sizeof(float) * hJh.size(), hipMemcpyHostToDevice, c->stream));
-    EnergyScratch es{scr, scr + nb, (uint32_t*)(scr + 2 * nb), scr + 3 * nb};
-    for (size_t p = 0; p < cnt; p++) {
+    HIPCHK(c, hipMemcpyAsync(e_inits, he.data(), sizeof(double) * cnt, hipMemcpyHostToDevice, c->stream));
+    EnergyScratch es{scr, scr + nb * nt, (uint32_t*)(scr + 2 * nb * nt), scr + 3 * nb * nt};
+    if (batched && launch_energies(n, (uint32_t)cnt, Jh, e_inits, mode == MQW_ENERGY_EXACT, (float*)E_out, nullptr,
+                                   1, es, info, c->stream))
+        return fail(c, MQW_EDEVICE, "energy kernels: %s", last_kernel_error());
+    for (size_t p = 0; !batched && p < cnt; p++) {
         const float* Jd = Jh + p * (n * n + n);
         const float* hd = Jd + n * n;
-        const double e_init = mode == MQW_ENERGY_EXACT ? e_init_exact(n, ps[p]->J, ps[p]->h)
-                                                       : e_init_compat(n, ps[p]->J, ps[p]->h);
         float* hp = precision == MQW_FP32 ? (float*)E_out + p * N : hp_tmp;
         double* e64 = precision == MQW_FP64 ? (double*)E_out + p * N : nullptr;
-        if (launch_energy(n, Jd, hd, e_init, mode == MQW_ENERGY_EXACT, hp, e64, 1, es, info + p, c->stream))
+        if (launch_energy(n, Jd, hd, he[p], mode == MQW_ENERGY_EXACT, hp, e64, 1, es, info + p, c->stream))
             return fail(c, MQW_EDEVICE, "energy kernels: %s", last_kernel_error());
     }
     std::vector<EnergyDev> hinfo(cnt);
@@ -319,8 +333,7 @@ mqw_status group_energies(mqw_ctx* c, int n, std::vector<Problem*>& ps, int mode
         o.eloc = hinfo[p].eloc;
         o.shift = hinfo[p].shift;
         o.e_abs = hinfo[p].e_abs;
-        o.e_init = mode == MQW_ENERGY_EXACT ? e_init_exact(n, ps[p]->J, ps[p]->h)
-                                            : e_init_compat(n, ps[p]->J, ps[p]->h);
+        o.e_init = he[p];
     }
     return MQW_OK;
 }

@@ -332,6 +332,12 @@ size_t energy_blocks(int n);
 // FP64 centre when e64_shift, else unshifted), info on the device.
 int launch_energy(int n, const float* J_dev, const float* h_dev, double e_init, int exact, float* hp,
                   double* e64, int e64_shift, const EnergyScratch& scr, EnergyDev* info_dev, void* stream);
+// The same for cnt problems of one n in one set of launches (the same kernels
+// and results): couplings then fields of problem p at Jh_dev + p (n^2 + n),
+// its initial energy at e_inits_dev[p], its outputs at hp + p N (e64 + p N)
+// and info_dev + p, its block tables at p energy_blocks(n) in each of scr's.
+int launch_energies(int n, uint32_t cnt, const float* Jh_dev, const double* e_inits_dev, int exact, float* hp,
+                    double* e64, int e64_shift, const EnergyScratch& scr, EnergyDev* info_dev, void* stream);
 
 int term_tile_bits_supported(int precision, int tb);
 int launch_term(int precision, int tb, int variant, const void* jobs_dev, uint32_t njobs, int n, int lc,

@@ -103,13 +103,19 @@ __device__ __forceinline__ double block_sum_fixed(double v, double* red)
     return r;
 }
 
+// Problem blockIdx.y of a batch: its couplings at J + y sJ (fields h at + n^2),
+// its block tables at + y sB (EnergyBatch below; one problem: gridDim.y = 1).
 template <bool EXACT>
 __global__ void __launch_bounds__(kEnergyThreads)
-energy_blocksum_kernel(int n, const float* __restrict__ J, const float* __restrict__ h, double* __restrict__ block_sum)
+energy_blocksum_kernel(int n, const float* __restrict__ J, const float* __restrict__ h, double* __restrict__ block_sum,
+                       uint32_t sJ_stride, uint32_t sB_stride)
 {
     __shared__ float sJ[kMaxN * kMaxN];
     __shared__ float sh[kMaxN];
     __shared__ double red[kEnergyThreads];
+    J += (size_t)blockIdx.y * sJ_stride;
+    h += (size_t)blockIdx.y * sJ_stride;
+    block_sum += (size_t)blockIdx.y * sB_stride;
     load_couplings(n, J, h, sJ, sh);
     const uint64_t N = 1ull << n;
     const uint64_t i0 = ((uint64_t)blockIdx.x * kEnergyThreads + threadIdx.x) * kEnergyPer;
@@ -123,9 +129,10 @@ energy_blocksum_kernel(int n, const float* __restrict__ J, const float* __restri
 }
 
 // Exclusive scan of the block sums in place, one workgroup, fixed order.
-__global__ void __launch_bounds__(1024) energy_scan_kernel(double* __restrict__ v, uint32_t nb)
+__global__ void __launch_bounds__(1024) energy_scan_kernel(double* __restrict__ v, uint32_t nb, uint32_t sB_stride)
 {
     __shared__ double part[1024];
+    v += (size_t)blockIdx.y * sB_stride;
     const uint32_t t = threadIdx.x;
     const uint32_t per = (nb + 1023) / 1024;
     const uint32_t lo = t * per, hi = min(nb, lo + per);
@@ -153,8 +160,10 @@ __global__ void __launch_bounds__(1024) energy_scan_kernel(double* __restrict__ 
 template <bool EXACT>
 __global__ void __launch_bounds__(kEnergyThreads)
 energy_write_kernel(int n, const float* __restrict__ J, const float* __restrict__ h, double e_init,
-                    const double* __restrict__ block_prefix, float* __restrict__ hp, double* __restrict__ e64,
-                    double* __restrict__ block_min, uint32_t* __restrict__ block_arg, double* __restrict__ block_max)
+                    const double* __restrict__ e_inits, const double* __restrict__ block_prefix,
+                    float* __restrict__ hp, double* __restrict__ e64, double* __restrict__ block_min,
+                    uint32_t* __restrict__ block_arg, double* __restrict__ block_max, uint32_t sJ_stride,
+                    uint32_t sB_stride)
 {
     __shared__ float sJ[kMaxN * kMaxN];
     __shared__ float sh[kMaxN];
@@ -162,8 +171,22 @@ energy_write_kernel(int n, const float* __restrict__ J, const float* __restrict_
     __shared__ double mn[kEnergyThreads];
     __shared__ double mx[kEnergyThreads];
     __shared__ uint32_t ag[kEnergyThreads];
-    load_couplings(n, J, h, sJ, sh);
     const uint64_t N = 1ull << n;
+    {
+        const size_t y = blockIdx.y;
+        J += y * sJ_stride;
+        h += y * sJ_stride;
+        if (e_inits) e_init = e_inits[y];
+        block_prefix += y * sB_stride;
+        block_min += y * sB_stride;
+        block_arg += y * sB_stride;
+        block_max += y * sB_stride;
+        if (gridDim.y > 1) {
+            hp += y * N;
+            if (e64) e64 += y * N;
+        }
+    }
+    load_couplings(n, J, h, sJ, sh);
     const int t = threadIdx.x;
     const uint64_t i0 = ((uint64_t)blockIdx.x * kEnergyThreads + t) * kEnergyPer;
     double d[kEnergyPer];
@@ -226,11 +249,15 @@ energy_write_kernel(int n, const float* __restrict__ J, const float* __restrict_
 
 __global__ void __launch_bounds__(256)
 energy_finalize_kernel(uint32_t nb, const double* __restrict__ block_min, const uint32_t* __restrict__ block_arg,
-                       const double* __restrict__ block_max, EnergyDev* __restrict__ info)
+                       const double* __restrict__ block_max, EnergyDev* __restrict__ info, uint32_t sB_stride)
 {
     __shared__ double mn[256], mx[256];
     __shared__ uint32_t ag[256];
     const int t = threadIdx.x;
+    block_min += (size_t)blockIdx.y * sB_stride;
+    block_arg += (size_t)blockIdx.y * sB_stride;
+    block_max += (size_t)blockIdx.y * sB_stride;
+    info += blockIdx.y;
     double lo = __builtin_inf(), hi = -__builtin_inf();
     uint32_t arg = 0xffffffffu;
     for (uint32_t k = t; k < nb; k += 256) {
@@ -274,6 +301,11 @@ energy_shift_kernel(uint64_t N, float* __restrict__ hp, double* __restrict__ e64
 {
     const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (k >= N) return;
+    if (gridDim.y > 1) {
+        hp += (size_t)blockIdx.y * N;
+        if (e64) e64 += (size_t)blockIdx.y * N;
+        info += blockIdx.y;
+    }
     const float s = info->shift;
     hp[k] = hp[k] - s;
     if (e64 && e64_shift) e64[k] = e64[k] - info->center;
@@ -287,22 +319,54 @@ int launch_energy(int n, const float* J_dev, const float* h_dev, double e_init, 
     const uint64_t N = 1ull << n;
     if (exact)
         hipLaunchKernelGGL(energy_blocksum_kernel<true>, dim3(nb), dim3(kEnergyThreads), 0, st, n, J_dev, h_dev,
-                           scr.block_sum);
+                           scr.block_sum, 0u, 0u);
     else
         hipLaunchKernelGGL(energy_blocksum_kernel<false>, dim3(nb), dim3(kEnergyThreads), 0, st, n, J_dev, h_dev,
-                           scr.block_sum);
-    hipLaunchKernelGGL(energy_scan_kernel, dim3(1), dim3(1024), 0, st, scr.block_sum, nb);
+                           scr.block_sum, 0u, 0u);
+    hipLaunchKernelGGL(energy_scan_kernel, dim3(1), dim3(1024), 0, st, scr.block_sum, nb, 0u);
     if (exact)
         hipLaunchKernelGGL(energy_write_kernel<true>, dim3(nb), dim3(kEnergyThreads), 0, st, n, J_dev, h_dev, e_init,
-                           scr.block_sum, hp, e64, scr.block_min, scr.block_arg, scr.block_max);
+                           nullptr, scr.block_sum, hp, e64, scr.block_min, scr.block_arg, scr.block_max, 0u, 0u);
     else
         hipLaunchKernelGGL(energy_write_kernel<false>, dim3(nb), dim3(kEnergyThreads), 0, st, n, J_dev, h_dev,
-                           e_init, scr.block_sum, hp, e64, scr.block_min, scr.block_arg, scr.block_max);
+                           e_init, nullptr, scr.block_sum, hp, e64, scr.block_min, scr.block_arg, scr.block_max, 0u,
+                           0u);
     hipLaunchKernelGGL(energy_finalize_kernel, dim3(1), dim3(256), 0, st, nb, scr.block_min, scr.block_arg,
-                       scr.block_max, info_dev);
+                       scr.block_max, info_dev, 0u);
     hipLaunchKernelGGL(energy_shift_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, hp, e64,
                        e64_shift, info_dev);
     return check_launch("energy");
+}
+
+int launch_energies(int n, uint32_t cnt, const float* Jh_dev, const double* e_inits_dev, int exact, float* hp,
+                    double* e64, int e64_shift, const EnergyScratch& scr, EnergyDev* info_dev, void* stream)
+{
+    if (cnt == 0) return 0;
+    if (cnt > 65535u || n < 1 || n > kMaxN) return -2;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t nb = (uint32_t)energy_blocks(n);
+    const uint64_t N = 1ull << n;
+    const uint32_t sj = (uint32_t)(n * n + n); // couplings then fields, per problem
+    const float* J = Jh_dev;
+    const float* h = Jh_dev + n * n;
+    if (exact)
+        hipLaunchKernelGGL(energy_blocksum_kernel<true>, dim3(nb, cnt), dim3(kEnergyThreads), 0, st, n, J, h,
+                           scr.block_sum, sj, nb);
+    else
+        hipLaunchKernelGGL(energy_blocksum_kernel<false>, dim3(nb, cnt), dim3(kEnergyThreads), 0, st, n, J, h,
+                           scr.block_sum, sj, nb);
+    hipLaunchKernelGGL(energy_scan_kernel, dim3(1, cnt), dim3(1024), 0, st, scr.block_sum, nb, nb);
+    if (exact)
+        hipLaunchKernelGGL(energy_write_kernel<true>, dim3(nb, cnt), dim3(kEnergyThreads), 0, st, n, J, h, 0.0,
+                           e_inits_dev, scr.block_sum, hp, e64, scr.block_min, scr.block_arg, scr.block_max, sj, nb);
+    else
+        hipLaunchKernelGGL(energy_write_kernel<false>, dim3(nb, cnt), dim3(kEnergyThreads), 0, st, n, J, h, 0.0,
+                           e_inits_dev, scr.block_sum, hp, e64, scr.block_min, scr.block_arg, scr.block_max, sj, nb);
+    hipLaunchKernelGGL(energy_finalize_kernel, dim3(1, cnt), dim3(256), 0, st, nb, scr.block_min, scr.block_arg,
+                       scr.block_max, info_dev, nb);
+    hipLaunchKernelGGL(energy_shift_kernel, dim3((unsigned)((N + 255) / 256), cnt), dim3(256), 0, st, N, hp, e64,
+                       e64_shift, info_dev);
+    return check_launch("energies");
 }
 
 // ===========================================================================
