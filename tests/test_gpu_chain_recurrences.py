@@ -127,3 +127,38 @@ def test_chain_wait_polls_below_minus_one_is_an_error(ctx):
     with pytest.raises(M.MQWError) as e:
         ctx.solve(16, 3, recs, samples=8, chain_wait_polls=-2)
     assert e.value.status == 1  # MQW_EINVAL
+
+
+@pytest.mark.parametrize("n,m,count,cap,compress", [
+    (16, 5, 2, 0, True),          # the compression's launch aborts; the third stage after the checks
+    (16, 3, 4, 2 << 30, True),    # the same over several batches
+    (16, 2, 2, 0, False),         # no compression: the batch's launch (checked after the host has
+    (16, 2, 24, 1 << 30, False),  # built its plan) aborts, in one batch and in the first of several
+])
+def test_forced_fallback_third_stage_and_batches(ctx, n, m, count, cap, compress):
+    """The fallback forced from the host (no read per wait): where the first on-chip launch is the
+    compression's (checked at once), and where it is a batch's, whose error word is read after the
+    host has built the batch's plan; before the compressed third stage, and over several batches of
+    bases.  Bit for bit the results of chain_recurrences=False."""
+    recs = M.generate_sk(n, count, 1900 + m)
+    kw = dict(samples=100, with_samples=True, tile_bits=14, compress_bases=compress)
+    if cap:
+        kw["max_device_bytes"] = cap
+    _say("forced fallback n", n, "m", m, "batches cap", cap)
+    c = M.Context(0)
+    try:
+        p, tt, sp = c.solve(n, m, recs, chain_wait_polls=-1, **kw)
+        st = c.last_stats()
+    finally:
+        c.close()
+    p2, tt2, sp2 = ctx.solve(n, m, recs, chain_recurrences=False, **kw)
+    st2 = ctx.last_stats()
+    print(f"[chain] forced fallback n={n} m={m}: batches {st['basis_batches']}, third stages "
+          f"{st['third_stage_instances']}, on-chip launches {st['chain_launches']}", flush=True)
+    assert st["chain_launches"] == 0
+    if m == 5:
+        assert st["third_stage_instances"] == count and st2["third_stage_instances"] == count
+    if cap:
+        assert st["basis_batches"] >= 2
+    assert np.array_equal(tt, tt2)
+    assert np.array_equal(sp, sp2) and np.array_equal(p, p2)
