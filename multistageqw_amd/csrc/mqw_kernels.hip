@@ -1594,6 +1594,31 @@ int launch_combine(const CombineJob* jobs_dev, uint32_t njobs, const CombineItem
 // ===========================================================================
 typedef float mqw_f32x16 __attribute__((ext_vector_type(16)));
 
+// One step of a reduce-scatter over the 32 lanes of each half of a wave, NV
+// values per lane: the lane keeps the half of its values whose index bit
+// (log2 NV - 1) equals its lane bit log2 D, plus its partner's at distance D;
+// once one value is left, the remaining steps add the partner's (all-reduce).
+template <int NV, int D>
+__device__ __forceinline__ void norm_reduce_step(float* v, int col)
+{
+    if constexpr (D > 0) {
+        if constexpr (NV >= 2) {
+            constexpr int H = NV / 2;
+            const bool up = (col & D) != 0;
+#pragma unroll
+            for (int i = 0; i < H; i++) {
+                const float send = up ? v[i] : v[i + H];
+                const float keep = up ? v[i + H] : v[i];
+                v[i] = keep + __shfl_xor(send, D, 64);
+            }
+            norm_reduce_step<H, D / 2>(v, col);
+        } else {
+            v[0] += __shfl_xor(v[0], D, 64);
+            norm_reduce_step<1, D / 2>(v, col);
+        }
+    }
+}
+
 template <int MB>
 __global__ void __launch_bounds__(512, 4)
 combine2_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const Combine2Item* __restrict__ items,
@@ -1713,6 +1738,7 @@ combine2_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const C
             }
         }
         // lane (col, half) holds rows 8 (r / 4) + 4 half + r % 4 of column col
+        float sq[16 * MB];
 #pragma unroll
         for (int mb = 0; mb < MB; mb++)
 #pragma unroll
@@ -1724,11 +1750,18 @@ combine2_kernel(const Combine2Tile* __restrict__ tiles, uint32_t ntiles, const C
                     o[a] = re;
                     o[N + a] = im;
                 }
-                float sq = re * re + im * im;
-#pragma unroll
-                for (int d = 1; d < 32; d <<= 1) sq += __shfl_xor(sq, d, 64);
-                if (col == 0) s_n[w][j] += (double)sq;
+                sq[16 * mb + r] = re * re + im * im;
             }
+        // summed over the half's 32 lanes (see combine2h_kernel)
+        norm_reduce_step<16 * MB, 16>(sq, col);
+        constexpr int KQ = MB >= 2 ? MB / 2 : 1;
+#pragma unroll
+        for (int k = 0; k < KQ; k++) {
+            const int q = MB >= 2 ? KQ * col + k : col >> 1;
+            const int mb = q >> 4, r = q & 15;
+            const int j = mb * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+            if (MB >= 2 || (col & 1) == 0) s_n[w][j] += (double)sq[k];
+        }
     }
     __syncthreads();
     if (t < ns) {
@@ -1845,31 +1878,6 @@ __device__ __forceinline__ void lds_copy16(const void* src, uint32_t lds)
                  :
                  : "s"(lds), "v"(src)
                  : "memory", "m0");
-}
-
-// One step of a reduce-scatter over the 32 lanes of each half of a wave, NV
-// values per lane: the lane keeps the half of its values whose index bit
-// (log2 NV - 1) equals its lane bit log2 D, plus its partner's at distance D;
-// once one value is left, the remaining steps add the partner's (all-reduce).
-template <int NV, int D>
-__device__ __forceinline__ void norm_reduce_step(float* v, int col)
-{
-    if constexpr (D > 0) {
-        if constexpr (NV >= 2) {
-            constexpr int H = NV / 2;
-            const bool up = (col & D) != 0;
-#pragma unroll
-            for (int i = 0; i < H; i++) {
-                const float send = up ? v[i] : v[i + H];
-                const float keep = up ? v[i + H] : v[i];
-                v[i] = keep + __shfl_xor(send, D, 64);
-            }
-            norm_reduce_step<H, D / 2>(v, col);
-        } else {
-            v[0] += __shfl_xor(v[0], D, 64);
-            norm_reduce_step<1, D / 2>(v, col);
-        }
-    }
 }
 
 template <int MB, int NW>
